@@ -1,0 +1,196 @@
+"""dg_amd — MI355X-native DG right-hand side and RK4 engine (Python bindings).
+
+Thin ctypes layer over the C ABI in include/dgx.h (libdgx.so, built in-tree by
+``python -m dg_amd.build``).  Device buffers are passed as raw pointers;
+tests and the benchmark use torch tensors for device memory and streams.
+
+The library must be present: nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdgx.so")
+
+SINGLEFLUID, MULTIFLUID = 0, 1
+RUSANOV, ROE = 0, 1
+PERIODIC, FARFIELD, REFLECTIVE = 1, 2, 3
+LIMIT_NONE, LIMIT_HR, LIMIT_HRI = 0, 1, 3
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("physics", ctypes.c_int), ("flux", ctypes.c_int)]
+
+    @property
+    def nfields(self) -> int:
+        return 2 + self.dim + self.physics
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libdgx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m dg_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp, i = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_double, ctypes.c_int
+        L.dgx_last_error.restype = ctypes.c_char_p
+        L.dgx_mesh_build.argtypes = [i, i, i, i, dp, dp, dp, dp, ip, i, ctypes.POINTER(vp)]
+        L.dgx_mesh_free.argtypes = [vp]
+        L.dgx_mesh_sizes.argtypes = [vp, ip]
+        L.dgx_mesh_dx.argtypes = [vp]
+        L.dgx_mesh_dx.restype = dp
+        L.dgx_mesh_f64.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]
+        L.dgx_mesh_f64.restype = ctypes.POINTER(ctypes.c_double)
+        L.dgx_mesh_i32.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]
+        L.dgx_mesh_i32.restype = ctypes.POINTER(ctypes.c_int)
+        L.dgx_solver_create.argtypes = [ctypes.POINTER(Config), vp, ctypes.POINTER(vp)]
+        L.dgx_solver_free.argtypes = [vp]
+        L.dgx_solver_state_size.argtypes = [vp]
+        L.dgx_solver_state_size.restype = ctypes.c_long
+        L.dgx_rhs.argtypes = [vp, vp, vp, vp]
+        L.dgx_set_sensor_thresholds.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double)]
+        L.dgx_limit.argtypes = [vp, i, vp, vp]
+        L.dgx_sensing.argtypes = [vp, vp, vp, vp, vp]
+        L.dgx_dt_from_cfl.argtypes = [vp, vp, dp, vp, vp]
+        L.dgx_rk4_step.argtypes = [vp, vp, vp, i, vp]
+        L.dgx_rk4_integrate.argtypes = [vp, vp, i, dp, i, dp, i, ip, ctypes.POINTER(dp), vp]
+        L.dgx_set_config.argtypes = [ctypes.POINTER(Config)]
+        _lib = L
+    return _lib
+
+
+class DgxError(RuntimeError):
+    pass
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().dgx_last_error().decode(errors="replace")
+        raise DgxError(f"{what} failed ({status}): {msg}")
+
+
+SIZE_NAMES = ["D", "order", "N_s", "N_G", "M_s", "M_G", "N_N", "N_E", "M_T", "M_B", "N_F", "N_s1D", "nx", "ny",
+              "rflctiveIdx"]
+F64_TABLES = ["phi", "dphi", "phi_w", "dphi_w", "psi", "psi_w", "weight", "points", "pointsF", "weightF",
+              "ref_nodes", "xyz", "J", "invJac", "JF", "normals", "Minv", "Lag2Mono", "Mono2Lag", "Lag2MonoX",
+              "MonoX2MonoY", "MonoY2Lag", "Lag2MonoY", "MonoX2Lag"]
+I32_TABLES = ["map", "invmap", "boundaryMap", "neighbors", "face_elem", "face_closure", "face_tag", "closures"]
+
+
+class Mesh:
+    """Cartesian mesh + DG tables in the reference layouts (host copies)."""
+
+    def __init__(self, dim: int, order: int, nx: int, ny: int = 1, x0: float = 0.0, x1: float = 1.0,
+                 y0: float = 0.0, y1: float = 1.0, bc=(FARFIELD, FARFIELD, FARFIELD, FARFIELD), nfields: int = 4):
+        L = lib()
+        h = ctypes.c_void_p()
+        bcs = (ctypes.c_int * 4)(*(list(bc) + [FARFIELD] * (4 - len(bc))))
+        _check(L.dgx_mesh_build(dim, order, nx, ny, x0, x1, y0, y1, bcs, nfields, ctypes.byref(h)), "dgx_mesh_build")
+        self._h = h
+        z = (ctypes.c_int * 16)()
+        L.dgx_mesh_sizes(h, z)
+        self.sizes = {n: z[k] for k, n in enumerate(SIZE_NAMES)}
+        self.dx = L.dgx_mesh_dx(h)
+        self.bc = tuple(bc)
+        self.extent = (x0, x1, y0, y1)
+
+    def __getattr__(self, name):
+        if name.startswith("_") or name not in F64_TABLES + I32_TABLES:
+            raise AttributeError(name)
+        return self.array(name)
+
+    def array(self, name: str) -> np.ndarray:
+        L = lib()
+        n = ctypes.c_long()
+        if name in F64_TABLES:
+            p = L.dgx_mesh_f64(self._h, name.encode(), ctypes.byref(n))
+            if not p or n.value == 0:
+                return np.zeros(0)
+            return np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+        p = L.dgx_mesh_i32(self._h, name.encode(), ctypes.byref(n))
+        if not p or n.value == 0:
+            return np.zeros(0, dtype=np.int32)
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+
+    def tables(self) -> dict:
+        d = {n: self.array(n) for n in F64_TABLES + I32_TABLES}
+        d.update(self.sizes)
+        d["Dx"] = self.dx
+        return d
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().dgx_mesh_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _ptr(x) -> int:
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+class Solver:
+    """Fused engine on one GPU (DG_SOLVER + RK + Limiting + SENSOR)."""
+
+    def __init__(self, config: Config, mesh: Mesh):
+        self.config = config
+        self.mesh = mesh
+        h = ctypes.c_void_p()
+        _check(lib().dgx_solver_create(ctypes.byref(config), mesh._h, ctypes.byref(h)), "dgx_solver_create")
+        self._h = h
+        self.state_size = lib().dgx_solver_state_size(h)
+
+    def set_sensor_thresholds(self, thresholds) -> None:
+        th = (ctypes.c_double * 3)(*(list(thresholds) + [0.0] * (3 - len(thresholds))))
+        _check(lib().dgx_set_sensor_thresholds(self._h, len(thresholds), th), "dgx_set_sensor_thresholds")
+
+    def rhs(self, U, f_out, stream=None) -> None:
+        _check(lib().dgx_rhs(self._h, _ptr(U), _ptr(f_out), _stream(stream)), "dgx_rhs")
+
+    def limit(self, limiter: int, U, stream=None) -> None:
+        _check(lib().dgx_limit(self._h, limiter, _ptr(U), _stream(stream)), "dgx_limit")
+
+    def sensing(self, U, sensors, Uavg=None, stream=None) -> None:
+        _check(lib().dgx_sensing(self._h, _ptr(U), _ptr(sensors), _ptr(Uavg), _stream(stream)), "dgx_sensing")
+
+    def dt_from_cfl(self, U, cfl: float, dt_dev, stream=None) -> None:
+        _check(lib().dgx_dt_from_cfl(self._h, _ptr(U), cfl, _ptr(dt_dev), _stream(stream)), "dgx_dt_from_cfl")
+
+    def rk4_step(self, U, dt_dev, limiter: int = 0, stream=None) -> None:
+        _check(lib().dgx_rk4_step(self._h, _ptr(U), _ptr(dt_dev), limiter, _stream(stream)), "dgx_rk4_step")
+
+    def rk4_integrate(self, U, mode: int, dt_or_cfl: float, nsteps: int = 0, tf: float = 0.0, limiter: int = 0,
+                      stream=None):
+        steps = ctypes.c_int()
+        tfin = ctypes.c_double()
+        _check(lib().dgx_rk4_integrate(self._h, _ptr(U), mode, dt_or_cfl, nsteps, tf, limiter, ctypes.byref(steps),
+                                       ctypes.byref(tfin), _stream(stream)), "dgx_rk4_integrate")
+        return steps.value, tfin.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().dgx_solver_free(self._h)
+                self._h = None
+        except Exception:
+            pass

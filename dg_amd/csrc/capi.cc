@@ -1,0 +1,346 @@
+// Host side of the C ABI (include/dgx.h): setup tables, solver lifecycle,
+// and the RK4 driver (RK::RK_integration, rk.cc:11-247).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dgx_internal.h"
+
+namespace dgx {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+}  // namespace dgx
+
+using namespace dgx;
+
+struct dgx_mesh {
+  Tables t;
+};
+
+struct dgx_solver {
+  Solver s;
+};
+
+namespace {
+
+template <typename T>
+bool upload(Solver& s, T** dst, const T* src, size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  if (!hip_ok(hipMalloc(&p, n * sizeof(T)), "hipMalloc")) return false;
+  s.allocations.push_back(p);
+  *dst = static_cast<T*>(p);
+  if (src) return hip_ok(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  return true;
+}
+
+template <typename T>
+bool alloc(Solver& s, T** dst, size_t n) {
+  return upload<T>(s, dst, nullptr, n);
+}
+
+void free_solver(Solver& s) {
+  for (void* p : s.allocations) (void)hipFree(p);
+  s.allocations.clear();
+}
+
+hipStream_t as_stream(void* st) { return reinterpret_cast<hipStream_t>(st); }
+
+dgx_status fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dgx_last_error(void) { return g_err.c_str(); }
+
+int dgx_nfields(const dgx_config* c) { return 2 + c->dim + c->physics; }
+
+// ------------------------------------------------------------------ mesh
+dgx_status dgx_mesh_build(int dim, int order, int nx, int ny, double x0, double x1, double y0, double y1,
+                          const int* bc, int nfields, dgx_mesh** out) {
+  if (!out) return fail(DGX_EINVAL, "dgx_mesh_build: null output");
+  MeshSpec sp;
+  sp.dim = dim;
+  sp.order = order;
+  sp.nx = nx;
+  sp.ny = ny;
+  sp.x0 = x0; sp.x1 = x1; sp.y0 = y0; sp.y1 = y1;
+  for (int k = 0; k < 4; ++k) sp.bc[k] = bc ? bc[k < 2 * dim ? k : 0] : 2;
+  sp.nfields = nfields;
+  dgx_mesh* m = new dgx_mesh;
+  std::string err = build_tables(sp, m->t);
+  if (!err.empty()) {
+    delete m;
+    return fail(DGX_EINVAL, "dgx_mesh_build: " + err);
+  }
+  *out = m;
+  return DGX_OK;
+}
+
+void dgx_mesh_free(dgx_mesh* m) { delete m; }
+
+void dgx_mesh_sizes(const dgx_mesh* m, int* z) {
+  const Tables& t = m->t;
+  int v[16] = {t.D, t.order, t.N_s, t.N_G, t.M_s, t.M_G, t.N_N, t.N_E, t.M_T, t.M_B, t.N_F, t.N_s1D, t.nx, t.ny,
+               t.rflctiveIdx, 0};
+  std::memcpy(z, v, sizeof(v));
+}
+
+double dgx_mesh_dx(const dgx_mesh* m) { return m->t.Dx; }
+
+const double* dgx_mesh_f64(const dgx_mesh* m, const char* name, long* count) {
+  const Tables& t = m->t;
+  const std::map<std::string, const std::vector<double>*> tab = {
+      {"phi", &t.phi}, {"dphi", &t.dphi}, {"phi_w", &t.phi_w}, {"dphi_w", &t.dphi_w}, {"psi", &t.psi},
+      {"psi_w", &t.psi_w}, {"weight", &t.weight}, {"points", &t.points}, {"pointsF", &t.pointsF},
+      {"weightF", &t.weightF}, {"ref_nodes", &t.ref_nodes}, {"xyz", &t.xyz}, {"J", &t.J}, {"invJac", &t.invJac},
+      {"JF", &t.JF}, {"normals", &t.normals}, {"Minv", &t.Minv}, {"Lag2Mono", &t.Lag2Mono},
+      {"Mono2Lag", &t.Mono2Lag}, {"Lag2MonoX", &t.Lag2MonoX}, {"MonoX2MonoY", &t.MonoX2MonoY},
+      {"MonoY2Lag", &t.MonoY2Lag}, {"Lag2MonoY", &t.Lag2MonoY}, {"MonoX2Lag", &t.MonoX2Lag}};
+  auto it = tab.find(name);
+  if (it == tab.end()) return nullptr;
+  if (count) *count = (long)it->second->size();
+  return it->second->data();
+}
+
+const int* dgx_mesh_i32(const dgx_mesh* m, const char* name, long* count) {
+  const Tables& t = m->t;
+  const std::map<std::string, const std::vector<int>*> tab = {
+      {"map", &t.map}, {"invmap", &t.invmap}, {"boundaryMap", &t.boundaryMap}, {"neighbors", &t.neighbors},
+      {"face_elem", &t.face_elem}, {"face_closure", &t.face_closure}, {"face_tag", &t.face_tag},
+      {"closures", &t.closures}};
+  auto it = tab.find(name);
+  if (it == tab.end()) return nullptr;
+  if (count) *count = (long)it->second->size();
+  return it->second->data();
+}
+
+// ---------------------------------------------------------------- solver
+dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_solver** out) {
+  if (!cfg || !mesh || !out) return fail(DGX_EINVAL, "dgx_solver_create: null argument");
+  const Tables& t = mesh->t;
+  if (cfg->dim != t.D || cfg->order != t.order)
+    return fail(DGX_EINVAL, "dgx_solver_create: configuration dim/order do not match the mesh");
+  if (cfg->physics < 0 || cfg->physics > 1 || cfg->flux < 0 || cfg->flux > 1)
+    return fail(DGX_EINVAL, "dgx_solver_create: unknown physics or flux");
+  dgx_solver* h = new dgx_solver;
+  Solver& s = h->s;
+  s.cfg = *cfg;
+  s.D = t.D; s.P = t.order; s.NF = dgx_nfields(cfg);
+  s.N_s = t.N_s; s.N_G = t.N_G; s.M_s = t.M_s; s.M_G = t.M_G; s.N_N = t.N_N; s.N_E = t.N_E; s.M_T = t.M_T;
+  s.N1 = t.order + 1;
+  s.Dx = t.Dx;
+  const int D = s.D, Ns = s.N_s, NN = s.N_N, NE = s.N_E;
+
+  // per-element geometry, compacted (affine elements: values at g=0)
+  std::vector<double> J(NE), ij((size_t)NE * D * D);
+  bool uniform_geo = true;
+  for (int e = 0; e < NE; ++e) {
+    J[e] = t.J[e];
+    for (int k = 0; k < D * D; ++k) ij[(size_t)e * D * D + k] = t.invJac[(size_t)e * t.N_G * D * D + k];
+    if (e > 0 && (std::memcmp(&J[e], &J[0], 8) != 0 ||
+                  std::memcmp(&ij[(size_t)e * D * D], &ij[0], 8 * D * D) != 0))
+      uniform_geo = false;
+  }
+  bool uniform_minv = true;
+  for (int e = 1; e < NE && uniform_minv; ++e)
+    if (std::memcmp(&t.Minv[(size_t)e * Ns * Ns], &t.Minv[0], 8 * (size_t)Ns * Ns) != 0) uniform_minv = false;
+  s.t.geo_stride = uniform_geo ? 0 : 1;
+  s.t.minv_stride = uniform_minv ? 0 : 1;
+
+  // element -> (face, side) per local edge, reversed-closure bit
+  std::vector<int> ef((size_t)NE * NN, -1);
+  for (int f = 0; f < t.M_T; ++f)
+    for (int d = 0; d < 2; ++d) {
+      if (d == 1 && t.face_tag[f] != 0) continue;
+      int e = t.face_elem[f * 2 + d], cl = t.face_closure[f * 2 + d];
+      int k = (D == 1) ? cl : cl % 4;
+      int rev = (D == 2 && cl >= 4) ? 1 : 0;
+      if (ef[(size_t)e * NN + k] != -1) {
+        free_solver(s);
+        delete h;
+        return fail(DGX_EINVAL, "dgx_solver_create: element edge shared by two contributing faces");
+      }
+      ef[(size_t)e * NN + k] = ((f * 2 + d) << 1) | rev;
+    }
+  for (size_t k = 0; k < ef.size(); ++k)
+    if (ef[k] < 0) {
+      free_solver(s);
+      delete h;
+      return fail(DGX_EINVAL, "dgx_solver_create: element edge without a face");
+    }
+
+  bool ok = upload(s, &s.t.phi, t.phi.data(), t.phi.size()) && upload(s, &s.t.dphi, t.dphi.data(), t.dphi.size()) &&
+            upload(s, &s.t.phi_w, t.phi_w.data(), t.phi_w.size()) &&
+            upload(s, &s.t.dphi_w, t.dphi_w.data(), t.dphi_w.size()) &&
+            upload(s, &s.t.psi, t.psi.data(), t.psi.size()) && upload(s, &s.t.psi_w, t.psi_w.data(), t.psi_w.size()) &&
+            upload(s, &s.t.face_elem, t.face_elem.data(), t.face_elem.size()) &&
+            upload(s, &s.t.face_closure, t.face_closure.data(), t.face_closure.size()) &&
+            upload(s, &s.t.face_tag, t.face_tag.data(), t.face_tag.size()) &&
+            upload(s, &s.t.normals, t.normals.data(), t.normals.size()) &&
+            upload(s, &s.t.JF, t.JF.data(), t.JF.size()) && upload(s, &s.t.elem_faces, ef.data(), ef.size()) &&
+            upload(s, &s.t.J, J.data(), uniform_geo ? 1 : J.size()) &&
+            upload(s, &s.t.invJac, ij.data(), uniform_geo ? (size_t)D * D : ij.size()) &&
+            upload(s, &s.t.Minv, t.Minv.data(), uniform_minv ? (size_t)Ns * Ns : t.Minv.size()) &&
+            upload(s, &s.t.neighbors, t.neighbors.data(), t.neighbors.size());
+  if (ok && D == 1)
+    ok = upload(s, &s.t.Lag2Mono, t.Lag2Mono.data(), t.Lag2Mono.size()) &&
+         upload(s, &s.t.Mono2Lag, t.Mono2Lag.data(), t.Mono2Lag.size());
+  if (ok && D == 2)
+    ok = upload(s, &s.t.Lag2MonoX, t.Lag2MonoX.data(), t.Lag2MonoX.size()) &&
+         upload(s, &s.t.MonoX2MonoY, t.MonoX2MonoY.data(), t.MonoX2MonoY.size()) &&
+         upload(s, &s.t.MonoY2Lag, t.MonoY2Lag.data(), t.MonoY2Lag.size()) &&
+         upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
+         upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
+  const size_t S = (size_t)s.state_size();
+  ok = ok && alloc(s, &s.Qtcj, (size_t)2 * s.NF * s.M_s * s.M_T) && alloc(s, &s.Uacc, S) && alloc(s, &s.Ustar, S) &&
+       alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
+       alloc(s, &s.sensors, (size_t)NE) && alloc(s, &s.sens_tgt, (size_t)2 * NE) &&
+       alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.dscratch, 8);
+  if (ok) ok = hip_ok(hipMemset(s.sensors, 0, sizeof(int) * NE), "hipMemset");
+  if (!ok) {
+    free_solver(s);
+    delete h;
+    return DGX_EDEVICE;
+  }
+  *out = h;
+  return DGX_OK;
+}
+
+void dgx_solver_free(dgx_solver* h) {
+  if (!h) return;
+  free_solver(h->s);
+  delete h;
+}
+
+long dgx_solver_state_size(const dgx_solver* h) { return h ? h->s.state_size() : 0; }
+
+dgx_status dgx_rhs(dgx_solver* h, const double* U, double* f_rk, void* stream) {
+  if (!h || !U || !f_rk) return fail(DGX_EINVAL, "dgx_rhs: null argument");
+  Solver& s = h->s;
+  hipStream_t st = as_stream(stream);
+  if (!hip_ok(launch_face(s, U, st), "dgx_rhs face kernel")) return DGX_EDEVICE;
+  if (!hip_ok(launch_elem(s, 0, 0, U, U, nullptr, nullptr, nullptr, f_rk, st), "dgx_rhs element kernel"))
+    return DGX_EDEVICE;
+  return DGX_OK;
+}
+
+dgx_status dgx_set_sensor_thresholds(dgx_solver* h, int n, const double* th) {
+  if (!h || n < 0 || n > 3 || (n > 0 && !th)) return fail(DGX_EINVAL, "dgx_set_sensor_thresholds: bad arguments");
+  h->s.n_thresh = n;
+  for (int k = 0; k < 3; ++k) h->s.thresh[k] = k < n ? th[k] : 0.0;
+  return DGX_OK;
+}
+
+static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
+  if (limiter == 0) return DGX_OK;
+  if (limiter == 1) return hip_ok(launch_limit_hr(s, U, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
+  if (limiter == 3) {
+    if (s.cfg.physics != DGX_MULTIFLUID)
+      return fail(DGX_EUNSUPPORTED, "hri limiting needs the multifluid sensor (single-fluid sensors read past Uavg in "
+                                    "the reference, sensor_kernels.cu:55-60)");
+    return hip_ok(launch_limit_hri(s, U, st), "HRI limiter") ? DGX_OK : DGX_EDEVICE;
+  }
+  return fail(DGX_EUNSUPPORTED, "limiter must be 0 (none), 1 (hrl) or 3 (hri)");
+}
+
+dgx_status dgx_limit(dgx_solver* h, int limiter, double* U, void* stream) {
+  if (!h || !U) return fail(DGX_EINVAL, "dgx_limit: null argument");
+  return do_limit(h->s, limiter, U, as_stream(stream));
+}
+
+dgx_status dgx_sensing(dgx_solver* h, const double* U, int* sensors, double* Uavg, void* stream) {
+  if (!h || !U || !sensors) return fail(DGX_EINVAL, "dgx_sensing: null argument");
+  return hip_ok(launch_sensing(h->s, U, sensors, Uavg, as_stream(stream)), "sensing") ? DGX_OK : DGX_EDEVICE;
+}
+
+dgx_status dgx_dt_from_cfl(dgx_solver* h, const double* U, double cfl, double* dt_dev, void* stream) {
+  if (!h || !U || !dt_dev) return fail(DGX_EINVAL, "dgx_dt_from_cfl: null argument");
+  return hip_ok(launch_upa_max(h->s, U, cfl, dt_dev, as_stream(stream)), "dt kernel") ? DGX_OK : DGX_EDEVICE;
+}
+
+static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, hipStream_t st) {
+  dgx_status r;
+  for (int k = 0; k < 4; ++k) {
+    const double* Uin = (k == 0) ? U : s.Ustar;
+    if (k > 0 && (r = do_limit(s, limiter, s.Ustar, st)) != DGX_OK) return r;
+    if (!hip_ok(launch_face(s, Uin, st), "face kernel")) return DGX_EDEVICE;
+    double* Uout = (k < 3) ? s.Ustar : U;
+    if (!hip_ok(launch_elem(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "element kernel")) return DGX_EDEVICE;
+  }
+  return do_limit(s, limiter, U, st);
+}
+
+dgx_status dgx_rk4_step(dgx_solver* h, double* U, const double* dt_dev, int limiter, void* stream) {
+  if (!h || !U || !dt_dev) return fail(DGX_EINVAL, "dgx_rk4_step: null argument");
+  return rk4_step(h->s, U, dt_dev, limiter, as_stream(stream));
+}
+
+dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cfl, int nsteps, double tf, int limiter,
+                             int* steps, double* t_final, void* stream) {
+  if (!h || !U) return fail(DGX_EINVAL, "dgx_rk4_integrate: null argument");
+  Solver& s = h->s;
+  hipStream_t st = as_stream(stream);
+  double* dt_dev = s.dscratch + 1;
+  dgx_status r;
+  int n = 0;
+  double T = 0;
+  if (mode == 0) {
+    if (!hip_ok(hipMemcpyAsync(dt_dev, &dt_or_cfl, sizeof(double), hipMemcpyHostToDevice, st), "dt upload") ||
+        !hip_ok(hipStreamSynchronize(st), "dt upload"))
+      return DGX_EDEVICE;
+    for (; n < nsteps; ++n) {
+      if ((r = rk4_step(s, U, dt_dev, limiter, st)) != DGX_OK) return r;
+      T = T + dt_or_cfl;
+    }
+  } else if (mode == 1) {
+    if ((r = do_limit(s, limiter, U, st)) != DGX_OK) return r;
+    bool done = false;
+    while (!done) {
+      if (!hip_ok(launch_upa_max(s, U, dt_or_cfl, dt_dev, st), "dt kernel")) return DGX_EDEVICE;
+      double dt = 0;
+      if (!hip_ok(hipMemcpyAsync(&dt, dt_dev, sizeof(double), hipMemcpyDeviceToHost, st), "dt readback") ||
+          !hip_ok(hipStreamSynchronize(st), "dt readback"))
+        return DGX_EDEVICE;
+      if (dt < 1e-14 || dt != dt) {
+        if (steps) *steps = n;
+        if (t_final) *t_final = T;
+        return fail(DGX_EDT, "time step too small or NaN (rk.cc:147-148)");
+      }
+      if (dt > (tf - T)) {
+        dt = tf - T;
+        done = true;
+        if (!hip_ok(hipMemcpyAsync(dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, st), "dt clamp") ||
+            !hip_ok(hipStreamSynchronize(st), "dt clamp"))
+          return DGX_EDEVICE;
+      }
+      if ((r = rk4_step(s, U, dt_dev, limiter, st)) != DGX_OK) return r;
+      T = T + dt;
+      ++n;
+    }
+  } else {
+    return fail(DGX_EINVAL, "dgx_rk4_integrate: mode must be 0 (fixed Dt) or 1 (CFL)");
+  }
+  if (!hip_ok(hipStreamSynchronize(st), "rk4 integrate")) return DGX_EDEVICE;
+  if (steps) *steps = n;
+  if (t_final) *t_final = T;
+  return DGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,578 @@
+// Hierarchical-reconstruction limiter and troubled-cell sensor kernels.
+//
+// Compiled without FMA contraction: the limiter branches on signs and
+// magnitudes (minmod, limiting_kernels.cu:1470-1486) and the sensor flags are
+// compared bit for bit, so these kernels reproduce the CPU path's rounding
+// exactly for identical inputs.
+//
+// Global HR (Limiting::HRlimiting, limiting.cc:11-50):
+//   A = L2M * U (k_apply), per-slice limiting (k_hrl = hrl1D,
+//   limiting_kernels.cu:245-319), U = M2L * Alim (k_apply); in 2D the x sweep
+//   is followed by MonoX2MonoY and the y sweep.
+// Sensor (SENSOR::sensing, sensor.cc:11-36 + calc_sensors,
+//   sensor_kernels.cu:19-163): the reference loops over elements in order and
+//   each element skips the test if an earlier element already flagged it,
+//   then overwrites its flagged neighbour.  The result depends on that order.
+//   Here it is reproduced deterministically in three phases:
+//     k_sensor_hit    per element, the first neighbour test that fires
+//                     (independent of order);
+//     k_sensor_relax  "active" = not flagged by an earlier active element,
+//                     iterated to its fixed point (a DAG in element order);
+//     k_sensor_final  each element takes the value of its last writer in
+//                     element order.
+// Sensor-gated HR (Limiting::HRIlimiting, limiting.cc:596-634; hri1D,
+//   limiting_kernels.cu:322-427; copy_detected, sensor_kernels.cu:167-200).
+#pragma clang fp contract(off)
+#include <type_traits>
+
+#include "dgx_internal.h"
+#include "physics.cuh"
+
+namespace dgx {
+
+__device__ inline int signum_d(double v) { return v > 0 ? 1 : (v < 0 ? -1 : 0); }
+
+__device__ inline double minmod_d(double a, double b) {
+  int sa = signum_d(a);
+  if (sa != signum_d(b)) return 0;
+  double fa = fabs(a), fb = fabs(b);
+  if (fa < fb) return sa * fa;
+  return sa * fb;
+}
+
+__host__ __device__ constexpr int ifact(int n) { return n <= 1 ? 1 : n * ifact(n - 1); }
+__host__ __device__ constexpr long ipow(long b, int n) { return n == 0 ? 1 : b * ipow(b, n - 1); }
+
+// integral over [-1,1] of the k-th derivative of x^n/n! (limiting_kernels.cu:1748-1760)
+__device__ inline double integ_deriv(int k, int n) {
+  int num = n - k + 1;
+  if (num % 2) return 2.0 / (double)ifact(num);
+  return 0.0;
+}
+// over [a,b] with integer a,b (limiting_kernels.cu:1762-1775); powers are exact
+__device__ inline double integ_deriv_bounds(int k, int n, int a, int b) {
+  int num = n - k + 1;
+  return ((double)ipow(b, num) - (double)ipow(a, num)) / (double)ifact(num);
+}
+
+template <int N>
+__device__ inline void limit_monomial_d(const double* AL, const double* AC, const double* AR, double* Alim) {
+  double avgLC = 0;
+#pragma unroll
+  for (int m = N; m > 0; m--) {
+    double avgdUL = 0, avgdUC = 0, avgdUR = 0, avgRL = 0, avgRC = 0, avgRR = 0;
+#pragma unroll
+    for (int n = m - 1; n <= N; n++) {
+      double integral = integ_deriv(m - 1, n);
+      avgdUL += AL[n] * integral;
+      avgdUC += AC[n] * integral;
+      avgdUR += AR[n] * integral;
+      if (n >= m + 1) {
+        double alim = Alim[n];
+        avgRL += alim * integ_deriv_bounds(m - 1, n, -3, -1);
+        avgRC += alim * integral;
+        avgRR += alim * integ_deriv_bounds(m - 1, n, 1, 3);
+      }
+    }
+    double avgLL = 0.5 * (avgdUL - avgRL);
+    avgLC = 0.5 * (avgdUC - avgRC);
+    double avgLR = 0.5 * (avgdUR - avgRR);
+    double c1 = 0.5 * (avgLC - avgLL);
+    double c2 = 0.5 * (avgLR - avgLC);
+    Alim[m] = minmod_d(c1, c2);
+  }
+  Alim[0] = avgLC;
+}
+
+template <int N>
+__device__ inline void average_monomial_d(const double* A, double* Alim) {
+  double avg = 0;
+#pragma unroll
+  for (int n = 0; n <= N; n++) {
+    avg += A[n] * integ_deriv(0, n);
+    Alim[n] = 0;
+  }
+  Alim[0] = 0.5 * avg;
+}
+
+// Out[:,col] = M * In[:,col] for every column (e,f), netlib dgemm order.
+template <int Ns>
+__global__ __launch_bounds__(256) void k_apply(const double* __restrict__ M, const double* __restrict__ In,
+                                               double* __restrict__ Out, long ncol) {
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  double x[Ns], y[Ns];
+#pragma unroll
+  for (int l = 0; l < Ns; ++l) x[l] = In[col * Ns + l];
+#pragma unroll
+  for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+#pragma unroll
+  for (int l = 0; l < Ns; ++l)
+#pragma unroll
+    for (int r = 0; r < Ns; ++r) y[r] = y[r] + x[l] * M[r + l * Ns];
+#pragma unroll
+  for (int r = 0; r < Ns; ++r) Out[col * Ns + r] = y[r];
+}
+
+// hrl1D: one thread per (element, slice, field); N = polynomial order,
+// slices of N+1 coefficients.  ns_slice = N+1, slicenum = 1 (1D) or N+1 (2D).
+template <int N>
+__global__ __launch_bounds__(256) void k_hrl(int N_E, int NF, int N_N, int slicenum, const int* __restrict__ nb,
+                                             int offxy, const double* __restrict__ A, double* __restrict__ Alim) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)N_E * slicenum * NF;
+  if (tid >= total) return;
+  const int fc = (int)(tid % NF);
+  const int slice = (int)((tid / NF) % slicenum);
+  const int e = (int)(tid / ((long)NF * slicenum));
+  constexpr int Nsl = N + 1;
+  const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
+  int physical = 0;
+  if (left < 0) physical = -left;
+  if (right < 0) physical = -right;
+  const long idx = ((long)e * NF + fc) * Nsl * slicenum + slice * Nsl;
+  if (physical == 4) {
+    // unchanged (reference leaves Alim as it was)
+  } else if (physical == 2) {
+    average_monomial_d<N>(&A[idx], &Alim[idx]);
+  } else if (physical == 3) {
+    int un_idx = (offxy == 0) ? 1 : (offxy == 2 ? 2 : 0);
+    if (fc != un_idx)
+      average_monomial_d<N>(&A[idx], &Alim[idx]);
+    else
+      for (int n = 0; n <= N; n++) Alim[idx + n] = A[idx + n];
+  } else {
+    const long idxL = ((long)left * NF + fc) * Nsl * slicenum + slice * Nsl;
+    const long idxR = ((long)right * NF + fc) * Nsl * slicenum + slice * Nsl;
+    double al[Nsl], ac[Nsl], ar[Nsl], lim[Nsl];
+#pragma unroll
+    for (int n = 0; n < Nsl; ++n) { al[n] = A[idxL + n]; ac[n] = A[idx + n]; ar[n] = A[idxR + n]; lim[n] = 0; }
+    limit_monomial_d<N>(al, ac, ar, lim);
+#pragma unroll
+    for (int n = 0; n < Nsl; ++n) Alim[idx + n] = lim[n];
+  }
+}
+
+// ---------------------------------------------------------------- sensors
+// Uavg = sum_l (1/N_s * U_l), sequential (blasGemm M=1, sensor.cc:29)
+template <int Ns>
+__global__ __launch_bounds__(256) void k_uavg(const double* __restrict__ U, long ncol, double* __restrict__ Uavg) {
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const double w = 1.0 / (double)Ns;
+  double c = 0.0;
+#pragma unroll
+  for (int l = 0; l < Ns; ++l) c = c + (w * U[col * Ns + l]) * 1.0;
+  Uavg[col] = c;
+}
+
+struct SensorTh {
+  int on[3];
+  double th[3];
+};
+
+// First test that fires for element e (as if e is tested): tgt = neighbour
+// index, -1 for a boundary neighbour (flags only e), -2 for none.
+template <int D, int PH>
+__global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int* __restrict__ nb,
+                                                    const double* __restrict__ Uavg, SensorTh s,
+                                                    int* __restrict__ tgt, int* __restrict__ val) {
+  constexpr int NF = Fields<D, PH>::NF;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N_E) return;
+  double vyL = 0, vyR = 0, betaL = 0, betaR = 0;
+  int fc = 0;
+  double rhoL = Uavg[e * NF + fc]; fc++;
+  double vxL = Uavg[e * NF + fc] / rhoL; fc++;
+  if (D == 2) { vyL = Uavg[e * NF + fc] / rhoL; fc++; }
+  double EtL = Uavg[e * NF + fc]; fc++;
+  double alphaL = (PH == 0) ? 1 / (kGammaSF - 1) : Uavg[e * NF + fc];
+  double gammaL = 1.0 + 1.0 / alphaL;
+  double pinfL = (1 - 1.0 / gammaL) * betaL;
+  double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
+  double iL = pL * alphaL;
+  double aL = sqrt((gammaL * (pL + pinfL)) / rhoL);
+  double HL = (EtL + pL) / rhoL;
+  int t_out = -2, v_out = 0;
+  for (int nn = 0; nn < N_N; nn++) {
+    int right = nb[e * N_N + nn];
+    if (right < 0) { t_out = -1; v_out = 1; break; }
+    fc = 0;
+    double rhoR = Uavg[right * NF + fc]; fc++;
+    double vxR = Uavg[right * NF + fc] / rhoR; fc++;
+    if (D == 2) { vyR = Uavg[right * NF + fc] / rhoR; fc++; }
+    double EtR = Uavg[right * NF + fc]; fc++;
+    double alphaR = (PH == 0) ? 1 / (kGammaSF - 1) : Uavg[right * NF + fc];
+    double gammaR = 1.0 + 1.0 / alphaR;
+    double pinfR = (1 - 1.0 / gammaR) * betaR;
+    double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
+    double iR = pR * alphaR;
+    double aR = sqrt((gammaR * (pR + pinfR)) / rhoR);
+    double HR = (EtR + pR) / rhoR;
+    double RT = sqrt(rhoR / rhoL);
+    double vx = (vxL + RT * vxR) / (1 + RT);
+    double vy = (vyL + RT * vyR) / (1 + RT);
+    double alpha = (alphaL + RT * alphaR) / (1 + RT);
+    double gamma = 1 + 1.0 / alpha;
+    double H = (HL + RT * HR) / (1 + RT);
+    double a = sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
+    double i = (iL + RT * iR) / (1 + RT);
+    if (s.on[0]) {
+      double drho = rhoR - rhoL;
+      double dp = (gamma - 1) * (gamma - 1) * (alpha * (iR - iL) - i * (alphaR - alphaL));
+      double dV2 = drho - dp / (a * a);
+      double G = fabs(dV2) / (rhoL + rhoR);
+      double PHI = 2 * G / ((1 + G) * (1 + G));
+      if (PHI > s.th[0]) { t_out = right; v_out = 1; break; }
+    }
+    if (s.on[1]) {
+      if (((vxL - aL > vx - a) && (vx - a > vxR - aR)) || ((vyL - aL > vy - a) && (vy - a > vyR - aR))) {
+        double PSI = fabs(pR - pL) / (pL + pR);
+        double W = 2 * PSI / ((1 + PSI) * (1 + PSI));
+        if (W > s.th[1]) { t_out = right; v_out = 2; break; }
+      }
+    }
+    if (s.on[2]) {
+      double G = fabs(gammaR - gammaL) / (gammaL + gammaR);
+      double PHI = 2 * G / ((1 + G) * (1 + G));
+      if (PHI > s.th[2]) { t_out = right; v_out = 3; break; }
+    }
+  }
+  tgt[e] = t_out;
+  val[e] = v_out;
+}
+
+// act_in/act_out: 1 if the element runs its tests in the sequential order.
+__global__ __launch_bounds__(256) void k_sensor_relax(int N_E, int N_N, const int* __restrict__ nb,
+                                                      const int* __restrict__ init, const int* __restrict__ tgt,
+                                                      const int* __restrict__ act_in, int* __restrict__ act_out,
+                                                      int* __restrict__ changed) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N_E) return;
+  int a = init[e] == 0 ? 1 : 0;
+  if (a) {
+    for (int nn = 0; nn < N_N; ++nn) {
+      int o = nb[e * N_N + nn];
+      if (o >= 0 && o < e && act_in[o] && tgt[o] == e) { a = 0; break; }
+    }
+  }
+  act_out[e] = a;
+  if (a != act_in[e]) atomicOr(changed, 1);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int* p, long n, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// last writer in element order wins; untouched elements keep init.
+__global__ __launch_bounds__(256) void k_sensor_final(int N_E, int N_N, const int* __restrict__ nb,
+                                                      const int* __restrict__ init, const int* __restrict__ tgt,
+                                                      const int* __restrict__ val, const int* __restrict__ act,
+                                                      int* __restrict__ sensors) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N_E) return;
+  int best = -1, bval = init[x];
+  if (act[x] && tgt[x] != -2) { best = x; bval = val[x]; }
+  for (int nn = 0; nn < N_N; ++nn) {
+    int o = nb[x * N_N + nn];
+    if (o >= 0 && act[o] && tgt[o] == x && o > best) { best = o; bval = val[o]; }
+  }
+  sensors[x] = bval;
+}
+
+// hri1D: one thread per (element, field); only flagged elements.
+template <int D, int P>
+__global__ __launch_bounds__(128) void k_hri(int N_E, int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                             const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                             const int* __restrict__ sensors, const double* __restrict__ U,
+                                             double* __restrict__ Unew) {
+  constexpr int Ns = Shape<D, P>::Ns, N1 = P + 1, N = P;
+  constexpr int slicenum = (D == 1) ? 1 : N1;
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)N_E * NF) return;
+  const int e = (int)(tid / NF), fc = (int)(tid % NF);
+  if (sensors[e] == 0) return;
+  const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
+  int physical = 0;
+  if (left < 0) physical = -left;
+  if (right < 0) physical = -right;
+  double UC[Ns];
+#pragma unroll
+  for (int i = 0; i < Ns; ++i) UC[i] = U[((long)e * NF + fc) * Ns + i];
+  bool avg = false, full = false;
+  if (physical == 4) {
+  } else if (physical == 2) {
+    avg = true;
+  } else if (physical == 3) {
+    int un_idx = (offxy == 0) ? 1 : (offxy == 2 ? 2 : 0);
+    avg = (fc != un_idx);
+  } else {
+    full = true;
+  }
+  if (avg) {
+    double A[Ns], Alim[Ns];
+#pragma unroll
+    for (int m = 0; m < Ns; ++m) {
+      double s = 0;
+#pragma unroll
+      for (int n = 0; n < Ns; ++n) s += L2M[n * Ns + m] * UC[n];
+      A[m] = s;
+    }
+#pragma unroll
+    for (int sl = 0; sl < slicenum; ++sl) average_monomial_d<N>(&A[sl * N1], &Alim[sl * N1]);
+#pragma unroll
+    for (int m = 0; m < Ns; ++m) {
+      double s = 0;
+#pragma unroll
+      for (int n = 0; n < Ns; ++n) s += M2L[n * Ns + m] * Alim[n];
+      UC[m] = s;
+    }
+  } else if (full) {
+    double UL[Ns], UR[Ns], AL[Ns], AC[Ns], AR[Ns], Alim[Ns];
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) {
+      UL[i] = U[((long)left * NF + fc) * Ns + i];
+      UR[i] = U[((long)right * NF + fc) * Ns + i];
+    }
+#pragma unroll
+    for (int m = 0; m < Ns; ++m) {
+      double s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+      for (int n = 0; n < Ns; ++n) {
+        double a = L2M[n * Ns + m];
+        s1 += a * UL[n];
+        s2 += a * UC[n];
+        s3 += a * UR[n];
+      }
+      AL[m] = s1; AC[m] = s2; AR[m] = s3;
+    }
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) Alim[i] = 0;
+#pragma unroll
+    for (int sl = 0; sl < slicenum; ++sl) limit_monomial_d<N>(&AL[sl * N1], &AC[sl * N1], &AR[sl * N1], &Alim[sl * N1]);
+#pragma unroll
+    for (int m = 0; m < Ns; ++m) {
+      double s = 0;
+#pragma unroll
+      for (int n = 0; n < Ns; ++n) s += M2L[n * Ns + m] * Alim[n];
+      UC[m] = s;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fc) * Ns + i] = UC[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy_detected(int N_E, int per_elem, const int* __restrict__ sensors,
+                                                       const double* __restrict__ Uold, double* __restrict__ U) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)N_E * per_elem) return;
+  const int e = (int)(tid / per_elem);
+  if (sensors[e] != 0) U[tid] = Uold[tid];
+}
+
+// ------------------------------------------------------------ host side
+static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename Fn>
+static hipError_t by_order_ns(int D, int P, Fn&& fn) {
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  if (D == 1) {
+    if (P == 1) return fn(I1{}, I1{});
+    if (P == 2) return fn(I1{}, I2{});
+    return fn(I1{}, I3{});
+  }
+  if (P == 1) return fn(I2{}, I1{});
+  if (P == 2) return fn(I2{}, I2{});
+  return fn(I2{}, I3{});
+}
+
+static hipError_t apply_mat(const Solver& s, const double* M, const double* In, double* Out, hipStream_t st) {
+  long ncol = (long)s.N_E * s.NF;
+  return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    constexpr int Ns = Shape<decltype(d)::value, decltype(p)::value>::Ns;
+    hipLaunchKernelGGL((k_apply<Ns>), dim3(nblk(ncol, 256)), dim3(256), 0, st, M, In, Out, ncol);
+    return hipGetLastError();
+  });
+}
+
+static hipError_t hrl(const Solver& s, int slicenum, int offxy, const double* A, double* Alim, hipStream_t st) {
+  long total = (long)s.N_E * slicenum * s.NF;
+  auto go = [&](auto n) -> hipError_t {
+    hipLaunchKernelGGL((k_hrl<decltype(n)::value>), dim3(nblk(total, 256)), dim3(256), 0, st, s.N_E, s.NF, s.N_N,
+                       slicenum, s.t.neighbors, offxy, A, Alim);
+    return hipGetLastError();
+  };
+  if (s.P == 1) return go(std::integral_constant<int, 1>{});
+  if (s.P == 2) return go(std::integral_constant<int, 2>{});
+  return go(std::integral_constant<int, 3>{});
+}
+
+hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
+  hipError_t e;
+  if (s.D == 1) {
+    if ((e = apply_mat(s, s.t.Lag2Mono, U, s.work, st)) != hipSuccess) return e;
+    if ((e = hrl(s, 1, 0, s.work, s.work2, st)) != hipSuccess) return e;
+    return apply_mat(s, s.t.Mono2Lag, s.work2, U, st);
+  }
+  if ((e = apply_mat(s, s.t.Lag2MonoX, U, s.work, st)) != hipSuccess) return e;
+  if ((e = hrl(s, s.N1, 0, s.work, s.work2, st)) != hipSuccess) return e;
+  if ((e = apply_mat(s, s.t.MonoX2MonoY, s.work2, s.work, st)) != hipSuccess) return e;
+  if ((e = hrl(s, s.N1, 2, s.work, s.work2, st)) != hipSuccess) return e;
+  return apply_mat(s, s.t.MonoY2Lag, s.work2, U, st);
+}
+
+// Resolve sequential-order flags from per-element first hits.
+static hipError_t sensor_resolve(Solver& s, const int* nb, const int* init, const int* tgt, const int* val,
+                                 int* sensors, hipStream_t st) {
+  int* act0 = s.sens_act;
+  int* act1 = s.sens_act + s.N_E;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, act0, (long)s.N_E, 1);
+  // Jacobi sweeps converge to the sequential fixed point after (longest chain + 1) sweeps.
+  for (int it = 0; it <= s.N_E; ++it) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag, 1L, 0);
+    hipLaunchKernelGGL(k_sensor_relax, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt, act0,
+                       act1, s.flag);
+    int h = 0;
+    hipError_t e = hipMemcpyAsync(&h, s.flag, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    std::swap(act0, act1);
+    if (!h) break;
+  }
+  hipLaunchKernelGGL(k_sensor_final, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt, val, act0,
+                     sensors);
+  return hipGetLastError();
+}
+
+hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st) {
+  if (!Uavg) Uavg = s.Uavg;
+  long ncol = (long)s.N_E * s.NF;
+  hipError_t e = by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    constexpr int Ns = Shape<decltype(d)::value, decltype(p)::value>::Ns;
+    hipLaunchKernelGGL((k_uavg<Ns>), dim3(nblk(ncol, 256)), dim3(256), 0, st, U, ncol, Uavg);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
+  SensorTh th{};
+  for (int k = 0; k < 3; ++k) { th.on[k] = k < s.n_thresh; th.th[k] = s.thresh[k]; }
+  int* tgt = s.sens_tgt;
+  int* val = s.sens_tgt + s.N_E;
+  auto hit = [&](auto d, auto ph) -> hipError_t {
+    hipLaunchKernelGGL((k_sensor_hit<decltype(d)::value, decltype(ph)::value>), dim3(nblk(s.N_E, 256)), dim3(256), 0,
+                       st, s.N_E, s.N_N, s.t.neighbors, Uavg, th, tgt, val);
+    return hipGetLastError();
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  if (s.D == 1) e = s.cfg.physics ? hit(I1{}, I1{}) : hit(I1{}, I0{});
+  else e = s.cfg.physics ? hit(I2{}, I1{}) : hit(I2{}, I0{});
+  if (e != hipSuccess) return e;
+  // sensors start at zero (sensor.cc:17-20)
+  int* init = s.sens_act + 2 * (long)s.N_E;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, init, (long)s.N_E, 0);
+  return sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st);
+}
+
+static hipError_t hri(Solver& s, int offxy, const double* L2M, const double* M2L, double* U, hipStream_t st) {
+  long n = (long)s.N_E * s.NF;
+  hipError_t e = by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    hipLaunchKernelGGL((k_hri<decltype(d)::value, decltype(p)::value>), dim3(nblk(n, 128)), dim3(128), 0, st, s.N_E,
+                       s.NF, s.N_N, s.t.neighbors, offxy, L2M, M2L, s.sensors, U, s.work);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
+  int per = s.NF * s.N_s;
+  hipLaunchKernelGGL(k_copy_detected, dim3(nblk((long)s.N_E * per, 256)), dim3(256), 0, st, s.N_E, per, s.sensors,
+                     s.work, U);
+  return hipGetLastError();
+}
+
+hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  if (e != hipSuccess) return e;
+  if (s.D == 1) return hri(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
+  if ((e = hri(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
+  return hri(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);
+}
+
+// exported for the reference-named launchers (compat.hip)
+hipError_t compat_hrl(int N_s, int N_E, int Nfields, int N_N, int slicenum, const int* nb, int offxy, const double* A,
+                      double* Alim) {
+  long total = (long)N_E * slicenum * Nfields;
+  int N = (slicenum == 1) ? N_s - 1 : N_s - 1;
+  auto go = [&](auto n) -> hipError_t {
+    hipLaunchKernelGGL((k_hrl<decltype(n)::value>), dim3(nblk(total, 256)), dim3(256), 0, 0, N_E, Nfields, N_N,
+                       slicenum, nb, offxy, A, Alim);
+    return hipGetLastError();
+  };
+  switch (N) {
+    case 1: return go(std::integral_constant<int, 1>{});
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t compat_sensors(Solver& tmp, const int* nb, const double* Uavg, int* sensors, const int* on, const double* th_in,
+                          int* scratch) {
+  // scratch: 5*N_E ints: tgt, val, act0, act1, init
+  SensorTh th{};
+  for (int k = 0; k < 3; ++k) { th.on[k] = on[k]; th.th[k] = th_in[k]; }
+  int* tgt = scratch;
+  int* val = scratch + tmp.N_E;
+  auto hit = [&](auto d, auto ph) -> hipError_t {
+    hipLaunchKernelGGL((k_sensor_hit<decltype(d)::value, decltype(ph)::value>), dim3(nblk(tmp.N_E, 256)), dim3(256),
+                       0, 0, tmp.N_E, tmp.N_N, nb, Uavg, th, tgt, val);
+    return hipGetLastError();
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  hipError_t e;
+  if (tmp.D == 1) e = tmp.cfg.physics ? hit(I1{}, I1{}) : hit(I1{}, I0{});
+  else e = tmp.cfg.physics ? hit(I2{}, I1{}) : hit(I2{}, I0{});
+  if (e != hipSuccess) return e;
+  tmp.sens_act = scratch + 2 * tmp.N_E;
+  int* init = scratch + 4 * tmp.N_E;
+  hipMemcpyAsync(init, sensors, sizeof(int) * tmp.N_E, hipMemcpyDeviceToDevice, 0);
+  return sensor_resolve(tmp, nb, init, tgt, val, sensors, 0);
+}
+
+hipError_t compat_hri(int N_s, int N_E, int NF, int N_N, const int* nb, int N_s1D, int slicenum, int offxy, int D,
+                      const double* L2M, const double* M2L, const int* sensors, const double* U, double* Unew) {
+  long n = (long)N_E * NF;
+  int P = N_s1D - 1;
+  (void)slicenum; (void)N_s;
+  return by_order_ns(D, P, [&](auto d, auto p) -> hipError_t {
+    hipLaunchKernelGGL((k_hri<decltype(d)::value, decltype(p)::value>), dim3(nblk(n, 128)), dim3(128), 0, 0, N_E, NF,
+                       N_N, nb, offxy, L2M, M2L, sensors, U, Unew);
+    return hipGetLastError();
+  });
+}
+
+hipError_t compat_uavg(int Ns, const double* U, long ncol, double* Uavg) {
+  auto go = [&](auto n) -> hipError_t {
+    hipLaunchKernelGGL((k_uavg<decltype(n)::value>), dim3(nblk(ncol, 256)), dim3(256), 0, 0, U, ncol, Uavg);
+    return hipGetLastError();
+  };
+  switch (Ns) {
+    case 2: return go(std::integral_constant<int, 2>{});
+    case 3: return go(std::integral_constant<int, 3>{});
+    case 4: return go(std::integral_constant<int, 4>{});
+    case 9: return go(std::integral_constant<int, 9>{});
+    case 16: return go(std::integral_constant<int, 16>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t compat_copy_detected(int N_s, int N_E, int NF, const int* sensors, const double* Uold, double* U) {
+  int per = NF * N_s;
+  hipLaunchKernelGGL(k_copy_detected, dim3(nblk((long)N_E * per, 256)), dim3(256), 0, 0, N_E, per, sensors, Uold, U);
+  return hipGetLastError();
+}
+
+}  // namespace dgx

@@ -1,0 +1,179 @@
+"""Initial states for the benchmark configurations and the parity tests.
+
+Like the reference's initial conditions (init_cond.cc), states are set by
+nodal interpolation at the mesh nodes and returned in the reference layout
+U[(e*N_F + f)*N_s + i].  Multifluid states carry alpha = 1/(gamma-1)
+(GAMNCON).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID, Config, Mesh
+
+GAMMA = 1.4
+
+
+def _pack(mesh: Mesh, fields: list[np.ndarray]) -> np.ndarray:
+    """fields[f] has shape (N_E, N_s) -> U[(e*NF+f)*N_s+i]."""
+    return np.ascontiguousarray(np.stack(fields, axis=1).reshape(-1))
+
+
+def node_xy(mesh: Mesh):
+    s = mesh.sizes
+    xyz = mesh.array("xyz").reshape(s["N_E"], s["N_s"], s["D"])
+    x = xyz[..., 0]
+    y = xyz[..., 1] if s["D"] == 2 else None
+    return x, y
+
+
+def conserved_1d(rho, u, p, gamma, physics):
+    E = p / (gamma - 1.0) + 0.5 * rho * u * u
+    f = [rho, rho * u, E]
+    if physics == MULTIFLUID:
+        f.append(np.broadcast_to(1.0 / (gamma - 1.0), np.shape(rho)).astype(float))
+    return f
+
+
+def conserved_2d(rho, u, v, p, gamma, physics):
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    f = [rho, rho * u, rho * v, E]
+    if physics == MULTIFLUID:
+        f.append(np.broadcast_to(1.0 / (gamma - 1.0), np.shape(rho)).astype(float))
+    return f
+
+
+# ----------------------------------------------------------------- 1D Sod
+def sod_mesh(nx: int = 1000, order: int = 2, physics: int = MULTIFLUID) -> Mesh:
+    return Mesh(1, order, nx, 1, -1.0, 1.0, bc=(FARFIELD, FARFIELD), nfields=2 + 1 + physics)
+
+
+def sod_state(mesh: Mesh, physics: int = MULTIFLUID) -> np.ndarray:
+    """Sod tube, gamma=1.4 both sides, split on each element's first node
+    (buildLRstates_multifluid, init_cond.cc:351-397)."""
+    x, _ = node_xy(mesh)
+    left = (x[:, :1] < 1e-8) & np.ones_like(x, dtype=bool)
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    u = np.zeros_like(x)
+    return _pack(mesh, conserved_1d(rho, u, p, GAMMA, physics))
+
+
+# -------------------------------------------------------- 2D Shu vortex
+def vortex_mesh(n: int = 512, order: int = 2) -> Mesh:
+    return Mesh(2, order, n, n, 0.0, 10.0, 0.0, 10.0, bc=(PERIODIC,) * 4, nfields=4)
+
+
+def vortex_fields(x, y, t: float = 0.0, beta: float = 5.0, xc: float = 5.0, yc: float = 5.0, L: float = 10.0):
+    """Isentropic vortex in a (1,1,1,1) free stream, advected to time t on a
+    periodic [0,L]^2 (exact solution)."""
+    g = GAMMA
+    dx = np.mod(x - xc - t + L / 2, L) - L / 2
+    dy = np.mod(y - yc - t + L / 2, L) - L / 2
+    r2 = dx * dx + dy * dy
+    ex = np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - beta / (2 * math.pi) * ex * dy
+    v = 1.0 + beta / (2 * math.pi) * ex * dx
+    T = 1.0 - (g - 1.0) * beta * beta / (8.0 * g * math.pi * math.pi) * ex * ex
+    rho = T ** (1.0 / (g - 1.0))
+    p = rho ** g
+    return rho, u, v, p
+
+
+def vortex_state(mesh: Mesh, t: float = 0.0) -> np.ndarray:
+    x, y = node_xy(mesh)
+    rho, u, v, p = vortex_fields(x, y, t)
+    return _pack(mesh, conserved_2d(rho, u, v, p, GAMMA, SINGLEFLUID))
+
+
+# ------------------------------------------------- 2D shock-bubble (5 fields)
+def shock_bubble_mesh(nx: int = 1024, ny: int = 512, order: int = 2) -> Mesh:
+    return Mesh(2, order, nx, ny, 0.0, 2.0, 0.0, 1.0, bc=(FARFIELD, FARFIELD, REFLECTIVE, REFLECTIVE), nfields=5)
+
+
+def shock_bubble_state(mesh: Mesh, mach: float = 1.22, xs: float = 0.1, xb: float = 0.5, yb: float = 0.5,
+                       rb: float = 0.25, rho_b: float = 0.138, gamma_b: float = 1.667) -> np.ndarray:
+    """Planar shock at x=xs moving right into quiescent air (rho=1, p=1,
+    gamma=1.4) with a light-gas bubble in pressure equilibrium; post-shock
+    state from the Rankine-Hugoniot relations."""
+    g = GAMMA
+    x, y = node_xy(mesh)
+    M2 = mach * mach
+    p2 = 1.0 + 2 * g / (g + 1) * (M2 - 1.0)
+    rho2 = (g + 1) * M2 / ((g - 1) * M2 + 2.0)
+    c1 = math.sqrt(g)
+    s = mach * c1
+    u2 = s * (1.0 - 1.0 / rho2)
+    # cell-based split on each element's centre, like the reference's per-element ICs
+    xc = x.mean(axis=1, keepdims=True) * np.ones_like(x)
+    yc = y.mean(axis=1, keepdims=True) * np.ones_like(y)
+    shocked = xc < xs
+    bubble = (xc - xb) ** 2 + (yc - yb) ** 2 < rb * rb
+    rho = np.where(shocked, rho2, np.where(bubble, rho_b, 1.0))
+    u = np.where(shocked, u2, 0.0)
+    v = np.zeros_like(x)
+    p = np.where(shocked, p2, 1.0)
+    gam = np.where(bubble & ~shocked, gamma_b, g)
+    E = p / (gam - 1.0) + 0.5 * rho * (u * u + v * v)
+    return _pack(mesh, [rho, rho * u, rho * v, E, 1.0 / (gam - 1.0)])
+
+
+# ------------------------------------------------------- parity test states
+def smooth_random_state(mesh: Mesh, physics: int, seed: int = 0, amp: float = 0.2) -> np.ndarray:
+    """Seeded smooth positive states with O(amp) variations (all fields vary)."""
+    rng = np.random.default_rng(seed)
+    x, y = node_xy(mesh)
+    D = mesh.sizes["D"]
+    x0, x1, y0, y1 = mesh.extent
+    X = (x - x0) / (x1 - x0) * 2 * math.pi
+    Y = ((y - y0) / (y1 - y0) * 2 * math.pi) if D == 2 else np.zeros_like(x)
+
+    def wave():
+        a, b, c, d = rng.uniform(-1, 1, 4)
+        return a * np.sin(X + 2 * math.pi * c) + b * np.cos(Y + 2 * math.pi * d) + 0.3 * np.sin(2 * X + Y)
+
+    rho = 1.0 + amp * wave()
+    p = 1.0 + amp * wave()
+    u = 0.5 * amp * wave() + rng.uniform(-0.3, 0.3)
+    gam = 1.4 + 0.2 * (0.5 + 0.5 * np.tanh(wave())) if physics == MULTIFLUID else np.full_like(x, GAMMA)
+    if D == 1:
+        E = p / (gam - 1.0) + 0.5 * rho * u * u
+        f = [rho, rho * u, E]
+    else:
+        v = 0.5 * amp * wave() + rng.uniform(-0.3, 0.3)
+        E = p / (gam - 1.0) + 0.5 * rho * (u * u + v * v)
+        f = [rho, rho * u, rho * v, E]
+    if physics == MULTIFLUID:
+        f.append(1.0 / (gam - 1.0))
+    return _pack(mesh, f)
+
+
+def riemann_state(mesh: Mesh, physics: int, seed: int = 0, x_split: float | None = None) -> np.ndarray:
+    """Seeded Riemann problem: rho, p log-uniform in [0.1, 10], u in [-2, 2],
+    gamma in [1.1, 1.7] per side (multifluid), split on element first node."""
+    rng = np.random.default_rng(seed)
+    x, y = node_xy(mesh)
+    D = mesh.sizes["D"]
+    x0, x1, _, _ = mesh.extent
+    xs = 0.5 * (x0 + x1) if x_split is None else x_split
+    left = (x[:, :1] < xs) & np.ones_like(x, dtype=bool)
+    rl, rr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+    pl, pr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+    ul, ur = rng.uniform(-2, 2, 2)
+    gl, gr = (rng.uniform(1.1, 1.7, 2) if physics == MULTIFLUID else (GAMMA, GAMMA))
+    rho = np.where(left, rl, rr)
+    p = np.where(left, pl, pr)
+    u = np.where(left, ul, ur)
+    gam = np.where(left, gl, gr)
+    if D == 1:
+        E = p / (gam - 1.0) + 0.5 * rho * u * u
+        f = [rho, rho * u, E]
+    else:
+        v = np.zeros_like(x)
+        E = p / (gam - 1.0) + 0.5 * rho * u * u
+        f = [rho, rho * u, v, E]
+    if physics == MULTIFLUID:
+        f.append(1.0 / (gam - 1.0))
+    return _pack(mesh, f)

@@ -1,0 +1,81 @@
+/* CPU oracle for the DG right-hand side, RK4, HR limiter and sensors.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain FP64 restatement of the
+ * reference's CPU path (marchdf/dg, ARCH=USE_CPU), used as the checker for
+ * the HIP engine by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg.  Nothing in the product links or calls it.
+ *
+ * The reference fixes dimension, physics, flux and field count at compile
+ * time (code/Makefile:147-156); here they are runtime fields of or_cfg.
+ * Supported: D=1,2; SINGLEFLUID and MULTIFLUID (GAMNCON, no mass fractions);
+ * RUS and ROE fluxes.  Every function names the reference lines it follows.
+ */
+#ifndef DG_ORACLE_H
+#define DG_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int D, N_F, physics, flux; /* physics: 0 single fluid, 1 multifluid; flux: 0 Rusanov, 1 Roe */
+  int N_s, N_G, M_s, M_G, N_N, N_E, M_T, N_s1D;
+  double gx, gy;             /* gravity (constants::GLOBAL_GX/GY), zero in every config here */
+} or_cfg;
+
+typedef struct {
+  const double *phi, *dphi, *phi_w, *dphi_w, *psi, *psi_w;
+  const double *J, *invJac, *JF, *normals, *Minv;
+  const int *map, *invmap, *boundaryMap, *neighbors;
+  int rflctiveIdx;
+  const double *Lag2Mono, *Mono2Lag;                                   /* 1D */
+  const double *Lag2MonoX, *MonoX2MonoY, *MonoY2Lag, *Lag2MonoY, *MonoX2Lag; /* 2D */
+} or_tables;
+
+/* netlib dgemm restatement, column-major, alpha/beta as in BLAS */
+void or_dgemm(char ta, char tb, int M, int N, int K, double alpha, const double* A, int lda,
+              const double* B, int ldb, double beta, double* C, int ldc);
+
+/* pieces of DG_SOLVER::dg_solver (reference launcher semantics) */
+void or_mapToFace(const or_cfg* c, const int* map, const double* U, double* UF);
+void or_rflctive(const or_cfg* c, int M_B, const int* boundaryMap, const double* normals, int start, double* UF);
+void or_evaluate_sf(const or_cfg* c, double* s, double* f, const double* Ug, const double* dUg, const double* invJac);
+void or_evaluate_q(const or_cfg* c, double* q, const double* UgF, const double* normals);
+void or_redistribute_sf(const or_cfg* c, double* sJ, double* fJ, const double* s, const double* f,
+                        const double* J, const double* invJac);
+void or_redistribute_q(const or_cfg* c, double* qJ, const double* q, const double* JF);
+void or_mapToElement(const or_cfg* c, const int* invmap, double* Q, const double* Qtcj);
+void or_addSFQ(const or_cfg* c, double* A, const double* S, const double* F, const double* Q);
+void or_solve(const or_cfg* c, double Dt, const double* Minv, const double* f, double* DU);
+void or_findUPA(const or_cfg* c, const double* U, double* UPA);
+
+/* whole right-hand side: f_rk = S + F + Q (dg_solver.h:292-364) */
+void or_rhs(const or_cfg* c, const or_tables* t, const double* U, double* f_rk);
+
+/* Dt = CFL / max|u|+a over nodes (rk.cc:250-272); CFL already scaled by Dx/(2p+1) */
+double or_dt_from_cfl(const or_cfg* c, const double* U, double CFL);
+
+/* limiter pieces */
+void or_hrl1D(int N_s, int N_E, int Nfields, int N_N, int slicenum, const int* neighbors, int offxy,
+              const double* A, double* Alim);
+void or_limit_hr(const or_cfg* c, const or_tables* t, double* U);      /* Limiting::HRlimiting */
+void or_sensing(const or_cfg* c, const int* neighbors, const double* U, const int* on, const double* thresh,
+                int* sensors, double* Uavg);                           /* SENSOR::sensing */
+void or_calc_sensors(const or_cfg* c, const int* on, const double* thresh, const int* neighbors,
+                     const double* Uavg, int* sensors);
+void or_hri1D(const or_cfg* c, const int* neighbors, int N_s1D, int slicenum, int offxy, const double* L2M,
+              const double* M2L, const int* sensors, const double* U, double* Unew);
+void or_copy_detected(const or_cfg* c, const int* sensors, const double* Uold, double* U);
+void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors); /* Limiting::HRIlimiting (sensors returned for inspection) */
+
+/* RK4 time integration (rk.cc:132-247).
+ * mode 0: fixed Dt for nsteps steps; mode 1: CFL-based Dt until T reaches tf (Dt clamped).
+ * limiter: 0 none, 1 hrl, 3 hri.  Returns number of steps taken, or -1 on Dt error.
+ * *t_out receives the final time. dt_hist (may be NULL) receives up to max_hist Dt values. */
+int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
+           int limiter, const int* on, const double* thresh, double* t_out, double* dt_hist, int max_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,148 @@
+"""Pin the CPU oracle to the reference's own right-hand-side kernel files.
+
+The reference's launchers (dg/kernels.cu, physics/physics.cu,
+time_integration/rk_kernels.cu, limiting/limiting_kernels.cu,
+sensors/sensor_kernels.cu, bc/boundaries.cu) are compiled from the reference
+sources as CPU loops (ARCH=USE_CPU), one library per physics configuration
+(oracle/Makefile -> oracle/_ref/).  Every oracle component must equal the
+reference's output bit for bit on the same inputs, and the oracle's whole
+right-hand side must equal the reference launcher sequence of
+DG_SOLVER::dg_solver (dg_solver.h:292-364) composed with netlib-order GEMMs.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from dg_amd import MULTIFLUID, problems
+from tests.helpers import CONFIGS, cfg_id, small_mesh
+
+pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="reference sources not present: _ref not built")
+
+vp = ctypes.c_void_p
+
+
+def P(a):
+    return a.ctypes.data_as(vp)
+
+
+def _setup(c, order, seed, riemann=False, bc=None):
+    d, ph, fl = c
+    mesh = small_mesh(d, order, ph, bc=bc)
+    t = mesh.tables()
+    prob = oracle.Problem(t, ph, fl)
+    U = (problems.riemann_state(mesh, ph, seed) if riemann else problems.smooth_random_state(mesh, ph, seed))
+    return mesh, t, prob, U
+
+
+def _ref_rhs(R, prob, t, U):
+    """DG_SOLVER::dg_solver with the reference launchers + netlib dgemm."""
+    c = prob.cfg
+    L = oracle.lib()
+    NF, Ns, NG, Ms, MG, NE, MT, D = prob.nf, c.N_s, c.N_G, c.M_s, c.M_G, c.N_E, c.M_T, c.D
+    z = lambda n: np.zeros(n)
+    UF, Ug, dUg, UgF = z(2 * NF * Ms * MT), z(NF * NG * NE), z(D * NG * NE * NF), z(2 * NF * MG * MT)
+    s, sJ, S, f, fJ, F = z(NG * NE * NF), z(NG * NE * NF), z(Ns * NE * NF), z(D * NF * NG * NE), z(D * NG * NE * NF), z(Ns * NE * NF)
+    q, qJ, Qt, Q, frk = z(MG * MT * NF * 2), z(MG * MT * NF * 2), z(Ms * MT * NF * 2), z(Ns * NE * NF), z(Ns * NE * NF)
+    k = prob._keep
+    R.LmapToFace(Ms, MT, Ns, P(k["map"]), P(U), P(UF))
+    R.LrflctiveBoundary(Ms, int(t["rflctiveIdx"]), P(k["boundaryMap"]), P(k["normals"]), 0, P(UF))
+    L.or_dgemm(b"N", b"N", NG, NE * NF, Ns, 1.0, P(k["phi"]), NG, P(U), Ns, 0.0, P(Ug), NG)
+    L.or_dgemm(b"N", b"N", NG * D, NE * NF, Ns, 1.0, P(k["dphi"]), NG * D, P(U), Ns, 0.0, P(dUg), NG * D)
+    L.or_dgemm(b"N", b"N", MG, MT * NF * 2, Ms, 1.0, P(k["psi"]), MG, P(UF), Ms, 0.0, P(UgF), MG)
+    R.Levaluate_sf(NG, NE, P(s), P(f), P(Ug), P(dUg), P(k["invJac"]))
+    R.Levaluate_q(MG, MT, P(q), P(UgF), P(k["normals"]))
+    R.Lredistribute_sf(NG, NE, P(sJ), P(fJ), P(s), P(f), P(k["J"]), P(k["invJac"]))
+    L.or_dgemm(b"T", b"N", Ns, NE * NF, NG, 1.0, P(k["phi_w"]), NG, P(sJ), NG, 0.0, P(S), Ns)
+    L.or_dgemm(b"T", b"N", Ns, NE * NF, NG * D, 1.0, P(k["dphi_w"]), NG * D, P(fJ), NG * D, 0.0, P(F), Ns)
+    R.Lredistribute_q(MG, MT, P(qJ), P(q), P(k["JF"]))
+    L.or_dgemm(b"T", b"N", Ms, MT * NF * 2, MG, 1.0, P(k["psi_w"]), MG, P(qJ), MG, 0.0, P(Qt), Ms)
+    R.LmapToElement(Ns, NE, Ms, c.N_N, P(k["invmap"]), P(Q), P(Qt))
+    R.LaddSFQ(Ns, NE, P(frk), P(S), P(F), P(Q))
+    return frk
+
+
+@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_rhs_bitwise(c, order):
+    R = oracle.ref_lib(*c)
+    for seed, riem in ((1, False), (2, True)):
+        mesh, t, prob, U = _setup(c, order, seed, riemann=riem)
+        f_ref = _ref_rhs(R, prob, t, U)
+        f_or = prob.rhs(U)
+        assert np.array_equal(f_ref, f_or), f"max diff {np.abs(f_ref - f_or).max()}"
+        assert np.abs(f_or).max() > 0
+
+
+@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+def test_solve_and_upa_bitwise(c):
+    R = oracle.ref_lib(*c)
+    mesh, t, prob, U = _setup(c, 2, 3)
+    f = prob.rhs(U)
+    du_ref = np.zeros_like(f)
+    R.Lsolver(prob.cfg.N_s, prob.cfg.N_E, ctypes.c_double(0.0123), P(prob._keep["Minv"]), P(f), P(du_ref))
+    assert np.array_equal(du_ref, prob.solve(0.0123, f))
+    upa = np.zeros(prob.cfg.N_E * prob.cfg.N_s)
+    R.LfindUPA(prob.cfg.N_s, prob.cfg.N_E, P(U), P(upa))
+    assert prob.dt_from_cfl(U, 0.7) == 0.7 / upa[np.argmax(np.abs(upa))]
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("dim", [1, 2])
+def test_hrl1D_bitwise(dim, order):
+    c = (dim, MULTIFLUID, 0)
+    R = oracle.ref_lib(*c)
+    mesh, t, prob, U = _setup(c, order, 4, riemann=True)
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal(U.size)
+    Ns, NE, NF, NN = prob.cfg.N_s, prob.cfg.N_E, prob.nf, prob.cfg.N_N
+    nb = prob._keep["neighbors"]
+    if dim == 1:
+        args = [(Ns, 1, 0)]
+    else:
+        args = [(order + 1, order + 1, 0), (order + 1, order + 1, 2)]
+    for ns, sl, off in args:
+        a1 = np.zeros_like(A)
+        a2 = np.zeros_like(A)
+        R.Lhrl1D(ns, NE, NF, NN, sl, P(nb), off, P(A), P(a1))
+        oracle.lib().or_hrl1D(ns, NE, NF, NN, sl, P(nb), off, P(A), P(a2))
+        assert np.array_equal(a1, a2)
+        assert np.abs(a1).max() > 0
+
+
+@pytest.mark.parametrize("c", [x for x in CONFIGS if x[1] == MULTIFLUID], ids=cfg_id)
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_sensors_and_hri_bitwise(c, order):
+    R = oracle.ref_lib(*c)
+    d = c[0]
+    for seed in range(4):
+        mesh, t, prob, U = _setup(c, order, 10 + seed, riemann=True)
+        th = (0.02, 0.02, 0.02)[: 1 + seed % 3]
+        sens_or, uavg_or = prob.sensing(U, th)
+        # reference: blas mean + Lcalc_sensors (sensor.cc:11-36)
+        uavg = np.zeros(prob.cfg.N_E * prob.nf)
+        ones = np.ones(prob.cfg.N_s)
+        oracle.lib().or_dgemm(b"N", b"N", 1, prob.cfg.N_E * prob.nf, prob.cfg.N_s, 1.0 / prob.cfg.N_s, P(ones), 1, P(U),
+                              prob.cfg.N_s, 0.0, P(uavg), 1)
+        assert np.array_equal(uavg, uavg_or)
+        sens = np.zeros(prob.cfg.N_E, np.int32)
+        on = [k < len(th) for k in range(3)]
+        tv = list(th) + [0.0] * (3 - len(th))
+        R.Lcalc_sensors(prob.cfg.N_E, prob.cfg.N_N, ctypes.c_bool(on[0]), ctypes.c_double(tv[0]), ctypes.c_bool(on[1]),
+                        ctypes.c_double(tv[1]), ctypes.c_bool(on[2]), ctypes.c_double(tv[2]), P(prob._keep["neighbors"]),
+                        P(uavg), P(sens))
+        assert np.array_equal(sens, sens_or)
+        # hri1D on the flagged elements, then copy_detected
+        Unew_ref = np.zeros_like(U)
+        L2M = prob._keep["Lag2Mono" if d == 1 else "Lag2MonoX"]
+        M2L = prob._keep["Mono2Lag" if d == 1 else "MonoX2Lag"]
+        ns1 = order + 1
+        R.Lhri1D(prob.cfg.N_s, prob.cfg.N_E, prob.cfg.N_N, P(prob._keep["neighbors"]), ns1 if d == 2 else prob.cfg.N_s,
+                 ns1 if d == 2 else 1, 0, P(L2M), P(M2L), P(sens), P(U), P(Unew_ref))
+        U_ref = U.copy()
+        R.Lcopy_detected(prob.cfg.N_s, prob.cfg.N_E, P(sens), P(Unew_ref), P(U_ref))
+        if d == 1:
+            U_or, s2 = prob.limit_hri(U, th)
+            assert np.array_equal(s2, sens)
+            assert np.array_equal(U_or, U_ref)
