@@ -63,6 +63,8 @@ def lib() -> ctypes.CDLL:
         L.dgx_rk4_step.argtypes = [vp, vp, vp, i, vp]
         L.dgx_rk4_integrate.argtypes = [vp, vp, i, dp, i, dp, i, ip, ctypes.POINTER(dp), vp]
         L.dgx_set_config.argtypes = [ctypes.POINTER(Config)]
+        L.dgx_profile_enable.argtypes = [vp, i]
+        L.dgx_profile_read.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_long)]
         _lib = L
     return _lib
 
@@ -186,6 +188,16 @@ class Solver:
         _check(lib().dgx_rk4_integrate(self._h, _ptr(U), mode, dt_or_cfl, nsteps, tf, limiter, ctypes.byref(steps),
                                        ctypes.byref(tfin), _stream(stream)), "dgx_rk4_integrate")
         return steps.value, tfin.value
+
+    def profile(self, enable: bool = True) -> None:
+        _check(lib().dgx_profile_enable(self._h, 1 if enable else 0), "dgx_profile_enable")
+
+    def profile_read(self, kernel: str):
+        """(total milliseconds, launches) recorded for kernel since the last read."""
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        _check(lib().dgx_profile_read(self._h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), "dgx_profile_read")
+        return ms.value, n.value
 
     def __del__(self):
         try:

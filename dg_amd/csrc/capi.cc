@@ -51,10 +51,51 @@ bool alloc(Solver& s, T** dst, size_t n) {
   return upload<T>(s, dst, nullptr, n);
 }
 
+void release_events(Solver& s) {
+  for (auto& v : s.prof)
+    for (auto& pr : v) {
+      s.ev_pool.push_back(pr.first);
+      s.ev_pool.push_back(pr.second);
+    }
+  for (auto& v : s.prof) v.clear();
+}
+
 void free_solver(Solver& s) {
+  release_events(s);
+  for (hipEvent_t e : s.ev_pool) (void)hipEventDestroy(e);
+  s.ev_pool.clear();
   for (void* p : s.allocations) (void)hipFree(p);
   s.allocations.clear();
 }
+
+hipEvent_t take_event(Solver& s) {
+  if (!s.ev_pool.empty()) {
+    hipEvent_t e = s.ev_pool.back();
+    s.ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with events when profiling is on.
+struct KernelTimer {
+  Solver& s;
+  int slot;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  KernelTimer(Solver& s_, int slot_, hipStream_t st_) : s(s_), slot(slot_), st(st_) {
+    if (s.prof_on && (a = take_event(s))) (void)hipEventRecord(a, st);
+  }
+  ~KernelTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(s);
+    if (!b) return;
+    (void)hipEventRecord(b, st);
+    s.prof[slot].emplace_back(a, b);
+  }
+};
 
 hipStream_t as_stream(void* st) { return reinterpret_cast<hipStream_t>(st); }
 
@@ -164,7 +205,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   s.t.geo_stride = uniform_geo ? 0 : 1;
   s.t.minv_stride = uniform_minv ? 0 : 1;
 
-  // element -> (face, side) per local edge, reversed-closure bit
+  // every element edge must receive exactly one contributing face side
   std::vector<int> ef((size_t)NE * NN, -1);
   for (int f = 0; f < t.M_T; ++f)
     for (int d = 0; d < 2; ++d) {
@@ -194,7 +235,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
             upload(s, &s.t.face_closure, t.face_closure.data(), t.face_closure.size()) &&
             upload(s, &s.t.face_tag, t.face_tag.data(), t.face_tag.size()) &&
             upload(s, &s.t.normals, t.normals.data(), t.normals.size()) &&
-            upload(s, &s.t.JF, t.JF.data(), t.JF.size()) && upload(s, &s.t.elem_faces, ef.data(), ef.size()) &&
+            upload(s, &s.t.JF, t.JF.data(), t.JF.size()) &&
             upload(s, &s.t.J, J.data(), uniform_geo ? 1 : J.size()) &&
             upload(s, &s.t.invJac, ij.data(), uniform_geo ? (size_t)D * D : ij.size()) &&
             upload(s, &s.t.Minv, t.Minv.data(), uniform_minv ? (size_t)Ns * Ns : t.Minv.size()) &&
@@ -209,7 +250,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
          upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
          upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
   const size_t S = (size_t)s.state_size();
-  ok = ok && alloc(s, &s.Qtcj, (size_t)2 * s.NF * s.M_s * s.M_T) && alloc(s, &s.Uacc, S) && alloc(s, &s.Ustar, S) &&
+  ok = ok && alloc(s, &s.Qe, (size_t)NE * NN * s.NF * s.M_s) && alloc(s, &s.Uacc, S) && alloc(s, &s.Ustar, S) &&
        alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
        alloc(s, &s.sensors, (size_t)NE) && alloc(s, &s.sens_tgt, (size_t)2 * NE) &&
        alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.dscratch, 8);
@@ -275,16 +316,28 @@ dgx_status dgx_dt_from_cfl(dgx_solver* h, const double* U, double cfl, double* d
   return hip_ok(launch_upa_max(h->s, U, cfl, dt_dev, as_stream(stream)), "dt kernel") ? DGX_OK : DGX_EDEVICE;
 }
 
+static dgx_status timed_limit(Solver& s, int limiter, double* U, hipStream_t st) {
+  if (limiter == 0) return DGX_OK;
+  KernelTimer kt(s, 2, st);
+  return do_limit(s, limiter, U, st);
+}
+
 static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, hipStream_t st) {
   dgx_status r;
   for (int k = 0; k < 4; ++k) {
     const double* Uin = (k == 0) ? U : s.Ustar;
-    if (k > 0 && (r = do_limit(s, limiter, s.Ustar, st)) != DGX_OK) return r;
-    if (!hip_ok(launch_face(s, Uin, st), "face kernel")) return DGX_EDEVICE;
+    if (k > 0 && (r = timed_limit(s, limiter, s.Ustar, st)) != DGX_OK) return r;
+    {
+      KernelTimer kt(s, 0, st);
+      if (!hip_ok(launch_face(s, Uin, st), "face kernel")) return DGX_EDEVICE;
+    }
     double* Uout = (k < 3) ? s.Ustar : U;
-    if (!hip_ok(launch_elem(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "element kernel")) return DGX_EDEVICE;
+    {
+      KernelTimer kt(s, 1, st);
+      if (!hip_ok(launch_elem(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "element kernel")) return DGX_EDEVICE;
+    }
   }
-  return do_limit(s, limiter, U, st);
+  return timed_limit(s, limiter, U, st);
 }
 
 dgx_status dgx_rk4_step(dgx_solver* h, double* U, const double* dt_dev, int limiter, void* stream) {
@@ -313,7 +366,10 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
     if ((r = do_limit(s, limiter, U, st)) != DGX_OK) return r;
     bool done = false;
     while (!done) {
-      if (!hip_ok(launch_upa_max(s, U, dt_or_cfl, dt_dev, st), "dt kernel")) return DGX_EDEVICE;
+      {
+        KernelTimer kt(s, 3, st);
+        if (!hip_ok(launch_upa_max(s, U, dt_or_cfl, dt_dev, st), "dt kernel")) return DGX_EDEVICE;
+      }
       double dt = 0;
       if (!hip_ok(hipMemcpyAsync(&dt, dt_dev, sizeof(double), hipMemcpyDeviceToHost, st), "dt readback") ||
           !hip_ok(hipStreamSynchronize(st), "dt readback"))
@@ -340,6 +396,37 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
   if (!hip_ok(hipStreamSynchronize(st), "rk4 integrate")) return DGX_EDEVICE;
   if (steps) *steps = n;
   if (t_final) *t_final = T;
+  return DGX_OK;
+}
+
+dgx_status dgx_profile_enable(dgx_solver* h, int enable) {
+  if (!h) return fail(DGX_EINVAL, "dgx_profile_enable: null solver");
+  h->s.prof_on = enable != 0;
+  return DGX_OK;
+}
+
+dgx_status dgx_profile_read(dgx_solver* h, const char* kernel, double* total_ms, long* launches) {
+  if (!h || !kernel) return fail(DGX_EINVAL, "dgx_profile_read: null argument");
+  static const char* names[4] = {"face", "elem", "limit", "dt"};
+  int slot = -1;
+  for (int k = 0; k < 4; ++k)
+    if (std::strcmp(kernel, names[k]) == 0) slot = k;
+  if (slot < 0) return fail(DGX_EINVAL, "dgx_profile_read: kernel must be face, elem, limit or dt");
+  Solver& s = h->s;
+  double tot = 0;
+  for (auto& pr : s.prof[slot]) {
+    if (!hip_ok(hipEventSynchronize(pr.second), "hipEventSynchronize")) return DGX_EDEVICE;
+    float ms = 0;
+    if (!hip_ok(hipEventElapsedTime(&ms, pr.first, pr.second), "hipEventElapsedTime")) return DGX_EDEVICE;
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (launches) *launches = (long)s.prof[slot].size();
+  for (auto& pr : s.prof[slot]) {
+    s.ev_pool.push_back(pr.first);
+    s.ev_pool.push_back(pr.second);
+  }
+  s.prof[slot].clear();
   return DGX_OK;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/dgx.h"
@@ -27,7 +28,6 @@ struct DevTables {
   double* normals = nullptr;    // [t*D+a]
   double* JF = nullptr;         // [t*2+d]
   // elements
-  int* elem_faces = nullptr;    // [e*N_N+k] = ((t*2+d)<<1) | reversed, k = local edge
   double* J = nullptr;          // [e*geo_stride]
   double* invJac = nullptr;     // [(e*geo_stride)*D*D + alpha*D + a]
   double* Minv = nullptr;       // [(e*minv_stride*N_s + k)*N_s + i]
@@ -45,7 +45,7 @@ struct Solver {
   double Dx = 0;
   DevTables t;
   // work buffers
-  double* Qtcj = nullptr;   // face contributions [((t*NF+f)*2+d)*M_s+j]
+  double* Qe = nullptr;     // face contributions, element-major [((e*N_N+k)*NF+f)*M_s+j]
   double* Uacc = nullptr;   // RK accumulator
   double* Ustar = nullptr;  // stage state
   double* work = nullptr;   // limiter scratch (state size)
@@ -59,6 +59,10 @@ struct Solver {
   int n_thresh = 0;
   double thresh[3] = {0, 0, 0};
   std::vector<void*> allocations;
+  // event timing (dgx_profile_*): [0] face, [1] elem, [2] limit, [3] dt
+  bool prof_on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[4];
+  std::vector<hipEvent_t> ev_pool;
   long state_size() const { return (long)N_E * NF * N_s; }
 };
 

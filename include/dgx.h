@@ -139,6 +139,13 @@ dgx_status dgx_rk4_step(dgx_solver* s, double* U, const double* dt_dev, int limi
 dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
                              int limiter, int* steps, double* t_final, void* stream);
 
+/* Per-kernel timing with HIP events recorded on the launch stream (the
+ * reference's TIMERS use std::clock and synchronise after every launch,
+ * timers.cc:71-90).  kernel: "face", "elem", "limit", "dt".  Reading
+ * synchronises the recorded events and resets the counters. */
+dgx_status dgx_profile_enable(dgx_solver* s, int enable);
+dgx_status dgx_profile_read(dgx_solver* s, const char* kernel, double* total_ms, long* launches);
+
 #ifdef __cplusplus
 }
 #endif
