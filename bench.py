@@ -34,8 +34,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="elements per direction")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=256)
-    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU sample mesh size (0: same as --n)")
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB buffer 5 times (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -53,6 +55,17 @@ def main() -> int:
         return 1
     device = torch.device("cuda:0")
     stream = torch.cuda.current_stream(device)
+
+    if args.copy_calibration:
+        nbytes = 256 << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=device)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": nbytes, "copy_calibration_bytes_written": nbytes,
+                          "dispatches": 5}))
+        return 0
 
     order, n = 2, args.n
     mesh = problems.vortex_mesh(n, order)
@@ -160,7 +173,7 @@ def cpu_baseline(args, device):
     import oracle
     from dg_amd import ROE, SINGLEFLUID, Config, Solver, problems
 
-    n = args.cpu_sample_n
+    n = args.cpu_sample_n if args.cpu_sample_n else args.n
     mesh = problems.vortex_mesh(n, 2)
     prob = oracle.Problem(mesh.tables(), SINGLEFLUID, ROE)
     U0 = problems.vortex_state(mesh)
