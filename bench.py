@@ -34,6 +34,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="elements per direction")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", choices=["auto", "general", "tiled"], default="auto",
+                    help="kernel path: fused tile kernel (auto on this mesh) or face + element kernels")
     ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU sample mesh size (0: same as --n)")
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -48,7 +50,7 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from dg_amd import ROE, SINGLEFLUID, Config, Solver, problems
+    from dg_amd import PATH_AUTO, PATH_GENERAL, PATH_TILED, ROE, SINGLEFLUID, Config, Solver, problems
 
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -71,6 +73,8 @@ def main() -> int:
     mesh = problems.vortex_mesh(n, order)
     cfg = Config(2, order, SINGLEFLUID, ROE)
     solver = Solver(cfg, mesh)
+    solver.set_path({"auto": PATH_AUTO, "general": PATH_GENERAL, "tiled": PATH_TILED}[args.path])
+    tiled = solver.path == PATH_TILED
     s = mesh.sizes
     N_E, N_s, NF = s["N_E"], s["N_s"], cfg.nfields
     n_dof = N_E * N_s * NF
@@ -103,29 +107,34 @@ def main() -> int:
 
     # per-kernel timing with HIP events on the launch stream (separate run)
     prof_steps = max(1, min(args.steps, 20))
+    names = ("stage", "elem", "face")
     solver.profile(True)
-    solver.profile_read("elem"), solver.profile_read("face")
+    for k in names:
+        solver.profile_read(k)
     solver.rk4_integrate(U, 0, dt, nsteps=prof_steps, stream=stream)
     torch.cuda.synchronize()
-    elem_ms, elem_n = solver.profile_read("elem")
-    face_ms, face_n = solver.profile_read("face")
+    avg = {}
+    for k in names:
+        ms, cnt = solver.profile_read(k)
+        if cnt:
+            avg["k_" + k] = ms / cnt
     solver.profile(False)
-    elem_avg = elem_ms / max(elem_n, 1)
-    face_avg = face_ms / max(face_n, 1)
-    dominant, dom_avg = ("k_elem", elem_avg) if elem_avg >= face_avg else ("k_face", face_avg)
-    # Algorithmic bytes of one element-kernel launch (one RK stage): the state
-    # vectors of the reference RK data flow with DU fused away, averaged over
-    # the four stages: stage 0 R(U) W(Uacc) W(U*) = 3, stages 1-2 R(U*) R(U)
-    # R(Uacc) W(Uacc) W(U*) = 5, stage 3 R(U*) R(Uacc) W(U) = 3 -> 4 state
-    # vectors of n_dof doubles per launch.
-    elem_bytes = 4.0 * 8.0 * n_dof
+    # Algorithmic bytes of one stage launch (k_stage, or k_elem on the general
+    # path): the state vectors of the reference RK data flow with DU fused
+    # away, averaged over the four stages: stage 0 R(U) W(Uacc) W(U*) = 3,
+    # stages 1-2 R(U*) R(U) R(Uacc) W(Uacc) W(U*) = 5, stage 3 R(U*) R(Uacc)
+    # W(U) = 3 -> 4 state vectors of n_dof doubles per launch.  Halo traces
+    # and geometry are not counted.
+    stage_bytes = 4.0 * 8.0 * n_dof
     # Face kernel: one read of the face traces (<= one state vector) and its
     # per-face nodal contributions (2 sides x N_F x M_s per face).
     face_bytes = 8.0 * (n_dof + 2 * NF * s["M_s"] * s["M_T"])
-    dom_bytes = elem_bytes if dominant == "k_elem" else face_bytes
+    dominant = max(avg, key=avg.get)
+    dom_avg = avg[dominant]
+    dom_bytes = face_bytes if dominant == "k_face" else stage_bytes
     achieved = dom_bytes / (dom_avg * 1e-3) / 1e9
-    # whole-step view (minimum RK4 traffic c = 17 state vectors per step without limiting)
-    step_gbps = 17.0 * 8.0 * n_dof / (ms_per_step * 1e-3) / 1e9
+    # whole-step view: minimum RK4 state traffic, c = 16 state vectors per step
+    step_gbps = 16.0 * 8.0 * n_dof / (ms_per_step * 1e-3) / 1e9
 
     out = {
         "metric": "DOF-updates/s (1 GPU) at fixed L2 err vs FP64 oracle; % HBM/MFMA roofline",
@@ -144,14 +153,15 @@ def main() -> int:
             "workload": f"2D isentropic vortex, periodic [0,10]^2, {n}x{n} quads, p=2 Lagrange, 4 fields, Roe, "
                         f"no limiter, RK4 fixed Dt from CFL 0.3",
             "elements": N_E, "dofs": n_dof, "dt": dt, "parallelism": "single GPU",
+            "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
             "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg,
-            "kernels_ms": {"k_elem": elem_avg, "k_face": face_avg},
+            "kernels_ms": avg,
         },
-        "step_roofline": {"c": 17, "achieved": step_gbps, "frac": step_gbps / HBM_PEAK_GBPS, "unit": "GB/s"},
+        "step_roofline": {"c": 16, "achieved": step_gbps, "frac": step_gbps / HBM_PEAK_GBPS, "unit": "GB/s"},
         "l2_rel_err_rho_vs_exact": l2_exact,
     }
 
@@ -171,7 +181,7 @@ def cpu_baseline(args, device):
     import torch
 
     import oracle
-    from dg_amd import ROE, SINGLEFLUID, Config, Solver, problems
+    from dg_amd import PATH_AUTO, PATH_GENERAL, PATH_TILED, ROE, SINGLEFLUID, Config, Solver, problems
 
     n = args.cpu_sample_n if args.cpu_sample_n else args.n
     mesh = problems.vortex_mesh(n, 2)

@@ -20,6 +20,7 @@ SINGLEFLUID, MULTIFLUID = 0, 1
 RUSANOV, ROE = 0, 1
 PERIODIC, FARFIELD, REFLECTIVE = 1, 2, 3
 LIMIT_NONE, LIMIT_HR, LIMIT_HRI = 0, 1, 3
+PATH_AUTO, PATH_GENERAL, PATH_TILED = 0, 1, 2
 
 
 class Config(ctypes.Structure):
@@ -44,6 +45,7 @@ def lib() -> ctypes.CDLL:
         L.dgx_last_error.restype = ctypes.c_char_p
         L.dgx_mesh_build.argtypes = [i, i, i, i, dp, dp, dp, dp, ip, i, ctypes.POINTER(vp)]
         L.dgx_mesh_free.argtypes = [vp]
+        L.dgx_mesh_tileable.argtypes = [vp]
         L.dgx_mesh_sizes.argtypes = [vp, ip]
         L.dgx_mesh_dx.argtypes = [vp]
         L.dgx_mesh_dx.restype = dp
@@ -63,6 +65,8 @@ def lib() -> ctypes.CDLL:
         L.dgx_rk4_step.argtypes = [vp, vp, vp, i, vp]
         L.dgx_rk4_integrate.argtypes = [vp, vp, i, dp, i, dp, i, ip, ctypes.POINTER(dp), vp]
         L.dgx_set_config.argtypes = [ctypes.POINTER(Config)]
+        L.dgx_solver_set_path.argtypes = [vp, i]
+        L.dgx_solver_path.argtypes = [vp]
         L.dgx_profile_enable.argtypes = [vp, i]
         L.dgx_profile_read.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_long)]
         _lib = L
@@ -103,6 +107,11 @@ class Mesh:
         self.dx = L.dgx_mesh_dx(h)
         self.bc = tuple(bc)
         self.extent = (x0, x1, y0, y1)
+
+    @property
+    def tileable(self) -> bool:
+        """True when solvers on this mesh run the fused tile kernel by default."""
+        return bool(lib().dgx_mesh_tileable(self._h))
 
     def __getattr__(self, name):
         if name.startswith("_") or name not in F64_TABLES + I32_TABLES:
@@ -188,6 +197,15 @@ class Solver:
         _check(lib().dgx_rk4_integrate(self._h, _ptr(U), mode, dt_or_cfl, nsteps, tf, limiter, ctypes.byref(steps),
                                        ctypes.byref(tfin), _stream(stream)), "dgx_rk4_integrate")
         return steps.value, tfin.value
+
+    def set_path(self, path: int) -> None:
+        """PATH_AUTO, PATH_GENERAL (face + element kernels) or PATH_TILED (one fused kernel per stage)."""
+        _check(lib().dgx_solver_set_path(self._h, path), "dgx_solver_set_path")
+
+    @property
+    def path(self) -> int:
+        """Kernel path in use: PATH_GENERAL or PATH_TILED."""
+        return lib().dgx_solver_path(self._h)
 
     def profile(self, enable: bool = True) -> None:
         _check(lib().dgx_profile_enable(self._h, 1 if enable else 0), "dgx_profile_enable")

@@ -34,6 +34,7 @@ SOURCES = [
     ("mesh.cc", "cxx", []),
     ("capi.cc", "hip-host", []),
     ("rhs.hip", "hipcc", []),
+    ("stage.hip", "hipcc", []),
     ("limiter.hip", "hipcc", ["-ffp-contract=off"]),
     ("compat.hip", "hipcc", []),
 ]

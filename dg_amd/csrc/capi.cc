@@ -2,6 +2,7 @@
 // and the RK4 driver (RK::RK_integration, rk.cc:11-247).
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -104,6 +105,72 @@ dgx_status fail(int code, const std::string& msg) {
   return code;
 }
 
+
+// Checks that the mesh is a Cartesian grid whose faces follow the structured
+// pattern the tile kernel assumes (element e = ey*nx + ex; side 0 is the
+// left/lower element, or the boundary element of a boundary face; closures
+// and periodic partners as built by build_tables; normals along the axes)
+// and gathers each face's normal and Jacobians into grid order.  Any
+// mismatch leaves the general path.
+bool cartesian_layout(const Tables& t, Cart& out) {
+  Cart c;
+  const int D = t.D, nx = t.nx, ny = (D == 2) ? t.ny : 1;
+  if (nx * ny != t.N_E) return false;
+  if (!stage_tiles(D, nx, ny, &c.tx, &c.ty)) return false;
+  c.nx = nx;
+  c.ny = ny;
+  for (int k = 0; k < 4; ++k) c.bc[k] = t.bc[k];
+  c.gx.assign((size_t)(nx + 1) * ny * 4, 0.0);
+  c.gy.assign(D == 2 ? (size_t)nx * (ny + 1) * 4 : 4, 0.0);
+  std::vector<char> seen_x((size_t)(nx + 1) * ny, 0), seen_y(D == 2 ? (size_t)nx * (ny + 1) : 0, 0);
+  const bool perx = c.bc[0] == BC_PERIODIC, pery = D == 2 && c.bc[2] == BC_PERIODIC;
+  for (int f = 0; f < t.M_T; ++f) {
+    const int e0 = t.face_elem[2 * f], c0 = t.face_closure[2 * f];
+    const int e1 = t.face_elem[2 * f + 1], c1 = t.face_closure[2 * f + 1], tag = t.face_tag[f];
+    const int ex = e0 % nx, ey = e0 / nx;
+    int we1, wc1, wtag;
+    double wn[2] = {0, 0};
+    size_t slot;
+    bool xface;
+    if (D == 1) {
+      xface = true;
+      if (c0 == 1 && ex < nx - 1) { wtag = 0; we1 = e0 + 1; wc1 = 0; wn[0] = 1; slot = ex + 1; }
+      else if (c0 == 1) { wtag = c.bc[1]; wn[0] = 1; we1 = perx ? 0 : e0; wc1 = perx ? 0 : 1; slot = nx; }
+      else if (c0 == 0 && ex == 0) { wtag = c.bc[0]; wn[0] = -1; we1 = perx ? nx - 1 : e0; wc1 = perx ? 1 : 0; slot = 0; }
+      else return false;
+    } else {
+      xface = (c0 == 1 || c0 == 3);
+      const size_t sx = (size_t)ey * (nx + 1), sy = (size_t)ey * nx + ex;
+      if (c0 == 1 && ex < nx - 1) { wtag = 0; we1 = e0 + 1; wc1 = 7; wn[0] = 1; slot = sx + ex + 1; }
+      else if (c0 == 1) { wtag = c.bc[1]; wn[0] = 1; we1 = perx ? ey * nx : e0; wc1 = perx ? 7 : 1; slot = sx + nx; }
+      else if (c0 == 3 && ex == 0) { wtag = c.bc[0]; wn[0] = -1; we1 = perx ? ey * nx + nx - 1 : e0; wc1 = perx ? 5 : 3; slot = sx; }
+      else if (c0 == 2 && ey < ny - 1) { wtag = 0; we1 = e0 + nx; wc1 = 4; wn[1] = 1; slot = sy + nx; }
+      else if (c0 == 2) { wtag = c.bc[3]; wn[1] = 1; we1 = pery ? ex : e0; wc1 = pery ? 4 : 2; slot = sy + nx; }
+      else if (c0 == 0 && ey == 0) { wtag = c.bc[2]; wn[1] = -1; we1 = pery ? (ny - 1) * nx + ex : e0; wc1 = pery ? 6 : 0; slot = sy; }
+      else return false;
+    }
+    if (tag != wtag || e1 != we1 || c1 != wc1) return false;
+    // normals of a Cartesian grid: the axis direction up to rounding
+    for (int a = 0; a < D; ++a)
+      if (!(std::fabs(t.normals[(size_t)f * D + a] - wn[a]) <= 1e-6)) return false;
+    std::vector<char>& seen = xface ? seen_x : seen_y;
+    if (seen[slot]) return false;
+    seen[slot] = 1;
+    double* g = (xface ? c.gx.data() : c.gy.data()) + slot * 4;
+    g[0] = t.normals[(size_t)f * D];
+    g[1] = (D == 2) ? t.normals[(size_t)f * D + 1] : 0.0;
+    g[2] = t.JF[2 * f];
+    g[3] = t.JF[2 * f + 1];
+  }
+  for (char v : seen_x)
+    if (!v) return false;
+  for (char v : seen_y)
+    if (!v) return false;
+  c.on = true;
+  out = std::move(c);
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -135,6 +202,11 @@ dgx_status dgx_mesh_build(int dim, int order, int nx, int ny, double x0, double 
 }
 
 void dgx_mesh_free(dgx_mesh* m) { delete m; }
+
+int dgx_mesh_tileable(const dgx_mesh* m) {
+  Cart c;
+  return (m && cartesian_layout(m->t, c)) ? 1 : 0;
+}
 
 void dgx_mesh_sizes(const dgx_mesh* m, int* z) {
   const Tables& t = m->t;
@@ -199,11 +271,26 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
                   std::memcmp(&ij[(size_t)e * D * D], &ij[0], 8 * D * D) != 0))
       uniform_geo = false;
   }
+  // one inverse mass matrix when Minv_e = Minv_0 * J_0 / J_e for every
+  // element to rounding (affine elements), else one per element
   bool uniform_minv = true;
-  for (int e = 1; e < NE && uniform_minv; ++e)
-    if (std::memcmp(&t.Minv[(size_t)e * Ns * Ns], &t.Minv[0], 8 * (size_t)Ns * Ns) != 0) uniform_minv = false;
+  {
+    double mmax = 0.0;
+    for (int k = 0; k < Ns * Ns; ++k) mmax = std::max(mmax, std::fabs(t.Minv[k]));
+    for (int e = 1; e < NE && uniform_minv; ++e) {
+      const double r = t.J[0] / t.J[e];
+      const double* Me = &t.Minv[(size_t)e * Ns * Ns];
+      for (int k = 0; k < Ns * Ns; ++k)
+        if (!(std::fabs(Me[k] - t.Minv[k] * r) <= 1e-13 * mmax * std::max(1.0, r))) {
+          uniform_minv = false;
+          break;
+        }
+    }
+  }
   s.t.geo_stride = uniform_geo ? 0 : 1;
   s.t.minv_stride = uniform_minv ? 0 : 1;
+  s.t.J0 = t.J[0];
+  cartesian_layout(t, s.cart);
 
   // every element edge must receive exactly one contributing face side
   std::vector<int> ef((size_t)NE * NN, -1);
@@ -240,6 +327,9 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
             upload(s, &s.t.invJac, ij.data(), uniform_geo ? (size_t)D * D : ij.size()) &&
             upload(s, &s.t.Minv, t.Minv.data(), uniform_minv ? (size_t)Ns * Ns : t.Minv.size()) &&
             upload(s, &s.t.neighbors, t.neighbors.data(), t.neighbors.size());
+  if (ok && s.cart.on)
+    ok = upload(s, &s.t.face_geo_x, s.cart.gx.data(), s.cart.gx.size()) &&
+         upload(s, &s.t.face_geo_y, s.cart.gy.data(), s.cart.gy.size());
   if (ok && D == 1)
     ok = upload(s, &s.t.Lag2Mono, t.Lag2Mono.data(), t.Lag2Mono.size()) &&
          upload(s, &s.t.Mono2Lag, t.Mono2Lag.data(), t.Mono2Lag.size());
@@ -253,7 +343,8 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   ok = ok && alloc(s, &s.Qe, (size_t)NE * NN * s.NF * s.M_s) && alloc(s, &s.Uacc, S) && alloc(s, &s.Ustar, S) &&
        alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
        alloc(s, &s.sensors, (size_t)NE) && alloc(s, &s.sens_tgt, (size_t)2 * NE) &&
-       alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.dscratch, 8);
+       alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.dscratch, 8) &&
+       (!s.cart.on || alloc(s, &s.Ustar2, S));
   if (ok) ok = hip_ok(hipMemset(s.sensors, 0, sizeof(int) * NE), "hipMemset");
   if (!ok) {
     free_solver(s);
@@ -276,6 +367,10 @@ dgx_status dgx_rhs(dgx_solver* h, const double* U, double* f_rk, void* stream) {
   if (!h || !U || !f_rk) return fail(DGX_EINVAL, "dgx_rhs: null argument");
   Solver& s = h->s;
   hipStream_t st = as_stream(stream);
+  if (s.tiled())
+    return hip_ok(launch_stage(s, 0, 0, U, U, nullptr, nullptr, nullptr, f_rk, st), "dgx_rhs tile kernel")
+               ? DGX_OK
+               : DGX_EDEVICE;
   if (!hip_ok(launch_face(s, U, st), "dgx_rhs face kernel")) return DGX_EDEVICE;
   if (!hip_ok(launch_elem(s, 0, 0, U, U, nullptr, nullptr, nullptr, f_rk, st), "dgx_rhs element kernel"))
     return DGX_EDEVICE;
@@ -324,14 +419,26 @@ static dgx_status timed_limit(Solver& s, int limiter, double* U, hipStream_t st)
 
 static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, hipStream_t st) {
   dgx_status r;
+  // The tile kernel reads halo traces of neighbouring tiles while other
+  // blocks write their stage output, so its stages ping-pong between two
+  // buffers (U -> Ustar -> Ustar2 -> Ustar -> U); the face + element path
+  // consumes all traces in the face kernel and updates Ustar in place.
+  const bool tiled = s.tiled();
+  double* cur = U;
   for (int k = 0; k < 4; ++k) {
-    const double* Uin = (k == 0) ? U : s.Ustar;
-    if (k > 0 && (r = timed_limit(s, limiter, s.Ustar, st)) != DGX_OK) return r;
+    const double* Uin = cur;
+    if (k > 0 && (r = timed_limit(s, limiter, cur, st)) != DGX_OK) return r;
+    double* Uout = (k == 3) ? U : ((tiled && k == 1) ? s.Ustar2 : s.Ustar);
+    cur = Uout;
+    if (tiled) {
+      KernelTimer kt(s, 4, st);
+      if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "tile kernel")) return DGX_EDEVICE;
+      continue;
+    }
     {
       KernelTimer kt(s, 0, st);
       if (!hip_ok(launch_face(s, Uin, st), "face kernel")) return DGX_EDEVICE;
     }
-    double* Uout = (k < 3) ? s.Ustar : U;
     {
       KernelTimer kt(s, 1, st);
       if (!hip_ok(launch_elem(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "element kernel")) return DGX_EDEVICE;
@@ -399,6 +506,17 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
   return DGX_OK;
 }
 
+dgx_status dgx_solver_set_path(dgx_solver* h, int path) {
+  if (!h || path < 0 || path > 2) return fail(DGX_EINVAL, "dgx_solver_set_path: path must be 0, 1 or 2");
+  if (path == PATH_TILED && !h->s.cart.on)
+    return fail(DGX_EUNSUPPORTED, "dgx_solver_set_path: the tile kernel needs a uniform Cartesian mesh whose "
+                                  "element counts are multiples of the tile shape");
+  h->s.path = path;
+  return DGX_OK;
+}
+
+int dgx_solver_path(const dgx_solver* h) { return (h && h->s.tiled()) ? PATH_TILED : PATH_GENERAL; }
+
 dgx_status dgx_profile_enable(dgx_solver* h, int enable) {
   if (!h) return fail(DGX_EINVAL, "dgx_profile_enable: null solver");
   h->s.prof_on = enable != 0;
@@ -407,11 +525,11 @@ dgx_status dgx_profile_enable(dgx_solver* h, int enable) {
 
 dgx_status dgx_profile_read(dgx_solver* h, const char* kernel, double* total_ms, long* launches) {
   if (!h || !kernel) return fail(DGX_EINVAL, "dgx_profile_read: null argument");
-  static const char* names[4] = {"face", "elem", "limit", "dt"};
+  static const char* names[5] = {"face", "elem", "limit", "dt", "stage"};
   int slot = -1;
-  for (int k = 0; k < 4; ++k)
+  for (int k = 0; k < 5; ++k)
     if (std::strcmp(kernel, names[k]) == 0) slot = k;
-  if (slot < 0) return fail(DGX_EINVAL, "dgx_profile_read: kernel must be face, elem, limit or dt");
+  if (slot < 0) return fail(DGX_EINVAL, "dgx_profile_read: kernel must be face, elem, limit, dt or stage");
   Solver& s = h->s;
   double tot = 0;
   for (auto& pr : s.prof[slot]) {

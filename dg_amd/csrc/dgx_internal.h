@@ -32,6 +32,14 @@ struct DevTables {
   double* invJac = nullptr;     // [(e*geo_stride)*D*D + alpha*D + a]
   double* Minv = nullptr;       // [(e*minv_stride*N_s + k)*N_s + i]
   int geo_stride = 1, minv_stride = 1;
+  // minv_stride == 0: one matrix, Minv_e = Minv * (J0 / J_e).  The mass
+  // matrix of an affine element is J_e times the reference one, so this holds
+  // to rounding for every element the mesh builder makes; it is verified when
+  // the solver is created.
+  double J0 = 1.0;
+  // tile kernel face geometry: [face*4 + {n_x, n_y, JF side 0, JF side 1}]
+  double* face_geo_x = nullptr;  // faces normal to x, index ey*(nx+1) + fx
+  double* face_geo_y = nullptr;  // faces normal to y, index fy*nx + ex
   // limiter / sensor
   int* neighbors = nullptr;     // [e*N_N + {L,R,D,U}]
   double *Lag2Mono = nullptr, *Mono2Lag = nullptr;
@@ -39,15 +47,32 @@ struct DevTables {
          *MonoX2Lag = nullptr;
 };
 
+// Cartesian meshes whose faces follow the structured pattern (element
+// e = ey*nx + ex, side 0 = left/lower element) run the fused tile kernel of
+// stage.hip, which finds faces from the tile structure and reads their
+// normals and Jacobians from per-face arrays in grid order.
+struct Cart {
+  bool on = false;
+  int nx = 0, ny = 1, tx = 1, ty = 1;
+  int bc[4] = {0, 0, 0, 0};
+  std::vector<double> gx, gy;  // host copies of DevTables::face_geo_x / face_geo_y
+};
+
+enum KernelPath { PATH_AUTO = 0, PATH_GENERAL = 1, PATH_TILED = 2 };
+
 struct Solver {
   dgx_config cfg{};
   int D = 1, P = 1, NF = 3, N_s = 0, N_G = 0, M_s = 0, M_G = 0, N_N = 0, N_E = 0, M_T = 0, N1 = 0;
   double Dx = 0;
   DevTables t;
+  Cart cart;
+  int path = PATH_AUTO;
+  bool tiled() const { return cart.on && path != PATH_GENERAL; }
   // work buffers
   double* Qe = nullptr;     // face contributions, element-major [((e*N_N+k)*NF+f)*M_s+j]
   double* Uacc = nullptr;   // RK accumulator
   double* Ustar = nullptr;  // stage state
+  double* Ustar2 = nullptr; // second stage state (tile path, see rk4_step)
   double* work = nullptr;   // limiter scratch (state size)
   double* work2 = nullptr;  // limiter scratch (state size)
   double* Uavg = nullptr;   // [e*NF+f]
@@ -59,9 +84,9 @@ struct Solver {
   int n_thresh = 0;
   double thresh[3] = {0, 0, 0};
   std::vector<void*> allocations;
-  // event timing (dgx_profile_*): [0] face, [1] elem, [2] limit, [3] dt
+  // event timing (dgx_profile_*): [0] face, [1] elem, [2] limit, [3] dt, [4] stage
   bool prof_on = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[4];
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[5];
   std::vector<hipEvent_t> ev_pool;
   long state_size() const { return (long)N_E * NF * N_s; }
 };
@@ -71,6 +96,10 @@ hipError_t launch_face(const Solver& s, const double* U, hipStream_t st);
 // mode 0: f_rk = S+F+Q to out; mode 1: RK stage `stage` update (see rhs.hip)
 hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st);
+// fused tile kernel (Cartesian meshes): same modes as launch_elem
+bool stage_tiles(int D, int nx, int ny, int* tx, int* ty);
+hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
+                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st);
 hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* dt_dev, hipStream_t st);
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st);

@@ -212,6 +212,7 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
   t.N_F = s.nfields;
   t.nx = s.nx;
   t.ny = (D == 2) ? s.ny : 1;
+  for (int k = 0; k < 4; ++k) t.bc[k] = (D == 2 || k < 2) ? s.bc[k] : 0;
   t.N_s1D = p + 1;
   const std::vector<double> ln = line_nodes(p);
 

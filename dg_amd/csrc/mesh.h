@@ -44,6 +44,7 @@ struct Tables {
   int D = 1, order = 1, N_s = 0, N_G = 0, M_s = 0, M_G = 0, N_N = 0;
   int N_E = 0, M_T = 0, M_B = 0, N_F = 0, N_s1D = 0, nx = 0, ny = 0;
   int rflctiveIdx = 0;  // number of reflective faces at the head of boundaryMap
+  int bc[4] = {0, 0, 0, 0};  // boundary tags of the domain sides (MeshSpec::bc)
   double Dx = 0;        // minimum cell size used by the CFL step (simpleMesh.cc:1140-1190)
   double refArea = 0;
 

@@ -23,6 +23,7 @@
 
 #include "dgx_internal.h"
 #include "physics.cuh"
+#include "volume.cuh"
 
 namespace dgx {
 
@@ -133,6 +134,7 @@ struct ElemArgs {
   const double* __restrict__ invJac;
   const double* __restrict__ Minv;
   int geo_stride, minv_stride;
+  double J0;  // minv_stride == 0: Minv_e = Minv * J0 / J_e
   const double* __restrict__ Qe;
   int N_E;
   int stage;
@@ -144,49 +146,31 @@ struct ElemArgs {
   double* fout;
 };
 
-__device__ constexpr double kRkBeta[4] = {0.0, 0.5, 0.5, 1.0};
-__device__ constexpr double kRkGamma[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
-
-constexpr int pow2ceil(int n) { return n <= 1 ? 1 : 2 * pow2ceil((n + 1) / 2); }
-
 // One block of 256 threads handles EPB elements.  Phase 1: lane = (element,
 // quadrature point); phase 2: lane = (element, node); phase 3: flat over the
 // block's contiguous state (coalesced).  Element state, face contributions,
 // per-point fluxes and the reference tables are staged in LDS.
 template <int D, int P, int PH, int FL, int MODE>
 __global__ __launch_bounds__(256) void k_elem(ElemArgs a) {
-  using S = Shape<D, P>;
-  constexpr int NF = Fields<D, PH>::NF, Ns = S::Ns, NG = S::NG, Ms = S::Ms, NN = S::NN;
-  constexpr int NFX = NF - PH;           // fields that carry a volume flux (alpha has none)
-  constexpr int NU = NF * Ns;            // state doubles per element
-  constexpr int NQE = NN * NF * Ms;      // face-contribution doubles per element
-  constexpr int NQ = NFX * D + PH;       // per-point values handed to phase 2
-  constexpr int TPE = pow2ceil(NG);      // lanes per element in phase 1
-  constexpr int EPB = 256 / TPE;         // elements per block
-
+  using V = Vol<D, P, PH>;
+  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, NU = V::NU, NQE = V::NQE, NQ = V::NQ;
+  constexpr int TPE = V::TPE, EPB = V::EPB;
   __shared__ double sU[EPB * NU];
   __shared__ double sQ[EPB * NQE];
   __shared__ double sF[EPB * NG * NQ];
-  __shared__ double sPhi[NG * Ns];
-  __shared__ double sDphi[PH ? NG * D * Ns : 1];
-  __shared__ double sDw[NG * D * Ns];   // transposed: [(g*D+al)*Ns + i]
-  __shared__ double sPw[PH ? NG * Ns : 1];  // transposed: [g*Ns + i]
+  __shared__ double sPhi[V::NTAB_PHI];
+  __shared__ double sDphi[PH ? V::NTAB_D : 1];
+  __shared__ double sDw[V::NTAB_D];
+  __shared__ double sPw[PH ? V::NTAB_PHI : 1];
   __shared__ double sMinv[Ns * Ns];
+  __shared__ double sScale[EPB];
+  const VolTables<D, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
 
   const int tid = threadIdx.x;
   const int e0 = blockIdx.x * EPB;
   const int n_el = min(EPB, a.N_E - e0);
-
-  for (int k = tid; k < NG * Ns; k += 256) sPhi[k] = a.phi[k];
-  for (int k = tid; k < NG * D * Ns; k += 256) {
-    const int i = k / (NG * D), r = k % (NG * D);
-    sDw[r * Ns + i] = a.dphi_w[k];
-    if constexpr (PH == 1) sDphi[k] = a.dphi[k];
-  }
-  if constexpr (PH == 1)
-    for (int k = tid; k < NG * Ns; k += 256) sPw[(k % NG) * Ns + k / NG] = a.phi_w[k];
-  if (MODE == 1 && a.minv_stride == 0)
-    for (int k = tid; k < Ns * Ns; k += 256) sMinv[k] = a.Minv[k];
+  const bool uniform_minv = a.minv_stride == 0;
+  load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && uniform_minv, tid);
   {
     const double* src = a.Uin + (size_t)e0 * NU;
     for (int k = tid; k < n_el * NU; k += 256) sU[k] = src[k];
@@ -195,127 +179,41 @@ __global__ __launch_bounds__(256) void k_elem(ElemArgs a) {
   }
   __syncthreads();
 
-  // ---- phase 1: fluxes at the quadrature points
-  {
+  {  // phase 1: fluxes at the quadrature points
     const int el = tid / TPE, g = tid % TPE;
     if (el < n_el && g < NG) {
-      const double* u = sU + el * NU;
-      double ug[NF];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < Ns; ++i) s += u[f * Ns + i] * sPhi[g + i * NG];
-        ug[f] = s;
-      }
-      double dua[D];
-#pragma unroll
-      for (int al = 0; al < D; ++al) {
-        double s = 0.0;
-        if constexpr (PH == 1) {
-#pragma unroll
-          for (int i = 0; i < Ns; ++i) s += u[(NF - 1) * Ns + i] * sDphi[(g * D + al) + i * NG * D];
-        }
-        dua[al] = s;
-      }
       const int eg = (e0 + el) * a.geo_stride;
-      const double Je = a.J[eg];
       double ij[D * D];
 #pragma unroll
       for (int k = 0; k < D * D; ++k) ij[k] = a.invJac[eg * D * D + k];
-      double fx[NF * D];
-      double salpha;
-      volume_flux<D, PH>(ug, dua, ij, fx, salpha);
-      double* out = sF + (el * NG + g) * NQ;
-#pragma unroll
-      for (int f = 0; f < NFX; ++f)
-#pragma unroll
-        for (int al = 0; al < D; ++al) {
-          double sol = 0.0;
-#pragma unroll
-          for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
-          out[f * D + al] = sol;
-        }
-      if constexpr (PH == 1) out[NFX * D] = salpha * Je;
+      vol_phase1<D, P, PH>(tb, g, sU + el * NU, a.J[eg], ij, sF + (el * NG + g) * NQ);
     }
   }
   __syncthreads();
-
-  // ---- phase 2: integrate against the test functions, add face terms
-  {
+  {  // phase 2: integrate against the test functions, add face terms
     const int el = tid / TPE, i = tid % TPE;
     double rhs[NF];
-    if (el < n_el && i < Ns) {
-      double acc[NF];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) acc[f] = 0.0;
-      const double* fe = sF + el * NG * NQ;
-#pragma unroll 4
-      for (int g = 0; g < NG; ++g) {
-#pragma unroll
-        for (int al = 0; al < D; ++al) {
-          const double w = sDw[(g * D + al) * Ns + i];
-#pragma unroll
-          for (int f = 0; f < NFX; ++f) acc[f] += w * fe[g * NQ + f * D + al];
-        }
-        if constexpr (PH == 1) acc[NF - 1] += sPw[g * Ns + i] * fe[g * NQ + NFX * D];
-      }
-      const double* qe = sQ + el * NQE;
-      // face contributions landing on node i: at most two (corners), so
-      // the sum does not depend on the order
-      double q[NF];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) q[f] = 0.0;
-#pragma unroll
-      for (int kk = 0; kk < NN; ++kk)
-#pragma unroll
-        for (int jj = 0; jj < Ms; ++jj)
-          if (closure_node<D, P>(kk, jj) == i)
-#pragma unroll
-            for (int f = 0; f < NF; ++f) q[f] += qe[(kk * NF + f) * Ms + jj];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) rhs[f] = acc[f] + q[f];
-    }
-    __syncthreads();  // sU is reused below for the right-hand side
-    if (el < n_el && i < Ns)
+    const bool act = el < n_el && i < Ns;
+    if (act) vol_phase2<D, P, PH>(tb, i, sF + el * NG * NQ, sQ + el * NQE, rhs);
+    if (MODE == 1 && uniform_minv && act && i == 0) sScale[el] = a.J0 / a.J[(e0 + el) * a.geo_stride];
+    __syncthreads();  // sU is reused for the right-hand side
+    if (act)
 #pragma unroll
       for (int f = 0; f < NF; ++f) sU[el * NU + f * Ns + i] = rhs[f];
   }
   __syncthreads();
 
-  // ---- phase 3: coalesced output
+  // phase 3: coalesced output
   const size_t base = (size_t)e0 * NU;
   if constexpr (MODE == 0) {
     for (int k = tid; k < n_el * NU; k += 256) a.fout[base + k] = sU[k];
   } else {
     const double dt = a.dt[0];
-    const int st = a.stage;
-    const double gk = kRkGamma[st];
-    const double bk = (st < 3) ? kRkBeta[st + 1] : 0.0;
     for (int k = tid; k < n_el * NU; k += 256) {
-      const int el = k / NU, r = k % NU, f = r / Ns, i = r % Ns;
-      const double* R = sU + el * NU + f * Ns;
-      double sol = 0.0;
-      if (a.minv_stride == 0) {
-#pragma unroll
-        for (int ii = 0; ii < Ns; ++ii) sol += sMinv[ii * Ns + i] * R[ii];
-      } else {
-        const double* Mi = a.Minv + (size_t)(e0 + el) * Ns * Ns;
-#pragma unroll
-        for (int ii = 0; ii < Ns; ++ii) sol += Mi[ii * Ns + i] * R[ii];
-      }
-      const double du = dt * sol;
-      const size_t idx = base + k;
-      if (st == 0) {
-        const double un = a.Un[idx];
-        a.Uacc[idx] = un + gk * du;
-        a.Uout[idx] = un + bk * du;
-      } else if (st < 3) {
-        a.Uacc[idx] = a.Uacc[idx] + gk * du;
-        a.Uout[idx] = a.Un[idx] + bk * du;
-      } else {
-        a.Uout[idx] = a.Uacc[idx] + gk * du;
-      }
+      const int el = k / NU;
+      const double dte = uniform_minv ? dt * sScale[el] : dt;
+      rk_update<D, P, PH>(tb, a.Minv + (size_t)(e0 + el) * Ns * Ns, uniform_minv, k % NU, sU + el * NU, dte, a.stage,
+                          base + k, a.Un, a.Uacc, a.Uout);
     }
   }
 }
@@ -394,10 +292,10 @@ hipError_t launch_face(const Solver& s, const double* U, hipStream_t st) {
 hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st) {
   ElemArgs a{s.t.phi, s.t.dphi, s.t.phi_w, s.t.dphi_w, s.t.J, s.t.invJac, s.t.Minv, s.t.geo_stride,
-             s.t.minv_stride, s.Qe, s.N_E, stage, Uin, Un, Uacc, Uout, dt_dev, f_out};
+             s.t.minv_stride, s.t.J0, s.Qe, s.N_E, stage, Uin, Un, Uacc, Uout, dt_dev, f_out};
   return dispatch(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
-    constexpr int EPB = 256 / pow2ceil(Shape<D, P>::NG);
+    constexpr int EPB = Vol<D, P, PH>::EPB;
     const unsigned nb = blocks_for(s.N_E, EPB);
     if (mode == 0)
       hipLaunchKernelGGL((k_elem<D, P, PH, FL, 0>), dim3(nb), dim3(256), 0, st, a);

@@ -1,0 +1,419 @@
+// Fused RK stage for Cartesian meshes: one kernel evaluates the whole right-
+// hand side of a tile of elements and applies Dt*Minv and the RK4 update.
+//
+// A 256-thread block owns a TX x TY tile (TX in 1D).  It stages the tile's
+// state and the face traces of its halo in LDS, evaluates every face that
+// touches the tile (faces on tile edges are evaluated by both neighbouring
+// tiles with identical inputs and code), integrates them against psi_w,
+// then runs the element phases of volume.cuh and writes the stage update.
+// Face sides, closures, normals and boundary treatment follow the reference
+// interface construction (simpleMesh.cc:770-989: side 0 = first element in
+// element order, its outward normal; boundary faces contribute to side 0
+// only, dg_functions.cc:225-233), so the result equals the general
+// face/element kernels.  Tiles are mapped to XCDs in contiguous bands so
+// that halo rows are shared through each XCD's L2.
+#include <type_traits>
+
+#include "dgx_internal.h"
+#include "physics.cuh"
+#include "volume.cuh"
+
+namespace dgx {
+
+struct StageArgs {
+  const double* __restrict__ phi;
+  const double* __restrict__ dphi;
+  const double* __restrict__ phi_w;
+  const double* __restrict__ dphi_w;
+  const double* __restrict__ psi;
+  const double* __restrict__ psi_w;
+  const double* __restrict__ Minv;    // [e*minv_stride] N_s x N_s
+  const double* __restrict__ J;       // [e*geo_stride]
+  const double* __restrict__ invJac;  // [(e*geo_stride)*D*D + alpha*D + a]
+  const double* __restrict__ gx;      // faces normal to x: [(ey*(nx+1) + fx)*4 + {n_x, n_y, JF0, JF1}]
+  const double* __restrict__ gy;      // faces normal to y: [(fy*nx + ex)*4 + {n_x, n_y, JF0, JF1}]
+  double J0;                          // Jacobian of the element whose Minv is stored when minv_stride == 0
+  int geo_stride, minv_stride;
+  int nx, ny;
+  int bc[4];  // left, right, bottom, top
+  int tiles_x, tiles_y, xcd_bands;
+  int stage;
+  const double* Uin;
+  const double* Un;
+  double* Uacc;
+  double* Uout;
+  const double* __restrict__ dt;
+  double* fout;
+};
+
+template <int D, int TX, int TY>
+struct Tile {
+  static constexpr int NT = TX * TY;
+  static constexpr int NFX = (D == 1) ? TX + 1 : (TX + 1) * TY;  // faces normal to x
+  static constexpr int NFY = (D == 1) ? 0 : TX * (TY + 1);       // faces normal to y
+  static constexpr int NFACE = NFX + NFY;
+  static constexpr int NH = (D == 1) ? 2 : 2 * TY + 2 * TX;      // halo trace slots: L, R, B, T
+};
+
+// Where each side of a tile face reads its trace and where it contributes.
+struct FaceDesc {
+  int src[2];     // tile slot (>= 0) or halo slot (-1 - h)
+  int cl[2];      // closure id used by the side (tile sources)
+  int dst[2];     // tile slot receiving this side's contribution, or -1
+  int edge[2];    // local edge of the receiving element
+  int tag;        // 0 interior, else boundary physical tag
+  int geo;        // index of the face in gx (x-faces) or gy (y-faces)
+  bool xf;        // face normal to x
+};
+
+template <int D, int TX, int TY>
+__device__ inline FaceDesc face_desc(int fi, int ex0, int ey0, const StageArgs& a) {
+  FaceDesc d;
+  d.dst[0] = d.dst[1] = -1;
+  d.edge[0] = d.edge[1] = 0;
+  if constexpr (D == 1) {
+    const int exl = ex0 + fi - 1, exr = ex0 + fi;
+    d.xf = true;
+    d.geo = ex0 + fi;
+    if (exl < 0) {  // left domain boundary: side 0 = element 0, closure 0
+      d.tag = a.bc[0];
+      d.src[0] = 0; d.cl[0] = 0;
+      if (d.tag == BC_PERIODIC) { d.src[1] = -1 - 0; d.cl[1] = 1; } else { d.src[1] = 0; d.cl[1] = 0; }
+      d.dst[0] = 0; d.edge[0] = 0;
+    } else if (exr >= a.nx) {  // right domain boundary: side 0 = last element, closure 1
+      d.tag = a.bc[1];
+      d.src[0] = TX - 1; d.cl[0] = 1;
+      if (d.tag == BC_PERIODIC) { d.src[1] = -1 - 1; d.cl[1] = 0; } else { d.src[1] = TX - 1; d.cl[1] = 1; }
+      d.dst[0] = TX - 1; d.edge[0] = 1;
+    } else {
+      d.tag = 0;
+      d.src[0] = (fi == 0) ? -1 - 0 : fi - 1; d.cl[0] = 1;
+      d.src[1] = (fi == TX) ? -1 - 1 : fi; d.cl[1] = 0;
+      if (fi > 0) { d.dst[0] = fi - 1; d.edge[0] = 1; }
+      if (fi < TX) { d.dst[1] = fi; d.edge[1] = 0; }
+    }
+  } else {
+    if (fi < Tile<D, TX, TY>::NFX) {
+      const int ty = fi / (TX + 1), fx = fi % (TX + 1);
+      const int exl = ex0 + fx - 1, exr = ex0 + fx;
+      d.xf = true;
+      d.geo = (ey0 + ty) * (a.nx + 1) + ex0 + fx;
+      if (exl < 0) {
+        d.tag = a.bc[0];
+        d.src[0] = ty * TX; d.cl[0] = 3;
+        if (d.tag == BC_PERIODIC) { d.src[1] = -1 - ty; d.cl[1] = 5; } else { d.src[1] = ty * TX; d.cl[1] = 3; }
+        d.dst[0] = ty * TX; d.edge[0] = 3;
+      } else if (exr >= a.nx) {
+        d.tag = a.bc[1];
+        d.src[0] = ty * TX + TX - 1; d.cl[0] = 1;
+        if (d.tag == BC_PERIODIC) { d.src[1] = -1 - (TY + ty); d.cl[1] = 7; } else { d.src[1] = ty * TX + TX - 1; d.cl[1] = 1; }
+        d.dst[0] = ty * TX + TX - 1; d.edge[0] = 1;
+      } else {
+        d.tag = 0;
+        d.src[0] = (fx == 0) ? -1 - ty : ty * TX + fx - 1; d.cl[0] = 1;
+        d.src[1] = (fx == TX) ? -1 - (TY + ty) : ty * TX + fx; d.cl[1] = 7;
+        if (fx > 0) { d.dst[0] = ty * TX + fx - 1; d.edge[0] = 1; }
+        if (fx < TX) { d.dst[1] = ty * TX + fx; d.edge[1] = 3; }
+      }
+    } else {
+      const int fj = fi - Tile<D, TX, TY>::NFX;
+      const int fy = fj / TX, tx = fj % TX;
+      const int eyl = ey0 + fy - 1, eyu = ey0 + fy;
+      d.xf = false;
+      d.geo = (ey0 + fy) * a.nx + ex0 + tx;
+      if (eyl < 0) {
+        d.tag = a.bc[2];
+        d.src[0] = tx; d.cl[0] = 0;
+        if (d.tag == BC_PERIODIC) { d.src[1] = -1 - (2 * TY + tx); d.cl[1] = 6; } else { d.src[1] = tx; d.cl[1] = 0; }
+        d.dst[0] = tx; d.edge[0] = 0;
+      } else if (eyu >= a.ny) {
+        d.tag = a.bc[3];
+        d.src[0] = (TY - 1) * TX + tx; d.cl[0] = 2;
+        if (d.tag == BC_PERIODIC) { d.src[1] = -1 - (2 * TY + TX + tx); d.cl[1] = 4; }
+        else { d.src[1] = (TY - 1) * TX + tx; d.cl[1] = 2; }
+        d.dst[0] = (TY - 1) * TX + tx; d.edge[0] = 2;
+      } else {
+        d.tag = 0;
+        d.src[0] = (fy == 0) ? -1 - (2 * TY + tx) : (fy - 1) * TX + tx; d.cl[0] = 2;
+        d.src[1] = (fy == TY) ? -1 - (2 * TY + TX + tx) : fy * TX + tx; d.cl[1] = 4;
+        if (fy > 0) { d.dst[0] = (fy - 1) * TX + tx; d.edge[0] = 2; }
+        if (fy < TY) { d.dst[1] = fy * TX + tx; d.edge[1] = 0; }
+      }
+    }
+  }
+  return d;
+}
+
+// Global element and closure of halo slot h, or -1 when the slot is unused.
+template <int D, int TX, int TY>
+__device__ inline int halo_source(int h, int ex0, int ey0, const StageArgs& a, int& cl) {
+  if constexpr (D == 1) {
+    if (h == 0) {
+      cl = 1;
+      if (ex0 > 0) return ex0 - 1;
+      return a.bc[0] == BC_PERIODIC ? a.nx - 1 : -1;
+    }
+    cl = 0;
+    if (ex0 + TX < a.nx) return ex0 + TX;
+    return a.bc[1] == BC_PERIODIC ? 0 : -1;
+  } else {
+    const int nx = a.nx, ny = a.ny;
+    if (h < TY) {  // left of row h
+      const int row = ey0 + h;
+      if (ex0 > 0) { cl = 1; return row * nx + ex0 - 1; }
+      cl = 5;
+      return a.bc[0] == BC_PERIODIC ? row * nx + nx - 1 : -1;
+    }
+    if (h < 2 * TY) {  // right of row h - TY
+      const int row = ey0 + h - TY;
+      cl = 7;
+      if (ex0 + TX < nx) return row * nx + ex0 + TX;
+      return a.bc[1] == BC_PERIODIC ? row * nx : -1;
+    }
+    if (h < 2 * TY + TX) {  // below column
+      const int col = ex0 + h - 2 * TY;
+      if (ey0 > 0) { cl = 2; return (ey0 - 1) * nx + col; }
+      cl = 6;
+      return a.bc[2] == BC_PERIODIC ? (ny - 1) * nx + col : -1;
+    }
+    const int col = ex0 + h - 2 * TY - TX;  // above column
+    cl = 4;
+    if (ey0 + TY < ny) return (ey0 + TY) * nx + col;
+    return a.bc[3] == BC_PERIODIC ? col : -1;
+  }
+}
+
+template <int D, int P, int PH, int FL, int MODE, int TX, int TY>
+__global__ __launch_bounds__(256) void k_stage(StageArgs a) {
+  using V = Vol<D, P, PH>;
+  using T = Tile<D, TX, TY>;
+  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, Ms = V::Ms, MG = Shape<D, P>::MG, NN = V::NN;
+  constexpr int NU = V::NU, NQE = V::NQE, NQ = V::NQ, TPE = V::TPE, EPB = V::EPB;
+  constexpr int NT = T::NT, NFACE = T::NFACE, NH = T::NH, HT = NF * Ms;
+  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * NG * NQ;
+  constexpr int NW = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
+
+  __shared__ double sU[NT * NU];
+  __shared__ double sH[NH * HT];
+  __shared__ double sQ[NT * NQE];
+  __shared__ double sW[NW];  // face-point fluxes, then per-point volume values
+  __shared__ double sPhi[V::NTAB_PHI];
+  __shared__ double sDphi[PH ? V::NTAB_D : 1];
+  __shared__ double sDw[V::NTAB_D];
+  __shared__ double sPw[PH ? V::NTAB_PHI : 1];
+  __shared__ double sMinv[Ns * Ns];
+  __shared__ double sPsi[MG * Ms], sPsiw[MG * Ms];
+  constexpr int NGEO = 1 + D * D;
+  __shared__ double sGeo[NT * NGEO];  // per element: J, invJac
+  const VolTables<D, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
+
+  const int tid = threadIdx.x;
+  int tile = blockIdx.x;
+  if (a.xcd_bands) {  // contiguous band of tiles per XCD (dispatch is round-robin over 8 XCDs)
+    const int per = (a.tiles_x * a.tiles_y) >> 3;
+    tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
+
+  const bool shared_minv = a.minv_stride == 0;
+  load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
+  for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
+  for (int k = tid; k < NT * NGEO; k += 256) {
+    const int el = k / NGEO, r = k % NGEO;
+    const int eg = ((ey0 + el / TX) * a.nx + ex0 + el % TX) * a.geo_stride;
+    sGeo[k] = r == 0 ? a.J[eg] : a.invJac[eg * D * D + r - 1];
+  }
+  // tile state: TY rows of TX contiguous elements
+#pragma unroll
+  for (int ty = 0; ty < TY; ++ty) {
+    const double* src = a.Uin + ((size_t)(ey0 + ty) * a.nx + ex0) * NU;
+    for (int k = tid; k < TX * NU; k += 256) sU[ty * TX * NU + k] = src[k];
+  }
+  // halo traces, stored in the closure order of the side that reads them
+  for (int k = tid; k < NH * HT; k += 256) {
+    const int h = k / HT, r = k % HT, f = r / Ms, j = r % Ms;
+    int cl = 0;
+    const int e = halo_source<D, TX, TY>(h, ex0, ey0, a, cl);
+    sH[k] = (e >= 0) ? a.Uin[((size_t)e * NF + f) * Ns + closure_node<D, P>(cl, j)] : 0.0;
+  }
+  __syncthreads();
+
+  // ---- faces: one lane per (face, Gauss point)
+  for (int it = tid; it < NFACE * MG; it += 256) {
+    const int fi = it / MG, g = it % MG;
+    const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
+    const double* fg = (fd.xf ? a.gx : a.gy) + (size_t)fd.geo * 4;
+    const double nxv = fg[0], nyv = (D == 2) ? fg[1] : 0.0, jf0 = fg[2], jf1 = fg[3];
+    double tr[2][NF][Ms];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const int s = fd.src[d];
+#pragma unroll
+      for (int j = 0; j < Ms; ++j) {
+        if (s >= 0) {
+          const int node = closure_node<D, P>(fd.cl[d], j);
+#pragma unroll
+          for (int f = 0; f < NF; ++f) tr[d][f][j] = sU[s * NU + f * Ns + node];
+        } else {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) tr[d][f][j] = sH[(-1 - s) * HT + f * Ms + j];
+        }
+      }
+    }
+    if (fd.tag == BC_REFLECTIVE) {  // boundaries.cu:20-67 on the nodal face values
+#pragma unroll
+      for (int j = 0; j < Ms; ++j) {
+        if constexpr (D == 1) {
+          tr[1][1][j] = -tr[1][1][j];
+        } else {
+          const double nx2 = nxv * nxv, ny2 = nyv * nyv, m2nxny = -2 * nxv * nyv;
+          const double inv = 1.0 / (nx2 + ny2);
+          const double vxL = tr[0][1][j], vyL = tr[0][2][j];
+          tr[1][1][j] = inv * ((-nx2 + ny2) * vxL + m2nxny * vyL);
+          tr[1][2][j] = inv * (m2nxny * vxL + (nx2 - ny2) * vyL);
+        }
+      }
+    }
+    double ul[NF], ur[NF], F[NF], nc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double sl = 0.0, sr = 0.0;
+#pragma unroll
+      for (int j = 0; j < Ms; ++j) {
+        sl += tr[0][f][j] * sPsi[g + j * MG];
+        sr += tr[1][f][j] * sPsi[g + j * MG];
+      }
+      ul[f] = sl;
+      ur[f] = sr;
+    }
+    riemann<D, PH, FL>(ul, ur, nxv, nyv, F, nc);
+    double* w = sW + (fi * MG + g) * 2 * NF;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      w[f] = (-F[f] + nc[f]) * jf0;
+      w[NF + f] = (F[f] + nc[f]) * jf1;
+    }
+  }
+  __syncthreads();
+  // ---- face integration against psi_w into the receiving elements' slots
+  for (int it = tid; it < NFACE * 2 * Ms; it += 256) {
+    const int fi = it / (2 * Ms), d = (it / Ms) % 2, j = it % Ms;
+    const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
+    const int dst = fd.dst[d];
+    if (dst < 0) continue;
+    const int jf = (D == 2 && fd.cl[d] >= 4) ? reversed_pos<P>(j) : j;
+    double* q = sQ + dst * NQE + fd.edge[d] * NF * Ms;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double s = 0.0;
+#pragma unroll
+      for (int g = 0; g < MG; ++g) s += sPsiw[g + j * MG] * sW[((fi * MG + g) * 2 + d) * NF + f];
+      q[f * Ms + jf] = s;
+    }
+  }
+  __syncthreads();
+
+  // ---- element phases, in batches of EPB elements
+  for (int b = 0; b < NT; b += EPB) {
+    {
+      const int el = b + tid / TPE, g = tid % TPE;
+      if (el < NT && g < NG)
+        vol_phase1<D, P, PH>(tb, g, sU + el * NU, sGeo[el * NGEO], sGeo + el * NGEO + 1, sW + ((el - b) * NG + g) * NQ);
+    }
+    __syncthreads();
+    {
+      const int el = b + tid / TPE, i = tid % TPE;
+      double rhs[NF];
+      const bool act = el < NT && i < Ns;
+      if (act) vol_phase2<D, P, PH>(tb, i, sW + (el - b) * NG * NQ, sQ + el * NQE, rhs);
+      __syncthreads();
+      if (act)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sU[el * NU + f * Ns + i] = rhs[f];
+    }
+    __syncthreads();
+  }
+
+  // ---- stage output, row by row (coalesced)
+  const double dt = (MODE == 1) ? a.dt[0] : 0.0;
+#pragma unroll
+  for (int ty = 0; ty < TY; ++ty) {
+    const size_t base = ((size_t)(ey0 + ty) * a.nx + ex0) * NU;
+    for (int k = tid; k < TX * NU; k += 256) {
+      const int el = ty * TX + k / NU;
+      if constexpr (MODE == 0) {
+        a.fout[base + k] = sU[ty * TX * NU + k];
+      } else {
+        // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
+        const size_t e = (size_t)(ey0 + ty) * a.nx + ex0 + el % TX;
+        const double dte = shared_minv ? dt * (a.J0 / sGeo[el * NGEO]) : dt;
+        rk_update<D, P, PH>(tb, a.Minv + e * a.minv_stride * Ns * Ns, shared_minv, k % NU, sU + el * NU, dte,
+                            a.stage, base + k, a.Un, a.Uacc, a.Uout);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- host
+template <int D>
+struct TileShape;
+template <>
+struct TileShape<1> { static constexpr int TX = 128, TY = 1; };
+template <>
+struct TileShape<2> { static constexpr int TX = 8, TY = 4; };
+
+bool stage_tiles(int D, int nx, int ny, int* tx, int* ty) {
+  *tx = D == 1 ? TileShape<1>::TX : TileShape<2>::TX;
+  *ty = D == 1 ? TileShape<1>::TY : TileShape<2>::TY;
+  return nx % *tx == 0 && (D == 1 || ny % *ty == 0);
+}
+
+template <typename Fn>
+static hipError_t dispatch_stage(const dgx_config& c, Fn&& fn) {
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  auto by_flux = [&](auto d, auto p, auto ph) -> hipError_t {
+    return c.flux == 0 ? fn(d, p, ph, I0{}) : fn(d, p, ph, I1{});
+  };
+  auto by_phys = [&](auto d, auto p) -> hipError_t {
+    return c.physics == 0 ? by_flux(d, p, I0{}) : by_flux(d, p, I1{});
+  };
+  auto by_order = [&](auto d) -> hipError_t {
+    switch (c.order) {
+      case 1: return by_phys(d, I1{});
+      case 2: return by_phys(d, I2{});
+      default: return by_phys(d, I3{});
+    }
+  };
+  return c.dim == 1 ? by_order(I1{}) : by_order(I2{});
+}
+
+hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
+                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st) {
+  const Cart& c = s.cart;
+  StageArgs a{};
+  a.phi = s.t.phi; a.dphi = s.t.dphi; a.phi_w = s.t.phi_w; a.dphi_w = s.t.dphi_w;
+  a.psi = s.t.psi; a.psi_w = s.t.psi_w; a.Minv = s.t.Minv;
+  a.J = s.t.J; a.invJac = s.t.invJac; a.gx = s.t.face_geo_x; a.gy = s.t.face_geo_y;
+  a.J0 = s.t.J0; a.geo_stride = s.t.geo_stride; a.minv_stride = s.t.minv_stride;
+  a.nx = c.nx; a.ny = c.ny;
+  for (int k = 0; k < 4; ++k) a.bc[k] = c.bc[k];
+  a.tiles_x = c.nx / c.tx;
+  a.tiles_y = c.ny / c.ty;
+  const int ntiles = a.tiles_x * a.tiles_y;
+  a.xcd_bands = (ntiles % 8 == 0) ? 1 : 0;
+  a.stage = stage; a.Uin = Uin; a.Un = Un; a.Uacc = Uacc; a.Uout = Uout; a.dt = dt_dev; a.fout = f_out;
+  return dispatch_stage(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
+    constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
+    constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
+    if (mode == 0)
+      hipLaunchKernelGGL((k_stage<D, P, PH, FL, 0, TX, TY>), dim3(ntiles), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_stage<D, P, PH, FL, 1, TX, TY>), dim3(ntiles), dim3(256), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+}  // namespace dgx

@@ -177,3 +177,32 @@ def riemann_state(mesh: Mesh, physics: int, seed: int = 0, x_split: float | None
     if physics == MULTIFLUID:
         f.append(1.0 / (gam - 1.0))
     return _pack(mesh, f)
+
+
+def oblique_riemann_state(mesh: Mesh, physics: int, seed: int = 0) -> np.ndarray:
+    """Seeded 2D Riemann problem across an oblique line through the domain
+    centre (random angle): rho, p log-uniform in [0.5, 2], u and v in
+    [-0.5, 0.5], gamma in [1.1, 1.7] per side (multifluid).  Every field,
+    both momenta included, varies, and elements are constant (the side is
+    chosen by the element's first node)."""
+    rng = np.random.default_rng(seed)
+    x, y = node_xy(mesh)
+    x0, x1, y0, y1 = mesh.extent
+    th = rng.uniform(0.2, 1.3)
+    side = (x[:, :1] - 0.5 * (x0 + x1)) * math.cos(th) + (y[:, :1] - 0.5 * (y0 + y1)) * math.sin(th)
+    left = (side < 0) & np.ones_like(x, dtype=bool)
+    rl, rr = np.exp(rng.uniform(np.log(0.5), np.log(2.0), 2))
+    pl, pr = np.exp(rng.uniform(np.log(0.5), np.log(2.0), 2))
+    ul, ur = rng.uniform(-0.5, 0.5, 2)
+    vl, vr = rng.uniform(-0.5, 0.5, 2)
+    gl, gr = (rng.uniform(1.1, 1.7, 2) if physics == MULTIFLUID else (GAMMA, GAMMA))
+    rho = np.where(left, rl, rr)
+    p = np.where(left, pl, pr)
+    u = np.where(left, ul, ur)
+    v = np.where(left, vl, vr)
+    gam = np.where(left, gl, gr)
+    E = p / (gam - 1.0) + 0.5 * rho * (u * u + v * v)
+    f = [rho, rho * u, rho * v, E]
+    if physics == MULTIFLUID:
+        f.append(1.0 / (gam - 1.0))
+    return _pack(mesh, f)

@@ -62,6 +62,9 @@ typedef struct dgx_mesh dgx_mesh;
 dgx_status dgx_mesh_build(int dim, int order, int nx, int ny, double x0, double x1, double y0, double y1,
                           const int* bc, int nfields, dgx_mesh** out);
 void dgx_mesh_free(dgx_mesh* m);
+/* 1 when solvers on this mesh run the fused tile kernel by default (uniform
+ * Cartesian grid, element counts multiples of the tile), else 0.  Host only. */
+int dgx_mesh_tileable(const dgx_mesh* m);
 /* sizes[0..15] = D, order, N_s, N_G, M_s, M_G, N_N, N_E, M_T, M_B, N_F, N_s1D, nx, ny, rflctiveIdx, 0 */
 void dgx_mesh_sizes(const dgx_mesh* m, int* sizes);
 double dgx_mesh_dx(const dgx_mesh* m); /* minimum cell size (simpleMesh.cc:1140-1190) */
@@ -139,9 +142,19 @@ dgx_status dgx_rk4_step(dgx_solver* s, double* U, const double* dt_dev, int limi
 dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
                              int limiter, int* steps, double* t_final, void* stream);
 
+/* Kernel path.  0 auto: the fused tile kernel (one launch per RK stage)
+ * on uniform Cartesian meshes whose element counts are multiples of the tile
+ * (8x4 elements in 2D, 128 in 1D), else the face + element kernels;
+ * 1 face + element kernels; 2 tile kernel (DGX_EUNSUPPORTED when the mesh
+ * does not qualify).  Both paths compute the same right-hand side as
+ * DG_SOLVER::dg_solver (dg_solver.h:292-364).  dgx_solver_path reports the
+ * path in use (1 or 2). */
+dgx_status dgx_solver_set_path(dgx_solver* s, int path);
+int dgx_solver_path(const dgx_solver* s);
+
 /* Per-kernel timing with HIP events recorded on the launch stream (the
  * reference's TIMERS use std::clock and synchronise after every launch,
- * timers.cc:71-90).  kernel: "face", "elem", "limit", "dt".  Reading
+ * timers.cc:71-90).  kernel: "face", "elem", "limit", "dt", "stage".  Reading
  * synchronises the recorded events and resets the counters. */
 dgx_status dgx_profile_enable(dgx_solver* s, int enable);
 dgx_status dgx_profile_read(dgx_solver* s, const char* kernel, double* total_ms, long* launches);

@@ -89,6 +89,24 @@ def test_cfl_mode_sod_with_hr_limiter(device, c):
         assert l2 < 1e-11 and li < 1e-11, (l2, li)
 
 
+@pytest.mark.parametrize("c", [x for x in CONFIGS if x[0] == 2], ids=cfg_id)
+def test_limited_2d_trajectory_general_path(device, c):
+    """Oblique 2D Riemann problem with the HR limiter on a mesh the tile
+    kernel does not take (12 x 7 elements): face + element kernels."""
+    d, ph, fl = c
+    order = 2
+    mesh = small_mesh(d, order, ph)
+    prob = oracle.Problem(mesh.tables(), ph, fl)
+    solver = Solver(Config(d, order, ph, fl), mesh)
+    U0 = problems.oblique_riemann_state(mesh, ph, seed=3)
+    dt = 0.3 * prob.dt_from_cfl(U0, mesh.dx / (2 * order + 1))
+    U_ref, n, T, _ = prob.rk4(U0, 0, dt, nsteps=12, limiter=LIMIT_HR)
+    Ud = dev(U0, device)
+    solver.rk4_integrate(Ud, 0, dt, nsteps=12, limiter=LIMIT_HR)
+    for l2, li in per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s):
+        assert l2 < 1e-11 and li < 1e-11, (l2, li)
+
+
 def _stepped_riemann(mesh, prob, ph, seed):
     """A Riemann state with a smooth 1% multiplicative perturbation in every
     field (limiter and sensor input: discontinuous, non-polynomial data)."""

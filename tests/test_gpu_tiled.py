@@ -1,0 +1,132 @@
+"""GPU parity of the fused tile kernel (one launch per RK stage on Cartesian meshes).
+
+The tile kernel must give the oracle's right-hand side to the same tolerance
+as the face + element kernels (relative 1e-12 on one RHS, 1e-11 on
+trajectories), for every physics/flux/order and every boundary combination,
+with and without the XCD band mapping of tiles (used when the tile count is
+a multiple of 8).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from dg_amd import (FARFIELD, LIMIT_HR, LIMIT_HRI, MULTIFLUID, PATH_GENERAL, PATH_TILED, PERIODIC, REFLECTIVE, ROE,
+                    Config, DgxError, Mesh, Solver, problems)
+from tests.helpers import CONFIGS, cfg_id, per_field_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, device):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+BCS_1D = [(FARFIELD, REFLECTIVE), (PERIODIC, PERIODIC), (REFLECTIVE, FARFIELD)]
+BCS_2D = [(PERIODIC, PERIODIC, REFLECTIVE, FARFIELD), (FARFIELD, REFLECTIVE, PERIODIC, PERIODIC),
+          (REFLECTIVE, REFLECTIVE, REFLECTIVE, REFLECTIVE), (PERIODIC, PERIODIC, PERIODIC, PERIODIC)]
+# element counts: tile count not a multiple of 8, then a multiple of 8 (XCD bands)
+SIZES_1D = [256, 1024]
+SIZES_2D = [(16, 8), (32, 16)]
+
+
+def tiled_mesh(d, order, ph, bc, n):
+    nf = 2 + d + ph
+    if d == 1:
+        return Mesh(1, order, n, 1, -1.0, 1.0, bc=bc, nfields=nf)
+    return Mesh(2, order, n[0], n[1], 0.0, 1.6, 0.0, 0.6, bc=bc, nfields=nf)
+
+
+@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_tiled_rhs_matches_oracle_and_general_path(device, c, order):
+    d, ph, fl = c
+    for k, bc in enumerate(BCS_1D if d == 1 else BCS_2D):
+        n = (SIZES_1D if d == 1 else SIZES_2D)[k % 2]
+        mesh = tiled_mesh(d, order, ph, bc, n)
+        prob = oracle.Problem(mesh.tables(), ph, fl)
+        solver = Solver(Config(d, order, ph, fl), mesh)
+        assert solver.path == PATH_TILED, (bc, n)
+        states = [problems.riemann_state(mesh, ph, seed=k + 5)]
+        if d == 2 or n <= 256:
+            # The RHS of a smooth state is O(h) times its volume and face terms,
+            # so rounding relative to max|f| grows with the element count per
+            # wavelength; 1024 elements in 1D exceed the 1e-12 bar for either path.
+            states.insert(0, problems.smooth_random_state(mesh, ph, seed=k))
+        for U in states:
+            f_ref = prob.rhs(U)
+            Ud = dev(U, device)
+            ft = torch.zeros_like(Ud)
+            solver.rhs(Ud, ft)
+            solver.set_path(PATH_GENERAL)
+            fg = torch.zeros_like(Ud)
+            solver.rhs(Ud, fg)
+            solver.set_path(PATH_TILED)
+            scale = np.abs(f_ref).max()
+            err_t = np.abs(host(ft) - f_ref).max() / scale
+            err_g = np.abs(host(fg) - f_ref).max() / scale
+            assert err_t <= 1e-12, (bc, n, err_t)
+            assert err_g <= 1e-12, (bc, n, err_g)
+
+
+@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+def test_tiled_rk4_trajectory_with_limiter(device, c):
+    d, ph, fl = c
+    order = 2
+    bc = BCS_1D[0] if d == 1 else BCS_2D[0]
+    mesh = tiled_mesh(d, order, ph, bc, SIZES_1D[0] if d == 1 else SIZES_2D[1])
+    prob = oracle.Problem(mesh.tables(), ph, fl)
+    solver = Solver(Config(d, order, ph, fl), mesh)
+    assert solver.path == PATH_TILED
+    # 2D: an oblique jump, so that both momenta carry signal (a jump along x
+    # leaves rho*v at rounding level, where relative errors mean nothing)
+    U0 = problems.riemann_state(mesh, ph, seed=2) if d == 1 else problems.oblique_riemann_state(mesh, ph, seed=2)
+    dt = 0.3 * prob.dt_from_cfl(U0, mesh.dx / (2 * order + 1))
+    nsteps = 12
+    U_ref, n, T, _ = prob.rk4(U0, 0, dt, nsteps=nsteps, limiter=LIMIT_HR)
+    Ud = dev(U0, device)
+    steps, _ = solver.rk4_integrate(Ud, 0, dt, nsteps=nsteps, limiter=LIMIT_HR)
+    assert steps == nsteps
+    for l2, li in per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s):
+        assert l2 < 1e-11 and li < 1e-11, (l2, li)
+
+
+def test_tiled_hri_cfl_run(device):
+    """Sensor-gated limiting with CFL time steps through the tile kernel."""
+    d, ph, fl, order = 2, MULTIFLUID, ROE, 2
+    mesh = tiled_mesh(d, order, ph, BCS_2D[1], SIZES_2D[1])
+    prob = oracle.Problem(mesh.tables(), ph, fl)
+    th = (0.01, 0.01, 0.01)
+    solver = Solver(Config(d, order, ph, fl), mesh)
+    solver.set_sensor_thresholds(th)
+    assert solver.path == PATH_TILED
+    U0 = problems.oblique_riemann_state(mesh, ph, seed=4)
+    cfl = 0.3 * mesh.dx / (2 * order + 1)
+    U_ref, n_ref, T_ref, dts = prob.rk4(U0, 1, cfl, tf=0.02, limiter=LIMIT_HRI, thresholds=th)
+    Ud = dev(U0, device)
+    n, T = solver.rk4_integrate(Ud, 1, cfl, tf=0.02, limiter=LIMIT_HRI)
+    assert n == n_ref and abs(T - T_ref) <= 1e-12 * T_ref
+    for l2, li in per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s):
+        assert l2 < 1e-11 and li < 1e-11, (l2, li)
+
+
+def test_path_selection(device):
+    ph = MULTIFLUID
+    # 12 x 7 elements is not a multiple of the 8 x 4 tile: general path only
+    mesh = tiled_mesh(2, 2, ph, BCS_2D[0], (12, 7))
+    solver = Solver(Config(2, 2, ph, ROE), mesh)
+    assert solver.path == PATH_GENERAL
+    with pytest.raises(DgxError):
+        solver.set_path(PATH_TILED)
+    mesh = tiled_mesh(2, 2, ph, BCS_2D[0], (16, 8))
+    solver = Solver(Config(2, 2, ph, ROE), mesh)
+    assert solver.path == PATH_TILED
+    solver.set_path(PATH_GENERAL)
+    assert solver.path == PATH_GENERAL
+    mesh = tiled_mesh(1, 2, ph, BCS_1D[0], 200)
+    assert Solver(Config(1, 2, ph, ROE), mesh).path == PATH_GENERAL
