@@ -38,9 +38,10 @@ def lib() -> ctypes.CDLL:
     """Load libdgx.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m dg_amd.build`")
-        L = ctypes.CDLL(LIB_PATH)
+        path = os.environ.get("DGX_LIBRARY", LIB_PATH)  # alternative builds (tuning variants)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m dg_amd.build`")
+        L = ctypes.CDLL(path)
         vp, ip, dp, i = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_double, ctypes.c_int
         L.dgx_last_error.restype = ctypes.c_char_p
         L.dgx_mesh_build.argtypes = [i, i, i, i, dp, dp, dp, dp, ip, i, ctypes.POINTER(vp)]

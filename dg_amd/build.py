@@ -40,8 +40,8 @@ SOURCES = [
 ]
 
 
-def _cmd(src: str, kind: str, extra: list[str]) -> list[str]:
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+def _cmd(src: str, kind: str, extra: list[str], build_dir: str = BUILD) -> list[str]:
+    obj = os.path.join(build_dir, os.path.basename(src) + ".o")
     path = os.path.join(CSRC, src)
     if kind == "cxx":
         return [CXX, "-O2", "-std=c++17", "-fPIC", "-c", path, "-o", obj] + extra
@@ -51,9 +51,13 @@ def _cmd(src: str, kind: str, extra: list[str]) -> list[str]:
     return [HIPCC] + HIP_FLAGS + extra + ["-c", path, "-o", obj]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
-    cmds = [_cmd(s, k, e) for s, k, e in SOURCES]
+def build(verbose: bool = False, out: str = LIB, defines: tuple = ()) -> str:
+    """Compile and link.  `defines` (NAME=VALUE strings) go to every HIP
+    translation unit; a non-default `out` gets its own object folder."""
+    build_dir = BUILD if out == LIB else os.path.join(BUILD, "variant_" + os.path.basename(out).replace(".", "_"))
+    os.makedirs(build_dir, exist_ok=True)
+    dflags = ["-D" + d for d in defines]
+    cmds = [_cmd(s, k, e + (dflags if k == "hipcc" else []), build_dir) for s, k, e in SOURCES]
 
     def run(c):
         r = subprocess.run(c, capture_output=True, text=True)
@@ -66,16 +70,23 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(" ".join(c) + "\n" + r.stdout + r.stderr)
         if r.returncode != 0:
             raise RuntimeError(f"compile failed: {' '.join(c)}")
-    objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s, _, _ in SOURCES]
-    tmp = LIB + ".tmp"
+    objs = [os.path.join(build_dir, os.path.basename(s) + ".o") for s, _, _ in SOURCES]
+    tmp = out + ".tmp"
     link = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(link, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(" ".join(link) + "\n" + r.stdout + r.stderr)
         raise RuntimeError("link failed")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-v", action="store_true")
+    ap.add_argument("--out", default=LIB)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    a = ap.parse_args()
+    print(build(verbose=a.v, out=os.path.abspath(a.out), defines=tuple(a.defines)))

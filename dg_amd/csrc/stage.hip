@@ -183,19 +183,32 @@ __device__ inline int halo_source(int h, int ex0, int ey0, const StageArgs& a, i
   }
 }
 
+#ifndef DGX_STAGE_PAD
+#define DGX_STAGE_PAD 1  // pad per-element LDS strides against bank conflicts
+#endif
+#ifndef DGX_STAGE_PREFETCH
+#define DGX_STAGE_PREFETCH 1  // load the RK operands (U^n, Uacc) at kernel entry
+#endif
+
+// Per-element LDS stride for n doubles: with padding, n rounded up to 2 mod 16
+// so that the elements a wave touches at once start on different banks.
+constexpr int lds_stride(int n) { return DGX_STAGE_PAD ? n + ((18 - n % 16) % 16) : n; }
+
 template <int D, int P, int PH, int FL, int MODE, int TX, int TY>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   using V = Vol<D, P, PH>;
   using T = Tile<D, TX, TY>;
-  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, Ms = V::Ms, MG = Shape<D, P>::MG, NN = V::NN;
+  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, Ms = V::Ms, MG = Shape<D, P>::MG;
   constexpr int NU = V::NU, NQE = V::NQE, NQ = V::NQ, TPE = V::TPE, EPB = V::EPB;
   constexpr int NT = T::NT, NFACE = T::NFACE, NH = T::NH, HT = NF * Ms;
-  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * NG * NQ;
+  constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
+  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * SW;
   constexpr int NW = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
+  constexpr int NOUT = (NT * NU + 255) / 256;  // output entries per thread
 
-  __shared__ double sU[NT * NU];
+  __shared__ double sU[NT * SU];
   __shared__ double sH[NH * HT];
-  __shared__ double sQ[NT * NQE];
+  __shared__ double sQ[NT * SQ];
   __shared__ double sW[NW];  // face-point fluxes, then per-point volume values
   __shared__ double sPhi[V::NTAB_PHI];
   __shared__ double sDphi[PH ? V::NTAB_D : 1];
@@ -214,7 +227,26 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   }
   const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
+  // global index of flat tile entry k = el*NU + r (rows of TX elements are contiguous)
+  auto gidx = [&](int k) -> size_t {
+    const int ty = k / (TX * NU);
+    return ((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU);
+  };
 
+  // tile state
+  for (int k = tid; k < NT * NU; k += 256) sU[(k / NU) * SU + k % NU] = a.Uin[gidx(k)];
+  // RK operands of this thread's output entries, in flight during the compute
+  double pre_un[NOUT], pre_acc[NOUT];
+#pragma unroll
+  for (int r = 0; r < NOUT; ++r) {
+    pre_un[r] = pre_acc[r] = 0.0;
+    const int k = tid + r * 256;
+    if (MODE == 1 && DGX_STAGE_PREFETCH && k < NT * NU) {
+      const size_t i = gidx(k);
+      if (a.stage < 3) pre_un[r] = a.Un[i];
+      if (a.stage > 0) pre_acc[r] = a.Uacc[i];
+    }
+  }
   const bool shared_minv = a.minv_stride == 0;
   load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
   for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
@@ -222,12 +254,6 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     const int el = k / NGEO, r = k % NGEO;
     const int eg = ((ey0 + el / TX) * a.nx + ex0 + el % TX) * a.geo_stride;
     sGeo[k] = r == 0 ? a.J[eg] : a.invJac[eg * D * D + r - 1];
-  }
-  // tile state: TY rows of TX contiguous elements
-#pragma unroll
-  for (int ty = 0; ty < TY; ++ty) {
-    const double* src = a.Uin + ((size_t)(ey0 + ty) * a.nx + ex0) * NU;
-    for (int k = tid; k < TX * NU; k += 256) sU[ty * TX * NU + k] = src[k];
   }
   // halo traces, stored in the closure order of the side that reads them
   for (int k = tid; k < NH * HT; k += 256) {
@@ -253,7 +279,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
         if (s >= 0) {
           const int node = closure_node<D, P>(fd.cl[d], j);
 #pragma unroll
-          for (int f = 0; f < NF; ++f) tr[d][f][j] = sU[s * NU + f * Ns + node];
+          for (int f = 0; f < NF; ++f) tr[d][f][j] = sU[s * SU + f * Ns + node];
         } else {
 #pragma unroll
           for (int f = 0; f < NF; ++f) tr[d][f][j] = sH[(-1 - s) * HT + f * Ms + j];
@@ -302,7 +328,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     const int dst = fd.dst[d];
     if (dst < 0) continue;
     const int jf = (D == 2 && fd.cl[d] >= 4) ? reversed_pos<P>(j) : j;
-    double* q = sQ + dst * NQE + fd.edge[d] * NF * Ms;
+    double* q = sQ + dst * SQ + fd.edge[d] * NF * Ms;
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       double s = 0.0;
@@ -318,38 +344,49 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     {
       const int el = b + tid / TPE, g = tid % TPE;
       if (el < NT && g < NG)
-        vol_phase1<D, P, PH>(tb, g, sU + el * NU, sGeo[el * NGEO], sGeo + el * NGEO + 1, sW + ((el - b) * NG + g) * NQ);
+        vol_phase1<D, P, PH>(tb, g, sU + el * SU, sGeo[el * NGEO], sGeo + el * NGEO + 1,
+                             sW + (el - b) * SW + g * NQ);
     }
     __syncthreads();
     {
       const int el = b + tid / TPE, i = tid % TPE;
       double rhs[NF];
       const bool act = el < NT && i < Ns;
-      if (act) vol_phase2<D, P, PH>(tb, i, sW + (el - b) * NG * NQ, sQ + el * NQE, rhs);
+      if (act) vol_phase2<D, P, PH>(tb, i, sW + (el - b) * SW, sQ + el * SQ, rhs);
       __syncthreads();
       if (act)
 #pragma unroll
-        for (int f = 0; f < NF; ++f) sU[el * NU + f * Ns + i] = rhs[f];
+        for (int f = 0; f < NF; ++f) sU[el * SU + f * Ns + i] = rhs[f];
     }
     __syncthreads();
   }
 
-  // ---- stage output, row by row (coalesced)
+  // ---- stage output (coalesced along the tile rows)
   const double dt = (MODE == 1) ? a.dt[0] : 0.0;
 #pragma unroll
-  for (int ty = 0; ty < TY; ++ty) {
-    const size_t base = ((size_t)(ey0 + ty) * a.nx + ex0) * NU;
-    for (int k = tid; k < TX * NU; k += 256) {
-      const int el = ty * TX + k / NU;
-      if constexpr (MODE == 0) {
-        a.fout[base + k] = sU[ty * TX * NU + k];
-      } else {
-        // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
-        const size_t e = (size_t)(ey0 + ty) * a.nx + ex0 + el % TX;
-        const double dte = shared_minv ? dt * (a.J0 / sGeo[el * NGEO]) : dt;
-        rk_update<D, P, PH>(tb, a.Minv + e * a.minv_stride * Ns * Ns, shared_minv, k % NU, sU + el * NU, dte,
-                            a.stage, base + k, a.Un, a.Uacc, a.Uout);
+  for (int r = 0; r < NOUT; ++r) {
+    const int k = tid + r * 256;
+    if (k >= NT * NU) break;
+    const int el = k / NU, rr = k % NU;
+    const size_t gi = gidx(k);
+    if constexpr (MODE == 0) {
+      a.fout[gi] = sU[el * SU + rr];
+    } else {
+      // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
+      const double dte = shared_minv ? dt * (a.J0 / sGeo[el * NGEO]) : dt;
+      const double* M = shared_minv ? tb.minv
+                                    : a.Minv + ((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * Ns * Ns;
+      const int f = rr / Ns, i = rr % Ns;
+      const double* R = sU + el * SU + f * Ns;
+      double sol = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
+      double un = pre_un[r], acc = pre_acc[r];
+      if (!DGX_STAGE_PREFETCH) {
+        if (a.stage < 3) un = a.Un[gi];
+        if (a.stage > 0) acc = a.Uacc[gi];
       }
+      rk_combine(a.stage, dte * sol, un, acc, a.Uacc + gi, a.Uout + gi);
     }
   }
 }

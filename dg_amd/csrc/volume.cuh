@@ -131,6 +131,21 @@ __device__ inline void vol_phase2(const VolTables<D, P, PH>& t, int i, const dou
   for (int f = 0; f < NF; ++f) rhs[f] = acc[f] + q[f];
 }
 
+// RK4 stage combination for one state entry (rk.cc:164-198 with DU fused):
+// un = U^n, acc = accumulator (read for stages 1-3), du = Dt*Minv*f.
+__device__ inline void rk_combine(int st, double du, double un, double acc, double* acc_out, double* out) {
+  const double gk = kRkGamma[st];
+  if (st == 0) {
+    *acc_out = un + gk * du;
+    *out = un + kRkBeta[1] * du;
+  } else if (st < 3) {
+    *acc_out = acc + gk * du;
+    *out = un + kRkBeta[st + 1] * du;
+  } else {
+    *out = acc + gk * du;
+  }
+}
+
 // Phase 3 for one state entry r = f*Ns + i of an element whose right-hand
 // side R (NF x Ns) is in LDS: du = Dt * Minv * R, then the RK4 stage update
 // at global index idx.
@@ -150,18 +165,9 @@ __device__ inline void rk_update(const VolTables<D, P, PH>& t, const double* Mgl
 #pragma unroll
     for (int ii = 0; ii < Ns; ++ii) sol += Mglob[ii * Ns + i] * Rf[ii];
   }
-  const double du = dt * sol;
-  const double gk = kRkGamma[st];
-  if (st == 0) {
-    const double un = Un[idx];
-    Uacc[idx] = un + gk * du;
-    Uout[idx] = un + kRkBeta[1] * du;
-  } else if (st < 3) {
-    Uacc[idx] = Uacc[idx] + gk * du;
-    Uout[idx] = Un[idx] + kRkBeta[st + 1] * du;
-  } else {
-    Uout[idx] = Uacc[idx] + gk * du;
-  }
+  const double un = st < 3 ? Un[idx] : 0.0;
+  const double acc = st > 0 ? Uacc[idx] : 0.0;
+  rk_combine(st, dt * sol, un, acc, Uacc + idx, Uout + idx);
 }
 
 }  // namespace dgx
