@@ -1,14 +1,22 @@
 #!/usr/bin/env python
 """DOF-updates/s of the FP64 DG RK4 step on one MI355X.
 
-Workload: 2D single-fluid isentropic vortex on a periodic [0,10]^2 mesh of
-512x512 quads, tensor Lagrange p=2 (9 nodes x 4 fields per element), Roe
-flux, no limiting, classic RK4 with a fixed Dt from CFL 0.3 (deck CFL x
-Dx/(2p+1) / max(|u|+a)).  Synthetic input (the exact vortex, interpolated at
-the nodes).  One DOF-update = one conserved coefficient advanced one full RK4
-step (four right-hand sides).
+Default workload (vortex): 2D single-fluid isentropic vortex on a periodic
+[0,10]^2 mesh of 512x512 quads, tensor Lagrange p=2 (9 nodes x 4 fields per
+element), Roe flux, no limiting, classic RK4 with a fixed Dt from CFL 0.3
+(deck CFL x Dx/(2p+1) / max(|u|+a)).  Synthetic input (the exact vortex,
+interpolated at the nodes).  One DOF-update = one conserved coefficient
+advanced one full RK4 step (four right-hand sides).
 
-    python bench.py [--steps K] [--warmup W] [--n 512] [--no-cpu-baseline]
+Other workloads (--config): `sod`, the 1D single-fluid Sod tube on 1000
+elements, p=2, Rusanov, HR limiter, CFL 0.4 steps (Dt from the state every
+step); `bubble`, the 2D two-fluid shock-bubble on 1024x512 quads, p=2, 5
+fields, Roe, HR limiter, fixed Dt from CFL 0.4 at t=0; `ensemble`, 256
+independent 1D two-fluid Riemann problems (element counts uniform in
+[200, 4000], orders uniform in {1, 2, 3}, seeded states), Roe, HR limiter,
+each with its own CFL 0.4 step, advanced by one graph launch of K steps.
+
+    python bench.py [--steps K] [--warmup W] [--config vortex|sod|bubble|ensemble] [--no-cpu-baseline]
 
 Prints one JSON line.  `value` is the timed region's throughput with the
 state already resident in HBM; `roofline` is the dominant kernel's achieved
@@ -32,12 +40,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=512, help="elements per direction")
+    ap.add_argument("--config", choices=["vortex", "sod", "bubble", "ensemble"], default="vortex")
+    ap.add_argument("--n", type=int, default=0, help="vortex: elements per direction (default 512)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", choices=["auto", "general", "tiled"], default="auto",
                     help="kernel path: fused tile kernel (auto on this mesh) or face + element kernels")
-    ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU sample mesh size (0: same as --n)")
-    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--cpu-sample-steps", type=int, default=0, help="CPU sample steps (0: workload default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB buffer 5 times (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -50,7 +58,7 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from dg_amd import PATH_AUTO, PATH_GENERAL, PATH_TILED, ROE, SINGLEFLUID, Config, Solver, problems
+    from dg_amd import PATH_AUTO, PATH_GENERAL, PATH_TILED, Config, Solver
 
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -69,9 +77,11 @@ def main() -> int:
                           "dispatches": 5}))
         return 0
 
-    order, n = 2, args.n
-    mesh = problems.vortex_mesh(n, order)
-    cfg = Config(2, order, SINGLEFLUID, ROE)
+    if args.config == "ensemble":
+        return bench_ensemble(args, device)
+    w = workload(args)
+    mesh, order = w["mesh"], w["order"]
+    cfg = Config(w["dim"], order, w["physics"], w["flux"])
     solver = Solver(cfg, mesh)
     solver.set_path({"auto": PATH_AUTO, "general": PATH_GENERAL, "tiled": PATH_TILED}[args.path])
     tiled = solver.path == PATH_TILED
@@ -79,39 +89,48 @@ def main() -> int:
     N_E, N_s, NF = s["N_E"], s["N_s"], cfg.nfields
     n_dof = N_E * N_s * NF
 
-    U0 = problems.vortex_state(mesh)
+    U0 = w["U0"]
     U = torch.from_numpy(U0).to(device)
-    cfl = 0.3 * mesh.dx / (2 * order + 1)
+    cfl = w["cfl"] * mesh.dx / (2 * order + 1)
     dt_dev = torch.zeros(1, dtype=torch.float64, device=device)
     solver.dt_from_cfl(U, cfl, dt_dev, stream=stream)
     torch.cuda.synchronize()
     dt = float(dt_dev.item())
+    lim = w["limiter"]
+
+    def advance(k):
+        if w["cfl_steps"]:
+            solver.rk4_integrate(U, 1, cfl, nsteps=k, tf=1e30, limiter=lim, stream=stream)
+        else:
+            solver.rk4_integrate(U, 0, dt, nsteps=k, limiter=lim, stream=stream)
 
     # warmup, then exactly K timed steps bracketed by device synchronisation
-    solver.rk4_integrate(U, 0, dt, nsteps=args.warmup, stream=stream)
+    advance(args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    solver.rk4_integrate(U, 0, dt, nsteps=args.steps, stream=stream)
+    advance(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_dof * args.steps / elapsed
 
-    # accuracy against the exact translated vortex after warmup + K steps
-    t_final = (args.warmup + args.steps) * dt
-    Uh = U.cpu().numpy().reshape(N_E, NF, N_s)
-    x, y = problems.node_xy(mesh)
-    rho_ex, _, _, _ = problems.vortex_fields(x, y, t=t_final)
-    l2_exact = float(np.linalg.norm(Uh[:, 0] - rho_ex) / np.linalg.norm(rho_ex))
+    extra = {}
+    if args.config == "vortex":  # accuracy against the exact translated vortex after warmup + K steps
+        from dg_amd import problems
+        t_final = (args.warmup + args.steps) * dt
+        Uh = U.cpu().numpy().reshape(N_E, NF, N_s)
+        x, y = problems.node_xy(mesh)
+        rho_ex, _, _, _ = problems.vortex_fields(x, y, t=t_final)
+        extra["l2_rel_err_rho_vs_exact"] = float(np.linalg.norm(Uh[:, 0] - rho_ex) / np.linalg.norm(rho_ex))
 
     # per-kernel timing with HIP events on the launch stream (separate run)
     prof_steps = max(1, min(args.steps, 20))
-    names = ("stage", "elem", "face")
+    names = ("stage", "elem", "face", "limit", "dt")
     solver.profile(True)
     for k in names:
         solver.profile_read(k)
-    solver.rk4_integrate(U, 0, dt, nsteps=prof_steps, stream=stream)
+    advance(prof_steps)
     torch.cuda.synchronize()
     avg = {}
     for k in names:
@@ -129,12 +148,16 @@ def main() -> int:
     # Face kernel: one read of the face traces (<= one state vector) and its
     # per-face nodal contributions (2 sides x N_F x M_s per face).
     face_bytes = 8.0 * (n_dof + 2 * NF * s["M_s"] * s["M_T"])
-    dominant = max(avg, key=avg.get)
+    rhs_kernels = {k: v for k, v in avg.items() if k in ("k_stage", "k_elem", "k_face")}
+    dominant = max(rhs_kernels, key=rhs_kernels.get)
     dom_avg = avg[dominant]
     dom_bytes = face_bytes if dominant == "k_face" else stage_bytes
     achieved = dom_bytes / (dom_avg * 1e-3) / 1e9
-    # whole-step view: minimum RK4 state traffic, c = 16 state vectors per step
-    step_gbps = 16.0 * 8.0 * n_dof / (ms_per_step * 1e-3) / 1e9
+    # whole-step view: minimum RK4 state traffic per step, in state vectors:
+    # 16 for the fused stage flow, plus 2 per limiter sweep (read + write) for
+    # each of the 4 limiter applications (1 sweep in 1D, 2 in 2D)
+    c = 16 + (0 if not lim else 4 * 2 * w["dim"])
+    step_gbps = c * 8.0 * n_dof / (ms_per_step * 1e-3) / 1e9
 
     out = {
         "metric": "DOF-updates/s (1 GPU) at fixed L2 err vs FP64 oracle; % HBM/MFMA roofline",
@@ -150,8 +173,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"2D isentropic vortex, periodic [0,10]^2, {n}x{n} quads, p=2 Lagrange, 4 fields, Roe, "
-                        f"no limiter, RK4 fixed Dt from CFL 0.3",
+            "workload": w["name"],
             "elements": N_E, "dofs": n_dof, "dt": dt, "parallelism": "single GPU",
             "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
         },
@@ -161,48 +183,142 @@ def main() -> int:
             "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg,
             "kernels_ms": avg,
         },
-        "step_roofline": {"c": 16, "achieved": step_gbps, "frac": step_gbps / HBM_PEAK_GBPS, "unit": "GB/s"},
-        "l2_rel_err_rho_vs_exact": l2_exact,
+        "step_roofline": {"c": c, "achieved": step_gbps, "frac": step_gbps / HBM_PEAK_GBPS, "unit": "GB/s"},
     }
+    out.update(extra)
 
     if not args.no_cpu_baseline:
-        out["cpu_baseline"], out["l2_rel_err_vs_oracle"] = cpu_baseline(args, device)
+        out["cpu_baseline"], out["l2_rel_err_vs_oracle"] = cpu_baseline(args, device, w)
 
     print(json.dumps(out))
     return 0
 
 
-def cpu_baseline(args, device):
-    """FP64 CPU oracle (one core) on a bounded sample of the same workload:
-    `cpu-sample-steps` RK4 steps of the vortex at cpu-sample-n^2; the GPU
-    engine runs the same sample and the largest per-field relative L2
-    difference is reported."""
+def bench_ensemble(args, device):
+    """The batched ensemble: autotune, W warmup steps, then K steps timed as
+    one graph launch.  DOF-updates count the problems that completed every
+    step (status 0); strong random Riemann problems can lose positivity and
+    stop, as they would in the reference."""
     import numpy as np
     import torch
 
     import oracle
-    from dg_amd import PATH_AUTO, PATH_GENERAL, PATH_TILED, ROE, SINGLEFLUID, Config, Solver, problems
+    from dg_amd import FARFIELD, LIMIT_HR, MULTIFLUID, ROE, Batch, Config, Mesh, problems
 
-    n = args.cpu_sample_n if args.cpu_sample_n else args.n
-    mesh = problems.vortex_mesh(n, 2)
-    prob = oracle.Problem(mesh.tables(), SINGLEFLUID, ROE)
-    U0 = problems.vortex_state(mesh)
-    dt = prob.dt_from_cfl(U0, 0.3 * mesh.dx / 5)
+    n, cfl = 256, 0.4
+    ne, p = problems.ensemble_spec(n, seed=20261202)
+    meshes = [Mesh(1, p[k], ne[k], 1, -1.0, 1.0, bc=(FARFIELD, FARFIELD), nfields=4) for k in range(n)]
+    states = [problems.two_fluid_riemann_1d(meshes[k], 20261202 + k) for k in range(n)]
+    batch = Batch(Config(1, 1, MULTIFLUID, ROE), ne, p)
+    U = torch.from_numpy(batch.pack(states)).to(device)
+    T = torch.zeros(n, dtype=torch.float64, device=device)
+    status = torch.zeros(n, dtype=torch.int32, device=device)
+    batch.autotune(U, cfl)
+    batch.rk4(U, cfl, args.warmup, T, status, limit_initial=True)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    U_ref, steps, _, _ = prob.rk4(U0, 0, dt, nsteps=args.cpu_sample_steps)
+    batch.rk4(U, cfl, args.steps, T, status, limit_initial=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = status.cpu().numpy()
+    dofs = np.array([batch.offsets[k + 1] - batch.offsets[k] for k in range(n)])
+    ok = st == 0
+    value = float(dofs[ok].sum()) * args.steps / elapsed
+    out = {
+        "metric": "DOF-updates/s (1 GPU) at fixed L2 err vs FP64 oracle; % HBM/MFMA roofline",
+        "value": value, "unit": "DOF-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 independent 1D two-fluid Riemann problems (N_E uniform in [200,4000], p uniform "
+                               "in {1,2,3}, rho,p log-uniform [0.1,10], u in [-2,2], gamma in [1.1,1.7] per side), "
+                               "Roe, HR limiter, per-problem Dt from CFL 0.4, one graph launch of K steps",
+                   "problems": n, "dofs": int(dofs.sum()), "completed": int(ok.sum()),
+                   "block_sizes": batch.block_sizes, "parallelism": "single GPU"},
+        "step_roofline": {"c": 16 + 8, "achieved": 24 * 8.0 * float(dofs[ok].sum()) / (elapsed / args.steps) / 1e9,
+                          "unit": "GB/s"},
+    }
+    out["step_roofline"]["frac"] = out["step_roofline"]["achieved"] / HBM_PEAK_GBPS
+    if not args.no_cpu_baseline:
+        # oracle on a bounded sample: the first 16 problems that complete, 20 steps each
+        Uh = batch.unpack(U.cpu().numpy())
+        t_cpu, dof_steps, errs = 0.0, 0, []
+        for k in [k for k in range(n) if ok[k]][:16]:
+            prob = oracle.Problem(meshes[k].tables(), MULTIFLUID, ROE)
+            c = cfl * meshes[k].dx / (2 * p[k] + 1)
+            t1 = time.perf_counter()
+            _, steps, _, _ = prob.rk4(states[k], 1, c, nsteps=20, tf=1e30, limiter=LIMIT_HR)
+            t_cpu += time.perf_counter() - t1
+            dof_steps += states[k].size * max(steps, 0)
+        out["cpu_baseline"] = {"value": dof_steps / t_cpu, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"16 of the problems, 20 RK4 steps each, FP64 oracle, {t_cpu:.1f} s"}
+    print(json.dumps(out))
+    return 0
+
+
+def workload(args):
+    """Mesh, configuration, initial state and stepping of the selected workload."""
+    from dg_amd import MULTIFLUID, ROE, RUSANOV, SINGLEFLUID, problems
+
+    if args.config == "vortex":
+        n = args.n or 512
+        mesh = problems.vortex_mesh(n, 2)
+        return dict(name=f"2D isentropic vortex, periodic [0,10]^2, {n}x{n} quads, p=2 Lagrange, 4 fields, Roe, "
+                         f"no limiter, RK4 fixed Dt from CFL 0.3", mesh=mesh, dim=2, order=2, physics=SINGLEFLUID,
+                    flux=ROE, U0=problems.vortex_state(mesh), cfl=0.3, limiter=0, cfl_steps=False,
+                    sample=lambda: problems.vortex_mesh(n, 2), sample_state=problems.vortex_state, sample_steps=4)
+    if args.config == "sod":
+        mesh = problems.sod_mesh(1000, 2, SINGLEFLUID)
+        return dict(name="1D single-fluid Sod tube, [-1,1], 1000 elements, p=2 Lagrange, 3 fields, Rusanov, HR "
+                         "limiter, RK4 with Dt from CFL 0.4 every step", mesh=mesh, dim=1, order=2,
+                    physics=SINGLEFLUID, flux=RUSANOV, U0=problems.sod_state(mesh, SINGLEFLUID), cfl=0.4, limiter=1,
+                    cfl_steps=True, sample=lambda: problems.sod_mesh(1000, 2, SINGLEFLUID),
+                    sample_state=lambda m: problems.sod_state(m, SINGLEFLUID), sample_steps=200)
+    mesh = problems.shock_bubble_mesh(1024, 512, 2)
+    return dict(name="2D two-fluid shock-bubble (Mach 1.22, bubble r=0.25 rho=0.138 gamma=1.667), [0,2]x[0,1], "
+                     "1024x512 quads, p=2 Lagrange, 5 fields, Roe, HR limiter, RK4 fixed Dt from CFL 0.4 at t=0",
+                mesh=mesh, dim=2, order=2, physics=MULTIFLUID, flux=ROE, U0=problems.shock_bubble_state(mesh),
+                cfl=0.4, limiter=1, cfl_steps=False, sample=lambda: problems.shock_bubble_mesh(1024, 512, 2),
+                sample_state=problems.shock_bubble_state, sample_steps=2)
+
+
+def cpu_baseline(args, device, w):
+    """FP64 CPU oracle (one core) on a bounded sample of the same workload
+    (a few RK4 steps); the GPU engine runs the same sample and the largest
+    per-field relative L2 difference is reported."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from dg_amd import Config, Solver
+
+    mesh = w["sample"]()
+    prob = oracle.Problem(mesh.tables(), w["physics"], w["flux"])
+    U0 = w["sample_state"](mesh)
+    order = w["order"]
+    cfl = w["cfl"] * mesh.dx / (2 * order + 1)
+    nst = args.cpu_sample_steps if args.cpu_sample_steps > 0 else w["sample_steps"]
+    dt = prob.dt_from_cfl(U0, cfl)
+    t0 = time.perf_counter()
+    if w["cfl_steps"]:
+        U_ref, steps, _, _ = prob.rk4(U0, 1, cfl, nsteps=nst, tf=1e30, limiter=w["limiter"])
+    else:
+        U_ref, steps, _, _ = prob.rk4(U0, 0, dt, nsteps=nst, limiter=w["limiter"])
     el = time.perf_counter() - t0
-    solver = Solver(Config(2, 2, SINGLEFLUID, ROE), mesh)
+    solver = Solver(Config(w["dim"], order, w["physics"], w["flux"]), mesh)
     Ud = torch.from_numpy(U0).to(device)
-    solver.rk4_integrate(Ud, 0, dt, nsteps=args.cpu_sample_steps)
+    if w["cfl_steps"]:
+        solver.rk4_integrate(Ud, 1, cfl, nsteps=nst, tf=1e30, limiter=w["limiter"])
+    else:
+        solver.rk4_integrate(Ud, 0, dt, nsteps=nst, limiter=w["limiter"])
     torch.cuda.synchronize()
     Ug = Ud.cpu().numpy()
-    nf, ns = 4, mesh.sizes["N_s"]
+    nf, ns = prob.nf, mesh.sizes["N_s"]
     A, B = Ug.reshape(-1, nf, ns), U_ref.reshape(-1, nf, ns)
     err = max(float(np.linalg.norm(A[:, f] - B[:, f]) / np.linalg.norm(B[:, f])) for f in range(nf))
     cpu = {
         "value": U0.size * steps / el, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-        "sample": f"{steps} RK4 step(s) of the same vortex workload at {n}x{n} ({U0.size} DOFs), "
-                  f"FP64 oracle (oracle/dg_oracle.cc, g++ -O2), {el:.1f} s",
+        "sample": f"{steps} RK4 step(s) of the same workload ({U0.size} DOFs), FP64 oracle "
+                  f"(oracle/dg_oracle.cc, g++ -O2), {el:.1f} s",
     }
     return cpu, err
 

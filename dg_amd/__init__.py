@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libdgx.so")
 SINGLEFLUID, MULTIFLUID = 0, 1
 RUSANOV, ROE = 0, 1
 PERIODIC, FARFIELD, REFLECTIVE = 1, 2, 3
-LIMIT_NONE, LIMIT_HR, LIMIT_HRI = 0, 1, 3
+LIMIT_NONE, LIMIT_HR, LIMIT_HRI, LIMIT_LEARNED = 0, 1, 3, 4
 PATH_AUTO, PATH_GENERAL, PATH_TILED = 0, 1, 2
 
 
@@ -66,6 +66,26 @@ def lib() -> ctypes.CDLL:
         L.dgx_rk4_step.argtypes = [vp, vp, vp, i, vp]
         L.dgx_rk4_integrate.argtypes = [vp, vp, i, dp, i, dp, i, ip, ctypes.POINTER(dp), vp]
         L.dgx_set_config.argtypes = [ctypes.POINTER(Config)]
+        L.dgx_batch_create.argtypes = [ctypes.POINTER(Config), i, ip, ip, dp, dp, ip, ctypes.POINTER(vp)]
+        L.dgx_batch_free.argtypes = [vp]
+        L.dgx_batch_state_offset.argtypes = [vp, i]
+        L.dgx_batch_state_offset.restype = ctypes.c_long
+        L.dgx_batch_block_sizes.argtypes = [vp, ip]
+        L.dgx_batch_set_block_sizes.argtypes = [vp, ip]
+        L.dgx_batch_autotune.argtypes = [vp, vp, dp, i, vp]
+        L.dgx_batched_rk4.argtypes = [vp, vp, dp, i, i, i, vp, vp, vp]
+        L.dgx_mlp_param_count.restype = i
+        L.dgx_mlp_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+        L.dgx_mlp_free.argtypes = [vp]
+        L.dgx_mlp_set_params.argtypes = [vp, vp]
+        L.dgx_mlp_get_params.argtypes = [vp, vp]
+        L.dgx_mlp_forward.argtypes = [vp, vp, ctypes.c_long, vp, vp]
+        L.dgx_mlp_gradient.argtypes = [vp, vp, vp, ctypes.c_long, ctypes.c_float, vp, vp]
+        L.dgx_mlp_train.argtypes = [vp, vp, vp, ctypes.c_long, i, i, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_ulonglong, vp, vp]
+        L.dgx_solver_set_indicator.argtypes = [vp, vp, ctypes.c_float]
+        L.dgx_mlp_features.argtypes = [vp, vp, vp, vp]
+        L.dgx_indicator_flags.argtypes = [vp, vp, vp, vp, vp]
         L.dgx_solver_set_path.argtypes = [vp, i]
         L.dgx_solver_path.argtypes = [vp]
         L.dgx_profile_enable.argtypes = [vp, i]
@@ -208,6 +228,20 @@ class Solver:
         """Kernel path in use: PATH_GENERAL or PATH_TILED."""
         return lib().dgx_solver_path(self._h)
 
+    def set_indicator(self, mlp, threshold: float) -> None:
+        """Attach a learned indicator (Indicator) for limiter 4; None detaches."""
+        self._mlp = mlp
+        _check(lib().dgx_solver_set_indicator(self._h, mlp._h if mlp else None, threshold),
+               "dgx_solver_set_indicator")
+
+    def features(self, U, X, stream=None) -> None:
+        """Indicator features of U into X (float32 device array N_E x 64)."""
+        _check(lib().dgx_mlp_features(self._h, _ptr(U), _ptr(X), _stream(stream)), "dgx_mlp_features")
+
+    def indicator_flags(self, U, flags, z=None, stream=None) -> None:
+        _check(lib().dgx_indicator_flags(self._h, _ptr(U), _ptr(flags), _ptr(z), _stream(stream)),
+               "dgx_indicator_flags")
+
     def profile(self, enable: bool = True) -> None:
         _check(lib().dgx_profile_enable(self._h, 1 if enable else 0), "dgx_profile_enable")
 
@@ -222,6 +256,112 @@ class Solver:
         try:
             if getattr(self, "_h", None):
                 lib().dgx_solver_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """Ensemble of independent 1D problems advanced together (dgx_batch_*).
+
+    Problem k has n_elem[k] elements of order orders[k] on [x0, x1] with the
+    boundary tags bc; its state occupies U[offset(k):offset(k+1)] of the
+    packed state, in the usual per-problem layout."""
+
+    def __init__(self, config: Config, n_elem, orders, x0: float = -1.0, x1: float = 1.0,
+                 bc=(FARFIELD, FARFIELD)):
+        n = len(n_elem)
+        ne = (ctypes.c_int * n)(*n_elem)
+        po = (ctypes.c_int * n)(*orders)
+        bcs = (ctypes.c_int * 2)(*bc)
+        h = ctypes.c_void_p()
+        _check(lib().dgx_batch_create(ctypes.byref(config), n, ne, po, x0, x1, bcs, ctypes.byref(h)),
+               "dgx_batch_create")
+        self._h = h
+        self.n = n
+        self.config = config
+        self.offsets = [lib().dgx_batch_state_offset(h, k) for k in range(n + 1)]
+        self.state_size = self.offsets[-1]
+
+    def pack(self, states) -> np.ndarray:
+        return np.concatenate([np.asarray(s, dtype=np.float64).ravel() for s in states])
+
+    def unpack(self, U):
+        U = np.asarray(U)
+        return [U[self.offsets[k]:self.offsets[k + 1]] for k in range(self.n)]
+
+    @property
+    def block_sizes(self):
+        z = (ctypes.c_int * 3)()
+        _check(lib().dgx_batch_block_sizes(self._h, z), "dgx_batch_block_sizes")
+        return list(z)
+
+    def set_block_sizes(self, sizes) -> None:
+        z = (ctypes.c_int * 3)(*sizes)
+        _check(lib().dgx_batch_set_block_sizes(self._h, z), "dgx_batch_set_block_sizes")
+
+    def autotune(self, U, cfl: float, limiter: int = LIMIT_HR, stream=None) -> None:
+        _check(lib().dgx_batch_autotune(self._h, _ptr(U), cfl, limiter, _stream(stream)), "dgx_batch_autotune")
+
+    def rk4(self, U, cfl: float, nsteps: int, T, status, limiter: int = LIMIT_HR, limit_initial: bool = True,
+            stream=None) -> None:
+        """nsteps RK4 steps of every problem with Dt = cfl*Dx_k/(2p_k+1)/max(|u|+a); T (float64[n]) and
+        status (int32[n]) are device arrays updated per problem."""
+        _check(lib().dgx_batched_rk4(self._h, _ptr(U), cfl, nsteps, limiter, 1 if limit_initial else 0, _ptr(T),
+                                     _ptr(status), _stream(stream)), "dgx_batched_rk4")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().dgx_batch_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class Indicator:
+    """Learned troubled-cell indicator (dgx_mlp_*): 49 -> 64 -> 64 -> 1 MLP."""
+
+    N_IN, N_PAD, N_HIDDEN = 49, 64, 64
+
+    def __init__(self, seed: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().dgx_mlp_create(seed, ctypes.byref(h)), "dgx_mlp_create")
+        self._h = h
+        self.n_params = lib().dgx_mlp_param_count()
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        _check(lib().dgx_mlp_get_params(self._h, out.ctypes.data), "dgx_mlp_get_params")
+        return out
+
+    @params.setter
+    def params(self, value) -> None:
+        v = np.ascontiguousarray(value, dtype=np.float32)
+        assert v.size == self.n_params
+        _check(lib().dgx_mlp_set_params(self._h, v.ctypes.data), "dgx_mlp_set_params")
+
+    def forward(self, X, z, n=None, stream=None) -> None:
+        n = X.shape[0] if n is None else n
+        _check(lib().dgx_mlp_forward(self._h, _ptr(X), n, _ptr(z), _stream(stream)), "dgx_mlp_forward")
+
+    def gradient(self, X, y, grad, pos_weight: float = 1.0, stream=None) -> None:
+        """grad: float32 device array of n_params + 1 (the last entry is the loss)."""
+        _check(lib().dgx_mlp_gradient(self._h, _ptr(X), _ptr(y), X.shape[0], pos_weight, _ptr(grad),
+                                      _stream(stream)), "dgx_mlp_gradient")
+
+    def train(self, X, y, epochs: int, batch: int = 1024, lr: float = 1e-3, pos_weight: float = 1.0,
+              seed: int = 0, stream=None) -> np.ndarray:
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        _check(lib().dgx_mlp_train(self._h, _ptr(X), _ptr(y), X.shape[0], epochs, batch, lr, pos_weight, seed,
+                                   losses.ctypes.data, _stream(stream)), "dgx_mlp_train")
+        return losses[:epochs]
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().dgx_mlp_free(self._h)
                 self._h = None
         except Exception:
             pass

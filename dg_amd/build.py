@@ -35,6 +35,8 @@ SOURCES = [
     ("capi.cc", "hip-host", []),
     ("rhs.hip", "hipcc", []),
     ("stage.hip", "hipcc", []),
+    ("batch.hip", "hipcc", []),
+    ("mlp.hip", "hipcc", []),
     ("limiter.hip", "hipcc", ["-ffp-contract=off"]),
     ("compat.hip", "hipcc", []),
 ]

@@ -1,0 +1,643 @@
+// Batched ensemble: many independent 1D problems of uneven size and order,
+// each advanced by the RK4 loop of RK::RK_integration (rk.cc:132-247) with
+// its own CFL time step (rk.cc:250-272), the HR limiter (limiting.cc:11-27)
+// and the DG right-hand side of DG_SOLVER::dg_solver (dg_solver.h:292-364).
+//
+// One persistent kernel per polynomial order: a workgroup owns one problem
+// and runs all its steps, stages and limiter sweeps, separated by workgroup
+// barriers only (the problems never exchange data).  Per step: Dt from the
+// workgroup maximum of |u|+a; per stage: a face pass (one lane per interface,
+// Riemann flux integrated into element-major edge slots), an element pass
+// (one lane per element, volume terms in registers with the element-local
+// helpers of volume.cuh, Dt*Minv and the RK4 combination), and the HR
+// limiter (Lagrange->Taylor, per-column limiting with the neighbours'
+// Taylor coefficients, Taylor->Lagrange, without FMA contraction as in the
+// CPU path).  The launches of all orders are captured once in a hipGraph
+// (one branch per order) and replayed; the workgroup size of each order is
+// chosen by a timing autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "dgx_internal.h"
+#include "hr.cuh"
+#include "physics.cuh"
+#include "volume.cuh"
+
+namespace dgx {
+
+struct BatchArgs {
+  // reference-element tables of this order
+  const double* phi;
+  const double* dphi;
+  const double* phi_w;
+  const double* dphi_w;
+  const double* psi_w;  // 1 x 1 face weight
+  const double* L2M;    // Lag2Mono, column-major N_s x N_s
+  const double* M2L;    // Mono2Lag
+  // problems of this order
+  int nprob;
+  const int* prob;          // problem ids
+  const int* n_elem;        // [problem]
+  const long* state_off;    // [problem] doubles into the packed state
+  const long* elem_off;     // [problem] elements into the packed per-element arrays
+  const double* J;          // [element]
+  const double* ij;         // [element]
+  const double* normals;    // [elem_off + problem + face]
+  const double* minv;       // [problem*16 + k]: Minv of the problem's element 0 (N_s x N_s)
+  const double* J0;         // [problem] Jacobian of that element
+  const double* cfl;        // [problem] CFL * Dx / (2p+1)
+  int bc[2];
+  double* U;
+  double* Uacc;
+  double* S;    // stage state
+  double* A;    // limiter: Taylor coefficients
+  double* Qe;   // [element][edge][field]
+  double* T;    // [problem] accumulated time
+  int* status;  // [problem] 0 ok, 1 Dt too small or NaN
+  int nsteps, limiter, limit_initial;
+};
+
+// max that keeps a NaN: a problem whose state holds a NaN gets a NaN Dt and
+// stops with status 1 at once (the reference's idamax skips NaNs that are
+// not first and stops only once its Dt turns NaN, rk.cc:147-148)
+__device__ inline double nanmax(double a, double b) { return (b != b || b > a) ? b : a; }
+
+template <int P, int PH>
+struct BatchShape {
+  static constexpr int NF = Fields<1, PH>::NF, Ns = P + 1, NG = P + 1, NU = NF * Ns;
+};
+
+// HR limiter of one problem (limiting.cc:11-27 in 1D): A = L2M U, then per
+// column (element, field) hrl1D and U = M2L Alim.  Contraction off.
+template <int P, int PH>
+__device__ void batch_limit(const BatchArgs& a, double* U, double* A, int NE, const double* sL2M,
+                            const double* sM2L) {
+#pragma clang fp contract(off)
+  using B = BatchShape<P, PH>;
+  constexpr int NF = B::NF, Ns = B::Ns;
+  for (int col = threadIdx.x; col < NE * NF; col += blockDim.x) {
+    double x[Ns], y[Ns];
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) x[l] = U[col * Ns + l];
+#pragma unroll
+    for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l)
+#pragma unroll
+      for (int r = 0; r < Ns; ++r) y[r] = y[r] + x[l] * sL2M[r + l * Ns];
+#pragma unroll
+    for (int r = 0; r < Ns; ++r) A[col * Ns + r] = y[r];
+  }
+  __syncthreads();
+  for (int col = threadIdx.x; col < NE * NF; col += blockDim.x) {
+    const int e = col / NF, fc = col % NF;
+    const int left = e > 0 ? e - 1 : -a.bc[0];
+    const int right = e < NE - 1 ? e + 1 : -a.bc[1];
+    int physical = 0;
+    if (left < 0) physical = -left;
+    if (right < 0) physical = -right;
+    double lim[Ns], ac[Ns];
+#pragma unroll
+    for (int n = 0; n < Ns; ++n) { ac[n] = A[col * Ns + n]; lim[n] = 0.0; }
+    if (physical == 2 || (physical == 3 && fc != 1)) {
+      average_monomial_d<P>(ac, lim);
+    } else if (physical == 3) {
+#pragma unroll
+      for (int n = 0; n < Ns; ++n) lim[n] = ac[n];
+    } else {
+      double al[Ns], ar[Ns];
+#pragma unroll
+      for (int n = 0; n < Ns; ++n) {
+        al[n] = A[(left * NF + fc) * Ns + n];
+        ar[n] = A[(right * NF + fc) * Ns + n];
+      }
+      limit_monomial_d<P>(al, ac, ar, lim);
+    }
+    double y[Ns];
+#pragma unroll
+    for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l)
+#pragma unroll
+      for (int r = 0; r < Ns; ++r) y[r] = y[r] + lim[l] * sM2L[r + l * Ns];
+#pragma unroll
+    for (int r = 0; r < Ns; ++r) U[col * Ns + r] = y[r];
+  }
+  __syncthreads();
+}
+
+template <int P, int PH, int FL>
+__global__ __launch_bounds__(512) void k_batch(BatchArgs a) {
+  using B = BatchShape<P, PH>;
+  using V = Vol<1, P, PH>;
+  constexpr int NF = B::NF, Ns = B::Ns, NG = B::NG, NU = B::NU, NQ = V::NQ;
+  __shared__ double sPhi[V::NTAB_PHI], sDphi[PH ? V::NTAB_D : 1], sDw[V::NTAB_D], sPw[PH ? V::NTAB_PHI : 1];
+  __shared__ double sMinv[Ns * Ns], sL2M[Ns * Ns], sM2L[Ns * Ns];
+  __shared__ double sRed[8];
+  const VolTables<1, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
+
+  const int pid = a.prob[blockIdx.x];
+  const int NE = a.n_elem[pid];
+  const long so = a.state_off[pid], eo = a.elem_off[pid];
+  double* U = a.U + so;
+  double* Uacc = a.Uacc + so;
+  double* S = a.S + so;
+  double* A = a.A + so;
+  double* Qe = a.Qe + eo * 2 * NF;
+  const double* J = a.J + eo;
+  const double* ij = a.ij + eo;
+  const double* nrm = a.normals + eo + pid;
+  const double cfl = a.cfl[pid], J0 = a.J0[pid], psiw = a.psi_w[0];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+
+  load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, nullptr, false, tid, nthr);
+  for (int k = tid; k < Ns * Ns; k += nthr) {
+    sMinv[k] = a.minv[(long)pid * 16 + k];
+    sL2M[k] = a.L2M[k];
+    sM2L[k] = a.M2L[k];
+  }
+  __syncthreads();
+  if (a.status[pid] != 0) return;
+  if (a.limiter && a.limit_initial) batch_limit<P, PH>(a, U, A, NE, sL2M, sM2L);
+
+  for (int step = 0; step < a.nsteps; ++step) {
+    // ---- Dt = cfl / max over nodes of |u|+a (rk.cc:250-272)
+    double m = 0.0;
+    for (int n = tid; n < NE * Ns; n += nthr) {
+      const int e = n / Ns, i = n % Ns;
+      double u[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) u[f] = U[(e * NF + f) * Ns + i];
+      m = nanmax(m, upa<1, PH>(u));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = nanmax(m, __shfl_xor(m, off));
+    if ((tid & 63) == 0) sRed[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      double mx = sRed[0];
+      for (int w = 1; w < (nthr + 63) / 64; ++w) mx = nanmax(mx, sRed[w]);
+      sRed[0] = cfl / mx;
+    }
+    __syncthreads();
+    const double dt = sRed[0];
+    __syncthreads();  // sRed is reused by the next step
+    if (!(dt >= 1e-14)) {
+      if (tid == 0) a.status[pid] = 1;
+      return;
+    }
+    if (tid == 0) a.T[pid] = a.T[pid] + dt;
+
+    double* cur = U;
+    for (int k = 0; k < 4; ++k) {
+      if (k > 0 && a.limiter) batch_limit<P, PH>(a, cur, A, NE, sL2M, sM2L);
+      // ---- faces: interface j between elements j-1 and j; boundary faces
+      // contribute to the boundary element only (dg_functions.cc:225-233)
+      for (int j = tid; j <= NE; j += nthr) {
+        const bool lb = j == 0, rb = j == NE;
+        const int e0 = lb ? 0 : j - 1, n0 = lb ? 0 : 1;  // side 0: element and node
+        const int e1 = (lb || rb) ? e0 : j, n1 = (lb || rb) ? n0 : 0;
+        const int tag = lb ? a.bc[0] : (rb ? a.bc[1] : 0);
+        double ul[NF], ur[NF], F[NF], nc[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          ul[f] = cur[(e0 * NF + f) * Ns + n0];
+          ur[f] = cur[(e1 * NF + f) * Ns + n1];
+        }
+        if (tag == BC_REFLECTIVE) ur[1] = -ur[1];  // boundaries.cu:20-35
+        riemann<1, PH, FL>(ul, ur, nrm[j], 0.0, F, nc);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          Qe[(e0 * 2 + n0) * NF + f] = psiw * (-F[f] + nc[f]);
+          if (!lb && !rb) Qe[(e1 * 2 + n1) * NF + f] = psiw * (F[f] + nc[f]);
+        }
+      }
+      __syncthreads();
+      // ---- elements: volume terms, face terms, Dt*Minv and the RK4 update
+      double* out = (k == 3) ? U : S;
+      for (int e = tid; e < NE; e += nthr) {
+        double u[NU], fe[NG * NQ], qe[2 * NF], R[NU];
+#pragma unroll
+        for (int r = 0; r < NU; ++r) u[r] = cur[e * NU + r];
+#pragma unroll
+        for (int r = 0; r < 2 * NF; ++r) qe[r] = Qe[e * 2 * NF + r];
+        const double Je = J[e], ije = ij[e];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) vol_phase1<1, P, PH>(tb, g, u, Je, &ije, fe + g * NQ);
+#pragma unroll
+        for (int i = 0; i < Ns; ++i) {
+          double rhs[NF];
+          vol_phase2<1, P, PH>(tb, i, fe, qe, rhs);
+#pragma unroll
+          for (int f = 0; f < NF; ++f) R[f * Ns + i] = rhs[f];
+        }
+        const double dte = dt * (J0 / Je);
+#pragma unroll
+        for (int r = 0; r < NU; ++r) {
+          const int f = r / Ns, i = r % Ns;
+          double sol = 0.0;
+#pragma unroll
+          for (int ii = 0; ii < Ns; ++ii) sol += sMinv[ii * Ns + i] * R[f * Ns + ii];
+          const long idx = (long)e * NU + r;
+          const double un = k < 3 ? U[idx] : 0.0;
+          const double acc = k > 0 ? Uacc[idx] : 0.0;
+          rk_combine(k, dte * sol, un, acc, Uacc + idx, out + idx);
+        }
+      }
+      __syncthreads();
+      cur = out;
+    }
+    if (a.limiter) batch_limit<P, PH>(a, U, A, NE, sL2M, sM2L);
+  }
+}
+
+__global__ void k_fill_zero_batch(double* T, int* status, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    T[k] = 0.0;
+    status[k] = 0;
+  }
+}
+
+// ------------------------------------------------------------------- host
+template <typename Fn>
+static hipError_t dispatch_batch(int P, const dgx_config& c, Fn&& fn) {
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  auto by_flux = [&](auto p, auto ph) -> hipError_t { return c.flux == 0 ? fn(p, ph, I0{}) : fn(p, ph, I1{}); };
+  auto by_phys = [&](auto p) -> hipError_t { return c.physics == 0 ? by_flux(p, I0{}) : by_flux(p, I1{}); };
+  switch (P) {
+    case 1: return by_phys(std::integral_constant<int, 1>{});
+    case 2: return by_phys(std::integral_constant<int, 2>{});
+    default: return by_phys(std::integral_constant<int, 3>{});
+  }
+}
+
+struct BatchGroup {
+  int order = 0;
+  int nprob = 0;
+  int block = 256;  // workgroup size (autotuned)
+  int max_ne = 0;
+  double *phi = nullptr, *dphi = nullptr, *phi_w = nullptr, *dphi_w = nullptr, *psi_w = nullptr;
+  double *L2M = nullptr, *M2L = nullptr;
+  int* prob = nullptr;
+};
+
+}  // namespace dgx
+
+using namespace dgx;
+
+struct dgx_batch {
+  dgx_config cfg{};
+  int n = 0;
+  int bc[2] = {2, 2};
+  std::vector<int> ne, order;
+  std::vector<long> state_off, elem_off;
+  BatchGroup groups[3];
+  int *d_ne = nullptr;
+  long *d_state_off = nullptr, *d_elem_off = nullptr;
+  double *d_J = nullptr, *d_ij = nullptr, *d_normals = nullptr, *d_minv = nullptr, *d_J0 = nullptr,
+         *d_dx = nullptr, *d_cfl = nullptr;
+  double *Uacc = nullptr, *S = nullptr, *A = nullptr, *Qe = nullptr;
+  double *tuneU = nullptr, *tuneT = nullptr;
+  int* tune_status = nullptr;
+  std::vector<void*> allocs;
+  // graph cache
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t cs[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  double* g_U = nullptr;
+  double* g_T = nullptr;
+  int* g_status = nullptr;
+  double g_cfl = 0;
+  int g_nsteps = -1, g_limiter = -1, g_init = -1;
+  std::vector<double> dx;
+};
+
+namespace {
+
+template <typename T>
+bool bupload(dgx_batch* b, T** dst, const T* src, size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  if (!hip_ok(hipMalloc(&p, n * sizeof(T)), "hipMalloc")) return false;
+  b->allocs.push_back(p);
+  *dst = static_cast<T*>(p);
+  if (src) return hip_ok(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+  return true;
+}
+
+void drop_graph(dgx_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void free_batch(dgx_batch* b) {
+  drop_graph(b);
+  for (int k = 0; k < 3; ++k) {
+    if (b->cs[k]) (void)hipStreamDestroy(b->cs[k]);
+    if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+  }
+  for (void* p : b->allocs) (void)hipFree(p);
+  b->allocs.clear();
+}
+
+dgx_status bfail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+BatchArgs make_args(dgx_batch* b, const BatchGroup& g, double* U, double cfl_unused, int nsteps, int limiter,
+                    int limit_initial, double* T, int* status) {
+  (void)cfl_unused;
+  BatchArgs a{};
+  a.phi = g.phi; a.dphi = g.dphi; a.phi_w = g.phi_w; a.dphi_w = g.dphi_w; a.psi_w = g.psi_w;
+  a.L2M = g.L2M; a.M2L = g.M2L;
+  a.nprob = g.nprob; a.prob = g.prob;
+  a.n_elem = b->d_ne; a.state_off = b->d_state_off; a.elem_off = b->d_elem_off;
+  a.J = b->d_J; a.ij = b->d_ij; a.normals = b->d_normals; a.minv = b->d_minv; a.J0 = b->d_J0; a.cfl = b->d_cfl;
+  a.bc[0] = b->bc[0]; a.bc[1] = b->bc[1];
+  a.U = U; a.Uacc = b->Uacc; a.S = b->S; a.A = b->A; a.Qe = b->Qe; a.T = T; a.status = status;
+  a.nsteps = nsteps; a.limiter = limiter; a.limit_initial = limit_initial;
+  return a;
+}
+
+hipError_t launch_group(dgx_batch* b, const BatchGroup& g, int block, const BatchArgs& a, hipStream_t st) {
+  return dispatch_batch(g.order, b->cfg, [&](auto p, auto ph, auto fl) -> hipError_t {
+    constexpr int P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
+    hipLaunchKernelGGL((k_batch<P, PH, FL>), dim3(g.nprob), dim3(block), 0, st, a);
+    return hipGetLastError();
+  });
+}
+
+// per-problem CFL step: deck CFL * Dx / (2p+1) (main.cc:896-897)
+bool upload_cfl(dgx_batch* b, double cfl) {
+  std::vector<double> c(b->n);
+  for (int k = 0; k < b->n; ++k) c[k] = cfl * b->dx[k] / (2 * b->order[k] + 1);
+  return hip_ok(hipMemcpy(b->d_cfl, c.data(), sizeof(double) * b->n, hipMemcpyHostToDevice), "cfl upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+dgx_status dgx_batch_create(const dgx_config* cfg, int n, const int* n_elem, const int* order, double x0, double x1,
+                            const int* bc, dgx_batch** out) {
+  if (!cfg || !n_elem || !order || !bc || !out || n <= 0) return bfail(DGX_EINVAL, "dgx_batch_create: bad arguments");
+  if (cfg->dim != 1) return bfail(DGX_EUNSUPPORTED, "dgx_batch_create: the ensemble runs 1D problems");
+  if (cfg->physics < 0 || cfg->physics > 1 || cfg->flux < 0 || cfg->flux > 1)
+    return bfail(DGX_EINVAL, "dgx_batch_create: unknown physics or flux");
+  for (int k = 0; k < 2; ++k)
+    if (bc[k] != BC_FARFIELD && bc[k] != BC_REFLECTIVE)
+      return bfail(DGX_EUNSUPPORTED, "dgx_batch_create: boundary tags must be 2 (farfield) or 3 (wall)");
+  dgx_batch* b = new dgx_batch;
+  b->cfg = *cfg;
+  b->n = n;
+  b->bc[0] = bc[0];
+  b->bc[1] = bc[1];
+  const int NF = 2 + 1 + cfg->physics;
+  std::vector<double> J, ij, nrm, minv((size_t)n * 16, 0.0), J0(n), dx(n);
+  std::vector<int> prob[3];
+  long so = 0, eo = 0;
+  for (int k = 0; k < n; ++k) {
+    if (n_elem[k] < 2 || order[k] < 1 || order[k] > 3) {
+      free_batch(b);
+      delete b;
+      return bfail(DGX_EINVAL, "dgx_batch_create: problems need >= 2 elements and order 1..3");
+    }
+    MeshSpec ms;
+    ms.dim = 1;
+    ms.order = order[k];
+    ms.nx = n_elem[k];
+    ms.x0 = x0;
+    ms.x1 = x1;
+    ms.bc[0] = bc[0];
+    ms.bc[1] = bc[1];
+    ms.nfields = NF;
+    Tables t;
+    std::string err = build_tables(ms, t);
+    if (!err.empty()) {
+      free_batch(b);
+      delete b;
+      return bfail(DGX_EINVAL, "dgx_batch_create: " + err);
+    }
+    const int Ns = t.N_s, NE = t.N_E;
+    b->ne.push_back(NE);
+    b->order.push_back(order[k]);
+    b->state_off.push_back(so);
+    b->elem_off.push_back(eo);
+    so += (long)NE * NF * Ns;
+    eo += NE;
+    for (int e = 0; e < NE; ++e) {
+      J.push_back(t.J[e]);
+      ij.push_back(t.invJac[(size_t)e * t.N_G]);
+    }
+    // normals in geometric face order: face j between elements j-1 and j
+    std::vector<double> nk(NE + 1, 0.0);
+    std::vector<char> seen(NE + 1, 0);
+    for (int f = 0; f < t.M_T; ++f) {
+      const int e = t.face_elem[2 * f], c = t.face_closure[2 * f];
+      const int j = (c == 1) ? e + 1 : e;
+      if (!(c == 0 && e == 0) && c != 1) {
+        free_batch(b);
+        delete b;
+        return bfail(DGX_EINVAL, "dgx_batch_create: unexpected face layout");
+      }
+      nk[j] = t.normals[f];
+      seen[j] = 1;
+    }
+    for (char v : seen)
+      if (!v) {
+        free_batch(b);
+        delete b;
+        return bfail(DGX_EINVAL, "dgx_batch_create: unexpected face layout");
+      }
+    nrm.insert(nrm.end(), nk.begin(), nk.end());
+    // one inverse mass matrix per problem, scaled by J_0/J_e (affine elements)
+    double mmax = 0.0;
+    for (int q = 0; q < Ns * Ns; ++q) mmax = std::max(mmax, std::fabs(t.Minv[q]));
+    for (int e = 1; e < NE; ++e) {
+      const double r = t.J[0] / t.J[e];
+      for (int q = 0; q < Ns * Ns; ++q)
+        if (!(std::fabs(t.Minv[(size_t)e * Ns * Ns + q] - t.Minv[q] * r) <= 1e-13 * mmax * std::max(1.0, r))) {
+          free_batch(b);
+          delete b;
+          return bfail(DGX_EUNSUPPORTED, "dgx_batch_create: element mass matrices are not J-scaled copies");
+        }
+    }
+    for (int q = 0; q < Ns * Ns; ++q) minv[(size_t)k * 16 + q] = t.Minv[q];
+    J0[k] = t.J[0];
+    dx[k] = t.Dx;
+    BatchGroup& g = b->groups[order[k] - 1];
+    prob[order[k] - 1].push_back(k);
+    g.max_ne = std::max(g.max_ne, NE);
+    if (!g.phi) {
+      g.order = order[k];
+      bool ok = bupload(b, &g.phi, t.phi.data(), t.phi.size()) && bupload(b, &g.dphi, t.dphi.data(), t.dphi.size()) &&
+                bupload(b, &g.phi_w, t.phi_w.data(), t.phi_w.size()) &&
+                bupload(b, &g.dphi_w, t.dphi_w.data(), t.dphi_w.size()) &&
+                bupload(b, &g.psi_w, t.psi_w.data(), t.psi_w.size()) &&
+                bupload(b, &g.L2M, t.Lag2Mono.data(), t.Lag2Mono.size()) &&
+                bupload(b, &g.M2L, t.Mono2Lag.data(), t.Mono2Lag.size());
+      if (!ok) {
+        free_batch(b);
+        delete b;
+        return DGX_EDEVICE;
+      }
+    }
+  }
+  b->state_off.push_back(so);
+  b->elem_off.push_back(eo);
+  b->dx = dx;
+  bool ok = bupload(b, &b->d_ne, b->ne.data(), n) && bupload(b, &b->d_state_off, b->state_off.data(), n + 1) &&
+            bupload(b, &b->d_elem_off, b->elem_off.data(), n + 1) && bupload(b, &b->d_J, J.data(), J.size()) &&
+            bupload(b, &b->d_ij, ij.data(), ij.size()) && bupload(b, &b->d_normals, nrm.data(), nrm.size()) &&
+            bupload(b, &b->d_J0, J0.data(), n) && bupload<double>(b, &b->d_cfl, nullptr, n) &&
+            bupload<double>(b, &b->Uacc, nullptr, so) && bupload<double>(b, &b->S, nullptr, so) &&
+            bupload<double>(b, &b->A, nullptr, so) && bupload<double>(b, &b->Qe, nullptr, (size_t)eo * 2 * NF) &&
+            bupload<double>(b, &b->tuneU, nullptr, so) && bupload<double>(b, &b->tuneT, nullptr, n) &&
+            bupload<int>(b, &b->tune_status, nullptr, n);
+  if (ok) ok = bupload(b, &b->d_minv, minv.data(), minv.size());
+  for (int q = 0; q < 3 && ok; ++q) {
+    BatchGroup& g = b->groups[q];
+    g.nprob = (int)prob[q].size();
+    if (g.nprob) ok = bupload(b, &g.prob, prob[q].data(), prob[q].size());
+  }
+  for (int q = 0; q < 3 && ok; ++q)
+    ok = hip_ok(hipStreamCreateWithFlags(&b->cs[q], hipStreamNonBlocking), "hipStreamCreate") &&
+         hip_ok(hipEventCreateWithFlags(&b->ev[q], hipEventDisableTiming), "hipEventCreate");
+  if (!ok) {
+    free_batch(b);
+    delete b;
+    return DGX_EDEVICE;
+  }
+  *out = b;
+  return DGX_OK;
+}
+
+void dgx_batch_free(dgx_batch* b) {
+  if (!b) return;
+  free_batch(b);
+  delete b;
+}
+
+long dgx_batch_state_offset(const dgx_batch* b, int k) {
+  if (!b || k < 0 || k > b->n) return -1;
+  return b->state_off[k];
+}
+
+dgx_status dgx_batch_block_sizes(const dgx_batch* b, int* sizes) {
+  if (!b || !sizes) return bfail(DGX_EINVAL, "dgx_batch_block_sizes: null argument");
+  for (int q = 0; q < 3; ++q) sizes[q] = b->groups[q].nprob ? b->groups[q].block : 0;
+  return DGX_OK;
+}
+
+dgx_status dgx_batch_set_block_sizes(dgx_batch* b, const int* sizes) {
+  if (!b || !sizes) return bfail(DGX_EINVAL, "dgx_batch_set_block_sizes: null argument");
+  for (int q = 0; q < 3; ++q)
+    if (sizes[q] < 64 || sizes[q] > 512 || sizes[q] % 64)
+      return bfail(DGX_EINVAL, "dgx_batch_set_block_sizes: sizes must be multiples of 64 in [64, 512]");
+  for (int q = 0; q < 3; ++q) b->groups[q].block = sizes[q];
+  drop_graph(b);
+  return DGX_OK;
+}
+
+dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int limiter, void* stream) {
+  if (!b || !U) return bfail(DGX_EINVAL, "dgx_batch_autotune: null argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!upload_cfl(b, cfl)) return DGX_EDEVICE;
+  hipEvent_t t0, t1;
+  if (!hip_ok(hipEventCreate(&t0), "hipEventCreate") || !hip_ok(hipEventCreate(&t1), "hipEventCreate"))
+    return DGX_EDEVICE;
+  const int cand[4] = {64, 128, 256, 512};
+  dgx_status r = DGX_OK;
+  for (int q = 0; q < 3 && r == DGX_OK; ++q) {
+    BatchGroup& g = b->groups[q];
+    if (!g.nprob) continue;
+    float best = 1e30f;
+    int best_b = g.block;
+    for (int c : cand) {
+      if (c > 64 && c / 2 >= g.max_ne + 1) break;  // more lanes than interfaces: no gain
+      float tot = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {
+        // scratch copies: the caller's state and times are left untouched
+        if (!hip_ok(hipMemcpyAsync(b->tuneU, U, sizeof(double) * b->state_off[b->n], hipMemcpyDeviceToDevice, st),
+                    "autotune copy")) { r = DGX_EDEVICE; break; }
+        hipLaunchKernelGGL(k_fill_zero_batch, dim3((b->n + 255) / 256), dim3(256), 0, st, b->tuneT, b->tune_status,
+                           b->n);
+        BatchArgs a = make_args(b, g, b->tuneU, cfl, 1, limiter, 0, b->tuneT, b->tune_status);
+        (void)hipEventRecord(t0, st);
+        if (!hip_ok(launch_group(b, g, c, a, st), "autotune launch")) { r = DGX_EDEVICE; break; }
+        (void)hipEventRecord(t1, st);
+        if (!hip_ok(hipEventSynchronize(t1), "autotune sync")) { r = DGX_EDEVICE; break; }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (rep == 1) tot += ms;  // first run warms the code object and caches
+      }
+      if (r != DGX_OK) break;
+      if (tot < best) {
+        best = tot;
+        best_b = c;
+      }
+    }
+    g.block = best_b;
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  drop_graph(b);
+  return r;
+}
+
+dgx_status dgx_batched_rk4(dgx_batch* b, double* U, double cfl, int nsteps, int limiter, int limit_initial, double* T,
+                           int* status, void* stream) {
+  if (!b || !U || !T || !status || nsteps < 0) return bfail(DGX_EINVAL, "dgx_batched_rk4: bad arguments");
+  if (limiter != 0 && limiter != 1) return bfail(DGX_EUNSUPPORTED, "dgx_batched_rk4: limiter must be 0 or 1 (hrl)");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const bool same = b->exec && b->g_U == U && b->g_T == T && b->g_status == status && b->g_cfl == cfl &&
+                    b->g_nsteps == nsteps && b->g_limiter == limiter && b->g_init == limit_initial;
+  if (!same) {
+    drop_graph(b);
+    if (!upload_cfl(b, cfl)) return DGX_EDEVICE;
+    // one graph, one branch per order: capture on internal streams
+    hipStream_t s0 = b->cs[0];
+    if (!hip_ok(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return DGX_EDEVICE;
+    bool ok = hip_ok(hipEventRecord(b->ev[0], s0), "hipEventRecord");
+    for (int q = 1; q < 3 && ok; ++q) ok = hip_ok(hipStreamWaitEvent(b->cs[q], b->ev[0], 0), "hipStreamWaitEvent");
+    for (int q = 0; q < 3 && ok; ++q) {
+      const BatchGroup& g = b->groups[q];
+      if (!g.nprob) continue;
+      BatchArgs a = make_args(b, g, U, cfl, nsteps, limiter, limit_initial, T, status);
+      ok = hip_ok(launch_group(b, g, g.block, a, b->cs[q]), "batched launch");
+    }
+    for (int q = 1; q < 3 && ok; ++q)
+      ok = hip_ok(hipEventRecord(b->ev[q], b->cs[q]), "hipEventRecord") &&
+           hip_ok(hipStreamWaitEvent(s0, b->ev[q], 0), "hipStreamWaitEvent");
+    hipGraph_t graph = nullptr;
+    const bool ended = hip_ok(hipStreamEndCapture(s0, &graph), "hipStreamEndCapture");
+    if (!ok || !ended) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return DGX_EDEVICE;
+    }
+    b->graph = graph;
+    if (!hip_ok(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+      drop_graph(b);
+      return DGX_EDEVICE;
+    }
+    b->g_U = U; b->g_T = T; b->g_status = status; b->g_cfl = cfl;
+    b->g_nsteps = nsteps; b->g_limiter = limiter; b->g_init = limit_initial;
+  }
+  if (!hip_ok(hipGraphLaunch(b->exec, st), "hipGraphLaunch")) return DGX_EDEVICE;
+  return DGX_OK;
+}
+
+}  // extern "C"

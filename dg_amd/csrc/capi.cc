@@ -32,6 +32,7 @@ struct dgx_mesh {
 
 struct dgx_solver {
   Solver s;
+  std::vector<double> ref_nodes;  // reference-element node coordinates [i*D + a]
 };
 
 namespace {
@@ -253,6 +254,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   if (cfg->physics < 0 || cfg->physics > 1 || cfg->flux < 0 || cfg->flux > 1)
     return fail(DGX_EINVAL, "dgx_solver_create: unknown physics or flux");
   dgx_solver* h = new dgx_solver;
+  h->ref_nodes = mesh->t.ref_nodes;
   Solver& s = h->s;
   s.cfg = *cfg;
   s.D = t.D; s.P = t.order; s.NF = dgx_nfields(cfg);
@@ -393,7 +395,12 @@ static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
                                     "the reference, sensor_kernels.cu:55-60)");
     return hip_ok(launch_limit_hri(s, U, st), "HRI limiter") ? DGX_OK : DGX_EDEVICE;
   }
-  return fail(DGX_EUNSUPPORTED, "limiter must be 0 (none), 1 (hrl) or 3 (hri)");
+  if (limiter == 4) {
+    if (!s.mlp) return fail(DGX_EUNSUPPORTED, "limiter 4 needs an indicator (dgx_solver_set_indicator)");
+    if (!hip_ok(mlp_indicator_flags(s, U, st), "indicator")) return DGX_EDEVICE;
+    return hip_ok(launch_limit_flagged(s, U, st), "flagged HR limiter") ? DGX_OK : DGX_EDEVICE;
+  }
+  return fail(DGX_EUNSUPPORTED, "limiter must be 0 (none), 1 (hrl), 3 (hri) or 4 (learned indicator + HR)");
 }
 
 dgx_status dgx_limit(dgx_solver* h, int limiter, double* U, void* stream) {
@@ -472,7 +479,7 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
   } else if (mode == 1) {
     if ((r = do_limit(s, limiter, U, st)) != DGX_OK) return r;
     bool done = false;
-    while (!done) {
+    while (!done && (nsteps <= 0 || n < nsteps)) {
       {
         KernelTimer kt(s, 3, st);
         if (!hip_ok(launch_upa_max(s, U, dt_or_cfl, dt_dev, st), "dt kernel")) return DGX_EDEVICE;
@@ -504,6 +511,72 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
   if (steps) *steps = n;
   if (t_final) *t_final = T;
   return DGX_OK;
+}
+
+// Tensor-Legendre transform of the 2D p=2 element: V[i][a*3+b] =
+// P_a(xi_i) P_b(eta_i) at the element's nodes, inverted by Gauss-Jordan with
+// partial pivoting; Vinv[q][i] maps nodal values to modal coefficients.
+static bool legendre_vinv(const std::vector<double>& nodes, int Ns, std::vector<double>& vinv) {
+  auto P = [](int a, double x) { return a == 0 ? 1.0 : (a == 1 ? x : 0.5 * (3 * x * x - 1)); };
+  std::vector<double> A((size_t)Ns * 2 * Ns, 0.0);
+  for (int i = 0; i < Ns; ++i) {
+    for (int q = 0; q < Ns; ++q) A[(size_t)i * 2 * Ns + q] = P(q / 3, nodes[i * 2]) * P(q % 3, nodes[i * 2 + 1]);
+    A[(size_t)i * 2 * Ns + Ns + i] = 1.0;
+  }
+  for (int c = 0; c < Ns; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < Ns; ++r)
+      if (std::fabs(A[(size_t)r * 2 * Ns + c]) > std::fabs(A[(size_t)piv * 2 * Ns + c])) piv = r;
+    if (std::fabs(A[(size_t)piv * 2 * Ns + c]) < 1e-12) return false;
+    for (int k = 0; k < 2 * Ns; ++k) std::swap(A[(size_t)c * 2 * Ns + k], A[(size_t)piv * 2 * Ns + k]);
+    const double d = A[(size_t)c * 2 * Ns + c];
+    for (int k = 0; k < 2 * Ns; ++k) A[(size_t)c * 2 * Ns + k] /= d;
+    for (int r = 0; r < Ns; ++r) {
+      if (r == c) continue;
+      const double f = A[(size_t)r * 2 * Ns + c];
+      for (int k = 0; k < 2 * Ns; ++k) A[(size_t)r * 2 * Ns + k] -= f * A[(size_t)c * 2 * Ns + k];
+    }
+  }
+  // inverse of V (nodes x modes) is modes x nodes: Vinv[q][i] = A[q][Ns + i]
+  vinv.assign((size_t)Ns * Ns, 0.0);
+  for (int q = 0; q < Ns; ++q)
+    for (int i = 0; i < Ns; ++i) vinv[(size_t)q * Ns + i] = A[(size_t)q * 2 * Ns + Ns + i];
+  return true;
+}
+
+dgx_status dgx_solver_set_indicator(dgx_solver* h, dgx_mlp* mlp, float threshold) {
+  if (!h) return fail(DGX_EINVAL, "dgx_solver_set_indicator: null solver");
+  Solver& s = h->s;
+  if (mlp && !(s.D == 2 && s.P == 2 && s.cfg.physics == DGX_MULTIFLUID))
+    return fail(DGX_EUNSUPPORTED, "dgx_solver_set_indicator: the indicator takes 2D p=2 two-fluid states");
+  if (mlp && !s.mlp_x) {
+    std::vector<double> vinv;
+    if (!legendre_vinv(h->ref_nodes, s.N_s, vinv)) return fail(DGX_EINVAL, "dgx_solver_set_indicator: singular transform");
+    if (!upload(s, &s.t.mlp_vinv, vinv.data(), vinv.size()) || !alloc(s, &s.mlp_avg, (size_t)s.N_E * 3) ||
+        !alloc(s, &s.mlp_x, (size_t)s.N_E * 64) || !alloc(s, &s.mlp_z, (size_t)s.N_E))
+      return DGX_EDEVICE;
+  }
+  s.mlp = mlp;
+  s.mlp_threshold = threshold;
+  return DGX_OK;
+}
+
+dgx_status dgx_mlp_features(dgx_solver* h, const double* U, float* X, void* stream) {
+  if (!h || !U || !X) return fail(DGX_EINVAL, "dgx_mlp_features: null argument");
+  Solver& s = h->s;
+  if (!s.mlp_x) return fail(DGX_EUNSUPPORTED, "dgx_mlp_features: attach an indicator first");
+  return hip_ok(mlp_features(s, U, X, as_stream(stream)), "features") ? DGX_OK : DGX_EDEVICE;
+}
+
+dgx_status dgx_indicator_flags(dgx_solver* h, const double* U, int* flags, float* z, void* stream) {
+  if (!h || !U || !flags) return fail(DGX_EINVAL, "dgx_indicator_flags: null argument");
+  Solver& s = h->s;
+  if (!s.mlp) return fail(DGX_EUNSUPPORTED, "dgx_indicator_flags: no indicator attached");
+  hipStream_t st = as_stream(stream);
+  if (!hip_ok(mlp_indicator_flags(s, U, st), "indicator")) return DGX_EDEVICE;
+  bool ok = hip_ok(hipMemcpyAsync(flags, s.sensors, sizeof(int) * s.N_E, hipMemcpyDeviceToDevice, st), "flags copy");
+  if (ok && z) ok = hip_ok(hipMemcpyAsync(z, s.mlp_z, sizeof(float) * s.N_E, hipMemcpyDeviceToDevice, st), "z copy");
+  return ok ? DGX_OK : DGX_EDEVICE;
 }
 
 dgx_status dgx_solver_set_path(dgx_solver* h, int path) {

@@ -9,6 +9,8 @@
 #include "../../include/dgx.h"
 #include "mesh.h"
 
+struct dgx_mlp;
+
 namespace dgx {
 
 void set_error(const std::string& msg);
@@ -40,6 +42,9 @@ struct DevTables {
   // tile kernel face geometry: [face*4 + {n_x, n_y, JF side 0, JF side 1}]
   double* face_geo_x = nullptr;  // faces normal to x, index ey*(nx+1) + fx
   double* face_geo_y = nullptr;  // faces normal to y, index fy*nx + ex
+  // learned indicator (2D p=2): tensor-Legendre modes from nodal values,
+  // Vinv[q*N_s + i]; row 0 gives the cell mean
+  double* mlp_vinv = nullptr;
   // limiter / sensor
   int* neighbors = nullptr;     // [e*N_N + {L,R,D,U}]
   double *Lag2Mono = nullptr, *Mono2Lag = nullptr;
@@ -84,6 +89,12 @@ struct Solver {
   int n_thresh = 0;
   double thresh[3] = {0, 0, 0};
   std::vector<void*> allocations;
+  // learned indicator (limiter 4): network (not owned), threshold, scratch
+  dgx_mlp* mlp = nullptr;
+  float mlp_threshold = 0.0f;
+  double* mlp_avg = nullptr;  // [e*3 + {rho, p, alpha}] cell means
+  float* mlp_x = nullptr;     // [e*64 + k] features
+  float* mlp_z = nullptr;     // [e] logits
   // event timing (dgx_profile_*): [0] face, [1] elem, [2] limit, [3] dt, [4] stage
   bool prof_on = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof[5];
@@ -104,6 +115,11 @@ hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* 
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st);
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st);
+// HR sweeps on the cells flagged in s.sensors (the HRI sweeps without the sensor)
+hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st);
+// learned indicator (mlp.hip): features of U into X, and flags into s.sensors
+hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st);
+hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st);
 
 // drop-in compat: process-wide configuration for the reference-named launchers
 const dgx_config& current_config();

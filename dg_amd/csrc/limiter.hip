@@ -26,74 +26,10 @@
 #include <type_traits>
 
 #include "dgx_internal.h"
+#include "hr.cuh"
 #include "physics.cuh"
 
 namespace dgx {
-
-__device__ inline int signum_d(double v) { return v > 0 ? 1 : (v < 0 ? -1 : 0); }
-
-__device__ inline double minmod_d(double a, double b) {
-  int sa = signum_d(a);
-  if (sa != signum_d(b)) return 0;
-  double fa = fabs(a), fb = fabs(b);
-  if (fa < fb) return sa * fa;
-  return sa * fb;
-}
-
-__host__ __device__ constexpr int ifact(int n) { return n <= 1 ? 1 : n * ifact(n - 1); }
-__host__ __device__ constexpr long ipow(long b, int n) { return n == 0 ? 1 : b * ipow(b, n - 1); }
-
-// integral over [-1,1] of the k-th derivative of x^n/n! (limiting_kernels.cu:1748-1760)
-__device__ inline double integ_deriv(int k, int n) {
-  int num = n - k + 1;
-  if (num % 2) return 2.0 / (double)ifact(num);
-  return 0.0;
-}
-// over [a,b] with integer a,b (limiting_kernels.cu:1762-1775); powers are exact
-__device__ inline double integ_deriv_bounds(int k, int n, int a, int b) {
-  int num = n - k + 1;
-  return ((double)ipow(b, num) - (double)ipow(a, num)) / (double)ifact(num);
-}
-
-template <int N>
-__device__ inline void limit_monomial_d(const double* AL, const double* AC, const double* AR, double* Alim) {
-  double avgLC = 0;
-#pragma unroll
-  for (int m = N; m > 0; m--) {
-    double avgdUL = 0, avgdUC = 0, avgdUR = 0, avgRL = 0, avgRC = 0, avgRR = 0;
-#pragma unroll
-    for (int n = m - 1; n <= N; n++) {
-      double integral = integ_deriv(m - 1, n);
-      avgdUL += AL[n] * integral;
-      avgdUC += AC[n] * integral;
-      avgdUR += AR[n] * integral;
-      if (n >= m + 1) {
-        double alim = Alim[n];
-        avgRL += alim * integ_deriv_bounds(m - 1, n, -3, -1);
-        avgRC += alim * integral;
-        avgRR += alim * integ_deriv_bounds(m - 1, n, 1, 3);
-      }
-    }
-    double avgLL = 0.5 * (avgdUL - avgRL);
-    avgLC = 0.5 * (avgdUC - avgRC);
-    double avgLR = 0.5 * (avgdUR - avgRR);
-    double c1 = 0.5 * (avgLC - avgLL);
-    double c2 = 0.5 * (avgLR - avgLC);
-    Alim[m] = minmod_d(c1, c2);
-  }
-  Alim[0] = avgLC;
-}
-
-template <int N>
-__device__ inline void average_monomial_d(const double* A, double* Alim) {
-  double avg = 0;
-#pragma unroll
-  for (int n = 0; n <= N; n++) {
-    avg += A[n] * integ_deriv(0, n);
-    Alim[n] = 0;
-  }
-  Alim[0] = 0.5 * avg;
-}
 
 // Out[:,col] = M * In[:,col] for every column (e,f), netlib dgemm order.
 template <int Ns>
@@ -151,6 +87,85 @@ __global__ __launch_bounds__(256) void k_hrl(int N_E, int NF, int N_N, int slice
 #pragma unroll
     for (int n = 0; n < Nsl; ++n) Alim[idx + n] = lim[n];
   }
+}
+
+// ------------------------------------------------- fused global HR sweeps
+// The same arithmetic as k_apply + k_hrl + k_apply, per column (element,
+// field), without the intermediate full-state passes: a sweep kernel forms
+// the Taylor coefficients of its own column and of the two neighbour columns
+// it needs, limits every slice and applies the next transform.
+
+// y = M x in netlib dgemm order
+template <int Ns>
+__device__ inline void col_apply(const double* __restrict__ M, const double* x, double* y) {
+#pragma unroll
+  for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+#pragma unroll
+  for (int l = 0; l < Ns; ++l)
+#pragma unroll
+    for (int r = 0; r < Ns; ++r) y[r] = y[r] + x[l] * M[r + l * Ns];
+}
+
+// hrl1D for every slice of one column (limiting_kernels.cu:245-319)
+template <int N, int SL>
+__device__ inline void hrl_column(int physical, int fc, int offxy, const double* AL, const double* AC,
+                                  const double* AR, double* Alim) {
+  constexpr int Nsl = N + 1;
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) {
+    const int o = sl * Nsl;
+    if (physical == 2 || (physical == 3 && fc != ((offxy == 0) ? 1 : 2))) {
+      average_monomial_d<N>(AC + o, Alim + o);
+    } else if (physical == 3 || physical == 4) {  // wall-normal velocity (3) / gravity tag (4): unchanged
+#pragma unroll
+      for (int n = 0; n < Nsl; ++n) Alim[o + n] = AC[o + n];
+    } else {
+      double lim[Nsl];
+#pragma unroll
+      for (int n = 0; n < Nsl; ++n) lim[n] = 0.0;
+      limit_monomial_d<N>(AL + o, AC + o, AR + o, lim);
+#pragma unroll
+      for (int n = 0; n < Nsl; ++n) Alim[o + n] = lim[n];
+    }
+  }
+}
+
+__device__ inline int hr_physical(int left, int right) {
+  int physical = 0;
+  if (left < 0) physical = -left;
+  if (right < 0) physical = -right;
+  return physical;
+}
+
+// first sweep: Out = Mnext * hrl(Mfirst * U) along offxy (x in 2D; the only
+// sweep in 1D, where Mnext = Mono2Lag)
+template <int Ns, int N, int SL>
+__global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                                  const double* __restrict__ Mfirst, const double* __restrict__ Mnext,
+                                                  const double* __restrict__ In, double* __restrict__ Out) {
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= (long)N_E * NF) return;
+  const int e = (int)(col / NF), fc = (int)(col % NF);
+  const int left = nb[e * N_N + offxy], right = nb[e * N_N + offxy + 1];
+  const int physical = hr_physical(left, right);
+  double x[Ns], AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
+#pragma unroll
+  for (int l = 0; l < Ns; ++l) x[l] = In[col * Ns + l];
+  if (Mfirst) col_apply<Ns>(Mfirst, x, AC); else for (int l = 0; l < Ns; ++l) AC[l] = x[l];
+  if (physical == 0) {
+    const double* pl = In + ((long)left * NF + fc) * Ns;
+    const double* pr = In + ((long)right * NF + fc) * Ns;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) x[l] = pl[l];
+    if (Mfirst) col_apply<Ns>(Mfirst, x, AL); else for (int l = 0; l < Ns; ++l) AL[l] = x[l];
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) x[l] = pr[l];
+    if (Mfirst) col_apply<Ns>(Mfirst, x, AR); else for (int l = 0; l < Ns; ++l) AR[l] = x[l];
+  }
+  hrl_column<N, SL>(physical, fc, offxy, AL, AC, AR, Alim);
+  col_apply<Ns>(Mnext, Alim, x);
+#pragma unroll
+  for (int r = 0; r < Ns; ++r) Out[col * Ns + r] = x[r];
 }
 
 // ---------------------------------------------------------------- sensors
@@ -389,39 +404,27 @@ static hipError_t by_order_ns(int D, int P, Fn&& fn) {
   return fn(I2{}, I3{});
 }
 
-static hipError_t apply_mat(const Solver& s, const double* M, const double* In, double* Out, hipStream_t st) {
-  long ncol = (long)s.N_E * s.NF;
+// Two launches per application.  1D: A = Lag2Mono U (k_apply), then
+// U = Mono2Lag hrl(A) (k_hr_sweep).  2D: work = MonoX2MonoY hrl_x(Lag2MonoX U),
+// then U = MonoY2Lag hrl_y(work).
+hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
+  const long ncol = (long)s.N_E * s.NF;
   return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
-    constexpr int Ns = Shape<decltype(d)::value, decltype(p)::value>::Ns;
-    hipLaunchKernelGGL((k_apply<Ns>), dim3(nblk(ncol, 256)), dim3(256), 0, st, M, In, Out, ncol);
+    constexpr int D = decltype(d)::value, P = decltype(p)::value;
+    constexpr int Ns = Shape<D, P>::Ns, SL = (D == 1) ? 1 : P + 1;
+    const dim3 g(nblk(ncol, 256)), b(256);
+    if (D == 1) {
+      hipLaunchKernelGGL((k_apply<Ns>), g, b, 0, st, s.t.Lag2Mono, U, s.work, ncol);
+      hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, nullptr,
+                         s.t.Mono2Lag, s.work, U);
+    } else {
+      hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, s.t.Lag2MonoX,
+                         s.t.MonoX2MonoY, U, s.work);
+      hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 2, nullptr,
+                         s.t.MonoY2Lag, s.work, U);
+    }
     return hipGetLastError();
   });
-}
-
-static hipError_t hrl(const Solver& s, int slicenum, int offxy, const double* A, double* Alim, hipStream_t st) {
-  long total = (long)s.N_E * slicenum * s.NF;
-  auto go = [&](auto n) -> hipError_t {
-    hipLaunchKernelGGL((k_hrl<decltype(n)::value>), dim3(nblk(total, 256)), dim3(256), 0, st, s.N_E, s.NF, s.N_N,
-                       slicenum, s.t.neighbors, offxy, A, Alim);
-    return hipGetLastError();
-  };
-  if (s.P == 1) return go(std::integral_constant<int, 1>{});
-  if (s.P == 2) return go(std::integral_constant<int, 2>{});
-  return go(std::integral_constant<int, 3>{});
-}
-
-hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
-  hipError_t e;
-  if (s.D == 1) {
-    if ((e = apply_mat(s, s.t.Lag2Mono, U, s.work, st)) != hipSuccess) return e;
-    if ((e = hrl(s, 1, 0, s.work, s.work2, st)) != hipSuccess) return e;
-    return apply_mat(s, s.t.Mono2Lag, s.work2, U, st);
-  }
-  if ((e = apply_mat(s, s.t.Lag2MonoX, U, s.work, st)) != hipSuccess) return e;
-  if ((e = hrl(s, s.N1, 0, s.work, s.work2, st)) != hipSuccess) return e;
-  if ((e = apply_mat(s, s.t.MonoX2MonoY, s.work2, s.work, st)) != hipSuccess) return e;
-  if ((e = hrl(s, s.N1, 2, s.work, s.work2, st)) != hipSuccess) return e;
-  return apply_mat(s, s.t.MonoY2Lag, s.work2, U, st);
 }
 
 // Resolve sequential-order flags from per-element first hits.
@@ -491,12 +494,17 @@ static hipError_t hri(Solver& s, int offxy, const double* L2M, const double* M2L
   return hipGetLastError();
 }
 
-hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {
-  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
-  if (e != hipSuccess) return e;
+hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st) {
+  hipError_t e;
   if (s.D == 1) return hri(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
   if ((e = hri(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
   return hri(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);
+}
+
+hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  if (e != hipSuccess) return e;
+  return launch_limit_flagged(s, U, st);
 }
 
 // exported for the reference-named launchers (compat.hip)

@@ -1,0 +1,592 @@
+// Learned troubled-cell indicator: a 49 -> 64 -> 64 -> 1 MLP on per-element
+// features of the 2D p=2 two-fluid state, evaluated ahead of the limiter and
+// trained on the flags of the reference sensor (calc_sensors,
+// sensor_kernels.cu:19-163, computed by limiter.hip).  The indicator has no
+// counterpart in the reference.
+//
+// Features of element e (49, padded with zeros to 64):
+//   45: the tensor-Legendre modal coefficients m[f][a*3+b] of the 5 fields,
+//       divided by a per-field scale (|mean| for rho, E and 1/(gamma-1);
+//       sqrt(|mean rho| |mean E|) for the momenta), clamped to [-8, 8];
+//    4: for each neighbour (L, R, D, U) the largest relative jump of the cell
+//       means of rho, p and 1/(gamma-1): |a_n - a_e| / (a_n + a_e); 0 at a
+//       domain boundary.
+// Network: h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), z = w3 . h2 + b3,
+// the cell is flagged when z > threshold.  Weights and activations are BF16
+// (round to nearest even), accumulation FP32: the two 64-wide layers run on
+// MFMA (v_mfma_f32_16x16x16_bf16, one wave per 16 elements, weights held in
+// registers, the layer-1 output fragments feed layer 2 without data
+// movement).  Training: weighted binary cross-entropy, gradients by the
+// backward kernels below (FP32, deterministic fixed-order reductions), Adam
+// on FP32 master weights.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dgx_internal.h"
+#include "physics.cuh"
+
+namespace dgx {
+
+constexpr int kIn = 49, kInP = 64, kH = 64;
+// parameter vector: W1 [64][64] (inputs >= 49 stay zero), b1, W2 [64][64], b2, w3, b3
+constexpr int kW1 = 0, kB1 = kW1 + kH * kInP, kW2 = kB1 + kH, kB2 = kW2 + kH * kH, kW3 = kB2 + kH, kB3 = kW3 + kH;
+constexpr int kNParam = kB3 + 1;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// ------------------------------------------------------------- features
+// cell means (tensor-Legendre mode 0) of rho, p and alpha = 1/(gamma-1)
+__global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U, const double* __restrict__ w0, int N_E,
+                                                   double* __restrict__ avg) {
+  constexpr int NF = 5, Ns = 9;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N_E) return;
+  double m[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) s += w0[i] * U[(e * NF + f) * Ns + i];
+    m[f] = s;
+  }
+  const double gam = 1.0 + 1.0 / m[4];
+  const double p = (gam - 1.0) * (m[3] - 0.5 * (m[1] * m[1] + m[2] * m[2]) / m[0]);
+  avg[e * 3 + 0] = m[0];
+  avg[e * 3 + 1] = p;
+  avg[e * 3 + 2] = m[4];
+}
+
+__device__ inline double rel_jump(double a, double b) {
+  const double s = fabs(a) + fabs(b);
+  return s > 0.0 ? fabs(a - b) / s : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__ U, const double* __restrict__ Vinv,
+                                                      const double* __restrict__ avg, const int* __restrict__ nb,
+                                                      int N_E, float* __restrict__ X) {
+  constexpr int NF = 5, Ns = 9;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N_E) return;
+  double u[NF * Ns];
+#pragma unroll
+  for (int r = 0; r < NF * Ns; ++r) u[r] = U[(size_t)e * NF * Ns + r];
+  double mean[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) s += Vinv[i] * u[f * Ns + i];
+    mean[f] = s;
+  }
+  const double sc_mom = sqrt(fabs(mean[0]) * fabs(mean[3]));
+  float* x = X + (size_t)e * kInP;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const double sc = (f == 1 || f == 2) ? sc_mom : fabs(mean[f]);
+    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+#pragma unroll
+    for (int q = 0; q < Ns; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < Ns; ++i) s += Vinv[q * Ns + i] * u[f * Ns + i];
+      double v = s * inv;
+      v = v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0));
+      x[f * Ns + q] = (float)v;
+    }
+  }
+  const double r0 = avg[e * 3 + 0], p0 = avg[e * 3 + 1], a0 = avg[e * 3 + 2];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const int o = nb[e * 4 + n];
+    double j = 0.0;
+    if (o >= 0) {
+      j = rel_jump(avg[o * 3 + 0], r0);
+      j = fmax(j, rel_jump(avg[o * 3 + 1], p0));
+      j = fmax(j, rel_jump(avg[o * 3 + 2], a0));
+    }
+    x[45 + n] = (float)j;
+  }
+#pragma unroll
+  for (int k = kIn; k < kInP; ++k) x[k] = 0.0f;
+}
+
+// --------------------------------------------------------------- forward
+// One wave per 16 elements (persistent over element groups).  MFMA operand
+// layouts (16x16x16 bf16): A lane l = row l%16, k 4*(l/16)+0..3; B lane l =
+// column l%16, k 4*(l/16)+0..3; D lane l = rows 4*(l/16)+0..3, column l%16.
+// With rows = hidden units and columns = elements, the D fragment of M-tile
+// t is the B fragment of k-step t of the next layer.
+__global__ __launch_bounds__(256) void k_mlp_fwd(const uint16_t* __restrict__ w1b, const uint16_t* __restrict__ w2b,
+                                                 const uint16_t* __restrict__ w3b, const float* __restrict__ P,
+                                                 const float* __restrict__ X, const int* __restrict__ idx, int N,
+                                                 float* __restrict__ z, uint16_t* __restrict__ h1s,
+                                                 uint16_t* __restrict__ h2s) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, q = lane >> 4;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  bf16x4 a1[4][4], a2[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a1[t][s][j] = (short)w1b[(16 * t + col) * kInP + 16 * s + 4 * q + j];
+        a2[t][s][j] = (short)w2b[(16 * t + col) * kH + 16 * s + 4 * q + j];
+      }
+  float b1v[4][4], b2v[4][4], w3v[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int u = 16 * t + 4 * q + v;
+      b1v[t][v] = P[kB1 + u];
+      b2v[t][v] = P[kB2 + u];
+      w3v[t][v] = bf2f(w3b[u]);
+    }
+  const float b3 = P[kB3];
+  for (int g = wave; g * 16 < N; g += nwaves) {
+    const int n = g * 16 + col;
+    const bool valid = n < N;
+    const int row = valid ? (idx ? idx[n] : n) : 0;
+    bf16x4 bx[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bx[s][j] = valid ? (short)f2bf(X[(size_t)row * kInP + 16 * s + 4 * q + j]) : (short)0;
+    bf16x4 bh1[4], bh2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t][s], bx[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float h = fmaxf(acc[v] + b1v[t][v], 0.0f);
+        bh1[t][v] = (short)f2bf(h);
+      }
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][s], bh1[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint16_t hb = f2bf(fmaxf(acc[v] + b2v[t][v], 0.0f));
+        bh2[t][v] = (short)hb;
+        part += w3v[t][v] * bf2f(hb);
+      }
+    }
+    // sum the four lanes that hold the same element (q = 0..3), fixed order
+    const float p1 = __shfl_xor(part, 16);
+    const float s01 = (q & 1) ? p1 + part : part + p1;
+    const float p2 = __shfl_xor(s01, 32);
+    const float tot = (q & 2) ? p2 + s01 : s01 + p2;
+    if (valid) {
+      if (q == 0) z[n] = tot + b3;
+      if (h1s) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            h1s[(size_t)n * kH + 16 * t + 4 * q + v] = (uint16_t)bh1[t][v];
+            h2s[(size_t)n * kH + 16 * t + 4 * q + v] = (uint16_t)bh2[t][v];
+          }
+      }
+    }
+  }
+}
+
+// -------------------------------------------------------------- backward
+// 32 samples per block: dz and the layer-2 deltas per sample, then layer-1
+// deltas per (sample, unit), then one lane per parameter summing its
+// gradient over the block's samples in sample order.  Partial sums per block
+// are reduced by k_mlp_reduce in block order.
+constexpr int kSB = 32;
+
+__global__ __launch_bounds__(256) void k_mlp_bwd(const uint16_t* __restrict__ w2b, const uint16_t* __restrict__ w3b,
+                                                 const float* __restrict__ X, const int* __restrict__ idx,
+                                                 const float* __restrict__ y, int N, float pos_weight, float inv_n,
+                                                 const float* __restrict__ z, const uint16_t* __restrict__ h1s,
+                                                 const uint16_t* __restrict__ h2s, float* __restrict__ partial) {
+  __shared__ float sX[kSB][kInP], sH1[kSB][kH], sH2[kSB][kH], sD1[kSB][kH], sD2[kSB][kH];
+  __shared__ float sDz[kSB], sLoss[kSB];
+  const int tid = threadIdx.x;
+  const int n0 = blockIdx.x * kSB;
+  for (int k = tid; k < kSB * kH; k += 256) {
+    const int s = k / kH, c = k % kH, n = n0 + s;
+    const bool v = n < N;
+    const int row = v ? (idx ? idx[n] : n) : 0;
+    sX[s][c] = v ? bf2f(f2bf(X[(size_t)row * kInP + c])) : 0.0f;
+    sH1[s][c] = v ? bf2f(h1s[(size_t)n * kH + c]) : 0.0f;
+    sH2[s][c] = v ? bf2f(h2s[(size_t)n * kH + c]) : 0.0f;
+  }
+  __syncthreads();
+  if (tid < kSB) {
+    const int n = n0 + tid;
+    float dz = 0.0f, loss = 0.0f;
+    if (n < N) {
+      const float zz = z[n], yy = y[idx ? idx[n] : n];
+      const float sg = 1.0f / (1.0f + expf(-zz));
+      dz = (pos_weight * yy * (sg - 1.0f) + (1.0f - yy) * sg) * inv_n;
+      // softplus(t) = log(1 + e^t), stable
+      const float sp_neg = fmaxf(-zz, 0.0f) + log1pf(expf(-fabsf(zz)));
+      const float sp_pos = fmaxf(zz, 0.0f) + log1pf(expf(-fabsf(zz)));
+      loss = (pos_weight * yy * sp_neg + (1.0f - yy) * sp_pos) * inv_n;
+    }
+    sDz[tid] = dz;
+    sLoss[tid] = loss;
+  }
+  __syncthreads();
+  for (int k = tid; k < kSB * kH; k += 256) {
+    const int s = k / kH, u = k % kH;
+    sD2[s][u] = sH2[s][u] > 0.0f ? sDz[s] * bf2f(w3b[u]) : 0.0f;
+  }
+  __syncthreads();
+  for (int k = tid; k < kSB * kH; k += 256) {
+    const int s = k / kH, v = k % kH;
+    float acc = 0.0f;
+    for (int u = 0; u < kH; ++u) acc += bf2f(w2b[u * kH + v]) * sD2[s][u];
+    sD1[s][v] = sH1[s][v] > 0.0f ? acc : 0.0f;
+  }
+  __syncthreads();
+  float* out = partial + (size_t)blockIdx.x * (kNParam + 1);
+  for (int p = tid; p < kNParam + 1; p += 256) {
+    float g = 0.0f;
+    if (p < kB1) {  // W1[v][k]
+      const int v = p / kInP, k = p % kInP;
+      for (int s = 0; s < kSB; ++s) g += sD1[s][v] * sX[s][k];
+    } else if (p < kW2) {
+      const int v = p - kB1;
+      for (int s = 0; s < kSB; ++s) g += sD1[s][v];
+    } else if (p < kB2) {  // W2[u][v]
+      const int u = (p - kW2) / kH, v = (p - kW2) % kH;
+      for (int s = 0; s < kSB; ++s) g += sD2[s][u] * sH1[s][v];
+    } else if (p < kW3) {
+      const int u = p - kB2;
+      for (int s = 0; s < kSB; ++s) g += sD2[s][u];
+    } else if (p < kB3) {
+      const int u = p - kW3;
+      for (int s = 0; s < kSB; ++s) g += sDz[s] * sH2[s][u];
+    } else if (p == kB3) {
+      for (int s = 0; s < kSB; ++s) g += sDz[s];
+    } else {
+      for (int s = 0; s < kSB; ++s) g += sLoss[s];
+    }
+    out[p] = g;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int nblocks,
+                                                    float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kNParam + 1) return;
+  float g = 0.0f;
+  for (int b = 0; b < nblocks; ++b) g += partial[(size_t)b * (kNParam + 1) + p];
+  grad[p] = g;
+}
+
+struct AdamArgs {
+  float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+};
+
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                                                  const float* __restrict__ grad, AdamArgs a, uint16_t* __restrict__ w1b,
+                                                  uint16_t* __restrict__ w2b, uint16_t* __restrict__ w3b,
+                                                  float* __restrict__ loss_acc) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0 && loss_acc) *loss_acc += grad[kNParam];
+  if (p >= kNParam) return;
+  const float g = grad[p];
+  const float mm = a.b1 * m[p] + (1.0f - a.b1) * g;
+  const float vv = a.b2 * v[p] + (1.0f - a.b2) * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  const float w = P[p] - a.lr * (mm / a.c1) / (sqrtf(vv / a.c2) + a.eps);
+  P[p] = w;
+  if (p < kB1) w1b[p - kW1] = f2bf(w);
+  else if (p >= kW2 && p < kB2) w2b[p - kW2] = f2bf(w);
+  else if (p >= kW3 && p < kB3) w3b[p - kW3] = f2bf(w);
+}
+
+__global__ void k_mlp_cast(const float* __restrict__ P, uint16_t* __restrict__ w1b, uint16_t* __restrict__ w2b,
+                           uint16_t* __restrict__ w3b) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kB1) w1b[p] = f2bf(P[kW1 + p]);
+  if (p < kH * kH) w2b[p] = f2bf(P[kW2 + p]);
+  if (p < kH) w3b[p] = f2bf(P[kW3 + p]);
+}
+
+__global__ void k_mlp_flags(const float* __restrict__ z, int N, float threshold, int* __restrict__ flags) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) flags[n] = z[n] > threshold ? 1 : 0;
+}
+
+__global__ void k_fill_f32(float* p, int n, float v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+static inline unsigned nblocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace dgx
+
+using namespace dgx;
+
+struct dgx_mlp {
+  float* P = nullptr;  // FP32 master parameters
+  float *m = nullptr, *v = nullptr, *grad = nullptr;
+  uint16_t *w1b = nullptr, *w2b = nullptr, *w3b = nullptr;
+  // training scratch, grown on demand
+  float* z = nullptr;
+  uint16_t *h1 = nullptr, *h2 = nullptr;
+  float* partial = nullptr;
+  int* idx = nullptr;
+  float* loss = nullptr;
+  long cap = 0, idx_cap = 0;
+  int step = 0;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+bool mlp_alloc(dgx_mlp* m, void** p, size_t bytes) {
+  if (!hip_ok(hipMalloc(p, bytes ? bytes : 4), "hipMalloc")) return false;
+  m->allocs.push_back(*p);
+  return true;
+}
+
+void mlp_release(dgx_mlp* m) {
+  for (void* p : m->allocs) (void)hipFree(p);
+  m->allocs.clear();
+}
+
+dgx_status mfail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+// scratch for n samples of activations (forward with store) and backward partials
+bool ensure_scratch(dgx_mlp* m, long n) {
+  if (n <= m->cap) return true;
+  const long c = std::max(n, 2 * m->cap);
+  void *z, *h1, *h2, *part;
+  if (!mlp_alloc(m, &z, sizeof(float) * c) || !mlp_alloc(m, &h1, 2 * kH * c) || !mlp_alloc(m, &h2, 2 * kH * c) ||
+      !mlp_alloc(m, &part, sizeof(float) * (kNParam + 1) * ((c + kSB - 1) / kSB)))
+    return false;
+  m->z = (float*)z;
+  m->h1 = (uint16_t*)h1;
+  m->h2 = (uint16_t*)h2;
+  m->partial = (float*)part;
+  m->cap = c;
+  return true;
+}
+
+unsigned fwd_grid(long n) {
+  const long waves = (n + 15) / 16;
+  return (unsigned)std::max<long>(1, std::min<long>((waves + 3) / 4, 2048));
+}
+
+hipError_t forward(dgx_mlp* m, const float* X, const int* idx, long n, float* z, uint16_t* h1, uint16_t* h2,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(fwd_grid(n)), dim3(256), 0, st, m->w1b, m->w2b, m->w3b, m->P, X, idx, (int)n, z,
+                     h1, h2);
+  return hipGetLastError();
+}
+
+// mean gradient of the weighted BCE over samples idx[0..n) (or rows 0..n)
+hipError_t gradient(dgx_mlp* m, const float* X, const int* idx, const float* y, long n, float pos_weight,
+                    hipStream_t st) {
+  hipError_t e = forward(m, X, idx, n, m->z, m->h1, m->h2, st);
+  if (e != hipSuccess) return e;
+  const unsigned nb = nblocks_for(n, kSB);
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(nb), dim3(256), 0, st, m->w2b, m->w3b, X, idx, y, (int)n, pos_weight,
+                     1.0f / (float)n, m->z, m->h1, m->h2, m->partial);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(nblocks_for(kNParam + 1, 256)), dim3(256), 0, st, m->partial, (int)nb,
+                     m->grad);
+  return hipGetLastError();
+}
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+}  // namespace
+
+namespace dgx {
+
+// Indicator state of a solver: features, logits and the flags that gate the
+// sensor-driven HR sweeps (limiter 4).
+hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st) {
+  dgx_mlp* m = s.mlp;
+  const int N = s.N_E;
+  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  hipLaunchKernelGGL(k_mlp_features, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg,
+                     s.t.neighbors, N, s.mlp_x);
+  hipError_t e = forward(m, s.mlp_x, nullptr, N, s.mlp_z, nullptr, nullptr, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mlp_flags, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_z, N, s.mlp_threshold,
+                     s.sensors);
+  return hipGetLastError();
+}
+
+hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st) {
+  const int N = s.N_E;
+  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  hipLaunchKernelGGL(k_mlp_features, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg,
+                     s.t.neighbors, N, X);
+  return hipGetLastError();
+}
+
+}  // namespace dgx
+
+extern "C" {
+
+int dgx_mlp_param_count(void) { return kNParam; }
+
+dgx_status dgx_mlp_create(unsigned long long seed, dgx_mlp** out) {
+  if (!out) return mfail(DGX_EINVAL, "dgx_mlp_create: null argument");
+  dgx_mlp* m = new dgx_mlp;
+  void *P, *mm, *vv, *g, *w1, *w2, *w3, *loss;
+  bool ok = mlp_alloc(m, &P, sizeof(float) * kNParam) && mlp_alloc(m, &mm, sizeof(float) * kNParam) &&
+            mlp_alloc(m, &vv, sizeof(float) * kNParam) && mlp_alloc(m, &g, sizeof(float) * (kNParam + 1)) &&
+            mlp_alloc(m, &w1, 2 * kH * kInP) && mlp_alloc(m, &w2, 2 * kH * kH) && mlp_alloc(m, &w3, 2 * kH) &&
+            mlp_alloc(m, &loss, sizeof(float) * 4);
+  if (!ok) {
+    mlp_release(m);
+    delete m;
+    return DGX_EDEVICE;
+  }
+  m->P = (float*)P; m->m = (float*)mm; m->v = (float*)vv; m->grad = (float*)g;
+  m->w1b = (uint16_t*)w1; m->w2b = (uint16_t*)w2; m->w3b = (uint16_t*)w3; m->loss = (float*)loss;
+  // He-uniform initialisation from a seeded splitmix64 stream; padded inputs stay zero
+  std::vector<float> h(kNParam, 0.0f);
+  uint64_t s = seed;
+  auto uni = [&]() { return (float)((splitmix(s) >> 11) * (1.0 / 9007199254740992.0)) * 2.0f - 1.0f; };
+  const float a1 = std::sqrt(6.0f / kIn), a2 = std::sqrt(6.0f / kH);
+  for (int u = 0; u < kH; ++u)
+    for (int k = 0; k < kIn; ++k) h[kW1 + u * kInP + k] = a1 * uni();
+  for (int u = 0; u < kH * kH; ++u) h[kW2 + u] = a2 * uni();
+  for (int u = 0; u < kH; ++u) h[kW3 + u] = a2 * uni();
+  *out = m;
+  const dgx_status r = dgx_mlp_set_params(m, h.data());
+  if (r != DGX_OK) {
+    dgx_mlp_free(m);
+    *out = nullptr;
+  }
+  return r;
+}
+
+void dgx_mlp_free(dgx_mlp* m) {
+  if (!m) return;
+  mlp_release(m);
+  delete m;
+}
+
+dgx_status dgx_mlp_set_params(dgx_mlp* m, const float* params) {
+  if (!m || !params) return mfail(DGX_EINVAL, "dgx_mlp_set_params: null argument");
+  if (!hip_ok(hipMemcpy(m->P, params, sizeof(float) * kNParam, hipMemcpyHostToDevice), "params upload") ||
+      !hip_ok(hipMemset(m->m, 0, sizeof(float) * kNParam), "hipMemset") ||
+      !hip_ok(hipMemset(m->v, 0, sizeof(float) * kNParam), "hipMemset"))
+    return DGX_EDEVICE;
+  m->step = 0;
+  hipLaunchKernelGGL(k_mlp_cast, dim3(nblocks_for(kH * kInP, 256)), dim3(256), 0, 0, m->P, m->w1b, m->w2b, m->w3b);
+  if (!hip_ok(hipGetLastError(), "cast") || !hip_ok(hipDeviceSynchronize(), "cast")) return DGX_EDEVICE;
+  return DGX_OK;
+}
+
+dgx_status dgx_mlp_get_params(const dgx_mlp* m, float* params) {
+  if (!m || !params) return mfail(DGX_EINVAL, "dgx_mlp_get_params: null argument");
+  return hip_ok(hipMemcpy(params, m->P, sizeof(float) * kNParam, hipMemcpyDeviceToHost), "params download")
+             ? DGX_OK
+             : DGX_EDEVICE;
+}
+
+dgx_status dgx_mlp_forward(dgx_mlp* m, const float* X, long n, float* z, void* stream) {
+  if (!m || !X || !z || n < 0) return mfail(DGX_EINVAL, "dgx_mlp_forward: bad arguments");
+  if (n == 0) return DGX_OK;
+  return hip_ok(forward(m, X, nullptr, n, z, nullptr, nullptr, reinterpret_cast<hipStream_t>(stream)), "mlp forward")
+             ? DGX_OK
+             : DGX_EDEVICE;
+}
+
+dgx_status dgx_mlp_gradient(dgx_mlp* m, const float* X, const float* y, long n, float pos_weight, float* grad,
+                            void* stream) {
+  if (!m || !X || !y || !grad || n <= 0) return mfail(DGX_EINVAL, "dgx_mlp_gradient: bad arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!ensure_scratch(m, n)) return DGX_EDEVICE;
+  if (!hip_ok(gradient(m, X, nullptr, y, n, pos_weight, st), "mlp gradient")) return DGX_EDEVICE;
+  return hip_ok(hipMemcpyAsync(grad, m->grad, sizeof(float) * (kNParam + 1), hipMemcpyDeviceToDevice, st),
+                "gradient copy")
+             ? DGX_OK
+             : DGX_EDEVICE;
+}
+
+dgx_status dgx_mlp_train(dgx_mlp* m, const float* X, const float* y, long n, int epochs, int batch, float lr,
+                         float pos_weight, unsigned long long seed, float* epoch_loss, void* stream) {
+  if (!m || !X || !y || n <= 0 || epochs < 0 || batch < kSB || !epoch_loss)
+    return mfail(DGX_EINVAL, "dgx_mlp_train: bad arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (!ensure_scratch(m, batch)) return DGX_EDEVICE;
+  if (n > m->idx_cap) {
+    void* p;
+    if (!mlp_alloc(m, &p, sizeof(int) * n)) return DGX_EDEVICE;
+    m->idx = (int*)p;
+    m->idx_cap = n;
+  }
+  std::vector<int> perm(n);
+  uint64_t s = seed;
+  AdamArgs a{lr, 0.9f, 0.999f, 1e-8f, 0.f, 0.f};
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (long k = 0; k < n; ++k) perm[k] = (int)k;
+    for (long k = n - 1; k > 0; --k) std::swap(perm[k], perm[splitmix(s) % (uint64_t)(k + 1)]);
+    if (!hip_ok(hipMemcpyAsync(m->idx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st), "idx upload"))
+      return DGX_EDEVICE;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st, m->loss, 1, 0.0f);
+    for (long b0 = 0; b0 < n; b0 += batch) {
+      const long nb = std::min<long>(batch, n - b0);
+      if (!hip_ok(gradient(m, X, m->idx + b0, y, nb, pos_weight, st), "mlp gradient")) return DGX_EDEVICE;
+      m->step += 1;
+      a.c1 = 1.0f - std::pow(a.b1, (float)m->step);
+      a.c2 = 1.0f - std::pow(a.b2, (float)m->step);
+      hipLaunchKernelGGL(k_mlp_adam, dim3(nblocks_for(kNParam, 256)), dim3(256), 0, st, m->P, m->m, m->v, m->grad, a,
+                         m->w1b, m->w2b, m->w3b, m->loss);
+    }
+    float l = 0.0f;
+    if (!hip_ok(hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, st), "loss readback") ||
+        !hip_ok(hipStreamSynchronize(st), "loss readback"))
+      return DGX_EDEVICE;
+    epoch_loss[ep] = l * (float)batch / (float)n;  // mean of the per-batch means, weighted by batch size
+  }
+  return DGX_OK;
+}
+
+}  // extern "C"

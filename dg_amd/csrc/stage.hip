@@ -12,6 +12,7 @@
 // only, dg_functions.cc:225-233), so the result equals the general
 // face/element kernels.  Tiles are mapped to XCDs in contiguous bands so
 // that halo rows are shared through each XCD's L2.
+#include <algorithm>
 #include <type_traits>
 
 #include "dgx_internal.h"
@@ -186,13 +187,27 @@ __device__ inline int halo_source(int h, int ex0, int ey0, const StageArgs& a, i
 #ifndef DGX_STAGE_PAD
 #define DGX_STAGE_PAD 1  // pad per-element LDS strides against bank conflicts
 #endif
-#ifndef DGX_STAGE_PREFETCH
-#define DGX_STAGE_PREFETCH 1  // load the RK operands (U^n, Uacc) at kernel entry
+#ifndef DGX_STAGE_PERSIST
+#define DGX_STAGE_PERSIST 0  // 1: persistent blocks, next tile loaded during the current one
 #endif
 
 // Per-element LDS stride for n doubles: with padding, n rounded up to 2 mod 16
 // so that the elements a wave touches at once start on different banks.
 constexpr int lds_stride(int n) { return DGX_STAGE_PAD ? n + ((18 - n % 16) % 16) : n; }
+
+// Tile origin of the i-th tile a block processes.  Blocks are dispatched
+// round-robin over the 8 XCDs; with XCD bands the tiles of each XCD form a
+// contiguous band of the grid so that halo rows are shared in its L2.
+__device__ inline int tile_of(const StageArgs& a, int i) {
+  const int ntiles = a.tiles_x * a.tiles_y;
+  if (a.xcd_bands) {
+    const int per = ntiles >> 3, nbx = gridDim.x >> 3;
+    const int x = blockIdx.x & 7, j = (blockIdx.x >> 3) + i * nbx;
+    return j < per ? x * per + j : -1;
+  }
+  const int t = blockIdx.x + i * gridDim.x;
+  return t < ntiles ? t : -1;
+}
 
 template <int D, int P, int PH, int FL, int MODE, int TX, int TY>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
@@ -204,7 +219,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
   constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * SW;
   constexpr int NW = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
-  constexpr int NOUT = (NT * NU + 255) / 256;  // output entries per thread
+  constexpr int NGEO = 1 + D * D;
+  // per-thread counts of the loaded quantities
+  constexpr int R_U = (NT * NU + 255) / 256, R_H = (NH * HT + 255) / 256;
+  constexpr int R_G = (NT * NGEO + 255) / 256, R_F = (NFACE * 4 + 255) / 256;
 
   __shared__ double sU[NT * SU];
   __shared__ double sH[NH * HT];
@@ -216,178 +234,234 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   __shared__ double sPw[PH ? V::NTAB_PHI : 1];
   __shared__ double sMinv[Ns * Ns];
   __shared__ double sPsi[MG * Ms], sPsiw[MG * Ms];
-  constexpr int NGEO = 1 + D * D;
   __shared__ double sGeo[NT * NGEO];  // per element: J, invJac
+  __shared__ double sFG[NFACE * 4];   // per tile face: n_x, n_y, JF side 0, JF side 1
   const VolTables<D, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
 
   const int tid = threadIdx.x;
-  int tile = blockIdx.x;
-  if (a.xcd_bands) {  // contiguous band of tiles per XCD (dispatch is round-robin over 8 XCDs)
-    const int per = (a.tiles_x * a.tiles_y) >> 3;
-    tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  }
-  const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
-  // global index of flat tile entry k = el*NU + r (rows of TX elements are contiguous)
-  auto gidx = [&](int k) -> size_t {
-    const int ty = k / (TX * NU);
-    return ((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU);
-  };
-
-  // tile state
-  for (int k = tid; k < NT * NU; k += 256) sU[(k / NU) * SU + k % NU] = a.Uin[gidx(k)];
-  // RK operands of this thread's output entries, in flight during the compute
-  double pre_un[NOUT], pre_acc[NOUT];
-#pragma unroll
-  for (int r = 0; r < NOUT; ++r) {
-    pre_un[r] = pre_acc[r] = 0.0;
-    const int k = tid + r * 256;
-    if (MODE == 1 && DGX_STAGE_PREFETCH && k < NT * NU) {
-      const size_t i = gidx(k);
-      if (a.stage < 3) pre_un[r] = a.Un[i];
-      if (a.stage > 0) pre_acc[r] = a.Uacc[i];
-    }
-  }
   const bool shared_minv = a.minv_stride == 0;
   load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
   for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
-  for (int k = tid; k < NT * NGEO; k += 256) {
-    const int el = k / NGEO, r = k % NGEO;
-    const int eg = ((ey0 + el / TX) * a.nx + ex0 + el % TX) * a.geo_stride;
-    sGeo[k] = r == 0 ? a.J[eg] : a.invJac[eg * D * D + r - 1];
-  }
-  // halo traces, stored in the closure order of the side that reads them
-  for (int k = tid; k < NH * HT; k += 256) {
-    const int h = k / HT, r = k % HT, f = r / Ms, j = r % Ms;
-    int cl = 0;
-    const int e = halo_source<D, TX, TY>(h, ex0, ey0, a, cl);
-    sH[k] = (e >= 0) ? a.Uin[((size_t)e * NF + f) * Ns + closure_node<D, P>(cl, j)] : 0.0;
-  }
-  __syncthreads();
-
-  // ---- faces: one lane per (face, Gauss point)
-  for (int it = tid; it < NFACE * MG; it += 256) {
-    const int fi = it / MG, g = it % MG;
-    const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
-    const double* fg = (fd.xf ? a.gx : a.gy) + (size_t)fd.geo * 4;
-    const double nxv = fg[0], nyv = (D == 2) ? fg[1] : 0.0, jf0 = fg[2], jf1 = fg[3];
-    double tr[2][NF][Ms];
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      const int s = fd.src[d];
-#pragma unroll
-      for (int j = 0; j < Ms; ++j) {
-        if (s >= 0) {
-          const int node = closure_node<D, P>(fd.cl[d], j);
-#pragma unroll
-          for (int f = 0; f < NF; ++f) tr[d][f][j] = sU[s * SU + f * Ns + node];
-        } else {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) tr[d][f][j] = sH[(-1 - s) * HT + f * Ms + j];
-        }
-      }
-    }
-    if (fd.tag == BC_REFLECTIVE) {  // boundaries.cu:20-67 on the nodal face values
-#pragma unroll
-      for (int j = 0; j < Ms; ++j) {
-        if constexpr (D == 1) {
-          tr[1][1][j] = -tr[1][1][j];
-        } else {
-          const double nx2 = nxv * nxv, ny2 = nyv * nyv, m2nxny = -2 * nxv * nyv;
-          const double inv = 1.0 / (nx2 + ny2);
-          const double vxL = tr[0][1][j], vyL = tr[0][2][j];
-          tr[1][1][j] = inv * ((-nx2 + ny2) * vxL + m2nxny * vyL);
-          tr[1][2][j] = inv * (m2nxny * vxL + (nx2 - ny2) * vyL);
-        }
-      }
-    }
-    double ul[NF], ur[NF], F[NF], nc[NF];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      double sl = 0.0, sr = 0.0;
-#pragma unroll
-      for (int j = 0; j < Ms; ++j) {
-        sl += tr[0][f][j] * sPsi[g + j * MG];
-        sr += tr[1][f][j] * sPsi[g + j * MG];
-      }
-      ul[f] = sl;
-      ur[f] = sr;
-    }
-    riemann<D, PH, FL>(ul, ur, nxv, nyv, F, nc);
-    double* w = sW + (fi * MG + g) * 2 * NF;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      w[f] = (-F[f] + nc[f]) * jf0;
-      w[NF + f] = (F[f] + nc[f]) * jf1;
-    }
-  }
-  __syncthreads();
-  // ---- face integration against psi_w into the receiving elements' slots
-  for (int it = tid; it < NFACE * 2 * Ms; it += 256) {
-    const int fi = it / (2 * Ms), d = (it / Ms) % 2, j = it % Ms;
-    const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
-    const int dst = fd.dst[d];
-    if (dst < 0) continue;
-    const int jf = (D == 2 && fd.cl[d] >= 4) ? reversed_pos<P>(j) : j;
-    double* q = sQ + dst * SQ + fd.edge[d] * NF * Ms;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      double s = 0.0;
-#pragma unroll
-      for (int g = 0; g < MG; ++g) s += sPsiw[g + j * MG] * sW[((fi * MG + g) * 2 + d) * NF + f];
-      q[f * Ms + jf] = s;
-    }
-  }
-  __syncthreads();
-
-  // ---- element phases, in batches of EPB elements
-  for (int b = 0; b < NT; b += EPB) {
-    {
-      const int el = b + tid / TPE, g = tid % TPE;
-      if (el < NT && g < NG)
-        vol_phase1<D, P, PH>(tb, g, sU + el * SU, sGeo[el * NGEO], sGeo + el * NGEO + 1,
-                             sW + (el - b) * SW + g * NQ);
-    }
-    __syncthreads();
-    {
-      const int el = b + tid / TPE, i = tid % TPE;
-      double rhs[NF];
-      const bool act = el < NT && i < Ns;
-      if (act) vol_phase2<D, P, PH>(tb, i, sW + (el - b) * SW, sQ + el * SQ, rhs);
-      __syncthreads();
-      if (act)
-#pragma unroll
-        for (int f = 0; f < NF; ++f) sU[el * SU + f * Ns + i] = rhs[f];
-    }
-    __syncthreads();
-  }
-
-  // ---- stage output (coalesced along the tile rows)
   const double dt = (MODE == 1) ? a.dt[0] : 0.0;
+
+  // registers holding the next tile's inputs while the current one computes
+  double pu[R_U], ph[R_H], pg[R_G], pf[R_F];
+  auto issue = [&](int tile) {
+    const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
 #pragma unroll
-  for (int r = 0; r < NOUT; ++r) {
-    const int k = tid + r * 256;
-    if (k >= NT * NU) break;
-    const int el = k / NU, rr = k % NU;
-    const size_t gi = gidx(k);
-    if constexpr (MODE == 0) {
-      a.fout[gi] = sU[el * SU + rr];
-    } else {
-      // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
-      const double dte = shared_minv ? dt * (a.J0 / sGeo[el * NGEO]) : dt;
-      const double* M = shared_minv ? tb.minv
-                                    : a.Minv + ((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * Ns * Ns;
-      const int f = rr / Ns, i = rr % Ns;
-      const double* R = sU + el * SU + f * Ns;
-      double sol = 0.0;
-#pragma unroll
-      for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
-      double un = pre_un[r], acc = pre_acc[r];
-      if (!DGX_STAGE_PREFETCH) {
-        if (a.stage < 3) un = a.Un[gi];
-        if (a.stage > 0) acc = a.Uacc[gi];
+    for (int r = 0; r < R_U; ++r) {
+      const int k = tid + r * 256;
+      if (k < NT * NU) {
+        const int ty = k / (TX * NU);
+        pu[r] = a.Uin[((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU)];
       }
-      rk_combine(a.stage, dte * sol, un, acc, a.Uacc + gi, a.Uout + gi);
     }
+#pragma unroll
+    for (int r = 0; r < R_H; ++r) {
+      const int k = tid + r * 256;
+      if (k < NH * HT) {
+        const int h = k / HT, q = k % HT, f = q / Ms, j = q % Ms;
+        int cl = 0;
+        const int e = halo_source<D, TX, TY>(h, ex0, ey0, a, cl);
+        ph[r] = (e >= 0) ? a.Uin[((size_t)e * NF + f) * Ns + closure_node<D, P>(cl, j)] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R_G; ++r) {
+      const int k = tid + r * 256;
+      if (k < NT * NGEO) {
+        const int el = k / NGEO, q = k % NGEO;
+        const int eg = ((ey0 + el / TX) * a.nx + ex0 + el % TX) * a.geo_stride;
+        pg[r] = q == 0 ? a.J[eg] : a.invJac[eg * D * D + q - 1];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R_F; ++r) {
+      const int k = tid + r * 256;
+      if (k < NFACE * 4) {
+        const int fi = k / 4, q = k % 4;
+        const double* src;
+        if (D == 1 || fi < T::NFX) {
+          const int ty = (D == 1) ? 0 : fi / (TX + 1), fx = (D == 1) ? fi : fi % (TX + 1);
+          src = a.gx + ((size_t)(ey0 + ty) * (a.nx + 1) + ex0 + fx) * 4;
+        } else {
+          const int fj = fi - T::NFX, fy = fj / TX, tx = fj % TX;
+          src = a.gy + ((size_t)(ey0 + fy) * a.nx + ex0 + tx) * 4;
+        }
+        pf[r] = src[q];
+      }
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int r = 0; r < R_U; ++r) {
+      const int k = tid + r * 256;
+      if (k < NT * NU) sU[(k / NU) * SU + k % NU] = pu[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R_H; ++r) {
+      const int k = tid + r * 256;
+      if (k < NH * HT) sH[k] = ph[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R_G; ++r) {
+      const int k = tid + r * 256;
+      if (k < NT * NGEO) sGeo[k] = pg[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R_F; ++r) {
+      const int k = tid + r * 256;
+      if (k < NFACE * 4) sFG[k] = pf[r];
+    }
+  };
+
+  int tile = tile_of(a, 0);
+  if (tile >= 0) issue(tile);
+  for (int it = 0; tile >= 0; ++it) {
+    const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
+    auto gidx = [&](int k) -> size_t {
+      const int ty = k / (TX * NU);
+      return ((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU);
+    };
+    commit();
+    __syncthreads();
+    // RK operands of this thread's output entries, then the next tile's
+    // inputs: all in flight during the compute below
+    double pre_un[R_U], pre_acc[R_U];
+#pragma unroll
+    for (int r = 0; r < R_U; ++r) {
+      pre_un[r] = pre_acc[r] = 0.0;
+      const int k = tid + r * 256;
+      if (MODE == 1 && k < NT * NU) {
+        const size_t i = gidx(k);
+        if (a.stage < 3) pre_un[r] = a.Un[i];
+        if (a.stage > 0) pre_acc[r] = a.Uacc[i];
+      }
+    }
+    const int next = DGX_STAGE_PERSIST ? tile_of(a, it + 1) : -1;
+    if (next >= 0) issue(next);
+
+    // ---- faces: one lane per (face, Gauss point)
+    for (int itf = tid; itf < NFACE * MG; itf += 256) {
+      const int fi = itf / MG, g = itf % MG;
+      const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
+      const double* fg = sFG + fi * 4;
+      const double nxv = fg[0], nyv = (D == 2) ? fg[1] : 0.0, jf0 = fg[2], jf1 = fg[3];
+      double tr[2][NF][Ms];
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const int s = fd.src[d];
+#pragma unroll
+        for (int j = 0; j < Ms; ++j) {
+          if (s >= 0) {
+            const int node = closure_node<D, P>(fd.cl[d], j);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) tr[d][f][j] = sU[s * SU + f * Ns + node];
+          } else {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) tr[d][f][j] = sH[(-1 - s) * HT + f * Ms + j];
+          }
+        }
+      }
+      if (fd.tag == BC_REFLECTIVE) {  // boundaries.cu:20-67 on the nodal face values
+#pragma unroll
+        for (int j = 0; j < Ms; ++j) {
+          if constexpr (D == 1) {
+            tr[1][1][j] = -tr[1][1][j];
+          } else {
+            const double nx2 = nxv * nxv, ny2 = nyv * nyv, m2nxny = -2 * nxv * nyv;
+            const double inv = 1.0 / (nx2 + ny2);
+            const double vxL = tr[0][1][j], vyL = tr[0][2][j];
+            tr[1][1][j] = inv * ((-nx2 + ny2) * vxL + m2nxny * vyL);
+            tr[1][2][j] = inv * (m2nxny * vxL + (nx2 - ny2) * vyL);
+          }
+        }
+      }
+      double ul[NF], ur[NF], F[NF], nc[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double sl = 0.0, sr = 0.0;
+#pragma unroll
+        for (int j = 0; j < Ms; ++j) {
+          sl += tr[0][f][j] * sPsi[g + j * MG];
+          sr += tr[1][f][j] * sPsi[g + j * MG];
+        }
+        ul[f] = sl;
+        ur[f] = sr;
+      }
+      riemann<D, PH, FL>(ul, ur, nxv, nyv, F, nc);
+      double* w = sW + (fi * MG + g) * 2 * NF;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        w[f] = (-F[f] + nc[f]) * jf0;
+        w[NF + f] = (F[f] + nc[f]) * jf1;
+      }
+    }
+    __syncthreads();
+    // ---- face integration against psi_w into the receiving elements' slots
+    for (int itf = tid; itf < NFACE * 2 * Ms; itf += 256) {
+      const int fi = itf / (2 * Ms), d = (itf / Ms) % 2, j = itf % Ms;
+      const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
+      const int dst = fd.dst[d];
+      if (dst < 0) continue;
+      const int jf = (D == 2 && fd.cl[d] >= 4) ? reversed_pos<P>(j) : j;
+      double* q = sQ + dst * SQ + fd.edge[d] * NF * Ms;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double s = 0.0;
+#pragma unroll
+        for (int g = 0; g < MG; ++g) s += sPsiw[g + j * MG] * sW[((fi * MG + g) * 2 + d) * NF + f];
+        q[f * Ms + jf] = s;
+      }
+    }
+    __syncthreads();
+
+    // ---- element phases, in batches of EPB elements
+    for (int b = 0; b < NT; b += EPB) {
+      {
+        const int el = b + tid / TPE, g = tid % TPE;
+        if (el < NT && g < NG)
+          vol_phase1<D, P, PH>(tb, g, sU + el * SU, sGeo[el * NGEO], sGeo + el * NGEO + 1,
+                               sW + (el - b) * SW + g * NQ);
+      }
+      __syncthreads();
+      {
+        const int el = b + tid / TPE, i = tid % TPE;
+        double rhs[NF];
+        const bool act = el < NT && i < Ns;
+        if (act) vol_phase2<D, P, PH>(tb, i, sW + (el - b) * SW, sQ + el * SQ, rhs);
+        __syncthreads();
+        if (act)
+#pragma unroll
+          for (int f = 0; f < NF; ++f) sU[el * SU + f * Ns + i] = rhs[f];
+      }
+      __syncthreads();
+    }
+
+    // ---- stage output (coalesced along the tile rows)
+#pragma unroll
+    for (int r = 0; r < R_U; ++r) {
+      const int k = tid + r * 256;
+      if (k >= NT * NU) break;
+      const int el = k / NU, rr = k % NU;
+      const size_t gi = gidx(k);
+      if constexpr (MODE == 0) {
+        a.fout[gi] = sU[el * SU + rr];
+      } else {
+        // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
+        const double dte = shared_minv ? dt * (a.J0 / sGeo[el * NGEO]) : dt;
+        const double* M = shared_minv ? tb.minv
+                                      : a.Minv + ((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * Ns * Ns;
+        const int f = rr / Ns, i = rr % Ns;
+        const double* R = sU + el * SU + f * Ns;
+        double sol = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
+        rk_combine(a.stage, dte * sol, pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+      }
+    }
+    __syncthreads();  // LDS is refilled for the next tile
+    tile = next;
   }
 }
 
@@ -440,15 +514,22 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
   a.tiles_x = c.nx / c.tx;
   a.tiles_y = c.ny / c.ty;
   const int ntiles = a.tiles_x * a.tiles_y;
-  a.xcd_bands = (ntiles % 8 == 0) ? 1 : 0;
   a.stage = stage; a.Uin = Uin; a.Un = Un; a.Uacc = Uacc; a.Uout = Uout; a.dt = dt_dev; a.fout = f_out;
   return dispatch_stage(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
-    if (mode == 0)
-      hipLaunchKernelGGL((k_stage<D, P, PH, FL, 0, TX, TY>), dim3(ntiles), dim3(256), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_stage<D, P, PH, FL, 1, TX, TY>), dim3(ntiles), dim3(256), 0, st, a);
+    auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY> : k_stage<D, P, PH, FL, 1, TX, TY>;
+    // persistent grid: as many blocks as fit on the device at once
+    int nb = ntiles;
+    if (DGX_STAGE_PERSIST) {
+      int per_cu = 0, dev = 0, cus = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) == hipSuccess && per_cu > 0)
+        nb = std::min(ntiles, per_cu * cus);
+    }
+    a.xcd_bands = (ntiles % 8 == 0 && nb % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, a);
     return hipGetLastError();
   });
 }

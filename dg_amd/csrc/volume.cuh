@@ -42,18 +42,18 @@ struct VolTables {
 template <int D, int P, int PH>
 __device__ inline void load_vol_tables(const VolTables<D, P, PH>& t, const double* phi, const double* dphi,
                                        const double* phi_w, const double* dphi_w, const double* Minv,
-                                       bool uniform_minv, int tid) {
+                                       bool uniform_minv, int tid, int nthr = 256) {
   using V = Vol<D, P, PH>;
-  for (int k = tid; k < V::NTAB_PHI; k += 256) t.phi[k] = phi[k];
-  for (int k = tid; k < V::NTAB_D; k += 256) {
+  for (int k = tid; k < V::NTAB_PHI; k += nthr) t.phi[k] = phi[k];
+  for (int k = tid; k < V::NTAB_D; k += nthr) {
     const int i = k / (V::NG * D), r = k % (V::NG * D);
     t.dw[r * V::Ns + i] = dphi_w[k];
     if constexpr (PH == 1) t.dphi[k] = dphi[k];
   }
   if constexpr (PH == 1)
-    for (int k = tid; k < V::NTAB_PHI; k += 256) t.pw[(k % V::NG) * V::Ns + k / V::NG] = phi_w[k];
+    for (int k = tid; k < V::NTAB_PHI; k += nthr) t.pw[(k % V::NG) * V::Ns + k / V::NG] = phi_w[k];
   if (uniform_minv && Minv)
-    for (int k = tid; k < V::Ns * V::Ns; k += 256) t.minv[k] = Minv[k];
+    for (int k = tid; k < V::Ns * V::Ns; k += nthr) t.minv[k] = Minv[k];
 }
 
 // Phase 1 for lane (element u, Gauss point g): fluxes with metric terms.

@@ -206,3 +206,32 @@ def oblique_riemann_state(mesh: Mesh, physics: int, seed: int = 0) -> np.ndarray
     if physics == MULTIFLUID:
         f.append(1.0 / (gam - 1.0))
     return _pack(mesh, f)
+
+
+# ------------------------------------------------ batched 1D ensemble
+def ensemble_spec(n: int = 256, seed: int = 20261202, ne_range=(200, 4000), orders=(1, 2, 3)):
+    """Seeded sizes of an ensemble of 1D problems: element counts uniform in
+    ne_range, orders uniform in `orders`."""
+    rng = np.random.default_rng(seed)
+    ne = rng.integers(ne_range[0], ne_range[1] + 1, n)
+    p = rng.choice(np.asarray(orders), n)
+    return [int(x) for x in ne], [int(x) for x in p]
+
+
+def two_fluid_riemann_1d(mesh: Mesh, seed: int) -> np.ndarray:
+    """Seeded 1D two-fluid Riemann problem: rho, p log-uniform in [0.1, 10],
+    u uniform in [-2, 2], gamma uniform in [1.1, 1.7] per side, split at x=0
+    on each element's first node (4 fields: rho, rho u, E, 1/(gamma-1))."""
+    rng = np.random.default_rng(seed)
+    x, _ = node_xy(mesh)
+    left = (x[:, :1] < 1e-12) & np.ones_like(x, dtype=bool)
+    rl, rr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+    pl, pr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+    ul, ur = rng.uniform(-2.0, 2.0, 2)
+    gl, gr = rng.uniform(1.1, 1.7, 2)
+    rho = np.where(left, rl, rr)
+    p = np.where(left, pl, pr)
+    u = np.where(left, ul, ur)
+    gam = np.where(left, gl, gr)
+    E = p / (gam - 1.0) + 0.5 * rho * u * u
+    return _pack(mesh, [rho, rho * u, E, 1.0 / (gam - 1.0)])

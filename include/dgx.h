@@ -122,7 +122,7 @@ dgx_status dgx_rhs(dgx_solver* s, const double* U, double* f_rk, void* stream);
 /* Sensor thresholds (deck "#sensor thresholds", sensor.h:49-53); n = 0..3. */
 dgx_status dgx_set_sensor_thresholds(dgx_solver* s, int n, const double* thresholds);
 
-/* limiter: 0 none, 1 hrl (global HR), 3 hri (sensor-gated HR) */
+/* limiter: 0 none, 1 hrl (global HR), 3 hri (sensor-gated HR), 4 HR gated by the learned indicator */
 dgx_status dgx_limit(dgx_solver* s, int limiter, double* U, void* stream);
 /* SENSOR::sensing: flags (int[N_E], device) and Uavg (double[N_E*N_F], device; may be NULL). */
 dgx_status dgx_sensing(dgx_solver* s, const double* U, int* sensors, double* Uavg, void* stream);
@@ -136,8 +136,9 @@ dgx_status dgx_rk4_step(dgx_solver* s, double* U, const double* dt_dev, int limi
 
 /* RK::RK_integration (rk.cc:11-247) without output/particles.
  * mode 0: fixed Dt = dt_or_cfl for nsteps steps (deck CFL < 0).
- * mode 1: Dt from CFL = dt_or_cfl (already scaled by Dx/(2p+1)) until tf;
- *         the initial condition is limited first (rk.cc:100-114).
+ * mode 1: Dt from CFL = dt_or_cfl (already scaled by Dx/(2p+1)) until tf,
+ *         or for at most nsteps steps when nsteps > 0; the initial condition
+ *         is limited first (rk.cc:100-114).
  * Returns the step count in *steps and the final time in *t_final. */
 dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
                              int limiter, int* steps, double* t_final, void* stream);
@@ -151,6 +152,69 @@ dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cf
  * path in use (1 or 2). */
 dgx_status dgx_solver_set_path(dgx_solver* s, int path);
 int dgx_solver_path(const dgx_solver* s);
+
+/* ------------------------------------------------- batched 1D ensemble */
+/* Many independent 1D problems advanced together: each runs the loop of
+ * RK::RK_integration (rk.cc:132-247) with its own Dt from CFL (rk.cc:250-272),
+ * the DG right-hand side (dg_solver.h:292-364) and, optionally, the HR
+ * limiter (limiting.cc:11-27).  No reference counterpart batches problems;
+ * each problem's result equals a separate run of the reference loop. */
+typedef struct dgx_batch dgx_batch;
+
+/* n problems on [x0, x1]: problem k has n_elem[k] >= 2 elements of order
+ * order[k] (1..3); cfg->dim must be 1 (cfg->order is ignored); bc = {left,
+ * right} tags, 2 (farfield) or 3 (wall). */
+dgx_status dgx_batch_create(const dgx_config* cfg, int n, const int* n_elem, const int* order, double x0,
+                            double x1, const int* bc, dgx_batch** out);
+void dgx_batch_free(dgx_batch* b);
+/* Offset (doubles) of problem k in the packed state, k = 0..n (n: total size).
+ * Each problem's slice uses the usual layout U[(e*N_F+f)*N_s+i]. */
+long dgx_batch_state_offset(const dgx_batch* b, int k);
+/* Workgroup size used for each order 1..3 (0 when no problem has that order). */
+dgx_status dgx_batch_block_sizes(const dgx_batch* b, int* sizes);
+dgx_status dgx_batch_set_block_sizes(dgx_batch* b, const int* sizes);
+/* Times one step of each order's kernel for every candidate workgroup size on
+ * scratch copies of U (U is not modified) and keeps the fastest. */
+dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int limiter, void* stream);
+/* nsteps RK4 steps of every problem in one graph launch.  Problem k uses
+ * Dt = cfl*Dx_k/(2p_k+1)/max(|u|+a) (deck CFL scaling, main.cc:896-897);
+ * limiter 0 none or 1 hrl; limit_initial limits the initial state first
+ * (rk.cc:100-114).  T (double[n], device) accumulates each problem's time;
+ * status (int[n], device, zero on entry) becomes 1 for a problem whose Dt
+ * fell below 1e-14 or was NaN (rk.cc:147-148); that problem stops. */
+dgx_status dgx_batched_rk4(dgx_batch* b, double* U, double cfl, int nsteps, int limiter, int limit_initial,
+                           double* T, int* status, void* stream);
+
+/* ------------------------------------------------ learned indicator */
+/* A 49 -> 64 -> 64 -> 1 MLP troubled-cell indicator for 2D p=2 two-fluid
+ * states, trained on the flags of calc_sensors (sensor_kernels.cu:19-163).
+ * No counterpart in the reference.  Parameters (dgx_mlp_param_count() floats):
+ * W1[64][64] (inputs 49..63 zero), b1[64], W2[64][64], b2[64], w3[64], b3.
+ * Features: float[64] per element (dgx_mlp_features).  BF16 weights and
+ * activations, FP32 accumulation. */
+typedef struct dgx_mlp dgx_mlp;
+int dgx_mlp_param_count(void);
+dgx_status dgx_mlp_create(unsigned long long seed, dgx_mlp** out); /* He-uniform init from seed */
+void dgx_mlp_free(dgx_mlp* m);
+dgx_status dgx_mlp_set_params(dgx_mlp* m, const float* params); /* host array; resets the optimizer */
+dgx_status dgx_mlp_get_params(const dgx_mlp* m, float* params);
+/* logits z[n] for n feature rows X[n*64] (device pointers) */
+dgx_status dgx_mlp_forward(dgx_mlp* m, const float* X, long n, float* z, void* stream);
+/* mean gradient of the weighted cross-entropy -[w y log s(z) + (1-y) log(1-s(z))]
+ * over the n rows, into grad (device float[param_count + 1], last = loss) */
+dgx_status dgx_mlp_gradient(dgx_mlp* m, const float* X, const float* y, long n, float pos_weight, float* grad,
+                            void* stream);
+/* Adam (lr, 0.9, 0.999, 1e-8) over shuffled minibatches (seeded); epoch_loss
+ * (host float[epochs]) receives the mean loss of each epoch */
+dgx_status dgx_mlp_train(dgx_mlp* m, const float* X, const float* y, long n, int epochs, int batch, float lr,
+                         float pos_weight, unsigned long long seed, float* epoch_loss, void* stream);
+/* Attach (not owned) an indicator to a solver; limiter 4 in dgx_limit and the
+ * RK drivers then flags cells with z > threshold and applies the HR sweeps
+ * of hri to them (limiting.cc:596-634 with the flags in place of the sensor). */
+dgx_status dgx_solver_set_indicator(dgx_solver* s, dgx_mlp* m, float threshold);
+dgx_status dgx_mlp_features(dgx_solver* s, const double* U, float* X, void* stream);
+/* flags (int[N_E], device) and optionally logits z (float[N_E], device) */
+dgx_status dgx_indicator_flags(dgx_solver* s, const double* U, int* flags, float* z, void* stream);
 
 /* Per-kernel timing with HIP events recorded on the launch stream (the
  * reference's TIMERS use std::clock and synchronise after every launch,

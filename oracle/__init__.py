@@ -63,6 +63,7 @@ def lib() -> ctypes.CDLL:
         L.or_limit_hr.argtypes = [cp, tp, vp]
         L.or_sensing.argtypes = [cp, vp, vp, vp, vp, vp, vp]
         L.or_limit_hri.argtypes = [cp, tp, vp, vp, vp, vp]
+        L.or_limit_flagged.argtypes = [cp, tp, vp, vp]
         L.or_rk4.argtypes = [cp, tp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                              vp, vp, vp, vp, ctypes.c_int]
         L.or_dgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -177,6 +178,14 @@ class Problem:
         sens = np.zeros(self.cfg.N_E, dtype=np.int32)
         lib().or_limit_hri(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
         return U, sens
+
+    def limit_flagged(self, U: np.ndarray, flags):
+        """HR sweeps of the sensor-gated limiter on the given flags."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        lib().or_limit_flagged(ctypes.byref(self.cfg), ctypes.byref(self.tab), f.ctypes.data_as(ctypes.c_void_p),
+                               _dptr(U))
+        return U
 
     def rk4(self, U: np.ndarray, mode: int, dt_or_cfl: float, nsteps: int = 0, tf: float = 0.0, limiter: int = 0,
             thresholds=(), max_hist: int = 100000):

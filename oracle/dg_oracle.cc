@@ -841,6 +841,23 @@ void or_copy_detected(const or_cfg* c, const int* sensors, const double* Uold, d
   }
 }
 
+// The HR sweeps of limiting.cc:596-634 on given flags (sensor values), without
+// the sensor: used with the flags of the learned indicator.
+void or_limit_flagged(const or_cfg* c, const or_tables* t, const int* flags, double* U) {
+  const int N_s = c->N_s, N_E = c->N_E, N_F = c->N_F;
+  std::vector<double> Utmp((size_t)N_s * N_E * N_F, 0.0);
+  if (c->D == 1) {
+    or_hri1D(c, t->neighbors, N_s, 1, 0, t->Lag2Mono, t->Mono2Lag, flags, U, Utmp.data());
+    or_copy_detected(c, flags, Utmp.data(), U);
+  } else {
+    const int n1 = c->N_s1D;
+    or_hri1D(c, t->neighbors, n1, n1, 0, t->Lag2MonoX, t->MonoX2Lag, flags, U, Utmp.data());
+    or_copy_detected(c, flags, Utmp.data(), U);
+    or_hri1D(c, t->neighbors, n1, n1, 2, t->Lag2MonoY, t->MonoY2Lag, flags, U, Utmp.data());
+    or_copy_detected(c, flags, Utmp.data(), U);
+  }
+}
+
 // limiting.cc:596-634
 void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors) {
@@ -881,6 +898,7 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
       if (steps >= nsteps) break;
       Dt = dt_or_cfl;
     } else {
+      if (nsteps > 0 && steps >= nsteps) break;  // optional step cap (not in the reference)
       Dt = or_dt_from_cfl(c, U, dt_or_cfl);
       if (Dt < 1e-14 || Dt != Dt) { if (t_out) *t_out = T; return -1; }
       if (Dt > (tf - T)) { Dt = tf - T; done = true; }

@@ -65,11 +65,14 @@ void or_calc_sensors(const or_cfg* c, const int* on, const double* thresh, const
 void or_hri1D(const or_cfg* c, const int* neighbors, int N_s1D, int slicenum, int offxy, const double* L2M,
               const double* M2L, const int* sensors, const double* U, double* Unew);
 void or_copy_detected(const or_cfg* c, const int* sensors, const double* Uold, double* U);
+/* HR sweeps of the sensor-gated limiter on given flags (no sensor). */
+void or_limit_flagged(const or_cfg* c, const or_tables* t, const int* flags, double* U);
 void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors); /* Limiting::HRIlimiting (sensors returned for inspection) */
 
 /* RK4 time integration (rk.cc:132-247).
- * mode 0: fixed Dt for nsteps steps; mode 1: CFL-based Dt until T reaches tf (Dt clamped).
+ * mode 0: fixed Dt for nsteps steps; mode 1: CFL-based Dt until T reaches tf (Dt clamped),
+ * or for at most nsteps steps when nsteps > 0 (a cap the reference does not have).
  * limiter: 0 none, 1 hrl, 3 hri.  Returns number of steps taken, or -1 on Dt error.
  * *t_out receives the final time. dt_hist (may be NULL) receives up to max_hist Dt values. */
 int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,

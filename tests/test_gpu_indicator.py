@@ -1,0 +1,120 @@
+"""GPU checks of the learned indicator against its CPU restatement.
+
+Features (float32 of FP64 arithmetic) within 1e-6; logits of the BF16 MFMA
+forward within 0.05 absolute of the oracle's BF16 forward (a BF16 rounding
+step of an activation moves z by ~1e-2 at most; a wrong operand layout moves
+it by O(1)); gradients within 2% relative L2 per parameter block; training
+is deterministic and learns a separable rule; the learned limiter applies
+the HR sweeps exactly (bit for bit) on the flags the network produced.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import oracle.mlp as om
+from dg_amd import LIMIT_LEARNED, MULTIFLUID, PERIODIC, REFLECTIVE, FARFIELD, ROE, Config, Indicator, Mesh, Solver, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(nx=16, ny=8, bc=(FARFIELD, REFLECTIVE, PERIODIC, PERIODIC)):
+    mesh = Mesh(2, 2, nx, ny, 0.0, 1.6, 0.0, 0.8, bc=bc, nfields=5)
+    return mesh, Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+
+
+def test_features_match_oracle(device):
+    mesh, solver = _solver()
+    t = mesh.tables()
+    net = Indicator(seed=1)
+    solver.set_indicator(net, 0.0)
+    vinv = om.legendre_vinv(t["ref_nodes"])
+    for U0 in (problems.oblique_riemann_state(mesh, MULTIFLUID, seed=3),
+               problems.smooth_random_state(mesh, MULTIFLUID, seed=4)):
+        U = torch.from_numpy(U0).to(device)
+        X = torch.zeros((t["N_E"], 64), dtype=torch.float32, device=device)
+        solver.features(U, X)
+        torch.cuda.synchronize()
+        ref = om.features(U0, t["neighbors"], vinv)
+        assert np.allclose(X.cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
+
+
+def _random_inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, 64), dtype=np.float32)
+    X[:, :49] = rng.standard_normal((n, 49)).astype(np.float32)
+    return X
+
+
+def test_forward_matches_oracle(device):
+    net = Indicator(seed=7)
+    P = net.params
+    X = _random_inputs(1000, 2)
+    z = torch.zeros(1000, dtype=torch.float32, device=device)
+    net.forward(torch.from_numpy(X).to(device), z)
+    torch.cuda.synchronize()
+    zr, _ = om.forward(P, X)
+    d = np.abs(z.cpu().numpy() - zr)
+    assert d.max() < 0.05, d.max()
+    assert np.mean(d > 1e-3) < 0.05
+
+
+def test_gradient_matches_oracle(device):
+    net = Indicator(seed=11)
+    P = net.params
+    X = _random_inputs(256, 5)
+    rng = np.random.default_rng(6)
+    y = (rng.random(256) < 0.3).astype(np.float32)
+    g = torch.zeros(net.n_params + 1, dtype=torch.float32, device=device)
+    net.gradient(torch.from_numpy(X).to(device), torch.from_numpy(y).to(device), g, pos_weight=2.5)
+    torch.cuda.synchronize()
+    gg = g.cpu().numpy().astype(np.float64)
+    gr = om.gradient(P, X, y.astype(np.float64), 2.5)
+    assert abs(gg[-1] - om.loss(P, X, y, 2.5)) <= 1e-3 * abs(om.loss(P, X, y, 2.5))
+    for a, b in ((om.W1, om.B1), (om.B1, om.W2), (om.W2, om.B2), (om.B2, om.W3), (om.W3, om.B3), (om.B3, om.N_PARAM)):
+        err = np.linalg.norm(gg[a:b] - gr[a:b]) / max(np.linalg.norm(gr[a:b]), 1e-12)
+        assert err < 0.02, (a, b, err)
+
+
+def test_training_is_deterministic_and_learns(device):
+    rng = np.random.default_rng(9)
+    X = _random_inputs(4096, 9)
+    y = ((X[:, 45] + 0.5 * X[:, 3]) > 0.4).astype(np.float32)
+    Xd, yd = torch.from_numpy(X).to(device), torch.from_numpy(y).to(device)
+    runs = []
+    for _ in range(2):
+        net = Indicator(seed=3)
+        losses = net.train(Xd, yd, epochs=12, batch=256, lr=3e-3, pos_weight=1.0, seed=5)
+        runs.append((net.params, losses))
+        z = torch.zeros(4096, dtype=torch.float32, device=device)
+        net.forward(Xd, z)
+        torch.cuda.synchronize()
+        acc = np.mean((z.cpu().numpy() > 0) == (y > 0.5))
+        assert acc > 0.95, acc
+    assert np.array_equal(runs[0][0], runs[1][0])
+    losses = runs[0][1]
+    assert losses[-1] < 0.5 * losses[0], losses
+
+
+def test_learned_limiter_is_exact_on_its_flags(device):
+    mesh, solver = _solver()
+    t = mesh.tables()
+    prob = oracle.Problem(t, MULTIFLUID, ROE)
+    net = Indicator(seed=2)
+    U0 = problems.oblique_riemann_state(mesh, MULTIFLUID, seed=8)
+    Ud = torch.from_numpy(U0).to(device)
+    z = torch.zeros(t["N_E"], dtype=torch.float32, device=device)
+    flags = torch.zeros(t["N_E"], dtype=torch.int32, device=device)
+    solver.set_indicator(net, 0.0)
+    solver.indicator_flags(Ud, flags, z)
+    torch.cuda.synchronize()
+    zz = z.cpu().numpy()
+    thr = float(np.median(zz))  # flag about half of the cells
+    solver.set_indicator(net, thr)
+    solver.indicator_flags(Ud, flags)
+    solver.limit(LIMIT_LEARNED, Ud)
+    torch.cuda.synchronize()
+    f = flags.cpu().numpy()
+    assert 0 < f.sum() < t["N_E"]
+    assert np.array_equal(f, (zz > thr).astype(np.int32))
+    assert np.array_equal(Ud.cpu().numpy(), prob.limit_flagged(U0, f))
