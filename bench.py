@@ -43,6 +43,8 @@ def main() -> int:
     ap.add_argument("--config", choices=["vortex", "sod", "bubble", "ensemble"], default="vortex")
     ap.add_argument("--n", type=int, default=0, help="vortex: elements per direction (default 512)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32mfma"], default="fp64",
+                    help="tile-kernel volume contractions: FP64, or FP32 on MFMA (2D p=2; the rest stays FP64)")
     ap.add_argument("--path", choices=["auto", "general", "tiled"], default="auto",
                     help="kernel path: fused tile kernel (auto on this mesh) or face + element kernels")
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="CPU sample steps (0: workload default)")
@@ -85,6 +87,8 @@ def main() -> int:
     solver = Solver(cfg, mesh)
     solver.set_path({"auto": PATH_AUTO, "general": PATH_GENERAL, "tiled": PATH_TILED}[args.path])
     tiled = solver.path == PATH_TILED
+    if args.precision == "fp32mfma":
+        solver.set_precision(1)
     s = mesh.sizes
     N_E, N_s, NF = s["N_E"], s["N_s"], cfg.nfields
     n_dof = N_E * N_s * NF
@@ -170,12 +174,13 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32 (mfma contractions)",
         "data": "synthetic",
         "config": {
             "workload": w["name"],
             "elements": N_E, "dofs": n_dof, "dt": dt, "parallelism": "single GPU",
             "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
+            "contractions": "FP32 MFMA (FP64 fluxes, RK and limiter)" if args.precision == "fp32mfma" else "FP64",
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -305,6 +310,8 @@ def cpu_baseline(args, device, w):
         U_ref, steps, _, _ = prob.rk4(U0, 0, dt, nsteps=nst, limiter=w["limiter"])
     el = time.perf_counter() - t0
     solver = Solver(Config(w["dim"], order, w["physics"], w["flux"]), mesh)
+    if args.precision == "fp32mfma":
+        solver.set_precision(1)
     Ud = torch.from_numpy(U0).to(device)
     if w["cfl_steps"]:
         solver.rk4_integrate(Ud, 1, cfl, nsteps=nst, tf=1e30, limiter=w["limiter"])

@@ -87,6 +87,7 @@ def lib() -> ctypes.CDLL:
         L.dgx_mlp_features.argtypes = [vp, vp, vp, vp]
         L.dgx_indicator_flags.argtypes = [vp, vp, vp, vp, vp]
         L.dgx_solver_set_path.argtypes = [vp, i]
+        L.dgx_solver_set_precision.argtypes = [vp, i]
         L.dgx_solver_path.argtypes = [vp]
         L.dgx_profile_enable.argtypes = [vp, i]
         L.dgx_profile_read.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_long)]
@@ -222,6 +223,10 @@ class Solver:
     def set_path(self, path: int) -> None:
         """PATH_AUTO, PATH_GENERAL (face + element kernels) or PATH_TILED (one fused kernel per stage)."""
         _check(lib().dgx_solver_set_path(self._h, path), "dgx_solver_set_path")
+
+    def set_precision(self, precision: int) -> None:
+        """0: FP64; 1: FP32-MFMA map/integrate contractions in the tile kernel (2D p=2)."""
+        _check(lib().dgx_solver_set_precision(self._h, precision), "dgx_solver_set_precision")
 
     @property
     def path(self) -> int:

@@ -585,6 +585,16 @@ dgx_status dgx_solver_set_path(dgx_solver* h, int path) {
     return fail(DGX_EUNSUPPORTED, "dgx_solver_set_path: the tile kernel needs a uniform Cartesian mesh whose "
                                   "element counts are multiples of the tile shape");
   h->s.path = path;
+  if (!h->s.tiled()) h->s.precision = 0;  // the face + element kernels are FP64
+  return DGX_OK;
+}
+
+dgx_status dgx_solver_set_precision(dgx_solver* h, int precision) {
+  if (!h || precision < 0 || precision > 1) return fail(DGX_EINVAL, "dgx_solver_set_precision: precision must be 0 or 1");
+  Solver& s = h->s;
+  if (precision == 1 && !(s.tiled() && s.D == 2 && s.P == 2))
+    return fail(DGX_EUNSUPPORTED, "dgx_solver_set_precision: FP32-MFMA contractions need the tile kernel, 2D, p=2");
+  s.precision = precision;
   return DGX_OK;
 }
 

@@ -72,6 +72,7 @@ struct Solver {
   DevTables t;
   Cart cart;
   int path = PATH_AUTO;
+  int precision = 0;  // tile kernel: 0 FP64, 1 FP32-MFMA map/integrate (2D p=2)
   bool tiled() const { return cart.on && path != PATH_GENERAL; }
   // work buffers
   double* Qe = nullptr;     // face contributions, element-major [((e*N_N+k)*NF+f)*M_s+j]

@@ -209,7 +209,7 @@ __device__ inline int tile_of(const StageArgs& a, int i) {
   return t < ntiles ? t : -1;
 }
 
-template <int D, int P, int PH, int FL, int MODE, int TX, int TY>
+template <int D, int P, int PH, int FL, int MODE, int TX, int TY, int PREC>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   using V = Vol<D, P, PH>;
   using T = Tile<D, TX, TY>;
@@ -217,9 +217,11 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int NU = V::NU, NQE = V::NQE, NQ = V::NQ, TPE = V::TPE, EPB = V::EPB;
   constexpr int NT = T::NT, NFACE = T::NFACE, NH = T::NH, HT = NF * Ms;
   constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
-  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * SW;
-  constexpr int NW = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
+  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * SW, NW_PREC = PREC ? 2400 : 0;
+  constexpr int NW0 = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
+  constexpr int NW = NW0 > NW_PREC ? NW0 : NW_PREC;
   constexpr int NGEO = 1 + D * D;
+  static_assert(PREC == 0 || (D == 2 && P == 2 && NT == 32), "mixed precision: 2D p=2 8x4 tiles");
   // per-thread counts of the loaded quantities
   constexpr int R_U = (NT * NU + 255) / 256, R_H = (NH * HT + 255) / 256;
   constexpr int R_G = (NT * NGEO + 255) / 256, R_F = (NFACE * 4 + 255) / 256;
@@ -416,7 +418,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     }
     __syncthreads();
 
-    // ---- element phases, in batches of EPB elements
+    // ---- element phases
+    if constexpr (PREC == 1) {
+      vol_mfma_tile<PH, NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, reinterpret_cast<float*>(sW), tid);
+    } else {
     for (int b = 0; b < NT; b += EPB) {
       {
         const int el = b + tid / TPE, g = tid % TPE;
@@ -436,6 +441,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
           for (int f = 0; f < NF; ++f) sU[el * SU + f * Ns + i] = rhs[f];
       }
       __syncthreads();
+    }
     }
 
     // ---- stage output (coalesced along the tile rows)
@@ -518,7 +524,11 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
   return dispatch_stage(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
-    auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY> : k_stage<D, P, PH, FL, 1, TX, TY>;
+    auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0> : k_stage<D, P, PH, FL, 1, TX, TY, 0>;
+    if constexpr (D == 2 && P == 2) {
+      if (s.precision == 1)
+        kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
+    }
     // persistent grid: as many blocks as fit on the device at once
     int nb = ntiles;
     if (DGX_STAGE_PERSIST) {

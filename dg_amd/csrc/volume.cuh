@@ -171,3 +171,155 @@ __device__ inline void rk_update(const VolTables<D, P, PH>& t, const double* Mgl
 }
 
 }  // namespace dgx
+
+namespace dgx {
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// FP32-MFMA volume terms of a tile of NT 2D p=2 elements (256 threads), for
+// the mixed-precision mode: the map U -> Gauss points (and the alpha
+// gradient) and the integration against the weighted derivative tables run
+// as v_mfma_f32_16x16x4f32 contractions with FP32 accumulation; the
+// pointwise flux, the face sum and everything after stay FP64.  Rows of the
+// MFMA tiles are (element, field) pairs, columns Gauss points (map) or nodes
+// (integration).  Operand layouts (16x16x4 f32): A lane l = row l%16,
+// k = l/16; B lane l = column l%16, k = l/16; D lane l = rows 4*(l/16)+v,
+// column l%16.  Writes S+F+Q of every element, in FP64, into sU.
+template <int PH, int NT, int SU, int SQ, int NGEO>
+__device__ void vol_mfma_tile(const VolTables<2, 2, PH>& t, double* sU, const double* sGeo, const double* sQ,
+                              float* sM, int tid) {
+  constexpr int NF = 4 + PH, NFX = 4, Ns = 9, NG = 16, D = 2, Ms = 3;
+  constexpr int NFP = NF + 2;          // per (element, point): fields, then d(alpha)/dxi, d(alpha)/deta
+  constexpr int KP = NG * D + 1;       // padded row of the flux operand
+  constexpr int NGP = NG + 1;          // padded row of the alpha-source operand
+  float* sUg = sM;                     // [(el*NG + g)*NFP + f]
+  float* sFe = sM;                     // [(el*NFX + f)*KP + g*D + al]  (after the flux phase)
+  float* sS = sM + NT * NFX * KP;      // [el*NGP + g]
+  const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
+
+  // ---- map: Ug[(el,f)][g] = sum_i U[el][f][i] phi[g][i]
+  float bphi[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int k = 4 * s + q;
+    bphi[s] = k < Ns ? (float)t.phi[c + k * NG] : 0.0f;
+  }
+  constexpr int RT = NT * NF / 16;
+  for (int rt = w; rt < RT; rt += 4) {
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    const int r = 16 * rt + c, el = r / NF, f = r % NF;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const int k = 4 * s + q;
+      const float a = k < Ns ? (float)sU[el * SU + f * Ns + k] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bphi[s], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = 16 * rt + 4 * q + v;
+      sUg[((rr / NF) * NG + c) * NFP + rr % NF] = acc[v];
+    }
+  }
+  if constexpr (PH == 1) {  // d(alpha)/dxi_al at the points: rows el, columns g for direction al
+    const int rt = w >> 1, al = w & 1;
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    const int el = 16 * rt + c;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const int k = 4 * s + q;
+      const float a = k < Ns ? (float)sU[el * SU + (NF - 1) * Ns + k] : 0.0f;
+      const float b = k < Ns ? (float)t.dphi[(c * D + al) + k * NG * D] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sUg[((16 * rt + 4 * q + v) * NG + c) * NFP + NF + al] = acc[v];
+  }
+  __syncthreads();
+
+  // ---- pointwise flux with metric terms (FP64), two (element, point) pairs per lane
+  double fe[2][NFX * D + PH];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = tid + h * 256, el = p / NG, g = p % NG;
+    double ug[NF], dua[D];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) ug[f] = sUg[(el * NG + g) * NFP + f];
+#pragma unroll
+    for (int al = 0; al < D; ++al) dua[al] = PH ? (double)sUg[(el * NG + g) * NFP + NF + al] : 0.0;
+    const double Je = sGeo[el * NGEO];
+    const double* ij = sGeo + el * NGEO + 1;
+    double fx[NF * D], salpha;
+    volume_flux<2, PH>(ug, dua, ij, fx, salpha);
+#pragma unroll
+    for (int f = 0; f < NFX; ++f)
+#pragma unroll
+      for (int al = 0; al < D; ++al) {
+        double sol = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
+        fe[h][f * D + al] = sol;
+      }
+    if constexpr (PH == 1) fe[h][NFX * D] = salpha * Je;
+  }
+  __syncthreads();  // the flux operand overwrites the point values
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = tid + h * 256, el = p / NG, g = p % NG;
+#pragma unroll
+    for (int f = 0; f < NFX; ++f)
+#pragma unroll
+      for (int al = 0; al < D; ++al) sFe[(el * NFX + f) * KP + g * D + al] = (float)fe[h][f * D + al];
+    if constexpr (PH == 1) sS[el * NGP + g] = (float)fe[h][NFX * D];
+  }
+  __syncthreads();
+
+  // face terms of node i, field f of element el (at most two, summed in edge order)
+  auto qsum = [&](int el, int f, int i) {
+    double s = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int jj = 0; jj < Ms; ++jj)
+        if (closure_node<2, 2>(kk, jj) == i) s += sQ[el * SQ + (kk * NF + f) * Ms + jj];
+    return s;
+  };
+  // ---- integration: F[(el,f)][i] = sum_{g,al} fe[(el,f)][g,al] dphi_w[(g,al)][i]
+  float bdw[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) bdw[s] = c < Ns ? (float)t.dw[(4 * s + q) * Ns + c] : 0.0f;
+  constexpr int RTI = NT * NFX / 16;
+  for (int rt = w; rt < RTI; rt += 4) {
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sFe[(16 * rt + c) * KP + 4 * s + q], bdw[s], acc, 0, 0, 0);
+    if (c < Ns) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rr = 16 * rt + 4 * q + v, el = rr / NFX, f = rr % NFX;
+        sU[el * SU + f * Ns + c] = (double)acc[v] + qsum(el, f, c);
+      }
+    }
+  }
+  if constexpr (PH == 1) {  // alpha source: S[el][i] = sum_g s[el][g] phi_w[g][i]
+    if (w < 2) {
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int k = 4 * s + q;
+        const float b = c < Ns ? (float)t.pw[k * Ns + c] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sS[(16 * w + c) * NGP + k], b, acc, 0, 0, 0);
+      }
+      if (c < Ns) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int el = 16 * w + 4 * q + v;
+          sU[el * SU + (NF - 1) * Ns + c] = (double)acc[v] + qsum(el, NF - 1, c);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace dgx

@@ -1,0 +1,186 @@
+"""Training data, training and evaluation of the learned troubled-cell indicator.
+
+The data come from the solver itself: a mosaic mesh of 2D p=2 two-fluid
+cells holds one seeded random Riemann problem per sub-square (half planar,
+jump normal to x or y, half oblique or circular), is advanced a few RK4
+steps with the global HR limiter on the GPU, and the reference sensor
+(calc_sensors semantics, `Solver.sensing`) labels every cell at a few
+snapshots.  Only cells away from the sub-square borders are kept, so the
+artificial jumps between neighbouring problems never enter the data.
+Features come from `Solver.features`; training runs on the GPU kernels of
+dg_amd/csrc/mlp.hip (`Indicator.train`).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (FARFIELD, LIMIT_HR, MULTIFLUID, ROE, Config, DgxError, Indicator, Mesh, Solver, problems)
+
+SENSOR_THRESHOLDS = (0.01, 0.01, 0.01)  # "#sensor thresholds" of the data and of the judge
+SEED = 20261202
+
+
+@dataclass
+class Dataset:
+    X: np.ndarray       # float32 [n, 64] features
+    y: np.ndarray       # float32 [n] 1 = flagged by the sensor
+    sensor: np.ndarray  # int32 [n] sensor value (1 contact, 2 shock, 3 gamma contact)
+
+
+def mosaic_state(mesh: Mesh, n_sub: int, sub: int, seed: int) -> np.ndarray:
+    """One Riemann problem per sub x sub block of elements (n_sub^2 problems):
+    rho, p log-uniform in [0.1, 10], velocities uniform in [-2, 2], gamma
+    uniform in [1.1, 1.7] on each side.  Even problem index: a planar jump
+    through the block centre, normal to x or y, with velocity normal to the
+    jump only (a 1D problem); odd: an oblique line (2 in 3) or a circle
+    through/around the centre, 2D velocities.  The side of each element is
+    decided by its first node, so elements are constant."""
+    rng = np.random.default_rng(seed)
+    K = n_sub * n_sub
+    off = rng.uniform(-1.0, 1.0, (K, 2))
+    rho = np.exp(rng.uniform(math.log(0.1), math.log(10.0), (K, 2)))
+    p = np.exp(rng.uniform(math.log(0.1), math.log(10.0), (K, 2)))
+    vel = rng.uniform(-2.0, 2.0, (K, 2, 2))
+    gam = rng.uniform(1.1, 1.7, (K, 2))
+    kind = rng.integers(0, 6, K)
+    th = rng.uniform(0.0, math.pi, K)
+    rfrac = rng.uniform(0.2, 0.4, K)
+
+    x, y = problems.node_xy(mesh)
+    s = mesh.sizes
+    nx = s["nx"]
+    e = np.arange(s["N_E"])
+    ex, ey = e % nx, e // nx
+    h = (mesh.extent[1] - mesh.extent[0]) / nx
+    k = (ey // sub) * n_sub + ex // sub
+    cx = ((ex // sub) * sub + sub / 2) * h + off[k, 0] * h
+    cy = ((ey // sub) * sub + sub / 2) * h + off[k, 1] * h
+    dx, dy = x[:, 0] - cx, y[:, 0] - cy
+    planar = k % 2 == 0
+    normal_x = kind[k] % 2 == 0
+    oblique = kind[k] < 4
+    side_planar = np.where(normal_x, dx < 0, dy < 0)
+    side_oblique = dx * np.cos(th[k]) + dy * np.sin(th[k]) < 0
+    side_circle = dx * dx + dy * dy < (rfrac[k] * sub * h) ** 2
+    left = np.where(planar, side_planar, np.where(oblique, side_oblique, side_circle))
+    j = np.where(left, 0, 1)
+    r_, p_, g_ = rho[k, j], p[k, j], gam[k, j]
+    u_, v_ = vel[k, j, 0], vel[k, j, 1]
+    u_ = np.where(planar & ~normal_x, 0.0, u_)
+    v_ = np.where(planar & normal_x, 0.0, v_)
+    E = p_ / (g_ - 1.0) + 0.5 * r_ * (u_ * u_ + v_ * v_)
+    f = np.stack([r_, r_ * u_, r_ * v_, E, 1.0 / (g_ - 1.0)], axis=1)
+    return np.ascontiguousarray(np.repeat(f[:, :, None], s["N_s"], axis=2).reshape(-1))
+
+
+def generate(n_sub: int = 64, sub: int = 8, snapshots=(0, 2, 4, 6), margin: int = 2, seed: int = SEED,
+             cfl: float = 0.3, device="cuda") -> Dataset:
+    """Features and sensor labels of the mosaic (n_sub^2 problems) at the
+    given step counts, interior cells only."""
+    import torch
+
+    n = n_sub * sub
+    mesh = Mesh(2, 2, n, n, 0.0, float(n_sub), 0.0, float(n_sub), bc=(FARFIELD,) * 4, nfields=5)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_sensor_thresholds(SENSOR_THRESHOLDS)
+    solver.set_indicator(Indicator(seed=0), 0.0)  # allocates the feature scratch
+    U = torch.from_numpy(mosaic_state(mesh, n_sub, sub, seed)).to(device)
+    NE = mesh.sizes["N_E"]
+    e = np.arange(NE)
+    ex, ey = e % n, e // n
+    keep = ((ex % sub) >= margin) & ((ex % sub) < sub - margin) & ((ey % sub) >= margin) & ((ey % sub) < sub - margin)
+    X = torch.zeros((NE, 64), dtype=torch.float32, device=device)
+    sens = torch.zeros(NE, dtype=torch.int32, device=device)
+    cfl_dt = cfl * mesh.dx / 5
+    out_X, out_s = [], []
+    done = 0
+    for snap in snapshots:
+        if snap > done:
+            try:
+                solver.rk4_integrate(U, 1, cfl_dt, nsteps=snap - done, tf=1e30, limiter=LIMIT_HR)
+            except DgxError:
+                break  # a problem lost positivity: keep the snapshots taken so far
+            done = snap
+        solver.features(U, X)
+        solver.sensing(U, sens)
+        torch.cuda.synchronize()
+        Xh, sh = X.cpu().numpy(), sens.cpu().numpy()
+        ok = keep & np.isfinite(Xh).all(axis=1)
+        out_X.append(Xh[ok])
+        out_s.append(sh[ok])
+    Xa = np.concatenate(out_X)
+    sa = np.concatenate(out_s).astype(np.int32)
+    return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
+
+
+def train(data: Dataset, seed: int = SEED, epochs: int = 30, batch: int = 1024, lr: float = 2e-3, device="cuda"):
+    """Train a fresh indicator on the GPU; returns (network, epoch losses,
+    threshold).  The threshold is set below the lowest logit of any training
+    cell the sensor flags as a shock, so that no shock cell is missed."""
+    import torch
+
+    net = Indicator(seed=seed & 0xFFFFFFFF)
+    Xd = torch.from_numpy(data.X).to(device)
+    yd = torch.from_numpy(data.y).to(device)
+    pos = max(float(data.y.sum()), 1.0)
+    pos_weight = (data.y.size - pos) / pos
+    losses = net.train(Xd, yd, epochs=epochs, batch=batch, lr=lr, pos_weight=pos_weight, seed=seed)
+    z = logits(net, data.X, device)
+    shock = data.sensor == 2
+    thr = float(np.min(z[shock]) - 1e-3) if shock.any() else 0.0
+    return net, losses, min(thr, 0.0)
+
+
+def logits(net: Indicator, X: np.ndarray, device="cuda") -> np.ndarray:
+    import torch
+
+    Xd = torch.from_numpy(np.ascontiguousarray(X)).to(device)
+    z = torch.zeros(X.shape[0], dtype=torch.float32, device=device)
+    net.forward(Xd, z)
+    torch.cuda.synchronize()
+    return z.cpu().numpy()
+
+
+def confusion(flags: np.ndarray, sensor: np.ndarray) -> dict:
+    """The plain sensor judges the indicator: missed = flagged by the sensor
+    only, spurious = flagged by the indicator only, per sensor kind."""
+    f = np.asarray(flags) != 0
+    s = np.asarray(sensor)
+    return {
+        "cells": int(s.size), "sensor_flagged": int((s != 0).sum()), "indicator_flagged": int(f.sum()),
+        "missed": int(((s != 0) & ~f).sum()), "spurious": int((f & (s == 0)).sum()),
+        "missed_shock": int(((s == 2) & ~f).sum()), "shock_cells": int((s == 2).sum()),
+    }
+
+
+def multimode_interface_mesh(nx: int = 2048, ny: int = 1024) -> Mesh:
+    return Mesh(2, 2, nx, ny, 0.0, 2.0, 0.0, 1.0, bc=(FARFIELD, FARFIELD, FARFIELD, FARFIELD), nfields=5)
+
+
+def multimode_interface_state(mesh: Mesh, modes: int = 8, mach: float = 1.22, seed: int = SEED) -> np.ndarray:
+    """A two-fluid interface x = 0.6 + sum_k a_k sin(2 pi k y + phi_k) between
+    air (rho=1, gamma=1.4, left) and a light gas (rho=0.138, gamma=1.667,
+    right) in pressure equilibrium, hit by a Mach `mach` shock at x = 0.3."""
+    rng = np.random.default_rng(seed)
+    x, y = problems.node_xy(mesh)
+    xe, ye = x[:, :1] * np.ones_like(x), y[:, :1] * np.ones_like(y)
+    amp = rng.uniform(0.005, 0.02, modes) / np.arange(1, modes + 1)
+    ph = rng.uniform(0.0, 2 * math.pi, modes)
+    xi = 0.6 + sum(amp[k] * np.sin(2 * math.pi * (k + 1) * ye + ph[k]) for k in range(modes))
+    g = 1.4
+    M2 = mach * mach
+    p2 = 1.0 + 2 * g / (g + 1) * (M2 - 1.0)
+    rho2 = (g + 1) * M2 / ((g - 1) * M2 + 2.0)
+    u2 = mach * math.sqrt(g) * (1.0 - 1.0 / rho2)
+    post = xe < 0.3
+    light = xe > xi
+    rho = np.where(post, rho2, np.where(light, 0.138, 1.0))
+    u = np.where(post, u2, 0.0)
+    p = np.where(post, p2, 1.0)
+    gam = np.where(light & ~post, 1.667, 1.4)
+    E = p / (gam - 1.0) + 0.5 * rho * u * u
+    f = np.stack([rho, rho * u, np.zeros_like(rho), E, 1.0 / (gam - 1.0)], axis=1)
+    return np.ascontiguousarray(f.reshape(-1))
