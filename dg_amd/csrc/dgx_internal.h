@@ -86,7 +86,8 @@ struct Solver {
   int* sens_tgt = nullptr;  // sensor resolution scratch [e*2]
   int* sens_act = nullptr;  // [e*2] (ping-pong)
   int* flag = nullptr;      // device flag / counters
-  double* dscratch = nullptr;  // device scalars: [0] max UPA bits, [1] Dt, [2] T
+  double* dscratch = nullptr;  // device scalars: [1] Dt
+  double* dt_partial = nullptr;  // per-block maxima of |u|+a (ceil(N_E*N_s/256))
   int n_thresh = 0;
   double thresh[3] = {0, 0, 0};
   std::vector<void*> allocations;

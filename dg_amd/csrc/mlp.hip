@@ -56,18 +56,24 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 }
 
 // ------------------------------------------------------------- features
-// cell means (tensor-Legendre mode 0) of rho, p and alpha = 1/(gamma-1)
+// cell means (tensor-Legendre mode 0) of rho, p and alpha = 1/(gamma-1); a
+// block stages 64 elements with coalesced loads, one lane per element reduces
 __global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U, const double* __restrict__ w0, int N_E,
                                                    double* __restrict__ avg) {
-  constexpr int NF = 5, Ns = 9;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N_E) return;
+  constexpr int NF = 5, Ns = 9, NU = NF * Ns;
+  __shared__ double sU[64 * NU];
+  const int e0 = blockIdx.x * 64, ne = min(64, N_E - e0);
+  for (int k = threadIdx.x; k < ne * NU; k += 256) sU[k] = U[(size_t)e0 * NU + k];
+  __syncthreads();
+  if ((int)threadIdx.x >= ne) return;
+  const int e = e0 + threadIdx.x;
+  const double* u = sU + threadIdx.x * NU;
   double m[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < Ns; ++i) s += w0[i] * U[(e * NF + f) * Ns + i];
+    for (int i = 0; i < Ns; ++i) s += w0[i] * u[f * Ns + i];
     m[f] = s;
   }
   const double gam = 1.0 + 1.0 / m[4];
@@ -218,6 +224,147 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const uint16_t* __restrict__ w1
             h2s[(size_t)n * kH + 16 * t + 4 * q + v] = (uint16_t)bh2[t][v];
           }
       }
+    }
+  }
+}
+
+// ------------------------------------------------- fused indicator pass
+// Features and network in one pass (the solver's limiter 4): a wave takes 16
+// consecutive elements, stages their state in LDS with coalesced loads, and
+// lane (column c, group q) builds exactly the 16 features of element c that
+// its MFMA B fragments hold (k = 16s + 4q + j), in FP64, straight from the
+// staged state; the neighbour jumps read the cell means of k_mlp_means.
+// Same features, rounding and network as k_mlp_features + k_mlp_fwd.
+__device__ inline double feature_of(int k, const double* u, const double* Vinv, const double* inv, double r0, double p0,
+                                    double a0, const int* nb, int e, const double* avg) {
+  // inv: the element's five field scales (LDS)
+  if (k < 45) {
+    const int f = k / 9, q = k % 9;
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s += Vinv[q * 9 + i] * u[f * 9 + i];
+    double v = s * inv[f];
+    return v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0));
+  }
+  if (k < kIn) {
+    const int o = nb[e * 4 + (k - 45)];
+    if (o < 0) return 0.0;
+    double j = rel_jump(avg[o * 3 + 0], r0);
+    j = fmax(j, rel_jump(avg[o * 3 + 1], p0));
+    return fmax(j, rel_jump(avg[o * 3 + 2], a0));
+  }
+  return 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict__ U, const double* __restrict__ Vinv,
+                                                       const double* __restrict__ avg, const int* __restrict__ nb,
+                                                       int N, const uint16_t* __restrict__ w1b,
+                                                       const uint16_t* __restrict__ w2b,
+                                                       const uint16_t* __restrict__ w3b, const float* __restrict__ P,
+                                                       float threshold, float* __restrict__ z,
+                                                       int* __restrict__ flags) {
+  constexpr int NU = 45;
+  __shared__ double sU[4][16 * NU];
+  __shared__ double sV[81];
+  __shared__ double sInv[64][5];
+  __shared__ double sAvg[64][3];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 15, q = lane >> 4;
+  for (int k = threadIdx.x; k < 81; k += 256) sV[k] = Vinv[k];
+  bf16x4 a1[4][4], a2[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a1[t][s][j] = (short)w1b[(16 * t + col) * kInP + 16 * s + 4 * q + j];
+        a2[t][s][j] = (short)w2b[(16 * t + col) * kH + 16 * s + 4 * q + j];
+      }
+  float b1v[4][4], b2v[4][4], w3v[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int u = 16 * t + 4 * q + v;
+      b1v[t][v] = P[kB1 + u];
+      b2v[t][v] = P[kB2 + u];
+      w3v[t][v] = bf2f(w3b[u]);
+    }
+  const float b3 = P[kB3];
+  // block-uniform trip count (64 elements per block and iteration), so the
+  // block barriers below are reached by every wave
+  for (int G = blockIdx.x; G * 64 < N; G += gridDim.x) {
+    const int e0 = G * 64 + wv * 16, ne = max(0, min(16, N - e0));
+    __syncthreads();  // previous iteration's reads of sU are done
+    for (int k = lane; k < ne * NU; k += 64) sU[wv][k] = U[(size_t)e0 * NU + k];
+    const int e = e0 + col;
+    const bool valid = col < ne;
+    if (q == 0 && valid) {  // per-element scales and cell means, once per element
+      const double* u = &sU[wv][col * NU];
+      double mean[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) s += sV[i] * u[f * 9 + i];
+        mean[f] = s;
+      }
+      const double sc_mom = sqrt(fabs(mean[0]) * fabs(mean[3]));
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        const double sc = (f == 1 || f == 2) ? sc_mom : fabs(mean[f]);
+        sInv[wv * 16 + col][f] = sc > 0.0 ? 1.0 / sc : 0.0;
+      }
+#pragma unroll
+      for (int c3 = 0; c3 < 3; ++c3) sAvg[wv * 16 + col][c3] = avg[e * 3 + c3];
+    }
+    __syncthreads();
+    bf16x4 bx[4];
+    if (valid) {
+      const double* u = &sU[wv][col * NU];
+      const double* inv = sInv[wv * 16 + col];
+      const double r0 = sAvg[wv * 16 + col][0], p0 = sAvg[wv * 16 + col][1], a0 = sAvg[wv * 16 + col][2];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          bx[s][j] = (short)f2bf((float)feature_of(16 * s + 4 * q + j, u, sV, inv, r0, p0, a0, nb, e, avg));
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bx[s][j] = 0;
+    }
+    bf16x4 bh1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t][s], bx[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bh1[t][v] = (short)f2bf(fmaxf(acc[v] + b1v[t][v], 0.0f));
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][s], bh1[s], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part += w3v[t][v] * bf2f(f2bf(fmaxf(acc[v] + b2v[t][v], 0.0f)));
+    }
+    const float p1 = __shfl_xor(part, 16);
+    const float s01 = (q & 1) ? p1 + part : part + p1;
+    const float p2 = __shfl_xor(s01, 32);
+    const float tot = (q & 2) ? p2 + s01 : s01 + p2;
+    if (valid && q == 0) {
+      const float zz = tot + b3;
+      if (z) z[e] = zz;
+      // cells next to a domain boundary are flagged by rule, as by the
+      // sensor (a boundary neighbour flags the cell, sensor_kernels.cu:19-163)
+      const bool wall = nb[e * 4 + 0] < 0 || nb[e * 4 + 1] < 0 || nb[e * 4 + 2] < 0 || nb[e * 4 + 3] < 0;
+      flags[e] = (wall || zz > threshold) ? 1 : 0;
     }
   }
 }
@@ -448,19 +595,16 @@ namespace dgx {
 hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st) {
   dgx_mlp* m = s.mlp;
   const int N = s.N_E;
-  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
-  hipLaunchKernelGGL(k_mlp_features, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg,
-                     s.t.neighbors, N, s.mlp_x);
-  hipError_t e = forward(m, s.mlp_x, nullptr, N, s.mlp_z, nullptr, nullptr, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mlp_flags, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_z, N, s.mlp_threshold,
-                     s.sensors);
+  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
+  hipLaunchKernelGGL(k_mlp_indicator, dim3(nb), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg, s.t.neighbors, N,
+                     m->w1b, m->w2b, m->w3b, m->P, s.mlp_threshold, s.mlp_z, s.sensors);
   return hipGetLastError();
 }
 
 hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st) {
   const int N = s.N_E;
-  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
   hipLaunchKernelGGL(k_mlp_features, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg,
                      s.t.neighbors, N, X);
   return hipGetLastError();

@@ -220,13 +220,27 @@ __global__ __launch_bounds__(256) void k_elem(ElemArgs a) {
 
 // ---------------------------------------------------------------------------
 // Dt from CFL: max over nodes of |u|+a, then Dt = cfl / max (rk.cc:250-272).
+// Two levels: per-block maxima, then one block reduces them (no atomics).
+// The max keeps NaNs, so a NaN state gives a NaN Dt and the driver stops.
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+__device__ inline double nan_max(double a, double b) { return (b != b || b > a) ? b : a; }
+
+__device__ inline double block_max(double v, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = nan_max(m, red[k]);
+  return m;
+}
 
 template <int D, int PH>
 __global__ __launch_bounds__(256) void k_upa_max(const double* __restrict__ U, int N_E, int Ns,
-                                                 unsigned long long* out) {
+                                                 double* __restrict__ partial) {
   constexpr int NF = Fields<D, PH>::NF;
+  __shared__ double red[4];
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (n < N_E * Ns) {
@@ -236,21 +250,17 @@ __global__ __launch_bounds__(256) void k_upa_max(const double* __restrict__ U, i
     for (int f = 0; f < NF; ++f) u[f] = U[(e * NF + f) * Ns + i];
     v = upa<D, PH>(u);
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off));
-  __shared__ double red[4];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = red[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = dmax(m, red[k]);
-    atomicMax(out, (unsigned long long)__double_as_longlong(m));
-  }
+  const double m = block_max(v, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
-__global__ void k_dt_finalize(const unsigned long long* mx, double cfl, double* dt) {
-  dt[0] = cfl / __longlong_as_double((long long)mx[0]);
+__global__ __launch_bounds__(256) void k_dt_finalize(const double* __restrict__ partial, int n, double cfl,
+                                                     double* dt) {
+  __shared__ double red[4];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v = nan_max(v, partial[k]);
+  const double m = block_max(v, red);
+  if (threadIdx.x == 0) dt[0] = cfl / m;
 }
 
 // ---------------------------------------------------------------------------
@@ -306,17 +316,16 @@ hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, 
 }
 
 hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* dt_dev, hipStream_t st) {
-  unsigned long long* mx = reinterpret_cast<unsigned long long*>(s.dscratch);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(1), 0, st, mx, 0ull);
-  long n = (long)s.N_E * s.N_s;
+  const long n = (long)s.N_E * s.N_s;
+  const unsigned nb = blocks_for(n, 256);
   hipError_t err = dispatch(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, PH = decltype(ph)::value;
     (void)p; (void)fl;
-    hipLaunchKernelGGL((k_upa_max<D, PH>), dim3(blocks_for(n, 256)), dim3(256), 0, st, U, s.N_E, s.N_s, mx);
+    hipLaunchKernelGGL((k_upa_max<D, PH>), dim3(nb), dim3(256), 0, st, U, s.N_E, s.N_s, s.dt_partial);
     return hipGetLastError();
   });
   if (err != hipSuccess) return err;
-  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, st, mx, cfl, dt_dev);
+  hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, st, s.dt_partial, (int)nb, cfl, dt_dev);
   return hipGetLastError();
 }
 

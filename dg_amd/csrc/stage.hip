@@ -191,9 +191,12 @@ __device__ inline int halo_source(int h, int ex0, int ey0, const StageArgs& a, i
 #define DGX_STAGE_PERSIST 0  // 1: persistent blocks, next tile loaded during the current one
 #endif
 
-// Per-element LDS stride for n doubles: with padding, n rounded up to 2 mod 16
-// so that the elements a wave touches at once start on different banks.
-constexpr int lds_stride(int n) { return DGX_STAGE_PAD ? n + ((18 - n % 16) % 16) : n; }
+// Per-element LDS stride for n doubles: with padding, the smallest stride
+// >= n whose element starts fall on distinct, non-overlapping 16-byte bank
+// groups for the four elements a wave reads at once (2*stride mod 32 dwords
+// in [4, 28]).
+constexpr bool bank_ok(int n) { return (2 * n) % 32 >= 4 && (2 * n) % 32 <= 28; }
+constexpr int lds_stride(int n) { return !DGX_STAGE_PAD || bank_ok(n) ? n : lds_stride(n + 1); }
 
 // Tile origin of the i-th tile a block processes.  Blocks are dispatched
 // round-robin over the 8 XCDs; with XCD bands the tiles of each XCD form a

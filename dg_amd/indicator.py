@@ -116,6 +116,62 @@ def generate(n_sub: int = 64, sub: int = 8, snapshots=(0, 2, 4, 6), margin: int 
     return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
 
 
+def interface_data(n_problems: int = 32, nx: int = 128, ny: int = 64, snapshots=(0, 5, 10, 20),
+                   neg_ratio: int = 10, seed: int = SEED + 1, cfl: float = 0.3, device="cuda") -> Dataset:
+    """Random two-fluid interface problems (multimode interfaces with random
+    modes, amplitudes, density and gamma ratios, half of them hit by a shock
+    of random Mach number), evolved with the global HR limiter; all flagged
+    cells are kept with a seeded sample of `neg_ratio` unflagged cells per
+    flagged one."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    mesh = multimode_interface_mesh(nx, ny)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_sensor_thresholds(SENSOR_THRESHOLDS)
+    solver.set_indicator(Indicator(seed=0), 0.0)
+    NE = mesh.sizes["N_E"]
+    X = torch.zeros((NE, 64), dtype=torch.float32, device=device)
+    sens = torch.zeros(NE, dtype=torch.int32, device=device)
+    # cells next to the domain boundary are flagged by rule (by the sensor and
+    # by the indicator alike) and carry no information: left out
+    interior = ~(mesh.array("neighbors").reshape(-1, 4) < 0).any(axis=1)
+    out_X, out_s = [], []
+    for k in range(n_problems):
+        U0 = multimode_interface_state(
+            mesh, modes=int(rng.integers(1, 9)), mach=float(rng.uniform(1.05, 2.0)) if k % 2 else 0.0,
+            seed=int(rng.integers(1 << 30)), rho_light=float(np.exp(rng.uniform(np.log(0.05), np.log(0.8)))),
+            gamma_light=float(rng.uniform(1.1, 1.7)), amp_scale=float(rng.uniform(0.3, 3.0)))
+        U = torch.from_numpy(U0).to(device)
+        done = 0
+        for snap in snapshots:
+            if snap > done:
+                try:
+                    solver.rk4_integrate(U, 1, cfl * mesh.dx / 5, nsteps=snap - done, tf=1e30, limiter=LIMIT_HR)
+                except DgxError:
+                    break
+                done = snap
+            solver.features(U, X)
+            solver.sensing(U, sens)
+            torch.cuda.synchronize()
+            Xh, sh = X.cpu().numpy(), sens.cpu().numpy()
+            fin = np.isfinite(Xh).all(axis=1) & interior
+            pos = np.nonzero(fin & (sh != 0))[0]
+            neg = np.nonzero(fin & (sh == 0))[0]
+            neg = rng.choice(neg, size=min(neg.size, max(neg_ratio * pos.size, 64)), replace=False)
+            sel = np.concatenate([pos, neg])
+            out_X.append(Xh[sel])
+            out_s.append(sh[sel])
+    Xa = np.concatenate(out_X)
+    sa = np.concatenate(out_s).astype(np.int32)
+    return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
+
+
+def merge(*parts: Dataset) -> Dataset:
+    return Dataset(X=np.concatenate([p.X for p in parts]), y=np.concatenate([p.y for p in parts]),
+                   sensor=np.concatenate([p.sensor for p in parts]))
+
+
 def train(data: Dataset, seed: int = SEED, epochs: int = 30, batch: int = 1024, lr: float = 2e-3, device="cuda"):
     """Train a fresh indicator on the GPU; returns (network, epoch losses,
     threshold).  The threshold is set below the lowest logit of any training
@@ -146,7 +202,8 @@ def logits(net: Indicator, X: np.ndarray, device="cuda") -> np.ndarray:
 
 def confusion(flags: np.ndarray, sensor: np.ndarray) -> dict:
     """The plain sensor judges the indicator: missed = flagged by the sensor
-    only, spurious = flagged by the indicator only, per sensor kind."""
+    only, spurious = flagged by the indicator only, per sensor kind (cells
+    next to a domain boundary are flagged by rule by both)."""
     f = np.asarray(flags) != 0
     s = np.asarray(sensor)
     return {
@@ -160,27 +217,31 @@ def multimode_interface_mesh(nx: int = 2048, ny: int = 1024) -> Mesh:
     return Mesh(2, 2, nx, ny, 0.0, 2.0, 0.0, 1.0, bc=(FARFIELD, FARFIELD, FARFIELD, FARFIELD), nfields=5)
 
 
-def multimode_interface_state(mesh: Mesh, modes: int = 8, mach: float = 1.22, seed: int = SEED) -> np.ndarray:
+def multimode_interface_state(mesh: Mesh, modes: int = 8, mach: float = 1.22, seed: int = SEED,
+                              rho_light: float = 0.138, gamma_light: float = 1.667,
+                              amp_scale: float = 1.0) -> np.ndarray:
     """A two-fluid interface x = 0.6 + sum_k a_k sin(2 pi k y + phi_k) between
     air (rho=1, gamma=1.4, left) and a light gas (rho=0.138, gamma=1.667,
-    right) in pressure equilibrium, hit by a Mach `mach` shock at x = 0.3."""
+    right) in pressure equilibrium, hit by a Mach `mach` shock at x = 0.3
+    (no shock when mach <= 1)."""
     rng = np.random.default_rng(seed)
     x, y = problems.node_xy(mesh)
     xe, ye = x[:, :1] * np.ones_like(x), y[:, :1] * np.ones_like(y)
-    amp = rng.uniform(0.005, 0.02, modes) / np.arange(1, modes + 1)
+    amp = amp_scale * rng.uniform(0.005, 0.02, modes) / np.arange(1, modes + 1)
     ph = rng.uniform(0.0, 2 * math.pi, modes)
     xi = 0.6 + sum(amp[k] * np.sin(2 * math.pi * (k + 1) * ye + ph[k]) for k in range(modes))
     g = 1.4
+    mach = max(mach, 1.0)
     M2 = mach * mach
     p2 = 1.0 + 2 * g / (g + 1) * (M2 - 1.0)
     rho2 = (g + 1) * M2 / ((g - 1) * M2 + 2.0)
     u2 = mach * math.sqrt(g) * (1.0 - 1.0 / rho2)
-    post = xe < 0.3
+    post = xe < (0.3 if mach > 1.0 else -1.0)
     light = xe > xi
-    rho = np.where(post, rho2, np.where(light, 0.138, 1.0))
+    rho = np.where(post, rho2, np.where(light, rho_light, 1.0))
     u = np.where(post, u2, 0.0)
     p = np.where(post, p2, 1.0)
-    gam = np.where(light & ~post, 1.667, 1.4)
+    gam = np.where(light & ~post, gamma_light, 1.4)
     E = p / (gam - 1.0) + 0.5 * rho * u * u
     f = np.stack([rho, rho * u, np.zeros_like(rho), E, 1.0 / (gam - 1.0)], axis=1)
     return np.ascontiguousarray(f.reshape(-1))

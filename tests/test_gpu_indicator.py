@@ -39,6 +39,25 @@ def test_features_match_oracle(device):
         assert np.allclose(X.cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
 
 
+def test_fused_indicator_equals_features_then_forward(device):
+    mesh, solver = _solver()
+    t = mesh.tables()
+    net = Indicator(seed=4)
+    solver.set_indicator(net, 0.25)
+    U = torch.from_numpy(problems.oblique_riemann_state(mesh, MULTIFLUID, seed=6)).to(device)
+    X = torch.zeros((t["N_E"], 64), dtype=torch.float32, device=device)
+    z1 = torch.zeros(t["N_E"], dtype=torch.float32, device=device)
+    z2 = torch.zeros_like(z1)
+    flags = torch.zeros(t["N_E"], dtype=torch.int32, device=device)
+    solver.features(U, X)
+    net.forward(X, z1)
+    solver.indicator_flags(U, flags, z2)
+    torch.cuda.synchronize()
+    assert np.array_equal(z1.cpu().numpy(), z2.cpu().numpy())
+    wall = (t["neighbors"].reshape(-1, 4) < 0).any(axis=1)
+    assert np.array_equal(flags.cpu().numpy(), ((z2.cpu().numpy() > 0.25) | wall).astype(np.int32))
+
+
 def _random_inputs(n, seed):
     rng = np.random.default_rng(seed)
     X = np.zeros((n, 64), dtype=np.float32)
@@ -116,5 +135,6 @@ def test_learned_limiter_is_exact_on_its_flags(device):
     torch.cuda.synchronize()
     f = flags.cpu().numpy()
     assert 0 < f.sum() < t["N_E"]
-    assert np.array_equal(f, (zz > thr).astype(np.int32))
+    wall = (t["neighbors"].reshape(-1, 4) < 0).any(axis=1)  # flagged by rule, as by the sensor
+    assert np.array_equal(f, ((zz > thr) | wall).astype(np.int32))
     assert np.array_equal(Ud.cpu().numpy(), prob.limit_flagged(U0, f))
