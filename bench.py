@@ -180,7 +180,9 @@ def main() -> int:
             "workload": w["name"],
             "elements": N_E, "dofs": n_dof, "dt": dt, "parallelism": "single GPU",
             "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
-            "contractions": "FP32 MFMA (FP64 fluxes, RK and limiter)" if args.precision == "fp32mfma" else "FP64",
+            "contractions": ("FP32 MFMA (FP64 fluxes, RK and limiter)" if args.precision == "fp32mfma" else
+                             "FP64 MFMA element phases" if tiled and w["physics"] == 0 and w["order"] == 2 and w["dim"] == 2
+                             else "FP64"),
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

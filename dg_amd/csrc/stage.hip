@@ -187,6 +187,9 @@ __device__ inline int halo_source(int h, int ex0, int ey0, const StageArgs& a, i
 #ifndef DGX_STAGE_PAD
 #define DGX_STAGE_PAD 1  // pad per-element LDS strides against bank conflicts
 #endif
+#ifndef DGX_STAGE_MFMA64
+#define DGX_STAGE_MFMA64 1  // single-fluid 2D p=2 element phases on the FP64 matrix cores
+#endif
 #ifndef DGX_STAGE_PERSIST
 #define DGX_STAGE_PERSIST 0  // 1: persistent blocks, next tile loaded during the current one
 #endif
@@ -220,11 +223,15 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int NU = V::NU, NQE = V::NQE, NQ = V::NQ, TPE = V::TPE, EPB = V::EPB;
   constexpr int NT = T::NT, NFACE = T::NFACE, NH = T::NH, HT = NF * Ms;
   constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
+  constexpr bool MF64 = DGX_STAGE_MFMA64 && PREC == 0 && D == 2 && P == 2 && PH == 0 && NT % 16 == 0;
   constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * SW, NW_PREC = PREC ? 2400 : 0;
+  constexpr int NW_MF = MF64 ? 4 * 16 * (NG * D + 1) : 0;  // FP64-MFMA flux rows, 16 per wave
   constexpr int NW0 = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
-  constexpr int NW = NW0 > NW_PREC ? NW0 : NW_PREC;
+  constexpr int NW1 = NW0 > NW_PREC ? NW0 : NW_PREC;
+  constexpr int NW = NW1 > NW_MF ? NW1 : NW_MF;
   constexpr int NGEO = 1 + D * D;
   static_assert(PREC == 0 || (D == 2 && P == 2 && NT == 32), "mixed precision: 2D p=2 8x4 tiles");
+  __shared__ int sNodeQ[18];
   // per-thread counts of the loaded quantities
   constexpr int R_U = (NT * NU + 255) / 256, R_H = (NH * HT + 255) / 256;
   constexpr int R_G = (NT * NGEO + 255) / 256, R_F = (NFACE * 4 + 255) / 256;
@@ -247,6 +254,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   const bool shared_minv = a.minv_stride == 0;
   load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
   for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
+  if constexpr (MF64) node_q_table<NF>(sNodeQ, tid);
   const double dt = (MODE == 1) ? a.dt[0] : 0.0;
 
   // registers holding the next tile's inputs while the current one computes
@@ -424,6 +432,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     // ---- element phases
     if constexpr (PREC == 1) {
       vol_mfma_tile<PH, NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, reinterpret_cast<float*>(sW), tid);
+    } else if constexpr (MF64) {
+      vol_mfma64_sf<NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, sW, sNodeQ, tid);
+      __syncthreads();
     } else {
     for (int b = 0; b < NT; b += EPB) {
       {
@@ -479,8 +490,14 @@ template <int D>
 struct TileShape;
 template <>
 struct TileShape<1> { static constexpr int TX = 128, TY = 1; };
+#ifndef DGX_TILE2_TX
+#define DGX_TILE2_TX 8
+#endif
+#ifndef DGX_TILE2_TY
+#define DGX_TILE2_TY 4
+#endif
 template <>
-struct TileShape<2> { static constexpr int TX = 8, TY = 4; };
+struct TileShape<2> { static constexpr int TX = DGX_TILE2_TX, TY = DGX_TILE2_TY; };
 
 bool stage_tiles(int D, int nx, int ny, int* tx, int* ty) {
   *tx = D == 1 ? TileShape<1>::TX : TileShape<2>::TX;
@@ -528,7 +545,7 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
     auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0> : k_stage<D, P, PH, FL, 1, TX, TY, 0>;
-    if constexpr (D == 2 && P == 2) {
+    if constexpr (D == 2 && P == 2 && TX * TY == 32) {
       if (s.precision == 1)
         kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
     }

@@ -323,3 +323,107 @@ __device__ void vol_mfma_tile(const VolTables<2, 2, PH>& t, double* sU, const do
 }
 
 }  // namespace dgx
+
+namespace dgx {
+
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+// Element phases of a tile of NT single-fluid 2D p=2 elements in FP64 on the
+// matrix cores (v_mfma_f64_16x16x4f64), same arithmetic precision as the
+// VALU path.  Wave w owns elements 4w..4w+3 of every group of 16: the map
+// (rows = element x field, K = 9 nodes padded to 12, columns = 16 Gauss
+// points) leaves in each lane the 4 fields of one element at one point
+// (D lane l, register v = row l/16 + 4v, column l%16), so the flux is evaluated in
+// registers; the metric-scaled fluxes go through this wave's LDS rows into
+// the integration (rows = element x field, K = 32 (point, direction),
+// columns = 9 nodes padded to 16), whose D fragments plus the face sums are
+// the right-hand side, written to sU.  No block barrier is needed: every
+// exchange stays inside the wave.  sF needs 4 * 16 * KP doubles.
+template <int NT, int SU, int SQ, int NGEO>
+__device__ void vol_mfma64_sf(const VolTables<2, 2, 0>& t, double* sU, const double* sGeo, const double* sQ,
+                              double* sF, const int* sNodeQ, int tid) {
+  constexpr int NF = 4, Ns = 9, NG = 16, D = 2, KP = NG * D + 1;
+  const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
+  double* F = sF + w * 16 * KP;
+  double bphi[3], bdw[8];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int k = 4 * s + q;
+    bphi[s] = k < Ns ? t.phi[c + k * NG] : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) bdw[s] = c < Ns ? t.dw[(4 * s + q) * Ns + c] : 0.0;
+#pragma unroll
+  for (int b = 0; b < NT; b += 16) {
+    const int e_base = b + 4 * w;  // this wave's four elements
+    // ---- map: Ug[(el,f)][g]. The f64 16x16x4 result holds row (l/16) + 4v of
+    // column l%16 in register v, so A row c is (element c%4, field c/4): lane
+    // (q, g) then receives the four fields of element q at point g.
+    f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+    {
+      const int el = e_base + (c & 3), f = c >> 2;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int k = 4 * s + q;
+        const double a = k < Ns ? sU[el * SU + f * Ns + k] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bphi[s], acc, 0, 0, 0);
+      }
+    }
+    // ---- flux of element e_base + q at point g = c, metric terms, into LDS
+    {
+      const int el = e_base + q;
+      double ug[NF] = {acc[0], acc[1], acc[2], acc[3]};
+      double dua[D] = {0.0, 0.0};
+      const double Je = sGeo[el * NGEO];
+      const double* ij = sGeo + el * NGEO + 1;
+      double fx[NF * D], salpha;
+      volume_flux<2, 0>(ug, dua, ij, fx, salpha);
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int al = 0; al < D; ++al) {
+          double sol = 0.0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
+          F[(f * 4 + q) * KP + c * D + al] = sol;  // row 4f + el, see the map
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // ---- integration: R[(el,f)][i] = sum_k F[(el,f)][k] dphi_w[k][i]
+    f64x4_t r = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) r = __builtin_amdgcn_mfma_f64_16x16x4f64(F[c * KP + 4 * s + q], bdw[s], r, 0, 0, 0);
+    __builtin_amdgcn_wave_barrier();  // F is rewritten by the next group
+    if (c < Ns) {
+      const int el = e_base + q;
+      const int q0 = sNodeQ[2 * c], q1 = sNodeQ[2 * c + 1];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double qs = 0.0;
+        if (q0 >= 0) qs += sQ[el * SQ + q0 + v * 3];
+        if (q1 >= 0) qs += sQ[el * SQ + q1 + v * 3];
+        sU[el * SU + v * Ns + c] = r[v] + qs;
+      }
+    }
+  }
+}
+
+// Offsets (kk*NF*Ms + jj) of the at most two face slots that land on node i
+// (closure_node(kk, jj) == i, edges in order), -1 when absent; field f adds f*Ms.
+template <int NF>
+__device__ inline void node_q_table(int* t, int tid) {
+  if (tid < 9) {
+    int n = 0, o[2] = {-1, -1};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+        if (closure_node<2, 2>(kk, jj) == tid && n < 2) o[n++] = kk * NF * 3 + jj;
+    t[2 * tid] = o[0];
+    t[2 * tid + 1] = o[1];
+  }
+}
+
+}  // namespace dgx
