@@ -188,7 +188,7 @@ __device__ inline int halo_source(int h, int ex0, int ey0, const StageArgs& a, i
 #define DGX_STAGE_PAD 1  // pad per-element LDS strides against bank conflicts
 #endif
 #ifndef DGX_STAGE_MFMA64
-#define DGX_STAGE_MFMA64 1  // single-fluid 2D p=2 element phases on the FP64 matrix cores
+#define DGX_STAGE_MFMA64 1  // 2D p=2 FP64 element phases on the matrix cores (0: VALU batches)
 #endif
 #ifndef DGX_STAGE_PERSIST
 #define DGX_STAGE_PERSIST 0  // 1: persistent blocks, next tile loaded during the current one
@@ -223,14 +223,15 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int NU = V::NU, NQE = V::NQE, NQ = V::NQ, TPE = V::TPE, EPB = V::EPB;
   constexpr int NT = T::NT, NFACE = T::NFACE, NH = T::NH, HT = NF * Ms;
   constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
-  constexpr bool MF64 = DGX_STAGE_MFMA64 && PREC == 0 && D == 2 && P == 2 && PH == 0 && NT % 16 == 0;
-  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = EPB * SW, NW_PREC = PREC ? 2400 : 0;
-  constexpr int NW_MF = MF64 ? 4 * 16 * (NG * D + 1) : 0;  // FP64-MFMA flux rows, 16 per wave
+  // element phases on the matrix cores (FP64, or FP32 in the mixed-precision mode) or as VALU batches
+  constexpr bool MFW = D == 2 && P == 2 && NT % 16 == 0 && (PREC == 1 || DGX_STAGE_MFMA64);
+  using TW = typename std::conditional<PREC == 1, float, double>::type;
+  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = MFW ? 0 : EPB * SW;
+  constexpr int NW_MF = MFW ? (4 * vol_wave_lds<PH>() * (int)sizeof(TW) + 7) / 8 : 0;
   constexpr int NW0 = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
-  constexpr int NW1 = NW0 > NW_PREC ? NW0 : NW_PREC;
-  constexpr int NW = NW1 > NW_MF ? NW1 : NW_MF;
+  constexpr int NW = NW0 > NW_MF ? NW0 : NW_MF;
   constexpr int NGEO = 1 + D * D;
-  static_assert(PREC == 0 || (D == 2 && P == 2 && NT == 32), "mixed precision: 2D p=2 8x4 tiles");
+  static_assert(PREC == 0 || MFW, "mixed precision: 2D p=2 tiles of a multiple of 16 elements");
   __shared__ int sNodeQ[18];
   // per-thread counts of the loaded quantities
   constexpr int R_U = (NT * NU + 255) / 256, R_H = (NH * HT + 255) / 256;
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   const bool shared_minv = a.minv_stride == 0;
   load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
   for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
-  if constexpr (MF64) node_q_table<NF>(sNodeQ, tid);
+  if constexpr (MFW) node_q_table<NF>(sNodeQ, tid);
   const double dt = (MODE == 1) ? a.dt[0] : 0.0;
 
   // registers holding the next tile's inputs while the current one computes
@@ -415,10 +416,12 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     for (int itf = tid; itf < NFACE * 2 * Ms; itf += 256) {
       const int fi = itf / (2 * Ms), d = (itf / Ms) % 2, j = itf % Ms;
       const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
-      const int dst = fd.dst[d];
+      // selects, not fd.x[d]: a runtime index into the descriptor puts it on the stack
+      const int dst = d ? fd.dst[1] : fd.dst[0];
       if (dst < 0) continue;
-      const int jf = (D == 2 && fd.cl[d] >= 4) ? reversed_pos<P>(j) : j;
-      double* q = sQ + dst * SQ + fd.edge[d] * NF * Ms;
+      const int cl = d ? fd.cl[1] : fd.cl[0], edge = d ? fd.edge[1] : fd.edge[0];
+      const int jf = (D == 2 && cl >= 4) ? reversed_pos<P>(j) : j;
+      double* q = sQ + dst * SQ + edge * NF * Ms;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         double s = 0.0;
@@ -430,10 +433,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     __syncthreads();
 
     // ---- element phases
-    if constexpr (PREC == 1) {
-      vol_mfma_tile<PH, NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, reinterpret_cast<float*>(sW), tid);
-    } else if constexpr (MF64) {
-      vol_mfma64_sf<NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, sW, sNodeQ, tid);
+    if constexpr (MFW) {
+      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, sW, sNodeQ, tid);
       __syncthreads();
     } else {
     for (int b = 0; b < NT; b += EPB) {

@@ -175,237 +175,148 @@ __device__ inline void rk_update(const VolTables<D, P, PH>& t, const double* Mgl
 namespace dgx {
 
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-// FP32-MFMA volume terms of a tile of NT 2D p=2 elements (256 threads), for
-// the mixed-precision mode: the map U -> Gauss points (and the alpha
-// gradient) and the integration against the weighted derivative tables run
-// as v_mfma_f32_16x16x4f32 contractions with FP32 accumulation; the
-// pointwise flux, the face sum and everything after stay FP64.  Rows of the
-// MFMA tiles are (element, field) pairs, columns Gauss points (map) or nodes
-// (integration).  Operand layouts (16x16x4 f32): A lane l = row l%16,
-// k = l/16; B lane l = column l%16, k = l/16; D lane l = rows 4*(l/16)+v,
-// column l%16.  Writes S+F+Q of every element, in FP64, into sU.
-template <int PH, int NT, int SU, int SQ, int NGEO>
-__device__ void vol_mfma_tile(const VolTables<2, 2, PH>& t, double* sU, const double* sGeo, const double* sQ,
-                              float* sM, int tid) {
-  constexpr int NF = 4 + PH, NFX = 4, Ns = 9, NG = 16, D = 2, Ms = 3;
-  constexpr int NFP = NF + 2;          // per (element, point): fields, then d(alpha)/dxi, d(alpha)/deta
-  constexpr int KP = NG * D + 1;       // padded row of the flux operand
-  constexpr int NGP = NG + 1;          // padded row of the alpha-source operand
-  float* sUg = sM;                     // [(el*NG + g)*NFP + f]
-  float* sFe = sM;                     // [(el*NFX + f)*KP + g*D + al]  (after the flux phase)
-  float* sS = sM + NT * NFX * KP;      // [el*NGP + g]
-  const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
-
-  // ---- map: Ug[(el,f)][g] = sum_i U[el][f][i] phi[g][i]
-  float bphi[3];
-#pragma unroll
-  for (int s = 0; s < 3; ++s) {
-    const int k = 4 * s + q;
-    bphi[s] = k < Ns ? (float)t.phi[c + k * NG] : 0.0f;
-  }
-  constexpr int RT = NT * NF / 16;
-  for (int rt = w; rt < RT; rt += 4) {
-    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-    const int r = 16 * rt + c, el = r / NF, f = r % NF;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      const int k = 4 * s + q;
-      const float a = k < Ns ? (float)sU[el * SU + f * Ns + k] : 0.0f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bphi[s], acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int rr = 16 * rt + 4 * q + v;
-      sUg[((rr / NF) * NG + c) * NFP + rr % NF] = acc[v];
-    }
-  }
-  if constexpr (PH == 1) {  // d(alpha)/dxi_al at the points: rows el, columns g for direction al
-    const int rt = w >> 1, al = w & 1;
-    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-    const int el = 16 * rt + c;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      const int k = 4 * s + q;
-      const float a = k < Ns ? (float)sU[el * SU + (NF - 1) * Ns + k] : 0.0f;
-      const float b = k < Ns ? (float)t.dphi[(c * D + al) + k * NG * D] : 0.0f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) sUg[((16 * rt + 4 * q + v) * NG + c) * NFP + NF + al] = acc[v];
-  }
-  __syncthreads();
-
-  // ---- pointwise flux with metric terms (FP64), two (element, point) pairs per lane
-  double fe[2][NFX * D + PH];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int p = tid + h * 256, el = p / NG, g = p % NG;
-    double ug[NF], dua[D];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) ug[f] = sUg[(el * NG + g) * NFP + f];
-#pragma unroll
-    for (int al = 0; al < D; ++al) dua[al] = PH ? (double)sUg[(el * NG + g) * NFP + NF + al] : 0.0;
-    const double Je = sGeo[el * NGEO];
-    const double* ij = sGeo + el * NGEO + 1;
-    double fx[NF * D], salpha;
-    volume_flux<2, PH>(ug, dua, ij, fx, salpha);
-#pragma unroll
-    for (int f = 0; f < NFX; ++f)
-#pragma unroll
-      for (int al = 0; al < D; ++al) {
-        double sol = 0.0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
-        fe[h][f * D + al] = sol;
-      }
-    if constexpr (PH == 1) fe[h][NFX * D] = salpha * Je;
-  }
-  __syncthreads();  // the flux operand overwrites the point values
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int p = tid + h * 256, el = p / NG, g = p % NG;
-#pragma unroll
-    for (int f = 0; f < NFX; ++f)
-#pragma unroll
-      for (int al = 0; al < D; ++al) sFe[(el * NFX + f) * KP + g * D + al] = (float)fe[h][f * D + al];
-    if constexpr (PH == 1) sS[el * NGP + g] = (float)fe[h][NFX * D];
-  }
-  __syncthreads();
-
-  // face terms of node i, field f of element el (at most two, summed in edge order)
-  auto qsum = [&](int el, int f, int i) {
-    double s = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-      for (int jj = 0; jj < Ms; ++jj)
-        if (closure_node<2, 2>(kk, jj) == i) s += sQ[el * SQ + (kk * NF + f) * Ms + jj];
-    return s;
-  };
-  // ---- integration: F[(el,f)][i] = sum_{g,al} fe[(el,f)][g,al] dphi_w[(g,al)][i]
-  float bdw[8];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) bdw[s] = c < Ns ? (float)t.dw[(4 * s + q) * Ns + c] : 0.0f;
-  constexpr int RTI = NT * NFX / 16;
-  for (int rt = w; rt < RTI; rt += 4) {
-    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < 8; ++s)
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sFe[(16 * rt + c) * KP + 4 * s + q], bdw[s], acc, 0, 0, 0);
-    if (c < Ns) {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int rr = 16 * rt + 4 * q + v, el = rr / NFX, f = rr % NFX;
-        sU[el * SU + f * Ns + c] = (double)acc[v] + qsum(el, f, c);
-      }
-    }
-  }
-  if constexpr (PH == 1) {  // alpha source: S[el][i] = sum_g s[el][g] phi_w[g][i]
-    if (w < 2) {
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int k = 4 * s + q;
-        const float b = c < Ns ? (float)t.pw[k * Ns + c] : 0.0f;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(sS[(16 * w + c) * NGP + k], b, acc, 0, 0, 0);
-      }
-      if (c < Ns) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int el = 16 * w + 4 * q + v;
-          sU[el * SU + (NF - 1) * Ns + c] = (double)acc[v] + qsum(el, NF - 1, c);
-        }
-      }
-    }
-  }
-  __syncthreads();
-}
-
-}  // namespace dgx
-
-namespace dgx {
-
 typedef double f64x4_t __attribute__((ext_vector_type(4)));
 
-// Element phases of a tile of NT single-fluid 2D p=2 elements in FP64 on the
-// matrix cores (v_mfma_f64_16x16x4f64), same arithmetic precision as the
-// VALU path.  Wave w owns elements 4w..4w+3 of every group of 16: the map
-// (rows = element x field, K = 9 nodes padded to 12, columns = 16 Gauss
-// points) leaves in each lane the 4 fields of one element at one point
-// (D lane l, register v = row l/16 + 4v, column l%16), so the flux is evaluated in
-// registers; the metric-scaled fluxes go through this wave's LDS rows into
-// the integration (rows = element x field, K = 32 (point, direction),
-// columns = 9 nodes padded to 16), whose D fragments plus the face sums are
-// the right-hand side, written to sU.  No block barrier is needed: every
-// exchange stays inside the wave.  sF needs 4 * 16 * KP doubles.
-template <int NT, int SU, int SQ, int NGEO>
-__device__ void vol_mfma64_sf(const VolTables<2, 2, 0>& t, double* sU, const double* sGeo, const double* sQ,
-                              double* sF, const int* sNodeQ, int tid) {
-  constexpr int NF = 4, Ns = 9, NG = 16, D = 2, KP = NG * D + 1;
+// 16x16x4 matrix-core step in T, and where its result rows land.  Operands
+// (both precisions): A lane l = row l%16, k = l/16; B lane l = column l%16,
+// k = l/16.  Result register v of lane l holds column l%16 of row
+//   v_mfma_f64_16x16x4f64: l/16 + 4v        v_mfma_f32_16x16x4f32: 4(l/16) + v
+// Rows are (element, field) pairs of four elements x four fields, numbered
+// by row(el, f) so that in both cases register v of lane l is field v of
+// element l/16.
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  using V4 = f64x4_t;
+  static __device__ inline int row(int el, int f) { return el + 4 * f; }
+  static __device__ inline int el(int r) { return r & 3; }
+  static __device__ inline int field(int r) { return r >> 2; }
+  static __device__ inline V4 mma(double a, double b, V4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma16<float> {
+  using V4 = f32x4_t;
+  static __device__ inline int row(int el, int f) { return 4 * el + f; }
+  static __device__ inline int el(int r) { return r >> 2; }
+  static __device__ inline int field(int r) { return r & 3; }
+  static __device__ inline V4 mma(float a, float b, V4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// T-values of per-wave LDS that vol_mfma_wave needs.
+template <int PH>
+constexpr int vol_wave_lds() { return 16 * (16 * 2 + 1) + (PH ? 4 * 16 : 0); }
+
+// Element phases of a tile of NT 2D p=2 elements (256 threads, NT a multiple
+// of 16) on the matrix cores, contractions in T: double for the FP64 path,
+// float for the mixed-precision mode (FP32 inputs and accumulation; the
+// pointwise flux, the face sums and everything after stay FP64).
+//   map        Ug[(el,f)][g]  = sum_i U[el][f][i] phi[g][i]         fields 0..3
+//   flux       fe[(el,f)][g,al] = J_e sum_k invJac[al][k] f_k(Ug)    in registers
+//   integrate  R[(el,f)][i]   = sum_{g,al} fe[(el,f)][g,al] dphi_w[g,al][i]
+// Wave w owns elements 4w..4w+3 of every group of 16.  After the map each
+// lane holds the four conserved fields of one element at one Gauss point, so
+// the flux needs no exchange; the scaled fluxes pass through the wave's own
+// LDS rows into the integration, whose result registers plus the face sums
+// are the right-hand side, written to sU.  Multifluid adds alpha and its
+// reference gradient at the point (sums over the 9 nodes in T) and the
+// alpha source integrated against phi_w.  No block barrier: every exchange
+// stays inside the wave.  sWork needs 4 * vol_wave_lds<PH>() T-values.
+template <typename T, int PH, int NT, int SU, int SQ, int NGEO>
+__device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, double* sU, const double* sGeo, const double* sQ,
+                              double* sWork, const int* sNodeQ, int tid) {
+  using M = Mfma16<T>;
+  constexpr int NF = 4 + PH, Ns = 9, NG = 16, D = 2, KP = NG * D + 1, Ms = 3;
+  static_assert(NT % 16 == 0, "groups of 16 elements, four per wave");
   const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
-  double* F = sF + w * 16 * KP;
-  double bphi[3], bdw[8];
+  T* F = reinterpret_cast<T*>(sWork) + w * vol_wave_lds<PH>();
+  T* S = F + 16 * KP;
+  T bphi[3], bdw[8];
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const int k = 4 * s + q;
-    bphi[s] = k < Ns ? t.phi[c + k * NG] : 0.0;
+    bphi[s] = k < Ns ? (T)t.phi[c + k * NG] : T(0);
   }
 #pragma unroll
-  for (int s = 0; s < 8; ++s) bdw[s] = c < Ns ? t.dw[(4 * s + q) * Ns + c] : 0.0;
-#pragma unroll
+  for (int s = 0; s < 8; ++s) bdw[s] = c < Ns ? (T)t.dw[(4 * s + q) * Ns + c] : T(0);
+  const int a_el = M::el(c), a_f = M::field(c);
+#pragma unroll 1
   for (int b = 0; b < NT; b += 16) {
-    const int e_base = b + 4 * w;  // this wave's four elements
-    // ---- map: Ug[(el,f)][g]. The f64 16x16x4 result holds row (l/16) + 4v of
-    // column l%16 in register v, so A row c is (element c%4, field c/4): lane
-    // (q, g) then receives the four fields of element q at point g.
-    f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
-    {
-      const int el = e_base + (c & 3), f = c >> 2;
+    const int e_base = b + 4 * w, el = e_base + q;
+    // ---- map
+    typename M::V4 acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-      for (int s = 0; s < 3; ++s) {
-        const int k = 4 * s + q;
-        const double a = k < Ns ? sU[el * SU + f * Ns + k] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bphi[s], acc, 0, 0, 0);
-      }
+    for (int s = 0; s < 3; ++s) {
+      const int k = 4 * s + q;
+      const T a = k < Ns ? (T)sU[(e_base + a_el) * SU + a_f * Ns + k] : T(0);
+      acc = M::mma(a, bphi[s], acc);
     }
-    // ---- flux of element e_base + q at point g = c, metric terms, into LDS
+    double ug[NF], dua[D] = {0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ug[v] = (double)acc[v];
+    if constexpr (PH == 1) {
+      const double* ua = sU + el * SU + 4 * Ns;
+      T al = 0, d0 = 0, d1 = 0;
+#pragma unroll
+      for (int k = 0; k < Ns; ++k) {
+        const T uk = (T)ua[k];
+        al += (T)t.phi[c + k * NG] * uk;
+        d0 += (T)t.dphi[(c * D + 0) + k * NG * D] * uk;
+        d1 += (T)t.dphi[(c * D + 1) + k * NG * D] * uk;
+      }
+      ug[4] = al;
+      dua[0] = d0;
+      dua[1] = d1;
+    }
+    // ---- flux of element el at point g = c with metric terms, into this wave's rows
     {
-      const int el = e_base + q;
-      double ug[NF] = {acc[0], acc[1], acc[2], acc[3]};
-      double dua[D] = {0.0, 0.0};
       const double Je = sGeo[el * NGEO];
       const double* ij = sGeo + el * NGEO + 1;
       double fx[NF * D], salpha;
-      volume_flux<2, 0>(ug, dua, ij, fx, salpha);
+      volume_flux<2, PH>(ug, dua, ij, fx, salpha);
 #pragma unroll
-      for (int f = 0; f < NF; ++f)
+      for (int f = 0; f < 4; ++f)
 #pragma unroll
         for (int al = 0; al < D; ++al) {
           double sol = 0.0;
 #pragma unroll
           for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
-          F[(f * 4 + q) * KP + c * D + al] = sol;  // row 4f + el, see the map
+          F[M::row(q, f) * KP + c * D + al] = (T)sol;
         }
+      if constexpr (PH == 1) S[q * NG + c] = (T)(salpha * Je);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- integration: R[(el,f)][i] = sum_k F[(el,f)][k] dphi_w[k][i]
-    f64x4_t r = {0.0, 0.0, 0.0, 0.0};
+    // ---- integration
+    typename M::V4 r = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int s = 0; s < 8; ++s) r = __builtin_amdgcn_mfma_f64_16x16x4f64(F[c * KP + 4 * s + q], bdw[s], r, 0, 0, 0);
-    __builtin_amdgcn_wave_barrier();  // F is rewritten by the next group
-    if (c < Ns) {
-      const int el = e_base + q;
-      const int q0 = sNodeQ[2 * c], q1 = sNodeQ[2 * c + 1];
+    for (int s = 0; s < 8; ++s) r = M::mma(F[c * KP + 4 * s + q], bdw[s], r);
+    T sa = 0;
+    if constexpr (PH == 1) {
+      if (c < Ns) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        double qs = 0.0;
-        if (q0 >= 0) qs += sQ[el * SQ + q0 + v * 3];
-        if (q1 >= 0) qs += sQ[el * SQ + q1 + v * 3];
-        sU[el * SU + v * Ns + c] = r[v] + qs;
+        for (int g = 0; g < NG; ++g) sa += S[q * NG + g] * (T)t.pw[g * Ns + c];
       }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // F and S are rewritten by the next group
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (c < Ns) {
+      const int q0 = sNodeQ[2 * c], q1 = sNodeQ[2 * c + 1];
+      auto face = [&](int f) {
+        double qs = 0.0;
+        if (q0 >= 0) qs += sQ[el * SQ + q0 + f * Ms];
+        if (q1 >= 0) qs += sQ[el * SQ + q1 + f * Ms];
+        return qs;
+      };
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sU[el * SU + v * Ns + c] = (double)r[v] + face(v);
+      if constexpr (PH == 1) sU[el * SU + 4 * Ns + c] = (double)sa + face(4);
     }
   }
 }
