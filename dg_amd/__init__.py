@@ -61,6 +61,7 @@ def lib() -> ctypes.CDLL:
         L.dgx_rhs.argtypes = [vp, vp, vp, vp]
         L.dgx_set_sensor_thresholds.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double)]
         L.dgx_limit.argtypes = [vp, i, vp, vp]
+        L.dgx_limit_into.argtypes = [vp, i, vp, vp, vp]
         L.dgx_sensing.argtypes = [vp, vp, vp, vp, vp]
         L.dgx_dt_from_cfl.argtypes = [vp, vp, dp, vp, vp]
         L.dgx_rk4_step.argtypes = [vp, vp, vp, i, vp]
@@ -202,6 +203,10 @@ class Solver:
 
     def limit(self, limiter: int, U, stream=None) -> None:
         _check(lib().dgx_limit(self._h, limiter, _ptr(U), _stream(stream)), "dgx_limit")
+
+    def limit_into(self, limiter: int, In, Out, stream=None) -> None:
+        """Out = limited In (In unchanged); 2D hrl on tile meshes runs both sweeps in one launch."""
+        _check(lib().dgx_limit_into(self._h, limiter, _ptr(In), _ptr(Out), _stream(stream)), "dgx_limit_into")
 
     def sensing(self, U, sensors, Uavg=None, stream=None) -> None:
         _check(lib().dgx_sensing(self._h, _ptr(U), _ptr(sensors), _ptr(Uavg), _stream(stream)), "dgx_sensing")

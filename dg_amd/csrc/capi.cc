@@ -341,6 +341,18 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
          upload(s, &s.t.MonoY2Lag, t.MonoY2Lag.data(), t.MonoY2Lag.size()) &&
          upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
          upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
+  if (s.cart.on && D == 2 && s.P <= 2) {
+    // the one-launch HR limiter takes the rows below and above a tile from
+    // the table and assumes e -/+ nx for the rows inside it (p = 3 would
+    // spill registers: two sweeps)
+    const int nx = s.cart.nx;
+    s.hr2d = true;
+    for (int e = 0; e < NE && s.hr2d; ++e) {
+      const int ey = e / nx;
+      if (ey > 0 && t.neighbors[(size_t)e * NN + 2] != e - nx) s.hr2d = false;
+      if (ey < s.cart.ny - 1 && t.neighbors[(size_t)e * NN + 3] != e + nx) s.hr2d = false;
+    }
+  }
   const size_t S = (size_t)s.state_size();
   ok = ok && alloc(s, &s.Qe, (size_t)NE * NN * s.NF * s.M_s) && alloc(s, &s.Uacc, S) && alloc(s, &s.Ustar, S) &&
        alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
@@ -409,6 +421,17 @@ dgx_status dgx_limit(dgx_solver* h, int limiter, double* U, void* stream) {
   return do_limit(h->s, limiter, U, as_stream(stream));
 }
 
+dgx_status dgx_limit_into(dgx_solver* h, int limiter, const double* In, double* Out, void* stream) {
+  if (!h || !In || !Out) return fail(DGX_EINVAL, "dgx_limit_into: null argument");
+  if (In == Out) return fail(DGX_EINVAL, "dgx_limit_into: In and Out must differ");
+  Solver& s = h->s;
+  hipStream_t st = as_stream(stream);
+  if (limiter == 1 && s.hr2d) return hip_ok(launch_limit_hr_oop(s, In, Out, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
+  if (!hip_ok(hipMemcpyAsync(Out, In, sizeof(double) * s.state_size(), hipMemcpyDeviceToDevice, st), "copy"))
+    return DGX_EDEVICE;
+  return do_limit(s, limiter, Out, st);
+}
+
 dgx_status dgx_sensing(dgx_solver* h, const double* U, int* sensors, double* Uavg, void* stream) {
   if (!h || !U || !sensors) return fail(DGX_EINVAL, "dgx_sensing: null argument");
   return hip_ok(launch_sensing(h->s, U, sensors, Uavg, as_stream(stream)), "sensing") ? DGX_OK : DGX_EDEVICE;
@@ -425,8 +448,37 @@ static dgx_status timed_limit(Solver& s, int limiter, double* U, hipStream_t st)
   return do_limit(s, limiter, U, st);
 }
 
+#ifndef DGX_RK_HR2D
+// 0: the RK driver limits with the two sweep launches in place.  The
+// one-launch kernel halves the limiter's HBM passes but measured slower on
+// the 1024x512 p=2 two-fluid grid (337 us against 218 us per application):
+// the limiter is bound by its FP64 arithmetic, which the halo rows raise by
+// half.  It serves dgx_limit_into.
+#define DGX_RK_HR2D 0
+#endif
+
+static dgx_status timed_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st) {
+  KernelTimer kt(s, 2, st);
+  return hip_ok(launch_limit_hr_oop(s, In, Out, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
+}
+
 static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, hipStream_t st) {
   dgx_status r;
+  if (DGX_RK_HR2D && s.tiled() && s.hr2d && limiter == 1) {
+    // One-launch HR limiter (out of place): every stage writes Ustar, the
+    // limiter takes Ustar to Ustar2 (the next stage's input) and, after the
+    // last stage, to U.  U keeps the step's initial state until then.
+    const double* Uin = U;
+    for (int k = 0; k < 4; ++k) {
+      if (k > 0) {
+        if ((r = timed_hr_oop(s, s.Ustar, s.Ustar2, st)) != DGX_OK) return r;
+        Uin = s.Ustar2;
+      }
+      KernelTimer kt(s, 4, st);
+      if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, s.Ustar, dt, nullptr, st), "tile kernel")) return DGX_EDEVICE;
+    }
+    return timed_hr_oop(s, s.Ustar, U, st);
+  }
   // The tile kernel reads halo traces of neighbouring tiles while other
   // blocks write their stage output, so its stages ping-pong between two
   // buffers (U -> Ustar -> Ustar2 -> Ustar -> U); the face + element path

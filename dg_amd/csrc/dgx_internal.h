@@ -74,6 +74,7 @@ struct Solver {
   int path = PATH_AUTO;
   int precision = 0;  // tile kernel: 0 FP64, 1 FP32-MFMA map/integrate (2D p=2)
   bool tiled() const { return cart.on && path != PATH_GENERAL; }
+  bool hr2d = false;  // 2D tile layout whose neighbour table follows it: one-launch HR limiter (k_hr2d)
   // work buffers
   double* Qe = nullptr;     // face contributions, element-major [((e*N_N+k)*NF+f)*M_s+j]
   double* Uacc = nullptr;   // RK accumulator
@@ -115,6 +116,8 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
                         double* Uout, const double* dt_dev, double* f_out, hipStream_t st);
 hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* dt_dev, hipStream_t st);
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
+// Out = HR(In), both 2D sweeps in one launch; In != Out, needs s.hr2d
+hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st);
 hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st);
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st);
 // HR sweeps on the cells flagged in s.sensors (the HRI sweeps without the sensor)

@@ -168,6 +168,79 @@ __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, cons
   for (int r = 0; r < Ns; ++r) Out[col * Ns + r] = x[r];
 }
 
+// Both sweeps of the 2D global HR limiter in one pass, out of place, on a
+// Cartesian grid (element e = ey*nx + ex): a block owns a tile of TXE x TYE
+// elements, runs the x sweep (Lag2MonoX, hrl along x, MonoX2MonoY) for the
+// tile rows and for the rows just below and above it (found through the
+// neighbour table, so periodic and wall rows need no special case) into
+// LDS, then the y sweep (hrl along y, MonoY2Lag) of the tile rows from LDS.
+// Every column sees exactly the arithmetic of the two k_hr_sweep launches,
+// so the result is bitwise the same; the intermediate state never reaches
+// HBM.  In and Out must differ: neighbouring tiles read In's rows.
+template <int Ns, int N, int SL, int TXE, int TYE>
+__global__ __launch_bounds__(256) void k_hr2d(int nx, int NF, int N_N, const int* __restrict__ nb,
+                                              const double* __restrict__ L2MX, const double* __restrict__ MX2MY,
+                                              const double* __restrict__ MY2L, const double* __restrict__ In,
+                                              double* __restrict__ Out) {
+  constexpr int ROWS = TYE + 2, NFMAX = 5;
+  __shared__ double sB[ROWS * TXE * NFMAX * Ns];
+  const int tiles_x = nx / TXE;
+  const int ex0 = (blockIdx.x % tiles_x) * TXE, ey0 = (blockIdx.x / tiles_x) * TYE;
+  // x sweep: rows 0 (below the tile) .. ROWS-1 (above it)
+  for (int t = threadIdx.x; t < ROWS * TXE * NF; t += blockDim.x) {
+    const int f = t % NF, i = (t / NF) % TXE, r = t / (NF * TXE);
+    int e;
+    if (r == 0) e = nb[(ey0 * nx + ex0 + i) * N_N + 2];
+    else if (r == ROWS - 1) e = nb[((ey0 + TYE - 1) * nx + ex0 + i) * N_N + 3];
+    else e = (ey0 + r - 1) * nx + ex0 + i;
+    if (e < 0) continue;  // a wall below or above: the y sweep does not read this row
+    const long col = (long)e * NF + f;
+    const int left = nb[e * N_N], right = nb[e * N_N + 1];
+    const int physical = hr_physical(left, right);
+    double x[Ns], AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) x[l] = In[col * Ns + l];
+    col_apply<Ns>(L2MX, x, AC);
+    if (physical == 0) {
+      const double* pl = In + ((long)left * NF + f) * Ns;
+      const double* pr = In + ((long)right * NF + f) * Ns;
+#pragma unroll
+      for (int l = 0; l < Ns; ++l) x[l] = pl[l];
+      col_apply<Ns>(L2MX, x, AL);
+#pragma unroll
+      for (int l = 0; l < Ns; ++l) x[l] = pr[l];
+      col_apply<Ns>(L2MX, x, AR);
+    }
+    hrl_column<N, SL>(physical, f, 0, AL, AC, AR, Alim);
+    double* b = sB + ((r * TXE + i) * NF + f) * Ns;
+    col_apply<Ns>(MX2MY, Alim, x);
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) b[l] = x[l];
+  }
+  __syncthreads();
+  // y sweep of the tile rows
+  for (int t = threadIdx.x; t < TYE * TXE * NF; t += blockDim.x) {
+    const int f = t % NF, i = (t / NF) % TXE, j = t / (NF * TXE);
+    const int e = (ey0 + j) * nx + ex0 + i;
+    const int physical = hr_physical(nb[e * N_N + 2], nb[e * N_N + 3]);
+    double AC[Ns], AL[Ns], AR[Ns], Alim[Ns], x[Ns];
+    const double* bc = sB + (((j + 1) * TXE + i) * NF + f) * Ns;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) AC[l] = bc[l];
+    if (physical == 0) {
+      const double* bl = sB + ((j * TXE + i) * NF + f) * Ns;
+      const double* br = sB + (((j + 2) * TXE + i) * NF + f) * Ns;
+#pragma unroll
+      for (int l = 0; l < Ns; ++l) { AL[l] = bl[l]; AR[l] = br[l]; }
+    }
+    hrl_column<N, SL>(physical, f, 2, AL, AC, AR, Alim);
+    col_apply<Ns>(MY2L, Alim, x);
+    double* o = Out + ((long)e * NF + f) * Ns;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) o[l] = x[l];
+  }
+}
+
 // ---------------------------------------------------------------- sensors
 // Uavg = sum_l (1/N_s * U_l), sequential (blasGemm M=1, sensor.cc:29)
 template <int Ns>
@@ -424,6 +497,24 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
                          s.t.MonoY2Lag, s.work, U);
     }
     return hipGetLastError();
+  });
+}
+
+// Out = HR(In) with both 2D sweeps in one launch (k_hr2d), p <= 2; needs the
+// tile layout and a neighbour table that follows it (Solver::hr2d).
+hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st) {
+  if (s.D != 2 || !s.hr2d || In == Out) return hipErrorInvalidValue;
+  constexpr int TXE = 8, TYE = 4;
+  const int tiles = (s.cart.nx / TXE) * (s.cart.ny / TYE);
+  return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    constexpr int D = decltype(d)::value, P = decltype(p)::value;
+    if constexpr (D == 2 && P <= 2) {
+      constexpr int Ns = Shape<D, P>::Ns, SL = P + 1;
+      hipLaunchKernelGGL((k_hr2d<Ns, P, SL, TXE, TYE>), dim3(tiles), dim3(256), 0, st, s.cart.nx, s.NF, s.N_N,
+                         s.t.neighbors, s.t.Lag2MonoX, s.t.MonoX2MonoY, s.t.MonoY2Lag, In, Out);
+      return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
   });
 }
 

@@ -226,8 +226,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   // element phases on the matrix cores (FP64, or FP32 in the mixed-precision mode) or as VALU batches
   constexpr bool MFW = D == 2 && P == 2 && NT % 16 == 0 && (PREC == 1 || DGX_STAGE_MFMA64);
   using TW = typename std::conditional<PREC == 1, float, double>::type;
-  constexpr int NW_FACE = NFACE * MG * 2 * NF, NW_VOL = MFW ? 0 : EPB * SW;
-  constexpr int NW_MF = MFW ? (4 * vol_wave_lds<PH>() * (int)sizeof(TW) + 7) / 8 : 0;
+  constexpr int NWP = NF + PH;  // per face point: F, and the alpha non-conservative term
+  constexpr int NW_FACE = NFACE * MG * NWP, NW_VOL = MFW ? 0 : EPB * SW;
+  constexpr int NW_MF = MFW ? 4 * vol_wave_lds<TW, PH>() : 0;
   constexpr int NW0 = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
   constexpr int NW = NW0 > NW_MF ? NW0 : NW_MF;
   constexpr int NGEO = 1 + D * D;
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   __shared__ double sW[NW];  // face-point fluxes, then per-point volume values
   __shared__ double sPhi[V::NTAB_PHI];
   __shared__ double sDphi[PH ? V::NTAB_D : 1];
-  __shared__ double sDw[V::NTAB_D];
+  __shared__ double sDw[MFW ? 1 : V::NTAB_D];  // the matrix-core path keeps its operand in registers
   __shared__ double sPw[PH ? V::NTAB_PHI : 1];
   __shared__ double sMinv[Ns * Ns];
   __shared__ double sPsi[MG * Ms], sPsiw[MG * Ms];
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 
   const int tid = threadIdx.x;
   const bool shared_minv = a.minv_stride == 0;
-  load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
+  load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid, 256, !MFW);
   for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
   if constexpr (MFW) node_q_table<NF>(sNodeQ, tid);
   const double dt = (MODE == 1) ? a.dt[0] : 0.0;
@@ -404,12 +405,12 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
         ur[f] = sr;
       }
       riemann<D, PH, FL>(ul, ur, nxv, nyv, F, nc);
-      double* w = sW + (fi * MG + g) * 2 * NF;
+      // the non-conservative term is zero except for alpha (multifluid):
+      // keep F and that one term; sides and face Jacobians at integration
+      double* w = sW + (fi * MG + g) * NWP;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        w[f] = (-F[f] + nc[f]) * jf0;
-        w[NF + f] = (F[f] + nc[f]) * jf1;
-      }
+      for (int f = 0; f < NF; ++f) w[f] = F[f];
+      if constexpr (PH == 1) w[NF] = nc[NF - 1];
     }
     __syncthreads();
     // ---- face integration against psi_w into the receiving elements' slots
@@ -422,11 +423,18 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       const int cl = d ? fd.cl[1] : fd.cl[0], edge = d ? fd.edge[1] : fd.edge[0];
       const int jf = (D == 2 && cl >= 4) ? reversed_pos<P>(j) : j;
       double* q = sQ + dst * SQ + edge * NF * Ms;
+      const double jfd = sFG[fi * 4 + 2 + d];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         double s = 0.0;
 #pragma unroll
-        for (int g = 0; g < MG; ++g) s += sPsiw[g + j * MG] * sW[((fi * MG + g) * 2 + d) * NF + f];
+        for (int g = 0; g < MG; ++g) {
+          const double* w = sW + (fi * MG + g) * NWP;
+          const double ncf = (PH == 1 && f == NF - 1) ? w[NF] : 0.0;
+          // q_L = (-F + nc) JF_0, q_R = (F + nc) JF_1 (physics.cu:800-804, kernels.cu:199-228)
+          const double qv = (d ? (w[f] + ncf) : (-w[f] + ncf)) * jfd;
+          s += sPsiw[g + j * MG] * qv;
+        }
         q[f * Ms + jf] = s;
       }
     }
@@ -434,7 +442,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 
     // ---- element phases
     if constexpr (MFW) {
-      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO>(tb, sU, sGeo, sQ, sW, sNodeQ, tid);
+      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO>(tb, a.dphi_w, sU, sGeo, sQ, sW, sNodeQ, tid);
       __syncthreads();
     } else {
     for (int b = 0; b < NT; b += EPB) {

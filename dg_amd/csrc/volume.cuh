@@ -42,12 +42,12 @@ struct VolTables {
 template <int D, int P, int PH>
 __device__ inline void load_vol_tables(const VolTables<D, P, PH>& t, const double* phi, const double* dphi,
                                        const double* phi_w, const double* dphi_w, const double* Minv,
-                                       bool uniform_minv, int tid, int nthr = 256) {
+                                       bool uniform_minv, int tid, int nthr = 256, bool with_dw = true) {
   using V = Vol<D, P, PH>;
   for (int k = tid; k < V::NTAB_PHI; k += nthr) t.phi[k] = phi[k];
   for (int k = tid; k < V::NTAB_D; k += nthr) {
     const int i = k / (V::NG * D), r = k % (V::NG * D);
-    t.dw[r * V::Ns + i] = dphi_w[k];
+    if (with_dw) t.dw[r * V::Ns + i] = dphi_w[k];
     if constexpr (PH == 1) t.dphi[k] = dphi[k];
   }
   if constexpr (PH == 1)
@@ -207,34 +207,39 @@ struct Mfma16<float> {
   }
 };
 
-// T-values of per-wave LDS that vol_mfma_wave needs.
-template <int PH>
-constexpr int vol_wave_lds() { return 16 * (16 * 2 + 1) + (PH ? 4 * 16 : 0); }
+// Doubles of per-wave LDS that vol_mfma_wave needs: the alpha source of four
+// elements (multifluid, FP64), then 16 flux rows of one direction in T
+// (16 points, padded).
+template <typename T, int PH>
+constexpr int vol_wave_lds() { return (PH ? 4 * 16 : 0) + (16 * 17 * (int)sizeof(T) + 7) / 8; }
 
 // Element phases of a tile of NT 2D p=2 elements (256 threads, NT a multiple
 // of 16) on the matrix cores, contractions in T: double for the FP64 path,
 // float for the mixed-precision mode (FP32 inputs and accumulation; the
-// pointwise flux, the face sums and everything after stay FP64).
+// pointwise flux, the alpha terms, the face sums and everything after stay
+// FP64).
 //   map        Ug[(el,f)][g]  = sum_i U[el][f][i] phi[g][i]         fields 0..3
 //   flux       fe[(el,f)][g,al] = J_e sum_k invJac[al][k] f_k(Ug)    in registers
-//   integrate  R[(el,f)][i]   = sum_{g,al} fe[(el,f)][g,al] dphi_w[g,al][i]
+//   integrate  R[(el,f)][i]   = sum_{al} sum_g fe[(el,f)][g,al] dphi_w[g,al][i]
 // Wave w owns elements 4w..4w+3 of every group of 16.  After the map each
 // lane holds the four conserved fields of one element at one Gauss point, so
 // the flux needs no exchange; the scaled fluxes pass through the wave's own
-// LDS rows into the integration, whose result registers plus the face sums
-// are the right-hand side, written to sU.  Multifluid adds alpha and its
-// reference gradient at the point (sums over the 9 nodes in T) and the
-// alpha source integrated against phi_w.  No block barrier: every exchange
-// stays inside the wave.  sWork needs 4 * vol_wave_lds<PH>() T-values.
+// LDS rows, one direction at a time, into the integration, whose result
+// registers plus the face sums are the right-hand side, written to sU.
+// The integration operand (dphi_w, [i*NG*D + g*D + al]) is read from global
+// memory into registers once.  Multifluid adds alpha and its reference
+// gradient at the point (FP64 sums over the 9 nodes) and the alpha source
+// integrated against phi_w (FP64).  No block barrier: every exchange stays
+// inside the wave.  sWork needs 4 * vol_wave_lds<T, PH>() doubles.
 template <typename T, int PH, int NT, int SU, int SQ, int NGEO>
-__device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, double* sU, const double* sGeo, const double* sQ,
-                              double* sWork, const int* sNodeQ, int tid) {
+__device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __restrict__ dphi_w, double* sU,
+                              const double* sGeo, const double* sQ, double* sWork, const int* sNodeQ, int tid) {
   using M = Mfma16<T>;
-  constexpr int NF = 4 + PH, Ns = 9, NG = 16, D = 2, KP = NG * D + 1, Ms = 3;
+  constexpr int NF = 4 + PH, Ns = 9, NG = 16, D = 2, KP = NG + 1, Ms = 3;
   static_assert(NT % 16 == 0, "groups of 16 elements, four per wave");
   const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
-  T* F = reinterpret_cast<T*>(sWork) + w * vol_wave_lds<PH>();
-  T* S = F + 16 * KP;
+  double* S = sWork + w * vol_wave_lds<T, PH>();
+  T* F = reinterpret_cast<T*>(S + (PH ? 4 * NG : 0));
   T bphi[3], bdw[8];
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
@@ -242,7 +247,10 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, double* sU, const do
     bphi[s] = k < Ns ? (T)t.phi[c + k * NG] : T(0);
   }
 #pragma unroll
-  for (int s = 0; s < 8; ++s) bdw[s] = c < Ns ? (T)t.dw[(4 * s + q) * Ns + c] : T(0);
+  for (int s = 0; s < 8; ++s) {  // direction s/4, points 4(s%4)+q
+    const int g = 4 * (s & 3) + q, al = s >> 2;
+    bdw[s] = c < Ns ? (T)dphi_w[c * NG * D + g * D + al] : T(0);
+  }
   const int a_el = M::el(c), a_f = M::field(c);
 #pragma unroll 1
   for (int b = 0; b < NT; b += 16) {
@@ -260,19 +268,20 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, double* sU, const do
     for (int v = 0; v < 4; ++v) ug[v] = (double)acc[v];
     if constexpr (PH == 1) {
       const double* ua = sU + el * SU + 4 * Ns;
-      T al = 0, d0 = 0, d1 = 0;
+      double al = 0, d0 = 0, d1 = 0;
 #pragma unroll
       for (int k = 0; k < Ns; ++k) {
-        const T uk = (T)ua[k];
-        al += (T)t.phi[c + k * NG] * uk;
-        d0 += (T)t.dphi[(c * D + 0) + k * NG * D] * uk;
-        d1 += (T)t.dphi[(c * D + 1) + k * NG * D] * uk;
+        const double uk = ua[k];
+        al += t.phi[c + k * NG] * uk;
+        d0 += t.dphi[(c * D + 0) + k * NG * D] * uk;
+        d1 += t.dphi[(c * D + 1) + k * NG * D] * uk;
       }
       ug[4] = al;
       dua[0] = d0;
       dua[1] = d1;
     }
-    // ---- flux of element el at point g = c with metric terms, into this wave's rows
+    // ---- flux of element el at point g = c with metric terms
+    T fe[4][D];
     {
       const double Je = sGeo[el * NGEO];
       const double* ij = sGeo + el * NGEO + 1;
@@ -285,22 +294,30 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, double* sU, const do
           double sol = 0.0;
 #pragma unroll
           for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
-          F[M::row(q, f) * KP + c * D + al] = (T)sol;
+          fe[f][al] = (T)sol;
         }
-      if constexpr (PH == 1) S[q * NG + c] = (T)(salpha * Je);
+      if constexpr (PH == 1) S[q * NG + c] = salpha * Je;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- integration
+    // ---- integration, one direction at a time through this wave's rows
     typename M::V4 r = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-    for (int s = 0; s < 8; ++s) r = M::mma(F[c * KP + 4 * s + q], bdw[s], r);
-    T sa = 0;
+    for (int al = 0; al < D; ++al) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) F[M::row(q, f) * KP + c] = fe[f][al];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int s = 0; s < 4; ++s) r = M::mma(F[c * KP + 4 * s + q], bdw[4 * al + s], r);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();  // the rows are rewritten next
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    double sa = 0;
     if constexpr (PH == 1) {
       if (c < Ns) {
 #pragma unroll
-        for (int g = 0; g < NG; ++g) sa += S[q * NG + g] * (T)t.pw[g * Ns + c];
+        for (int g = 0; g < NG; ++g) sa += S[q * NG + g] * t.pw[g * Ns + c];
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -316,7 +333,7 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, double* sU, const do
       };
 #pragma unroll
       for (int v = 0; v < 4; ++v) sU[el * SU + v * Ns + c] = (double)r[v] + face(v);
-      if constexpr (PH == 1) sU[el * SU + 4 * Ns + c] = (double)sa + face(4);
+      if constexpr (PH == 1) sU[el * SU + 4 * Ns + c] = sa + face(4);
     }
   }
 }

@@ -124,6 +124,11 @@ dgx_status dgx_set_sensor_thresholds(dgx_solver* s, int n, const double* thresho
 
 /* limiter: 0 none, 1 hrl (global HR), 3 hri (sensor-gated HR), 4 HR gated by the learned indicator */
 dgx_status dgx_limit(dgx_solver* s, int limiter, double* U, void* stream);
+/* Out = limited In (In unchanged; In != Out, both state-sized device arrays).
+ * hrl on a 2D Cartesian tile mesh runs both sweeps in one launch (the same
+ * arithmetic as dgx_limit, bit for bit); otherwise In is copied to Out and
+ * limited in place.  Same limiter codes as dgx_limit. */
+dgx_status dgx_limit_into(dgx_solver* s, int limiter, const double* In, double* Out, void* stream);
 /* SENSOR::sensing: flags (int[N_E], device) and Uavg (double[N_E*N_F], device; may be NULL). */
 dgx_status dgx_sensing(dgx_solver* s, const double* U, int* sensors, double* Uavg, void* stream);
 

@@ -1,8 +1,8 @@
 """Mixed-precision mode of the tile kernel (FP32-MFMA map/integrate).
 
 The map to the Gauss points and the integration run in FP32 on MFMA; the
-fluxes, the Riemann solver, Dt*Minv, the RK4 accumulation and the limiter
-stay FP64.  Bars: one RHS within 2e-6 relative (max norm) of the FP64
+fluxes, the alpha terms, the Riemann solver, Dt*Minv, the RK4 accumulation
+and the limiter stay FP64.  Bars: one RHS within 2e-6 relative (max norm) of the FP64
 oracle; 12 limited RK4 steps within 1e-6 relative L2 per field.
 """
 import numpy as np
@@ -35,6 +35,7 @@ def test_mixed_precision_rhs(device, ph, fl):
         torch.cuda.synchronize()
         err = np.abs(f.cpu().numpy() - f_ref).max() / np.abs(f_ref).max()
         assert err < 2e-6, err
+        assert err > 1e-10, err  # the contractions really ran in FP32
 
 
 def test_mixed_precision_limited_trajectory(device):

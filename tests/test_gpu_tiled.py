@@ -96,6 +96,32 @@ def test_tiled_rk4_trajectory_with_limiter(device, c):
         assert l2 < 1e-11 and li < 1e-11, (l2, li)
 
 
+@pytest.mark.parametrize("ph", [0, MULTIFLUID])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_one_launch_hr_limiter_bitwise(device, ph, order):
+    """dgx_limit_into on a tile mesh (both 2D sweeps in one launch, out of
+    place) equals the oracle's two sweeps and the in-place limiter bit for
+    bit, for every boundary combination, and leaves its input unchanged."""
+    for k, bc in enumerate(BCS_2D):
+        mesh = tiled_mesh(2, order, ph, bc, SIZES_2D[k % 2])
+        prob = oracle.Problem(mesh.tables(), ph, ROE)
+        solver = Solver(Config(2, order, ph, ROE), mesh)
+        U0 = problems.oblique_riemann_state(mesh, ph, seed=30 + k)
+        x = np.arange(U0.size, dtype=np.float64)
+        U0 = U0 * (1.0 + 0.01 * np.sin(0.37 * x + k))  # every mode non-trivial
+        U_ref = prob.limit_hr(U0)
+        assert not np.array_equal(U_ref, U0)
+        Ud = dev(U0, device)
+        out = torch.full_like(Ud, float("nan"))
+        solver.limit_into(LIMIT_HR, Ud, out)
+        assert np.array_equal(host(out), U_ref), (bc, order)
+        assert np.array_equal(host(Ud), U0)
+        solver.limit(LIMIT_HR, Ud)
+        assert np.array_equal(host(Ud), U_ref)
+    with pytest.raises(DgxError):
+        solver.limit_into(LIMIT_HR, Ud, Ud)
+
+
 def test_tiled_hri_cfl_run(device):
     """Sensor-gated limiting with CFL time steps through the tile kernel."""
     d, ph, fl, order = 2, MULTIFLUID, ROE, 2
