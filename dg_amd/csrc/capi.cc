@@ -606,7 +606,8 @@ dgx_status dgx_solver_set_indicator(dgx_solver* h, dgx_mlp* mlp, float threshold
     std::vector<double> vinv;
     if (!legendre_vinv(h->ref_nodes, s.N_s, vinv)) return fail(DGX_EINVAL, "dgx_solver_set_indicator: singular transform");
     if (!upload(s, &s.t.mlp_vinv, vinv.data(), vinv.size()) || !alloc(s, &s.mlp_avg, (size_t)s.N_E * 3) ||
-        !alloc(s, &s.mlp_x, (size_t)s.N_E * 64) || !alloc(s, &s.mlp_z, (size_t)s.N_E))
+        !alloc(s, &s.mlp_x, (size_t)s.N_E * 64) || !alloc(s, &s.mlp_z, (size_t)s.N_E) ||
+        !alloc(s, &s.mlp_jumps, (size_t)s.N_E * 4))
       return DGX_EDEVICE;
   }
   s.mlp = mlp;

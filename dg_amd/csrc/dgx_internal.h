@@ -96,6 +96,7 @@ struct Solver {
   dgx_mlp* mlp = nullptr;
   float mlp_threshold = 0.0f;
   double* mlp_avg = nullptr;  // [e*3 + {rho, p, alpha}] cell means
+  float* mlp_jumps = nullptr;  // [e*4 + n] neighbour jumps L, R, D, U (-1: boundary)
   float* mlp_x = nullptr;     // [e*64 + k] features
   float* mlp_z = nullptr;     // [e] logits
   // event timing (dgx_profile_*): [0] face, [1] elem, [2] limit, [3] dt, [4] stage

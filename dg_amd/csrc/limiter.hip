@@ -451,12 +451,18 @@ __global__ __launch_bounds__(128) void k_hri(int N_E, int NF, int N_N, const int
   for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fc) * Ns + i] = UC[i];
 }
 
+// U[e] = Uold[e] for the flagged elements (copy_detected, sensor_kernels.cu:167-200);
+// a block covers 64 elements and reads each flag once.
 __global__ __launch_bounds__(256) void k_copy_detected(int N_E, int per_elem, const int* __restrict__ sensors,
                                                        const double* __restrict__ Uold, double* __restrict__ U) {
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (long)N_E * per_elem) return;
-  const int e = (int)(tid / per_elem);
-  if (sensors[e] != 0) U[tid] = Uold[tid];
+  __shared__ int sF[64];
+  const long e0 = (long)blockIdx.x * 64;
+  const int ne = (int)min(64L, (long)N_E - e0);
+  if (threadIdx.x < 64) sF[threadIdx.x] = (int)threadIdx.x < ne ? sensors[e0 + threadIdx.x] : 0;
+  __syncthreads();
+  const long base = e0 * per_elem;
+  for (int k = threadIdx.x; k < ne * per_elem; k += 256)
+    if (sF[k / per_elem] != 0) U[base + k] = Uold[base + k];
 }
 
 // ------------------------------------------------------------ host side
@@ -580,8 +586,7 @@ static hipError_t hri(Solver& s, int offxy, const double* L2M, const double* M2L
   });
   if (e != hipSuccess) return e;
   int per = s.NF * s.N_s;
-  hipLaunchKernelGGL(k_copy_detected, dim3(nblk((long)s.N_E * per, 256)), dim3(256), 0, st, s.N_E, per, s.sensors,
-                     s.work, U);
+  hipLaunchKernelGGL(k_copy_detected, dim3(nblk(s.N_E, 64)), dim3(256), 0, st, s.N_E, per, s.sensors, s.work, U);
   return hipGetLastError();
 }
 
@@ -670,7 +675,7 @@ hipError_t compat_uavg(int Ns, const double* U, long ncol, double* Uavg) {
 
 hipError_t compat_copy_detected(int N_s, int N_E, int NF, const int* sensors, const double* Uold, double* U) {
   int per = NF * N_s;
-  hipLaunchKernelGGL(k_copy_detected, dim3(nblk((long)N_E * per, 256)), dim3(256), 0, 0, N_E, per, sensors, Uold, U);
+  hipLaunchKernelGGL(k_copy_detected, dim3(nblk(N_E, 64)), dim3(256), 0, 0, N_E, per, sensors, Uold, U);
   return hipGetLastError();
 }
 

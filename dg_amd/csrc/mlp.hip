@@ -40,6 +40,7 @@ constexpr int kNParam = kB3 + 1;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4m __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
   uint32_t u;
@@ -63,7 +64,18 @@ __global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U,
   constexpr int NF = 5, Ns = 9, NU = NF * Ns;
   __shared__ double sU[64 * NU];
   const int e0 = blockIdx.x * 64, ne = min(64, N_E - e0);
-  for (int k = threadIdx.x; k < ne * NU; k += 256) sU[k] = U[(size_t)e0 * NU + k];
+  constexpr int R = (64 * NU + 255) / 256;
+  double r[R];  // all loads in flight before the first LDS store
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int k = threadIdx.x + 256 * i;
+    r[i] = k < ne * NU ? U[(size_t)e0 * NU + k] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int k = threadIdx.x + 256 * i;
+    if (k < 64 * NU) sU[k] = r[i];
+  }
   __syncthreads();
   if ((int)threadIdx.x >= ne) return;
   const int e = e0 + threadIdx.x;
@@ -88,53 +100,154 @@ __device__ inline double rel_jump(double a, double b) {
   return s > 0.0 ? fabs(a - b) / s : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__ U, const double* __restrict__ Vinv,
-                                                      const double* __restrict__ avg, const int* __restrict__ nb,
-                                                      int N_E, float* __restrict__ X) {
-  constexpr int NF = 5, Ns = 9;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= N_E) return;
-  double u[NF * Ns];
+// Modal coefficients of the 16 elements a wave has staged in LDS (element c
+// at u + c*45): the tensor-Legendre transform m[f][mode] = sum_i
+// Vinv[mode][i] u[f][i] of each field as v_mfma_f64_16x16x4f64 tiles (rows =
+// modes padded to 16, K = nodes padded to 12, columns = elements; result
+// register v of lane l holds mode l/16 + 4v of element l%16).  The modes
+// replace the nodal values in place; the five field scales of element c
+// (|mean| for rho, E and alpha, sqrt(|mean rho| |mean E|) for the momenta,
+// the mean being mode 0) go to inv[c*5 + f] as reciprocals (0 for a zero
+// scale).  Wave-local: no block barrier.
+__device__ inline void wave_modes(double* u, const double* Vinv, double* inv, int lane) {
+  const int c = lane & 15, q = lane >> 4;
+  double a[3];
 #pragma unroll
-  for (int r = 0; r < NF * Ns; ++r) u[r] = U[(size_t)e * NF * Ns + r];
-  double mean[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < Ns; ++i) s += Vinv[i] * u[f * Ns + i];
-    mean[f] = s;
+  for (int s = 0; s < 3; ++s) {
+    const int k = 4 * s + q;
+    a[s] = (c < 9 && k < 9) ? Vinv[c * 9 + k] : 0.0;
   }
-  const double sc_mom = sqrt(fabs(mean[0]) * fabs(mean[3]));
-  float* x = X + (size_t)e * kInP;
+  f64x4m d[5];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    const double sc = (f == 1 || f == 2) ? sc_mom : fabs(mean[f]);
-    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+  for (int f = 0; f < 5; ++f) {
+    f64x4m acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < Ns; ++q) {
-      double s = 0.0;
+    for (int s = 0; s < 3; ++s) {
+      const int k = 4 * s + q;
+      const double b = k < 9 ? u[c * 45 + f * 9 + k] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b, acc, 0, 0, 0);
+    }
+    d[f] = acc;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();  // every lane has read its nodal values
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-      for (int i = 0; i < Ns; ++i) s += Vinv[q * Ns + i] * u[f * Ns + i];
-      double v = s * inv;
-      v = v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0));
-      x[f * Ns + q] = (float)v;
+  for (int f = 0; f < 5; ++f)
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const int mode = q + 4 * v;
+      if (mode < 9) u[c * 45 + f * 9 + mode] = d[f][v];
+    }
+  if (q == 0) {
+    const double sc_mom = sqrt(fabs(d[0][0]) * fabs(d[3][0]));
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+      const double sc = (f == 1 || f == 2) ? sc_mom : fabs(d[f][0]);
+      inv[c * 5 + f] = sc > 0.0 ? 1.0 / sc : 0.0;
     }
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The four neighbour features of every element (L, R, D, U): the largest
+// relative jump of the cell means of rho, p and alpha (avg, k_mlp_means),
+// as the float the network input takes; -1 marks a domain boundary (the
+// feature is 0 there and the cell is flagged by rule).
+__global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ avg, const int* __restrict__ nb, int N,
+                                                   float4* __restrict__ jumps) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  int o[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) o[n] = nb[e * 4 + n];
   const double r0 = avg[e * 3 + 0], p0 = avg[e * 3 + 1], a0 = avg[e * 3 + 2];
+  double an[4][3];
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) an[n][c] = o[n] >= 0 ? avg[o[n] * 3 + c] : 0.0;
+  float jv[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) {
-    const int o = nb[e * 4 + n];
-    double j = 0.0;
-    if (o >= 0) {
-      j = rel_jump(avg[o * 3 + 0], r0);
-      j = fmax(j, rel_jump(avg[o * 3 + 1], p0));
-      j = fmax(j, rel_jump(avg[o * 3 + 2], a0));
-    }
-    x[45 + n] = (float)j;
+    double j = rel_jump(an[n][0], r0);
+    j = fmax(j, rel_jump(an[n][1], p0));
+    j = fmax(j, rel_jump(an[n][2], a0));
+    jv[n] = o[n] < 0 ? -1.0f : (float)j;
   }
+  jumps[e] = make_float4(jv[0], jv[1], jv[2], jv[3]);
+}
+
+__device__ inline bool at_wall(float4 j) { return j.x < 0.f || j.y < 0.f || j.z < 0.f || j.w < 0.f; }
+
+// Feature k of an element from its modes m and scales inv (wave_modes) and
+// its neighbour jumps (k_mlp_jumps): scaled modes clamped to [-8, 8] (NaN to
+// 0), then the four jumps (0 toward a boundary), then zeros.
+__device__ inline float feature_of(int k, const double* m, const double* inv, float4 jmp) {
+  if (k < 45) {
+    const double v = m[k] * inv[k / 9];
+    return (float)(v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0)));
+  }
+  const float j = k == 45 ? jmp.x : k == 46 ? jmp.y : k == 47 ? jmp.z : k == 48 ? jmp.w : 0.0f;
+  return j < 0.0f ? 0.0f : j;
+}
+
+// Staging of 16 elements for a wave: the loads of elements e0 .. e0+ne-1
+// (ne <= 16, absent ones zero) go to registers (issue), then to the wave's
+// LDS rows (commit), so that a group's loads can be in flight while the
+// previous group computes.
+constexpr int kStageR = (16 * 45 + 63) / 64;
+__device__ inline void wave_issue(const double* __restrict__ U, long e0, int ne, double (&r)[kStageR], int lane) {
 #pragma unroll
-  for (int k = kIn; k < kInP; ++k) x[k] = 0.0f;
+  for (int i = 0; i < kStageR; ++i) {
+    const int k = lane + 64 * i;
+    r[i] = k < ne * 45 ? U[(size_t)e0 * 45 + k] : 0.0;
+  }
+}
+__device__ inline void wave_commit(const double (&r)[kStageR], double* u, int lane) {
+#pragma unroll
+  for (int i = 0; i < kStageR; ++i) {
+    const int k = lane + 64 * i;
+    if (k < 16 * 45) u[k] = r[i];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Feature rows X[e][64] for training and inspection: one wave per 16
+// elements, lane (column c, group q) writes the features k = 16s + 4q + j of
+// element c, exactly the values the fused indicator feeds its network.
+__global__ __launch_bounds__(256) void k_mlp_features(const double* __restrict__ U, const double* __restrict__ Vinv,
+                                                      const float4* __restrict__ jumps, int N, float* __restrict__ X) {
+  __shared__ double sU[4][16 * 45];
+  __shared__ double sV[81];
+  __shared__ double sInv[4][16 * 5];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, col = lane & 15, q = lane >> 4;
+  for (int k = threadIdx.x; k < 81; k += 256) sV[k] = Vinv[k];
+  __syncthreads();
+  const int nwaves = gridDim.x * 4;
+  for (int g = blockIdx.x * 4 + wv; g * 16 < N; g += nwaves) {
+    const int e0 = g * 16, ne = min(16, N - e0);
+    const float4 jmp = col < ne ? jumps[e0 + col] : make_float4(0.f, 0.f, 0.f, 0.f);
+    double r[kStageR];
+    wave_issue(U, e0, ne, r, lane);
+    __builtin_amdgcn_wave_barrier();  // the previous group's LDS reads are done
+    wave_commit(r, sU[wv], lane);
+    wave_modes(sU[wv], sV, sInv[wv], lane);
+    if (col < ne) {
+      const int e = e0 + col;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = 16 * s + 4 * q + j;
+          X[(size_t)e * kInP + k] = feature_of(k, &sU[wv][col * 45], &sInv[wv][col * 5], jmp);
+        }
+    }
+  }
 }
 
 // --------------------------------------------------------------- forward
@@ -229,107 +342,84 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const uint16_t* __restrict__ w1
 }
 
 // ------------------------------------------------- fused indicator pass
-// Features and network in one pass (the solver's limiter 4): a wave takes 16
-// consecutive elements, stages their state in LDS with coalesced loads, and
-// lane (column c, group q) builds exactly the 16 features of element c that
-// its MFMA B fragments hold (k = 16s + 4q + j), in FP64, straight from the
-// staged state; the neighbour jumps read the cell means of k_mlp_means.
-// Same features, rounding and network as k_mlp_features + k_mlp_fwd.
-__device__ inline double feature_of(int k, const double* u, const double* Vinv, const double* inv, double r0, double p0,
-                                    double a0, const int* nb, int e, const double* avg) {
-  // inv: the element's five field scales (LDS)
-  if (k < 45) {
-    const int f = k / 9, q = k % 9;
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) s += Vinv[q * 9 + i] * u[f * 9 + i];
-    double v = s * inv[f];
-    return v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0));
-  }
-  if (k < kIn) {
-    const int o = nb[e * 4 + (k - 45)];
-    if (o < 0) return 0.0;
-    double j = rel_jump(avg[o * 3 + 0], r0);
-    j = fmax(j, rel_jump(avg[o * 3 + 1], p0));
-    return fmax(j, rel_jump(avg[o * 3 + 2], a0));
-  }
-  return 0.0;
-}
-
+// Features and network in one pass (the solver's limiter 4): each wave
+// stages 16 consecutive elements, forms their modes on the FP64 matrix
+// cores (wave_modes) and lane (column c, group q) picks up exactly the 16
+// features of element c that its MFMA B fragments hold (k = 16s + 4q + j);
+// the neighbour jumps come from k_mlp_jumps.  Same features, rounding and
+// network as k_mlp_features + k_mlp_fwd.  Waves work
+// independently (no block barrier after the table load).
 __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict__ U, const double* __restrict__ Vinv,
-                                                       const double* __restrict__ avg, const int* __restrict__ nb,
-                                                       int N, const uint16_t* __restrict__ w1b,
+                                                       const float4* __restrict__ jumps, int N,
+                                                       const uint16_t* __restrict__ w1b,
                                                        const uint16_t* __restrict__ w2b,
                                                        const uint16_t* __restrict__ w3b, const float* __restrict__ P,
                                                        float threshold, float* __restrict__ z,
                                                        int* __restrict__ flags) {
-  constexpr int NU = 45;
-  __shared__ double sU[4][16 * NU];
+  __shared__ double sU[4][16 * 45];
   __shared__ double sV[81];
-  __shared__ double sInv[64][5];
-  __shared__ double sAvg[64][3];
+  __shared__ double sInv[4][16 * 5];
+  // the layers' A fragments per lane ([t*4 + s][lane], 8 bytes: conflict-free
+  // reads) and the biases and output weights: LDS, not registers, so that
+  // three waves fit on a SIMD
+  __shared__ bf16x4 sA1[16][64], sA2[16][64];
+  __shared__ float sB1[kH], sB2[kH], sW3[kH];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, q = lane >> 4;
   for (int k = threadIdx.x; k < 81; k += 256) sV[k] = Vinv[k];
-  bf16x4 a1[4][4], a2[4][4];
+  if (wv == 0) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < 4; ++s) {
+        bf16x4 f1, f2;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        a1[t][s][j] = (short)w1b[(16 * t + col) * kInP + 16 * s + 4 * q + j];
-        a2[t][s][j] = (short)w2b[(16 * t + col) * kH + 16 * s + 4 * q + j];
+        for (int j = 0; j < 4; ++j) {
+          f1[j] = (short)w1b[(16 * t + col) * kInP + 16 * s + 4 * q + j];
+          f2[j] = (short)w2b[(16 * t + col) * kH + 16 * s + 4 * q + j];
+        }
+        sA1[t * 4 + s][lane] = f1;
+        sA2[t * 4 + s][lane] = f2;
       }
-  float b1v[4][4], b2v[4][4], w3v[4][4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int u = 16 * t + 4 * q + v;
-      b1v[t][v] = P[kB1 + u];
-      b2v[t][v] = P[kB2 + u];
-      w3v[t][v] = bf2f(w3b[u]);
-    }
+  }
+  for (int k = threadIdx.x; k < kH; k += 256) {
+    sB1[k] = P[kB1 + k];
+    sB2[k] = P[kB2 + k];
+    sW3[k] = bf2f(w3b[k]);
+  }
   const float b3 = P[kB3];
-  // block-uniform trip count (64 elements per block and iteration), so the
-  // block barriers below are reached by every wave
-  for (int G = blockIdx.x; G * 64 < N; G += gridDim.x) {
-    const int e0 = G * 64 + wv * 16, ne = max(0, min(16, N - e0));
-    __syncthreads();  // previous iteration's reads of sU are done
-    for (int k = lane; k < ne * NU; k += 64) sU[wv][k] = U[(size_t)e0 * NU + k];
+  __syncthreads();
+  const int nwaves = gridDim.x * 4;
+  // software pipeline: group g + nwaves is loading while group g computes
+  int g = blockIdx.x * 4 + wv;
+  double r[kStageR];
+  float4 jn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (g * 16 < N) {
+    const int ne0 = min(16, N - g * 16);
+    wave_issue(U, (long)g * 16, ne0, r, lane);
+    if (col < ne0) jn = jumps[g * 16 + col];
+  }
+  for (; g * 16 < N; g += nwaves) {
+    const int e0 = g * 16, ne = min(16, N - e0);
     const int e = e0 + col;
     const bool valid = col < ne;
-    if (q == 0 && valid) {  // per-element scales and cell means, once per element
-      const double* u = &sU[wv][col * NU];
-      double mean[5];
-#pragma unroll
-      for (int f = 0; f < 5; ++f) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) s += sV[i] * u[f * 9 + i];
-        mean[f] = s;
-      }
-      const double sc_mom = sqrt(fabs(mean[0]) * fabs(mean[3]));
-#pragma unroll
-      for (int f = 0; f < 5; ++f) {
-        const double sc = (f == 1 || f == 2) ? sc_mom : fabs(mean[f]);
-        sInv[wv * 16 + col][f] = sc > 0.0 ? 1.0 / sc : 0.0;
-      }
-#pragma unroll
-      for (int c3 = 0; c3 < 3; ++c3) sAvg[wv * 16 + col][c3] = avg[e * 3 + c3];
+    const float4 jmp = jn;
+    __builtin_amdgcn_wave_barrier();  // the previous group's LDS reads are done
+    wave_commit(r, sU[wv], lane);
+    const int gn = g + nwaves;
+    if (gn * 16 < N) {
+      const int nen = min(16, N - gn * 16);
+      wave_issue(U, (long)gn * 16, nen, r, lane);
+      jn = col < nen ? jumps[gn * 16 + col] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    __syncthreads();
+    wave_modes(sU[wv], sV, sInv[wv], lane);
     bf16x4 bx[4];
     if (valid) {
-      const double* u = &sU[wv][col * NU];
-      const double* inv = sInv[wv * 16 + col];
-      const double r0 = sAvg[wv * 16 + col][0], p0 = sAvg[wv * 16 + col][1], a0 = sAvg[wv * 16 + col][2];
 #pragma unroll
       for (int s = 0; s < 4; ++s)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          bx[s][j] = (short)f2bf((float)feature_of(16 * s + 4 * q + j, u, sV, inv, r0, p0, a0, nb, e, avg));
+          bx[s][j] = (short)f2bf(feature_of(16 * s + 4 * q + j, &sU[wv][col * 45], &sInv[wv][col * 5], jmp));
     } else {
 #pragma unroll
       for (int s = 0; s < 4; ++s)
@@ -341,18 +431,21 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
     for (int t = 0; t < 4; ++t) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[t][s], bx[s], acc, 0, 0, 0);
+      for (int s = 0; s < 4; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(sA1[t * 4 + s][lane], bx[s], acc, 0, 0, 0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) bh1[t][v] = (short)f2bf(fmaxf(acc[v] + b1v[t][v], 0.0f));
+      for (int v = 0; v < 4; ++v) bh1[t][v] = (short)f2bf(fmaxf(acc[v] + sB1[16 * t + 4 * q + v], 0.0f));
     }
     float part = 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t][s], bh1[s], acc, 0, 0, 0);
+      for (int s = 0; s < 4; ++s)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(sA2[t * 4 + s][lane], bh1[s], acc, 0, 0, 0);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) part += w3v[t][v] * bf2f(f2bf(fmaxf(acc[v] + b2v[t][v], 0.0f)));
+      for (int v = 0; v < 4; ++v)
+        part += sW3[16 * t + 4 * q + v] * bf2f(f2bf(fmaxf(acc[v] + sB2[16 * t + 4 * q + v], 0.0f)));
     }
     const float p1 = __shfl_xor(part, 16);
     const float s01 = (q & 1) ? p1 + part : part + p1;
@@ -363,8 +456,7 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
       if (z) z[e] = zz;
       // cells next to a domain boundary are flagged by rule, as by the
       // sensor (a boundary neighbour flags the cell, sensor_kernels.cu:19-163)
-      const bool wall = nb[e * 4 + 0] < 0 || nb[e * 4 + 1] < 0 || nb[e * 4 + 2] < 0 || nb[e * 4 + 3] < 0;
-      flags[e] = (wall || zz > threshold) ? 1 : 0;
+      flags[e] = (at_wall(jmp) || zz > threshold) ? 1 : 0;
     }
   }
 }
@@ -596,17 +688,23 @@ hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st) {
   dgx_mlp* m = s.mlp;
   const int N = s.N_E;
   hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
+                     reinterpret_cast<float4*>(s.mlp_jumps));
   const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
-  hipLaunchKernelGGL(k_mlp_indicator, dim3(nb), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg, s.t.neighbors, N,
-                     m->w1b, m->w2b, m->w3b, m->P, s.mlp_threshold, s.mlp_z, s.sensors);
+  hipLaunchKernelGGL(k_mlp_indicator, dim3(nb), dim3(256), 0, st, U, s.t.mlp_vinv,
+                     reinterpret_cast<const float4*>(s.mlp_jumps), N, m->w1b, m->w2b, m->w3b, m->P, s.mlp_threshold,
+                     s.mlp_z, s.sensors);
   return hipGetLastError();
 }
 
 hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st) {
   const int N = s.N_E;
   hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
-  hipLaunchKernelGGL(k_mlp_features, dim3(nblocks_for(N, 256)), dim3(256), 0, st, U, s.t.mlp_vinv, s.mlp_avg,
-                     s.t.neighbors, N, X);
+  hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
+                     reinterpret_cast<float4*>(s.mlp_jumps));
+  const unsigned nbf = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
+  hipLaunchKernelGGL(k_mlp_features, dim3(nbf), dim3(256), 0, st, U, s.t.mlp_vinv,
+                     reinterpret_cast<const float4*>(s.mlp_jumps), N, X);
   return hipGetLastError();
 }
 

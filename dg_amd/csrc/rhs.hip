@@ -171,11 +171,21 @@ __global__ __launch_bounds__(256) void k_elem(ElemArgs a) {
   const int n_el = min(EPB, a.N_E - e0);
   const bool uniform_minv = a.minv_stride == 0;
   load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && uniform_minv, tid);
-  {
+  {  // every load in flight before the first LDS store
+    constexpr int RU = (EPB * NU + 255) / 256, RQ = (EPB * NQE + 255) / 256;
     const double* src = a.Uin + (size_t)e0 * NU;
-    for (int k = tid; k < n_el * NU; k += 256) sU[k] = src[k];
     const double* q = a.Qe + (size_t)e0 * NQE;
-    for (int k = tid; k < n_el * NQE; k += 256) sQ[k] = q[k];
+    double ru[RU], rq[RQ];
+#pragma unroll
+    for (int i = 0; i < RU; ++i) ru[i] = tid + 256 * i < n_el * NU ? src[tid + 256 * i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < RQ; ++i) rq[i] = tid + 256 * i < n_el * NQE ? q[tid + 256 * i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < RU; ++i)
+      if (tid + 256 * i < n_el * NU) sU[tid + 256 * i] = ru[i];
+#pragma unroll
+    for (int i = 0; i < RQ; ++i)
+      if (tid + 256 * i < n_el * NQE) sQ[tid + 256 * i] = rq[i];
   }
   __syncthreads();
 

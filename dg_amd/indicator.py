@@ -7,6 +7,8 @@ steps with the global HR limiter on the GPU, and the reference sensor
 (calc_sensors semantics, `Solver.sensing`) labels every cell at a few
 snapshots.  Only cells away from the sub-square borders are kept, so the
 artificial jumps between neighbouring problems never enter the data.
+Random multimode interfaces (some hit by a shock) and smooth random flows,
+labelled the same way, complete the data.
 Features come from `Solver.features`; training runs on the GPU kernels of
 dg_amd/csrc/mlp.hip (`Indicator.train`).
 """
@@ -17,7 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (FARFIELD, LIMIT_HR, MULTIFLUID, ROE, Config, DgxError, Indicator, Mesh, Solver, problems)
+from . import (FARFIELD, LIMIT_HR, MULTIFLUID, PERIODIC, ROE, Config, DgxError, Indicator, Mesh, Solver, problems)
 
 SENSOR_THRESHOLDS = (0.01, 0.01, 0.01)  # "#sensor thresholds" of the data and of the judge
 SEED = 20261202
@@ -162,6 +164,47 @@ def interface_data(n_problems: int = 32, nx: int = 128, ny: int = 64, snapshots=
             sel = np.concatenate([pos, neg])
             out_X.append(Xh[sel])
             out_s.append(sh[sel])
+    Xa = np.concatenate(out_X)
+    sa = np.concatenate(out_s).astype(np.int32)
+    return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
+
+
+def smooth_data(n_problems: int = 16, nx: int = 64, ny: int = 32, snapshots=(0, 5), amp_range=(0.005, 0.1),
+                seed: int = SEED + 2, cfl: float = 0.3, device="cuda") -> Dataset:
+    """Smooth random two-fluid flows on a periodic box (seeded waves of random
+    amplitude in all fields and in gamma), evolved without limiting and
+    labelled by the sensor: the smooth cells (most of them) teach the network
+    that higher modes alone do not make a troubled cell."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    mesh = Mesh(2, 2, nx, ny, 0.0, 2.0, 0.0, 1.0, bc=(PERIODIC,) * 4, nfields=5)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_sensor_thresholds(SENSOR_THRESHOLDS)
+    solver.set_indicator(Indicator(seed=0), 0.0)
+    NE = mesh.sizes["N_E"]
+    X = torch.zeros((NE, 64), dtype=torch.float32, device=device)
+    sens = torch.zeros(NE, dtype=torch.int32, device=device)
+    out_X, out_s = [], []
+    for _ in range(n_problems):
+        amp = float(np.exp(rng.uniform(np.log(amp_range[0]), np.log(amp_range[1]))))
+        U = torch.from_numpy(problems.smooth_random_state(mesh, MULTIFLUID, seed=int(rng.integers(1 << 30)),
+                                                          amp=amp)).to(device)
+        done = 0
+        for snap in snapshots:
+            if snap > done:
+                try:
+                    solver.rk4_integrate(U, 1, cfl * mesh.dx / 5, nsteps=snap - done, tf=1e30, limiter=0)
+                except DgxError:
+                    break
+                done = snap
+            solver.features(U, X)
+            solver.sensing(U, sens)
+            torch.cuda.synchronize()
+            Xh, sh = X.cpu().numpy(), sens.cpu().numpy()
+            fin = np.isfinite(Xh).all(axis=1)
+            out_X.append(Xh[fin])
+            out_s.append(sh[fin])
     Xa = np.concatenate(out_X)
     sa = np.concatenate(out_s).astype(np.int32)
     return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def trained(device):
-    data = indicator.merge(indicator.generate(n_sub=32, sub=8), indicator.interface_data(8))
+    data = indicator.merge(indicator.generate(n_sub=32, sub=8), indicator.interface_data(8), indicator.smooth_data(8))
     net, losses, thr = indicator.train(data, epochs=30)
     assert losses[-1] < 0.2 * losses[0], losses
     return net, thr
