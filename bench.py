@@ -47,6 +47,9 @@ def main() -> int:
                     help="tile-kernel volume contractions: FP64, or FP32 on MFMA (2D p=2; the rest stays FP64)")
     ap.add_argument("--path", choices=["auto", "general", "tiled"], default="auto",
                     help="kernel path: fused tile kernel (auto on this mesh) or face + element kernels")
+    ap.add_argument("--limiter", choices=["auto", "none", "hr", "hri", "learned"], default="auto",
+                    help="limiter between stages: the workload's own (auto), none, global HR, sensor-gated HR, or HR "
+                         "gated by the learned indicator (trained on the GPU before the timed run; 2D two-fluid p=2)")
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="CPU sample steps (0: workload default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB buffer 5 times (HBM counter calibration for rocprofv3 --pmc)")
@@ -100,13 +103,29 @@ def main() -> int:
     solver.dt_from_cfl(U, cfl, dt_dev, stream=stream)
     torch.cuda.synchronize()
     dt = float(dt_dev.item())
-    lim = w["limiter"]
+    lim = {"auto": w["limiter"], "none": 0, "hr": 1, "hri": 3, "learned": 4}[args.limiter]
+    extra = {}
+    net = None
+    if lim in (3, 4):
+        from dg_amd import indicator
+        solver.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
+    if lim == 4:  # train the indicator on the GPU (own forward/backward kernels), outside the timed region
+        t_tr = time.perf_counter()
+        data = indicator.merge(indicator.generate(), indicator.interface_data(), indicator.smooth_data())
+        net, losses, thr = indicator.train(data, epochs=40)
+        solver.set_indicator(net, thr)
+        extra["indicator"] = {"training_s": round(time.perf_counter() - t_tr, 2), "samples": int(data.y.size),
+                              "final_loss": float(losses[-1]), "threshold": thr}
 
-    def advance(k):
-        if w["cfl_steps"]:
-            solver.rk4_integrate(U, 1, cfl, nsteps=k, tf=1e30, limiter=lim, stream=stream)
-        else:
-            solver.rk4_integrate(U, 0, dt, nsteps=k, limiter=lim, stream=stream)
+    def make_advance(sv, Ux, lim_x):
+        def adv(k):
+            if w["cfl_steps"]:
+                sv.rk4_integrate(Ux, 1, cfl, nsteps=k, tf=1e30, limiter=lim_x, stream=stream)
+            else:
+                sv.rk4_integrate(Ux, 0, dt, nsteps=k, limiter=lim_x, stream=stream)
+        return adv
+
+    advance = make_advance(solver, U, lim)
 
     # warmup, then exactly K timed steps bracketed by device synchronisation
     advance(args.warmup)
@@ -119,7 +138,49 @@ def main() -> int:
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_dof * args.steps / elapsed
 
-    extra = {}
+    if lim in (3, 4):  # the plain sensor judges the flags of the final state
+        from dg_amd import indicator
+        sens = torch.zeros(N_E, dtype=torch.int32, device=device)
+        solver.sensing(U, sens)
+        if lim == 4:
+            fl = torch.zeros_like(sens)
+            solver.indicator_flags(U, fl)
+            torch.cuda.synchronize()
+            extra["indicator"]["flags_vs_sensor_final_state"] = indicator.confusion(fl.cpu().numpy(),
+                                                                                     sens.cpu().numpy())
+        else:
+            torch.cuda.synchronize()
+            extra["sensor_flagged_final_state"] = int((sens.cpu().numpy() != 0).sum())
+    if args.precision == "fp32mfma" or lim in (3, 4):
+        # the plain FP64 GPU form (global HR) on the same workload and steps, timed here too
+        sp = Solver(cfg, mesh)
+        Up = torch.from_numpy(U0).to(device)
+        adv_p = make_advance(sp, Up, w["limiter"])
+        adv_p(args.warmup)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        adv_p(args.steps)
+        torch.cuda.synchronize()
+        ms_plain = 1e3 * (time.perf_counter() - tp) / args.steps
+        extra["plain_fp64_form"] = {"limiter": {0: "none", 1: "hr"}.get(w["limiter"], w["limiter"]),
+                                    "ms_per_step": ms_plain, "speedup_of_this_form": ms_plain / ms_per_step}
+    if args.precision == "fp32mfma":
+        # precision error: the same limiter in FP64 from the same state, warmup + K steps
+        sf = Solver(cfg, mesh)
+        if lim in (3, 4):
+            sf.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
+        if lim == 4:
+            sf.set_indicator(net, extra["indicator"]["threshold"])
+        Uf = torch.from_numpy(U0).to(device)
+        make_advance(sf, Uf, lim)(args.warmup + args.steps)
+        torch.cuda.synchronize()
+        a64 = Uf.cpu().numpy().reshape(N_E, NF, N_s)
+        a32 = U.cpu().numpy().reshape(N_E, NF, N_s)
+        nrm = float(np.linalg.norm(a64))
+        extra["rel_l2_vs_fp64_same_limiter"] = {
+            "state": float(np.linalg.norm(a32 - a64) / nrm),
+            "fields_rel_to_state_norm": [float(np.linalg.norm(a32[:, f] - a64[:, f]) / nrm) for f in range(NF)]}
+
     if args.config == "vortex":  # accuracy against the exact translated vortex after warmup + K steps
         from dg_amd import problems
         t_final = (args.warmup + args.steps) * dt
@@ -160,7 +221,9 @@ def main() -> int:
     # whole-step view: minimum RK4 state traffic per step, in state vectors:
     # 16 for the fused stage flow, plus 2 per limiter sweep (read + write) for
     # each of the 4 limiter applications (1 sweep in 1D, 2 in 2D)
-    c = 16 + (0 if not lim else 4 * 2 * w["dim"])
+    # (flag-gated limiters: one state read per application for the sensor or
+    # indicator, the flagged cells' sweeps not counted)
+    c = 16 + (0 if not lim else 4 * 2 * w["dim"] if lim == 1 else 4)
     step_gbps = c * 8.0 * n_dof / (ms_per_step * 1e-3) / 1e9
 
     out = {
@@ -180,6 +243,7 @@ def main() -> int:
             "workload": w["name"],
             "elements": N_E, "dofs": n_dof, "dt": dt, "parallelism": "single GPU",
             "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
+            "limiter": {0: "none", 1: "global HR", 3: "sensor-gated HR", 4: "HR gated by the learned indicator"}[lim],
             "contractions": ("FP32 MFMA (FP64 fluxes, RK and limiter)" if args.precision == "fp32mfma" else
                              "FP64 MFMA element phases" if tiled and w["physics"] == 0 and w["order"] == 2 and w["dim"] == 2
                              else "FP64"),

@@ -86,7 +86,8 @@ struct Solver {
   int* sensors = nullptr;   // [e]
   int* sens_tgt = nullptr;  // sensor resolution scratch [e*2]
   int* sens_act = nullptr;  // [e*2] (ping-pong)
-  int* flag = nullptr;      // device flag / counters
+  int* flag = nullptr;      // device flag / counters: [0] sensor relaxation, [2] flagged-element count
+  int* flag_list = nullptr; // [N_E] flagged elements (launch_limit_flagged)
   double* dscratch = nullptr;  // device scalars: [1] Dt
   double* dt_partial = nullptr;  // per-block maxima of |u|+a (ceil(N_E*N_s/256))
   int n_thresh = 0;

@@ -369,18 +369,15 @@ __global__ __launch_bounds__(256) void k_sensor_final(int N_E, int N_N, const in
   sensors[x] = bval;
 }
 
-// hri1D: one thread per (element, field); only flagged elements.
+// hri1D (limiting_kernels.cu:322-427) for one column (element, field) of a
+// flagged element: HR in Lagrange space through Lag2Mono / Mono2Lag, or the
+// boundary rules; Unew receives the column.
 template <int D, int P>
-__global__ __launch_bounds__(128) void k_hri(int N_E, int NF, int N_N, const int* __restrict__ nb, int offxy,
-                                             const double* __restrict__ L2M, const double* __restrict__ M2L,
-                                             const int* __restrict__ sensors, const double* __restrict__ U,
-                                             double* __restrict__ Unew) {
+__device__ inline void hri_column(int e, int fc, int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                  const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                  const double* __restrict__ U, double* __restrict__ Unew) {
   constexpr int Ns = Shape<D, P>::Ns, N1 = P + 1, N = P;
   constexpr int slicenum = (D == 1) ? 1 : N1;
-  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= (long)N_E * NF) return;
-  const int e = (int)(tid / NF), fc = (int)(tid % NF);
-  if (sensors[e] == 0) return;
   const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
   int physical = 0;
   if (left < 0) physical = -left;
@@ -449,6 +446,62 @@ __global__ __launch_bounds__(128) void k_hri(int N_E, int NF, int N_N, const int
   }
 #pragma unroll
   for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fc) * Ns + i] = UC[i];
+}
+
+// one thread per (element, field); unflagged elements return at once
+template <int D, int P>
+__global__ __launch_bounds__(128) void k_hri(int N_E, int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                             const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                             const int* __restrict__ sensors, const double* __restrict__ U,
+                                             double* __restrict__ Unew) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= (long)N_E * NF) return;
+  const int e = (int)(tid / NF), fc = (int)(tid % NF);
+  if (sensors[e] == 0) return;
+  hri_column<D, P>(e, fc, NF, N_N, nb, offxy, L2M, M2L, U, Unew);
+}
+
+// The same over a list of flagged elements (k_flag_list): a fixed grid
+// strides over list[0 .. count) x fields.
+template <int D, int P>
+__global__ __launch_bounds__(128) void k_hri_list(int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                                  const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                                  const int* __restrict__ list, const int* __restrict__ count,
+                                                  const double* __restrict__ U, double* __restrict__ Unew) {
+  const long n = (long)count[0] * NF;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    hri_column<D, P>(list[t / NF], (int)(t % NF), NF, N_N, nb, offxy, L2M, M2L, U, Unew);
+}
+
+// list[0 .. count) = the flagged elements (sensors[e] != 0), in no fixed
+// order (each is limited independently); count must be zero on entry.  One
+// atomic add per block, on a global counter.
+__global__ __launch_bounds__(256) void k_flag_list(int N_E, const int* __restrict__ sensors, int* __restrict__ list,
+                                                   int* __restrict__ count) {
+  __shared__ int sN, sBase;
+  __shared__ int sIdx[256];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (threadIdx.x == 0) sN = 0;
+  __syncthreads();
+  const bool f = e < N_E && sensors[e] != 0;
+  int slot = -1;
+  if (f) slot = atomicAdd(&sN, 1);
+  if (f) sIdx[slot] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) sBase = sN ? atomicAdd(count, sN) : 0;
+  __syncthreads();
+  if ((int)threadIdx.x < sN) list[sBase + threadIdx.x] = sIdx[threadIdx.x];
+}
+
+// U[e] = Uold[e] for the listed elements
+__global__ __launch_bounds__(256) void k_copy_list(int per_elem, const int* __restrict__ list,
+                                                   const int* __restrict__ count, const double* __restrict__ Uold,
+                                                   double* __restrict__ U) {
+  const long n = (long)count[0] * per_elem;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const long i = (long)list[t / per_elem] * per_elem + t % per_elem;
+    U[i] = Uold[i];
+  }
 }
 
 // U[e] = Uold[e] for the flagged elements (copy_detected, sensor_kernels.cu:167-200);
@@ -577,24 +630,32 @@ hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg
   return sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st);
 }
 
-static hipError_t hri(Solver& s, int offxy, const double* L2M, const double* M2L, double* U, hipStream_t st) {
-  long n = (long)s.N_E * s.NF;
+// One sweep of the flagged HR over the compacted list (see launch_limit_flagged).
+static hipError_t hri_list(Solver& s, int offxy, const double* L2M, const double* M2L, double* U, hipStream_t st) {
+  const int* count = s.flag + 2;
   hipError_t e = by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
-    hipLaunchKernelGGL((k_hri<decltype(d)::value, decltype(p)::value>), dim3(nblk(n, 128)), dim3(128), 0, st, s.N_E,
-                       s.NF, s.N_N, s.t.neighbors, offxy, L2M, M2L, s.sensors, U, s.work);
+    hipLaunchKernelGGL((k_hri_list<decltype(d)::value, decltype(p)::value>), dim3(2048), dim3(128), 0, st, s.NF,
+                       s.N_N, s.t.neighbors, offxy, L2M, M2L, s.flag_list, count, U, s.work);
     return hipGetLastError();
   });
   if (e != hipSuccess) return e;
-  int per = s.NF * s.N_s;
-  hipLaunchKernelGGL(k_copy_detected, dim3(nblk(s.N_E, 64)), dim3(256), 0, st, s.N_E, per, s.sensors, s.work, U);
+  hipLaunchKernelGGL(k_copy_list, dim3(1024), dim3(256), 0, st, s.NF * s.N_s, s.flag_list, count, s.work, U);
   return hipGetLastError();
 }
 
+// HR sweeps on the cells flagged in s.sensors: the flagged elements are
+// listed once (k_flag_list), then each sweep and its copy-back run over the
+// list only, so the cost follows the number of flagged cells.  The same
+// arithmetic per column as k_hri over the whole grid.
 hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag + 2, 1L, 0);
+  hipLaunchKernelGGL(k_flag_list, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.sensors, s.flag_list,
+                     s.flag + 2);
   hipError_t e;
-  if (s.D == 1) return hri(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
-  if ((e = hri(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
-  return hri(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (s.D == 1) return hri_list(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
+  if ((e = hri_list(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
+  return hri_list(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);
 }
 
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {
@@ -642,7 +703,8 @@ hipError_t compat_sensors(Solver& tmp, const int* nb, const double* Uavg, int* s
   if (e != hipSuccess) return e;
   tmp.sens_act = scratch + 2 * tmp.N_E;
   int* init = scratch + 4 * tmp.N_E;
-  hipMemcpyAsync(init, sensors, sizeof(int) * tmp.N_E, hipMemcpyDeviceToDevice, 0);
+  hipError_t ec = hipMemcpyAsync(init, sensors, sizeof(int) * tmp.N_E, hipMemcpyDeviceToDevice, 0);
+  if (ec != hipSuccess) return ec;
   return sensor_resolve(tmp, nb, init, tgt, val, sensors, 0);
 }
 

@@ -213,6 +213,7 @@ struct Mfma16<float> {
 template <typename T, int PH>
 constexpr int vol_wave_lds() { return (PH ? 4 * 16 : 0) + (16 * 17 * (int)sizeof(T) + 7) / 8; }
 
+
 // Element phases of a tile of NT 2D p=2 elements (256 threads, NT a multiple
 // of 16) on the matrix cores, contractions in T: double for the FP64 path,
 // float for the mixed-precision mode (FP32 inputs and accumulation; the
@@ -231,37 +232,43 @@ constexpr int vol_wave_lds() { return (PH ? 4 * 16 : 0) + (16 * 17 * (int)sizeof
 // gradient at the point (FP64 sums over the 9 nodes) and the alpha source
 // integrated against phi_w (FP64).  No block barrier: every exchange stays
 // inside the wave.  sWork needs 4 * vol_wave_lds<T, PH>() doubles.
-template <typename T, int PH, int NT, int SU, int SQ, int NGEO>
+template <typename T, int PH, int NT, int SU, int SQ, int NGEO, typename TM = T>
 __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __restrict__ dphi_w, double* sU,
                               const double* sGeo, const double* sQ, double* sWork, const int* sNodeQ, int tid) {
+  // TM: type of the map (T unless set), T: type of the integration.  On gfx950
+  // the FP32 16x16x4 integration measured slower than the FP64 one (volume
+  // phase of a 2D p=2 tile: 11.0k against 7.3k cycles), the FP32 map about
+  // as fast as the FP64 map.
   using M = Mfma16<T>;
+  using MM = Mfma16<TM>;
   constexpr int NF = 4 + PH, Ns = 9, NG = 16, D = 2, KP = NG + 1, Ms = 3;
   static_assert(NT % 16 == 0, "groups of 16 elements, four per wave");
   const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
   double* S = sWork + w * vol_wave_lds<T, PH>();
   T* F = reinterpret_cast<T*>(S + (PH ? 4 * NG : 0));
-  T bphi[3], bdw[8];
+  TM bphi[3];
+  T bdw[8];
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const int k = 4 * s + q;
-    bphi[s] = k < Ns ? (T)t.phi[c + k * NG] : T(0);
+    bphi[s] = k < Ns ? (TM)t.phi[c + k * NG] : TM(0);
   }
 #pragma unroll
   for (int s = 0; s < 8; ++s) {  // direction s/4, points 4(s%4)+q
     const int g = 4 * (s & 3) + q, al = s >> 2;
     bdw[s] = c < Ns ? (T)dphi_w[c * NG * D + g * D + al] : T(0);
   }
-  const int a_el = M::el(c), a_f = M::field(c);
+  const int a_el = MM::el(c), a_f = MM::field(c);
 #pragma unroll 1
   for (int b = 0; b < NT; b += 16) {
     const int e_base = b + 4 * w, el = e_base + q;
     // ---- map
-    typename M::V4 acc = {T(0), T(0), T(0), T(0)};
+    typename MM::V4 acc = {TM(0), TM(0), TM(0), TM(0)};
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
       const int k = 4 * s + q;
-      const T a = k < Ns ? (T)sU[(e_base + a_el) * SU + a_f * Ns + k] : T(0);
-      acc = M::mma(a, bphi[s], acc);
+      const TM a = k < Ns ? (TM)sU[(e_base + a_el) * SU + a_f * Ns + k] : TM(0);
+      acc = MM::mma(a, bphi[s], acc);
     }
     double ug[NF], dua[D] = {0.0, 0.0};
 #pragma unroll
