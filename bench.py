@@ -250,7 +250,9 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": (measured_traffic(args.config, dominant)
+                        if args.precision == "fp64" and args.path != "general" and args.n in (0, 512) else None),
             "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg,
             "kernels_ms": avg,
         },
@@ -324,6 +326,28 @@ def bench_ensemble(args, device):
                                "sample": f"16 of the problems, 20 RK4 steps each, FP64 oracle, {t_cpu:.1f} s"}
     print(json.dumps(out))
     return 0
+
+
+def measured_traffic(config: str, kernel: str):
+    """HBM bytes per dispatch of `kernel` on `config` from the committed
+    rocprofv3 --pmc table (profiles/traffic.txt: FETCH_SIZE and WRITE_SIZE in
+    KiB with their copy-calibrated scales), or None when it has no row."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    scale, rows = {}, {}
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] == "scale":
+            scale[f[1]] = float(f[2])
+        else:
+            rows[(f[0], f[1])] = (float(f[2]), float(f[3]))
+    if (config, kernel) not in rows or len(scale) != 2:
+        return None
+    fetch, write = rows[(config, kernel)]
+    return 1024.0 * (scale["FETCH_SIZE"] * fetch + scale["WRITE_SIZE"] * write)
 
 
 def workload(args):
