@@ -61,6 +61,8 @@ def lib() -> ctypes.CDLL:
         L.dgx_rhs.argtypes = [vp, vp, vp, vp]
         L.dgx_set_sensor_thresholds.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double)]
         L.dgx_limit.argtypes = [vp, i, vp, vp]
+        L.dgx_mesh_read_gmsh.argtypes = [ctypes.c_char_p, i, vp]
+        L.dgx_mesh_write_gmsh.argtypes = [vp, ctypes.c_char_p]
         L.dgx_limit_into.argtypes = [vp, i, vp, vp, vp]
         L.dgx_sensing.argtypes = [vp, vp, vp, vp, vp]
         L.dgx_dt_from_cfl.argtypes = [vp, vp, dp, vp, vp]
@@ -123,13 +125,34 @@ class Mesh:
         h = ctypes.c_void_p()
         bcs = (ctypes.c_int * 4)(*(list(bc) + [FARFIELD] * (4 - len(bc))))
         _check(L.dgx_mesh_build(dim, order, nx, ny, x0, x1, y0, y1, bcs, nfields, ctypes.byref(h)), "dgx_mesh_build")
+        self._adopt(h)
+        self.bc = tuple(bc)
+        self.extent = (x0, x1, y0, y1)
+
+    def _adopt(self, h) -> None:
+        L = lib()
         self._h = h
         z = (ctypes.c_int * 16)()
         L.dgx_mesh_sizes(h, z)
         self.sizes = {n: z[k] for k, n in enumerate(SIZE_NAMES)}
         self.dx = L.dgx_mesh_dx(h)
-        self.bc = tuple(bc)
-        self.extent = (x0, x1, y0, y1)
+
+    @classmethod
+    def from_gmsh(cls, path: str, nfields: int = 4) -> "Mesh":
+        """Mesh from a Gmsh 2.2 ASCII file (the reference's mesh input)."""
+        h = ctypes.c_void_p()
+        _check(lib().dgx_mesh_read_gmsh(str(path).encode(), nfields, ctypes.byref(h)), "dgx_mesh_read_gmsh")
+        m = cls.__new__(cls)
+        m._adopt(h)
+        m.bc = ()
+        xyz = m.array("xyz").reshape(-1, m.sizes["D"])
+        lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+        m.extent = (lo[0], hi[0], lo[1], hi[1]) if m.sizes["D"] == 2 else (lo[0], hi[0], 0.0, 1.0)
+        return m
+
+    def write_gmsh(self, path: str) -> None:
+        """Write the mesh as Gmsh 2.2 ASCII (boundary faces as tagged elements)."""
+        _check(lib().dgx_mesh_write_gmsh(self._h, str(path).encode()), "dgx_mesh_write_gmsh")
 
     @property
     def tileable(self) -> bool:

@@ -202,6 +202,26 @@ dgx_status dgx_mesh_build(int dim, int order, int nx, int ny, double x0, double 
   return DGX_OK;
 }
 
+dgx_status dgx_mesh_read_gmsh(const char* path, int nfields, dgx_mesh** out) {
+  if (!path || !out) return fail(DGX_EINVAL, "dgx_mesh_read_gmsh: null argument");
+  GmshMesh g;
+  std::string err = read_gmsh(path, g);
+  dgx_mesh* m = new dgx_mesh;
+  if (err.empty()) err = build_tables_gmsh(g, nfields, m->t);
+  if (!err.empty()) {
+    delete m;
+    return fail(DGX_EINVAL, std::string("dgx_mesh_read_gmsh: ") + err);
+  }
+  *out = m;
+  return DGX_OK;
+}
+
+dgx_status dgx_mesh_write_gmsh(const dgx_mesh* m, const char* path) {
+  if (!m || !path) return fail(DGX_EINVAL, "dgx_mesh_write_gmsh: null argument");
+  std::string err = write_gmsh(m->t, path);
+  return err.empty() ? DGX_OK : fail(DGX_EINVAL, "dgx_mesh_write_gmsh: " + err);
+}
+
 void dgx_mesh_free(dgx_mesh* m) { delete m; }
 
 int dgx_mesh_tileable(const dgx_mesh* m) {

@@ -7,6 +7,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
 #include <unordered_map>
 
 namespace dgx {
@@ -190,12 +194,240 @@ static std::vector<double> taylor_to_lagrange(const std::vector<double>& nodes) 
 }
 
 // ---------------------------------------------------------------------------
+// Gmsh 2.2 ASCII meshes (simpleMesh::load, simpleMesh.cc:141-204)
+// ---------------------------------------------------------------------------
+// Gmsh element types: lines of 2, 3, 4 nodes; quadrilaterals of 4, 9, 16; point.
+int gmsh_element_type(int D, int p) {
+  static const int line[4] = {0, 1, 8, 26}, quad[4] = {0, 3, 10, 36};
+  return D == 1 ? line[p] : quad[p];
+}
+static int gmsh_boundary_type(int D, int p) { return D == 1 ? 15 : gmsh_element_type(1, p); }
+
+bool gmsh_dim_order(const GmshMesh& g, int& D, int& p) {
+  for (int dim = 2; dim >= 1; --dim)
+    for (int q = 1; q <= 3; ++q) {
+      const int ty = gmsh_element_type(dim, q);
+      for (const auto& el : g.elems)
+        if (el.type == ty) {
+          D = dim;
+          p = q;
+          return true;
+        }
+    }
+  return false;
+}
+
+std::string read_gmsh(const std::string& path, GmshMesh& g) {
+  std::ifstream in(path);
+  if (!in) return "cannot open " + path;
+  g = GmshMesh();
+  std::string line;
+  bool nodes = false, elems = false;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line == "$MeshFormat") {
+      std::getline(in, line);
+      std::istringstream f(line);
+      double version = 0;
+      int ascii = -1;
+      f >> version >> ascii;
+      if (version < 2.0 || version >= 3.0 || ascii != 0) return "only Gmsh 2.x ASCII meshes are supported";
+    } else if (line == "$Nodes") {
+      long n = 0;
+      if (!(in >> n) || n < 0) return "bad $Nodes count";
+      g.xyz.resize((size_t)n * 3);
+      for (long i = 0; i < n; ++i) {
+        long id;
+        if (!(in >> id >> g.xyz[i * 3] >> g.xyz[i * 3 + 1] >> g.xyz[i * 3 + 2])) return "bad node line";
+        if (!g.node_index.emplace(id, i).second) return "duplicate node id";
+      }
+      nodes = true;
+    } else if (line == "$Elements") {
+      long n = 0;
+      if (!(in >> n) || n < 0) return "bad $Elements count";
+      std::getline(in, line);
+      for (long i = 0; i < n; ++i) {
+        if (!std::getline(in, line)) return "missing element lines";
+        std::istringstream f(line);
+        GmshMesh::Elem el;
+        long id;
+        int ntags;
+        if (!(f >> id >> el.type >> ntags) || ntags < 0) return "bad element line";
+        for (int j = 0; j < ntags; ++j) {
+          int tag;
+          if (!(f >> tag)) return "bad element tags";
+          if (j == 0) el.tag = tag;  // physical tag (simpleMesh.cc:184)
+          if (j == 3 && tag != 1) return "partitioned meshes (partition tag > 1) are not supported";
+        }
+        long node;
+        while (f >> node) el.nodes.push_back(node);
+        g.elems.push_back(std::move(el));
+      }
+      elems = true;
+    }
+  }
+  if (!nodes || !elems) return "no $Nodes or $Elements section";
+  return "";
+}
+
+// pairPeriodic (simpleMesh.cc:21-91): boundary faces that are images of one
+// another across the domain.  1D: any two distinct points; 2D: a vertical
+// face pairs with one of the same y extent, a horizontal with the same x.
+static bool pair_periodic(const GmshMesh& g, int D, const std::vector<int64_t>& n1, const std::vector<int64_t>& n2) {
+  const double eps = 1e-8;
+  auto xy = [&](int64_t gid, int a) { return g.xyz[(size_t)g.node_index.at(gid + 1) * 3 + a]; };
+  if (D == 1) return std::fabs(xy(n1[0], 0) - xy(n2[0], 0)) >= eps;
+  double b1[4] = {xy(n1[0], 0), xy(n1[0], 0), xy(n1[0], 1), xy(n1[0], 1)};  // xs, xe, ys, ye
+  double b2[4] = {xy(n2[0], 0), xy(n2[0], 0), xy(n2[0], 1), xy(n2[0], 1)};
+  for (size_t j = 1; j < n1.size(); ++j) {
+    b1[0] = std::min(b1[0], xy(n1[j], 0)); b1[1] = std::max(b1[1], xy(n1[j], 0));
+    b1[2] = std::min(b1[2], xy(n1[j], 1)); b1[3] = std::max(b1[3], xy(n1[j], 1));
+    b2[0] = std::min(b2[0], xy(n2[j], 0)); b2[1] = std::max(b2[1], xy(n2[j], 0));
+    b2[2] = std::min(b2[2], xy(n2[j], 1)); b2[3] = std::max(b2[3], xy(n2[j], 1));
+  }
+  bool same = true;
+  for (int k = 0; k < 4; ++k) same = same && std::fabs(b1[k] - b2[k]) < eps;
+  if (same) return false;
+  if (std::fabs(b1[2] - b1[3]) > eps) return std::fabs(b1[2] - b2[2]) < eps && std::fabs(b1[3] - b2[3]) < eps;
+  if (std::fabs(b1[0] - b1[1]) > eps) return std::fabs(b1[0] - b2[0]) < eps && std::fabs(b1[1] - b2[1]) < eps;
+  return false;
+}
+
+// BuildInterfaces (simpleMesh.cc:770-989) for one partition: faces keyed by
+// their sorted node ids; a boundary element gives its face the physical tag
+// (2, 3, 4: the element is its own neighbour through the same closure);
+// the second element found uses closure j + nsides; periodic faces (tag 1)
+// are paired by geometry.  Faces come out in key order.
+std::string gmsh_interfaces(const GmshMesh& g, int D, int p, int N_E, int N_s, int M_s, int nsides,
+                            const std::vector<int64_t>& gid, Tables& t) {
+  struct Face { int tag = 0, e0 = -1, c0 = -1, e1 = -1, c1 = -1; };
+  std::map<std::vector<int64_t>, Face> faces;
+  const int bt = gmsh_boundary_type(D, p);
+  for (const auto& el : g.elems)
+    if (el.type == bt) {
+      std::vector<int64_t> key;
+      for (long n : el.nodes) key.push_back(n - 1);
+      std::sort(key.begin(), key.end());
+      faces[key] = Face{el.tag};
+    }
+  const int ncl = (D == 1) ? 2 : 4;
+  std::vector<int64_t> key(M_s);
+  for (int e = 0; e < N_E; ++e)
+    for (int j = 0; j < ncl; ++j) {
+      for (int k = 0; k < M_s; ++k) key[k] = gid[(size_t)e * N_s + t.closures[j * M_s + k]];
+      std::sort(key.begin(), key.end());
+      Face& f = faces[key];
+      if (f.e0 < 0) {
+        f.e0 = e;
+        f.c0 = j;
+        if (f.tag == 2 || f.tag == 3 || f.tag == 4) {
+          f.e1 = e;
+          f.c1 = j;
+        }
+      } else if (f.e0 != e) {
+        if (f.e1 < 0) {
+          f.e1 = e;
+          f.c1 = j + nsides;
+        } else if (f.e1 != e) {
+          return "a face shared by more than two elements";
+        }
+      }
+    }
+  for (auto& a : faces)
+    if (a.second.tag == 1 && a.second.e1 < 0)
+      for (auto& b : faces)
+        if (b.second.tag == 1 && b.second.e1 < 0 && pair_periodic(g, D, a.first, b.first)) {
+          a.second.e1 = b.second.e0;
+          a.second.c1 = b.second.c0 + nsides;
+          b.second.e1 = a.second.e0;
+          b.second.c1 = a.second.c0 + nsides;
+        }
+  for (const auto& a : faces) {
+    const Face& f = a.second;
+    if (f.e0 < 0) return "a boundary element that touches no element";
+    if (f.e1 < 0) return "a face with one element and no boundary tag (or an unpaired periodic face)";
+    if (f.tag < 0 || f.tag > 4) return "boundary physical tags must be 1..4";
+    t.face_elem.push_back(f.e0);
+    t.face_elem.push_back(f.e1);
+    t.face_closure.push_back(f.c0);
+    t.face_closure.push_back(f.c1);
+    t.face_tag.push_back(f.tag);
+  }
+  return "";
+}
+
+std::string write_gmsh(const Tables& t, const std::string& path) {
+  const int D = t.D, N_s = t.N_s, M_s = t.M_s;
+  // file node ids: the rank of each node's lattice id, so that the sorted
+  // face keys (and hence the face order) are the same when read back; a
+  // node shared by several elements is the same lattice point
+  std::vector<std::pair<double, double>> pts;
+  std::map<std::pair<long long, long long>, long> id_of;  // exact coordinates (bits) -> id
+  auto bits = [](double v) {
+    long long b;
+    std::memcpy(&b, &v, 8);
+    return b;
+  };
+  std::vector<long> node_id((size_t)t.N_E * N_s);
+  std::vector<std::pair<std::pair<double, double>, size_t>> order;  // ((y, x), slot): lattice order is y-major
+  for (int e = 0; e < t.N_E; ++e)
+    for (int i = 0; i < N_s; ++i) {
+      const double x = t.xyz[((size_t)e * N_s + i) * D], y = D == 2 ? t.xyz[((size_t)e * N_s + i) * D + 1] : 0.0;
+      order.push_back({{y, x}, (size_t)e * N_s + i});
+    }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const auto& a, const auto& b) { return a.first < b.first; });
+  long next = 0;
+  for (const auto& o : order) {
+    const auto key = std::make_pair(bits(o.first.first), bits(o.first.second));
+    auto it = id_of.find(key);
+    if (it == id_of.end()) {
+      it = id_of.emplace(key, ++next).first;
+      pts.push_back({o.first.second, o.first.first});
+    }
+    node_id[o.second] = it->second;
+  }
+  std::ofstream out(path);
+  if (!out) return "cannot write " + path;
+  out << "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n" << pts.size() << "\n";
+  char buf[128];
+  for (size_t i = 0; i < pts.size(); ++i) {
+    std::snprintf(buf, sizeof buf, "%zu %.17g %.17g 0\n", i + 1, pts[i].first, pts[i].second);
+    out << buf;
+  }
+  out << "$EndNodes\n$Elements\n";
+  long nb = 0;
+  for (int f = 0; f < t.M_T; ++f) nb += t.face_tag[f] != 0;
+  out << nb + t.N_E << "\n";
+  long id = 0;
+  const int bt = gmsh_boundary_type(D, t.order), vt = gmsh_element_type(D, t.order);
+  for (int f = 0; f < t.M_T; ++f) {
+    if (!t.face_tag[f]) continue;
+    const int e = t.face_elem[2 * f], cl = t.face_closure[2 * f];
+    out << ++id << " " << bt << " 2 " << t.face_tag[f] << " " << t.face_tag[f];
+    for (int k = 0; k < M_s; ++k) out << " " << node_id[(size_t)e * N_s + t.closures[cl * M_s + k]];
+    out << "\n";
+  }
+  for (int e = 0; e < t.N_E; ++e) {
+    out << ++id << " " << vt << " 2 88 1";
+    for (int i = 0; i < N_s; ++i) out << " " << node_id[(size_t)e * N_s + i];
+    out << "\n";
+  }
+  out << "$EndElements\n";
+  return out ? "" : "write failed: " + path;
+}
+
+// ---------------------------------------------------------------------------
 // Table builder
 // ---------------------------------------------------------------------------
-std::string build_tables(const MeshSpec& s, Tables& t) {
+// Structured mesh (gm == nullptr) or a Gmsh mesh: the two differ only in
+// where the element nodes come from and how boundary faces find their tag
+// and periodic partner; every table after that is built by the same code.
+static std::string build_tables_impl(const MeshSpec& s, const GmshMesh* gm, Tables& t) {
   const int D = s.dim, p = s.order;
   if (D != 1 && D != 2) return "dim must be 1 or 2";
   if (p < 1 || p > 3) return "order must be in 1..3";
+  if (!gm) {
   if (s.nx < 1 || (D == 2 && s.ny < 1)) return "mesh needs at least one element per direction";
   for (int k = 0; k < 2 * D; ++k)
     if (s.bc[k] < 1 || s.bc[k] > 3) return "boundary tags must be 1 (periodic), 2 (farfield) or 3 (wall)";
@@ -205,6 +437,7 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
     return "periodic tags must pair";
   if (D == 2 && ((s.bc[0] == BC_PERIODIC && s.nx < 2) || (s.bc[2] == BC_PERIODIC && s.ny < 2)))
     return "periodic direction needs at least two elements";
+  }
 
   t = Tables();
   t.D = D;
@@ -335,14 +568,38 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
 
   // --- elements and physical nodes
   const int nx = s.nx, ny = t.ny;
-  t.N_E = nx * ny;
+  t.N_E = gm ? 0 : nx * ny;
+  std::vector<const GmshMesh::Elem*> gvol;  // Gmsh: volume elements in file order
+  if (gm) {
+    const int vt = gmsh_element_type(D, p);
+    for (const auto& el : gm->elems)
+      if (el.type == vt) gvol.push_back(&el);
+    if (gvol.empty()) return "no elements of the mesh's type";
+    t.N_E = (int)gvol.size();
+    t.nx = t.ny = 0;  // not a structured mesh
+    for (int k = 0; k < 4; ++k) t.bc[k] = 0;
+  }
   const int N_E = t.N_E;
   const double hx = (s.x1 - s.x0) / nx;
   const double hy = (D == 2) ? (s.y1 - s.y0) / ny : 0.0;
   const int W = p * nx + 1;  // lattice width (global node ids)
   std::vector<int64_t> gid((size_t)N_E * N_s);
   t.xyz.assign((size_t)N_E * N_s * D, 0.0);
-  for (int e = 0; e < N_E; ++e) {
+  if (gm) {
+    // node index = Gmsh node id - 1 (simpleMesh.cc:160-164, 188-190)
+    for (int e = 0; e < N_E; ++e) {
+      const GmshMesh::Elem& el = *gvol[e];
+      if ((int)el.nodes.size() != N_s) return "element with the wrong number of nodes";
+      for (int i = 0; i < N_s; ++i) {
+        const int64_t id = el.nodes[i];
+        auto it = gm->node_index.find(id);
+        if (it == gm->node_index.end()) return "element refers to an undefined node";
+        gid[(size_t)e * N_s + i] = id - 1;
+        for (int a = 0; a < D; ++a) t.xyz[(e * N_s + i) * D + a] = gm->xyz[(size_t)it->second * 3 + a];
+      }
+    }
+  }
+  for (int e = 0; e < (gm ? 0 : N_E); ++e) {
     int ex = e % nx, ey = e / nx;
     for (int i = 0; i < N_s; ++i) {
       int la = (int)std::lround((t.ref_nodes[i * D] + 1.0) * p / 2.0);
@@ -387,6 +644,24 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
         for (int a = 0; a < D; ++a) t.invJac[((size_t)(e * N_G + g) * D + al) * D + a] = inv[al][a];
   }
 
+  if (gm && D == 2) {
+    // The kernels keep one invJac per element, as the reference's affine
+    // elements need (dg_functions.cc:33-70 takes det at the first point):
+    // refuse elements whose corners are not a parallelogram.
+    for (int e = 0; e < N_E; ++e) {
+      const double* x = &t.xyz[(size_t)e * N_s * 2];
+      const double sc = std::fabs(x[4] - x[0]) + std::fabs(x[5] - x[1]) + std::fabs(x[6] - x[2]) + std::fabs(x[7] - x[3]);
+      for (int i = 0; i < N_s; ++i) {  // every node where the corners' affine map puts it
+        const double xi = t.ref_nodes[2 * i], eta = t.ref_nodes[2 * i + 1];
+        for (int a = 0; a < 2; ++a) {
+          const double xa = x[a] + 0.5 * (xi + 1.0) * (x[2 + a] - x[a]) + 0.5 * (eta + 1.0) * (x[6 + a] - x[a]);
+          if (std::fabs(xa - x[2 * i + a]) > 1e-10 * sc)
+            return "element is not affine (a parallelogram with equispaced nodes); the kernels need one invJac per element";
+        }
+      }
+    }
+  }
+
   // --- inverse mass matrices (dg_functions.cc:98-174), shared across equal J
   t.Minv.assign((size_t)N_E * N_s * N_s, 0.0);
   {
@@ -414,6 +689,11 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
   }
 
   // --- interfaces (simpleMesh.cc:770-989)
+  t.face_elem.clear(); t.face_closure.clear(); t.face_tag.clear();
+  if (gm) {
+    std::string err = gmsh_interfaces(*gm, D, p, N_E, N_s, M_s, nsides, gid, t);
+    if (!err.empty()) return err;
+  } else {
   struct Visit { std::array<int64_t, 4> key; int e, cl; };
   std::vector<Visit> visits;
   visits.reserve((size_t)N_E * (D == 1 ? 2 : 4));
@@ -450,7 +730,6 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
       default: pcl = 0; return 0 * nx + ex;
     }
   };
-  t.face_elem.clear(); t.face_closure.clear(); t.face_tag.clear();
   for (size_t k = 0; k < visits.size();) {
     size_t k2 = k + 1;
     while (k2 < visits.size() && visits[k2].key == visits[k].key) ++k2;
@@ -480,6 +759,7 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
       return "internal error: face shared by more than two elements";
     }
     k = k2;
+  }
   }
   t.M_T = (int)t.face_tag.size();
   const int M_T = t.M_T;
@@ -640,6 +920,15 @@ std::string build_tables(const MeshSpec& s, Tables& t) {
     matmul(N_s, N_s, N_s, t.MonoX2MonoY, t.Lag2MonoX, t.Lag2MonoY);
   }
   return "";
+}
+
+std::string build_tables(const MeshSpec& s, Tables& t) { return build_tables_impl(s, nullptr, t); }
+
+std::string build_tables_gmsh(const GmshMesh& g, int nfields, Tables& t) {
+  MeshSpec s;
+  if (!gmsh_dim_order(g, s.dim, s.order)) return "no line or quadrilateral elements of order 1..3";
+  s.nfields = nfields;
+  return build_tables_impl(s, &g, t);
 }
 
 }  // namespace dgx

@@ -23,8 +23,10 @@
 //  * boundary physical tags: 1 periodic, 2 farfield (zero gradient),
 //    3 reflective wall (simpleMesh.cc:855-868, 695-703).
 #pragma once
-#include <vector>
+#include <cstdint>
 #include <string>
+#include <unordered_map>
+#include <vector>
 
 namespace dgx {
 
@@ -83,6 +85,31 @@ struct Tables {
 
 // Builds the tables; returns an empty string on success or an error message.
 std::string build_tables(const MeshSpec& spec, Tables& out);
+
+// A Gmsh 2.2 ASCII mesh as the reference reads it (simpleMesh::load,
+// simpleMesh.cc:141-204): nodes (x, y, z by file position, id -> position)
+// and elements (Gmsh type, physical tag, node ids), single partition.
+struct GmshMesh {
+  struct Elem {
+    int type = 0, tag = 0;
+    std::vector<long> nodes;
+  };
+  std::vector<double> xyz;
+  std::unordered_map<long, long> node_index;
+  std::vector<Elem> elems;
+};
+std::string read_gmsh(const std::string& path, GmshMesh& out);
+// Dimension and order from the element types present (quadrilaterals first).
+bool gmsh_dim_order(const GmshMesh& g, int& dim, int& order);
+int gmsh_element_type(int dim, int order);
+// Tables of a Gmsh mesh: elements in file order, node index = id - 1,
+// boundary physical tags 1 periodic, 2 farfield, 3 wall, 4 (as 2).
+std::string build_tables_gmsh(const GmshMesh& g, int nfields, Tables& out);
+// Writes a mesh built by build_tables as Gmsh 2.2 ASCII (boundary faces as
+// tagged boundary elements); reading it back gives the same tables.
+std::string write_gmsh(const Tables& t, const std::string& path);
+std::string gmsh_interfaces(const GmshMesh& g, int D, int p, int N_E, int N_s, int M_s, int nsides,
+                            const std::vector<int64_t>& gid, Tables& t);
 
 // 1D Lagrange value/derivative helpers on arbitrary nodes (exposed for tests).
 double lagrange1d(const std::vector<double>& nodes, int i, double x);

@@ -62,6 +62,16 @@ typedef struct dgx_mesh dgx_mesh;
 dgx_status dgx_mesh_build(int dim, int order, int nx, int ny, double x0, double x1, double y0, double y1,
                           const int* bc, int nfields, dgx_mesh** out);
 void dgx_mesh_free(dgx_mesh* m);
+/* Mesh from a Gmsh 2.2 ASCII file, as simpleMesh::load reads it
+ * (simpleMesh.cc:141-204): quadrilaterals (2D) or lines (1D) of order 1..3
+ * in file order, node index = node id - 1, boundary elements with physical
+ * tags 1 periodic (paired by geometry, simpleMesh.cc:21-91), 2 farfield,
+ * 3 wall, 4 (as farfield); faces built as BuildInterfaces does
+ * (simpleMesh.cc:770-989).  Elements must be affine (one invJac each). */
+dgx_status dgx_mesh_read_gmsh(const char* path, int nfields, dgx_mesh** out);
+/* Writes a mesh as Gmsh 2.2 ASCII, boundary faces as tagged boundary
+ * elements; dgx_mesh_read_gmsh of the file gives the same tables. */
+dgx_status dgx_mesh_write_gmsh(const dgx_mesh* m, const char* path);
 /* 1 when solvers on this mesh run the fused tile kernel by default (uniform
  * Cartesian grid, element counts multiples of the tile), else 0.  Host only. */
 int dgx_mesh_tileable(const dgx_mesh* m);
