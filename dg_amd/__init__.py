@@ -62,6 +62,9 @@ def lib() -> ctypes.CDLL:
         L.dgx_set_sensor_thresholds.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double)]
         L.dgx_limit.argtypes = [vp, i, vp, vp]
         L.dgx_mesh_read_gmsh.argtypes = [ctypes.c_char_p, i, vp]
+        L.dgx_format.argtypes = [vp, vp, vp, i, vp]
+        L.dgx_pos_write.argtypes = [vp, i, vp, ctypes.c_char_p, i, ctypes.c_double]
+        L.dgx_pos_read.argtypes = [vp, i, ctypes.c_char_p, i, vp, vp]
         L.dgx_mesh_write_gmsh.argtypes = [vp, ctypes.c_char_p]
         L.dgx_limit_into.argtypes = [vp, i, vp, vp, vp]
         L.dgx_sensing.argtypes = [vp, vp, vp, vp, vp]
@@ -113,7 +116,8 @@ SIZE_NAMES = ["D", "order", "N_s", "N_G", "M_s", "M_G", "N_N", "N_E", "M_T", "M_
 F64_TABLES = ["phi", "dphi", "phi_w", "dphi_w", "psi", "psi_w", "weight", "points", "pointsF", "weightF",
               "ref_nodes", "xyz", "J", "invJac", "JF", "normals", "Minv", "Lag2Mono", "Mono2Lag", "Lag2MonoX",
               "MonoX2MonoY", "MonoY2Lag", "Lag2MonoY", "MonoX2Lag"]
-I32_TABLES = ["map", "invmap", "boundaryMap", "neighbors", "face_elem", "face_closure", "face_tag", "closures"]
+I32_TABLES = ["map", "invmap", "boundaryMap", "neighbors", "face_elem", "face_closure", "face_tag", "closures",
+              "elem_id"]
 
 
 class Mesh:
@@ -149,6 +153,21 @@ class Mesh:
         lo, hi = xyz.min(axis=0), xyz.max(axis=0)
         m.extent = (lo[0], hi[0], lo[1], hi[1]) if m.sizes["D"] == 2 else (lo[0], hi[0], 0.0, 1.0)
         return m
+
+    def write_pos(self, physics: int, out, prefix: str, step: int, time: float) -> None:
+        """Printed fields (host, [(e*N_F+f)*N_s+i]) as .pos files "<prefix><fname><step>.pos"."""
+        out = np.ascontiguousarray(out, dtype=np.float64)
+        _check(lib().dgx_pos_write(self._h, physics, out.ctypes.data, str(prefix).encode(), step, time),
+               "dgx_pos_write")
+
+    def read_pos(self, physics: int, prefix: str, step: int):
+        """(printed fields, time) read from .pos files."""
+        nf = 2 + self.sizes["D"] + physics
+        out = np.zeros(self.sizes["N_E"] * nf * self.sizes["N_s"])
+        t = ctypes.c_double()
+        _check(lib().dgx_pos_read(self._h, physics, str(prefix).encode(), step, ctypes.byref(t), out.ctypes.data),
+               "dgx_pos_read")
+        return out, t.value
 
     def write_gmsh(self, path: str) -> None:
         """Write the mesh as Gmsh 2.2 ASCII (boundary faces as tagged elements)."""
@@ -230,6 +249,29 @@ class Solver:
     def limit_into(self, limiter: int, In, Out, stream=None) -> None:
         """Out = limited In (In unchanged); 2D hrl on tile meshes runs both sweeps in one launch."""
         _check(lib().dgx_limit_into(self._h, limiter, _ptr(In), _ptr(Out), _stream(stream)), "dgx_limit_into")
+
+    def format(self, U, out, inverse: bool = False, stream=None) -> None:
+        """PRINTER formatter on the device: out = printed fields of U, or U from out when inverse."""
+        _check(lib().dgx_format(self._h, _ptr(U), _ptr(out), 1 if inverse else 0, _stream(stream)), "dgx_format")
+
+    def checkpoint(self, U, prefix: str, step: int, time: float) -> None:
+        """Write U as the reference's .pos files (printed fields, one file per field)."""
+        import torch
+
+        out = torch.empty_like(U)
+        self.format(U, out)
+        torch.cuda.synchronize()
+        self.mesh.write_pos(self.config.physics, out.cpu().numpy(), prefix, step, time)
+
+    def restart(self, prefix: str, step: int, U) -> float:
+        """Read .pos files into U (device state) through the inverse formatter; returns the time."""
+        import torch
+
+        out_h, time = self.mesh.read_pos(self.config.physics, prefix, step)
+        out = torch.from_numpy(out_h).to(U.device)
+        self.format(U, out, inverse=True)
+        torch.cuda.synchronize()
+        return time
 
     def sensing(self, U, sensors, Uavg=None, stream=None) -> None:
         _check(lib().dgx_sensing(self._h, _ptr(U), _ptr(sensors), _ptr(Uavg), _stream(stream)), "dgx_sensing")

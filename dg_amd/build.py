@@ -38,6 +38,7 @@ SOURCES = [
     ("batch.hip", "hipcc", []),
     ("mlp.hip", "hipcc", []),
     ("limiter.hip", "hipcc", ["-ffp-contract=off"]),
+    ("output.hip", "hipcc", ["-ffp-contract=off"]),
     ("compat.hip", "hipcc", []),
 ]
 

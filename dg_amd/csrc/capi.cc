@@ -257,7 +257,7 @@ const int* dgx_mesh_i32(const dgx_mesh* m, const char* name, long* count) {
   const Tables& t = m->t;
   const std::map<std::string, const std::vector<int>*> tab = {
       {"map", &t.map}, {"invmap", &t.invmap}, {"boundaryMap", &t.boundaryMap}, {"neighbors", &t.neighbors},
-      {"face_elem", &t.face_elem}, {"face_closure", &t.face_closure}, {"face_tag", &t.face_tag},
+      {"face_elem", &t.face_elem}, {"face_closure", &t.face_closure}, {"face_tag", &t.face_tag}, {"elem_id", &t.elem_id},
       {"closures", &t.closures}};
   auto it = tab.find(name);
   if (it == tab.end()) return nullptr;
@@ -451,6 +451,26 @@ dgx_status dgx_limit_into(dgx_solver* h, int limiter, const double* In, double* 
   if (!hip_ok(hipMemcpyAsync(Out, In, sizeof(double) * s.state_size(), hipMemcpyDeviceToDevice, st), "copy"))
     return DGX_EDEVICE;
   return do_limit(s, limiter, Out, st);
+}
+
+dgx_status dgx_format(dgx_solver* h, double* U, double* out, int inverse, void* stream) {
+  if (!h || !U || !out) return fail(DGX_EINVAL, "dgx_format: null argument");
+  return hip_ok(launch_format(h->s, U, out, inverse ? 1 : 0, as_stream(stream)), "formatter") ? DGX_OK : DGX_EDEVICE;
+}
+
+dgx_status dgx_pos_write(const dgx_mesh* m, int physics, const double* out, const char* prefix, int step,
+                         double time) {
+  if (!m || !out || !prefix) return fail(DGX_EINVAL, "dgx_pos_write: null argument");
+  if (physics != DGX_SINGLEFLUID && physics != DGX_MULTIFLUID) return fail(DGX_EINVAL, "dgx_pos_write: physics");
+  const std::string err = pos_write(m->t, physics, out, prefix, step, time);
+  return err.empty() ? DGX_OK : fail(DGX_EINVAL, "dgx_pos_write: " + err);
+}
+
+dgx_status dgx_pos_read(const dgx_mesh* m, int physics, const char* prefix, int step, double* time, double* out) {
+  if (!m || !out || !prefix) return fail(DGX_EINVAL, "dgx_pos_read: null argument");
+  if (physics != DGX_SINGLEFLUID && physics != DGX_MULTIFLUID) return fail(DGX_EINVAL, "dgx_pos_read: physics");
+  const std::string err = pos_read(m->t, physics, prefix, step, time, out);
+  return err.empty() ? DGX_OK : fail(DGX_EINVAL, "dgx_pos_read: " + err);
 }
 
 dgx_status dgx_sensing(dgx_solver* h, const double* U, int* sensors, double* Uavg, void* stream) {

@@ -127,6 +127,11 @@ hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st);
 // learned indicator (mlp.hip): features of U into X, and flags into s.sensors
 hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st);
 hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st);
+// solution output and restart (output.hip): PRINTER formatter, .pos files
+hipError_t launch_format(const Solver& s, double* U, double* out, int inverse, hipStream_t st);
+std::string pos_write(const Tables& t, int physics, const double* out, const std::string& prefix, int step,
+                      double time);
+std::string pos_read(const Tables& t, int physics, const std::string& prefix, int step, double* time, double* out);
 
 // drop-in compat: process-wide configuration for the reference-named launchers
 const dgx_config& current_config();

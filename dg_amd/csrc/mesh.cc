@@ -250,9 +250,8 @@ std::string read_gmsh(const std::string& path, GmshMesh& g) {
         if (!std::getline(in, line)) return "missing element lines";
         std::istringstream f(line);
         GmshMesh::Elem el;
-        long id;
         int ntags;
-        if (!(f >> id >> el.type >> ntags) || ntags < 0) return "bad element line";
+        if (!(f >> el.id >> el.type >> ntags) || ntags < 0) return "bad element line";
         for (int j = 0; j < ntags; ++j) {
           int tag;
           if (!(f >> tag)) return "bad element tags";
@@ -399,7 +398,11 @@ std::string write_gmsh(const Tables& t, const std::string& path) {
   long nb = 0;
   for (int f = 0; f < t.M_T; ++f) nb += t.face_tag[f] != 0;
   out << nb + t.N_E << "\n";
+  // boundary elements first, numbered 1..nb unless the element ids (kept for
+  // .pos files) take those numbers, then after the largest element id
   long id = 0;
+  if (!t.elem_id.empty() && *std::min_element(t.elem_id.begin(), t.elem_id.end()) <= nb)
+    id = *std::max_element(t.elem_id.begin(), t.elem_id.end());
   const int bt = gmsh_boundary_type(D, t.order), vt = gmsh_element_type(D, t.order);
   for (int f = 0; f < t.M_T; ++f) {
     if (!t.face_tag[f]) continue;
@@ -409,7 +412,7 @@ std::string write_gmsh(const Tables& t, const std::string& path) {
     out << "\n";
   }
   for (int e = 0; e < t.N_E; ++e) {
-    out << ++id << " " << vt << " 2 88 1";
+    out << (t.elem_id.empty() ? ++id : (long)t.elem_id[e]) << " " << vt << " 2 88 1";
     for (int i = 0; i < N_s; ++i) out << " " << node_id[(size_t)e * N_s + i];
     out << "\n";
   }
@@ -763,6 +766,16 @@ static std::string build_tables_impl(const MeshSpec& s, const GmshMesh* gm, Tabl
   }
   t.M_T = (int)t.face_tag.size();
   const int M_T = t.M_T;
+  // element ids for .pos files: the file's, or those write_gmsh gives (after
+  // the boundary elements it writes first)
+  t.elem_id.resize(N_E);
+  if (gm) {
+    for (int e = 0; e < N_E; ++e) t.elem_id[e] = (int)gvol[e]->id;
+  } else {
+    int nb = 0;
+    for (int f = 0; f < M_T; ++f) nb += t.face_tag[f] != 0;
+    for (int e = 0; e < N_E; ++e) t.elem_id[e] = nb + e + 1;
+  }
 
   // --- normals (simpleMesh.cc:206-319): sum of physical gradients of the
   // closure nodes' own basis functions at their nodes, normalised.

@@ -76,6 +76,7 @@ struct Tables {
   std::vector<int> face_elem;      // [t*2+d] element on each side
   std::vector<int> face_closure;   // [t*2+d] closure id used by each side
   std::vector<int> face_tag;       // [t] physical tag (0 interior)
+  std::vector<int> elem_id;        // [e] Gmsh element id (.pos files): the file's, or the one write_gmsh gives
 
   // limiter transforms (column-major N_s x N_s)
   std::vector<double> Lag2Mono, Mono2Lag;                       // 1D
@@ -91,6 +92,7 @@ std::string build_tables(const MeshSpec& spec, Tables& out);
 // and elements (Gmsh type, physical tag, node ids), single partition.
 struct GmshMesh {
   struct Elem {
+    long id = 0;
     int type = 0, tag = 0;
     std::vector<long> nodes;
   };

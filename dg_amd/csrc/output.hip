@@ -1,0 +1,164 @@
+// Solution output and restart: the PRINTER formatter on the GPU and the
+// reference's .pos files (Gmsh ElementNodeData views, one file per printed
+// field).
+//
+// Formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110,
+// GAMNCON): node by node, conserved -> printed primitive fields
+//   1D single fluid  rho, ux, p          1D two-fluid  rho, ux, G, p
+//   2D single fluid  rho, ux, uy, p      2D two-fluid  rho, ux, uy, G, p
+// (G = gamma = 1 + 1/alpha), and the inverse for restart.  Compiled without
+// FMA contraction: the same operations in the same order as the reference,
+// so the printed values are its values bit for bit.
+//
+// .pos files (simpleMesh::writeSolution / readSolution, simpleMesh.cc:322-459):
+// "<prefix><fname><step %010d>.pos" per field, header
+//   $MeshFormat / 2.1 0 8 / $EndMeshFormat / $ElementNodeData / 1 / "<Name>" /
+//   1 / <time> / 4 / <step> / 1 / <N_E> / <partition 0>
+// then one line per element "<element id> <N_s> v_0 ... v_{N_s-1} " with 20
+// significant digits, and $EndElementNodeData.
+#pragma clang fp contract(off)
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <string>
+
+#include "dgx_internal.h"
+#include "physics.cuh"
+
+namespace dgx {
+
+template <int D, int PH>
+__global__ __launch_bounds__(256) void k_format(int N_s, int N_E, double* __restrict__ U, double* __restrict__ out,
+                                                int inverse) {
+  constexpr int NF = 2 + D + PH;
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)N_E * N_s) return;
+  const long e = k / N_s, i = k % N_s;
+  double* u = U + e * NF * N_s + i;
+  double* o = out + e * NF * N_s + i;
+  if (!inverse) {
+    const double rho = u[0];
+    const double ux = u[1 * N_s] / rho;
+    const double uy = D == 2 ? u[2 * N_s] / rho : 0.0;
+    const double et = u[(D + 1) * N_s];
+    const double gamma = PH ? 1 + 1.0 / u[(D + 2) * N_s] : kGammaSF;
+    o[0] = rho;
+    o[1 * N_s] = ux;
+    if (D == 2) o[2 * N_s] = uy;
+    if (PH) o[(D + 1) * N_s] = gamma;
+    if (D == 2)
+      o[(D + 1 + PH) * N_s] = (gamma - 1) * (et - 0.5 * (ux * ux + uy * uy) * rho);
+    else
+      o[(D + 1 + PH) * N_s] = (gamma - 1) * (et - 0.5 * ux * ux * rho);
+  } else {
+    const double rho = o[0], ux = o[1 * N_s];
+    const double uy = D == 2 ? o[2 * N_s] : 0.0;
+    const double gamma = PH ? o[(D + 1) * N_s] : kGammaSF;
+    const double p = o[(D + 1 + PH) * N_s];
+    u[0] = rho;
+    u[1 * N_s] = rho * ux;
+    if (D == 2) {
+      u[2 * N_s] = rho * uy;
+      u[(D + 1) * N_s] = p / (gamma - 1) + 0.5 * rho * (ux * ux + uy * uy);
+    } else {
+      u[(D + 1) * N_s] = p / (gamma - 1) + 0.5 * rho * ux * ux;
+    }
+    if (PH) u[(D + 2) * N_s] = 1.0 / (gamma - 1);
+  }
+}
+
+hipError_t launch_format(const Solver& s, double* U, double* out, int inverse, hipStream_t st) {
+  const long n = (long)s.N_E * s.N_s;
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+  const bool mf = s.cfg.physics == DGX_MULTIFLUID;
+  if (s.D == 1) {
+    if (mf) hipLaunchKernelGGL((k_format<1, 1>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
+    else hipLaunchKernelGGL((k_format<1, 0>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
+  } else {
+    if (mf) hipLaunchKernelGGL((k_format<2, 1>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
+    else hipLaunchKernelGGL((k_format<2, 0>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
+  }
+  return hipGetLastError();
+}
+
+// PRINTER::set_names (printer.cc:11-80): printed field names and file names
+void pos_field_names(int D, int physics, std::vector<std::string>& names, std::vector<std::string>& fnames) {
+  names = {"Rho", "Ux"};
+  fnames = {"rho", "ux"};
+  if (D == 2) { names.push_back("Uy"); fnames.push_back("uy"); }
+  if (physics == DGX_MULTIFLUID) { names.push_back("G"); fnames.push_back("g"); }
+  names.push_back("P");
+  fnames.push_back("p");
+}
+
+static std::string pos_path(const std::string& prefix, const std::string& fname, int step) {
+  char st[24];
+  std::snprintf(st, sizeof st, "%010i", step);
+  return prefix + fname + st + ".pos";
+}
+
+std::string pos_write(const Tables& t, int physics, const double* out, const std::string& prefix, int step,
+                      double time) {
+  std::vector<std::string> names, fnames;
+  pos_field_names(t.D, physics, names, fnames);
+  const int NF = (int)names.size(), Ns = t.N_s;
+  for (int fc = 0; fc < NF; ++fc) {
+    const std::string path = pos_path(prefix, fnames[fc], step);
+    std::ofstream o(path);
+    if (!o) return "cannot write " + path;
+    o.precision(20);
+    o << "$MeshFormat\n2.1 0 8\n$EndMeshFormat\n";
+    o << "$ElementNodeData\n";
+    o << "1\n\"" << names[fc] << "\"\n";
+    o << "1\n" << time << "\n";
+    o << "4\n" << step << "\n1\n" << t.N_E << "\n" << 0 << "\n";
+    for (int e = 0; e < t.N_E; ++e) {
+      o << t.elem_id[e] << " " << Ns << " ";
+      for (int i = 0; i < Ns; ++i) o << out[((size_t)e * NF + fc) * Ns + i] << " ";
+      o << "\n";
+    }
+    o << "$EndElementNodeData\n";
+    if (!o) return "write failed: " + path;
+  }
+  return "";
+}
+
+std::string pos_read(const Tables& t, int physics, const std::string& prefix, int step, double* time,
+                     double* out) {
+  std::vector<std::string> names, fnames;
+  pos_field_names(t.D, physics, names, fnames);
+  const int NF = (int)names.size(), Ns = t.N_s;
+  for (int fc = 0; fc < NF; ++fc) {
+    const std::string path = pos_path(prefix, fnames[fc], step);
+    std::ifstream in(path);
+    if (!in) return "no file named " + path;
+    std::string line;
+    for (int k = 0; k < 7; ++k) std::getline(in, line);  // format block, view header, name
+    double tm;
+    in >> tm;
+    std::getline(in, line);
+    for (int k = 0; k < 3; ++k) std::getline(in, line);  // 4, step, 1
+    long fsize, part;
+    in >> fsize;
+    std::getline(in, line);
+    in >> part;
+    std::getline(in, line);
+    if (!in) return path + ": bad header";
+    if (fsize != t.N_E) return path + ": different number of elements than the mesh";
+    if (part != 0) return path + ": different partition";
+    for (int e = 0; e < t.N_E; ++e) {
+      long id, nn;
+      if (!(in >> id >> nn)) return path + ": missing element data";
+      if (id != t.elem_id[e] || nn != Ns) return path + ": element id or number of nodes differs from the mesh";
+      for (int i = 0; i < Ns; ++i)
+        if (!(in >> out[((size_t)e * NF + fc) * Ns + i])) return path + ": bad value";
+      std::getline(in, line);
+    }
+    if (time) *time = tm;
+  }
+  return "";
+}
+
+}  // namespace dgx

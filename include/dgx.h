@@ -139,6 +139,19 @@ dgx_status dgx_limit(dgx_solver* s, int limiter, double* U, void* stream);
  * arithmetic as dgx_limit, bit for bit); otherwise In is copied to Out and
  * limited in place.  Same limiter codes as dgx_limit. */
 dgx_status dgx_limit_into(dgx_solver* s, int limiter, const double* In, double* Out, void* stream);
+/* Solution output and restart.
+ * dgx_format replaces Lformater (printer_kernels.h:14; printer_{oned,twod}_*.cu:20-110): inverse 0 fills
+ * out (device, state-sized) with the printed fields node by node (rho, ux[, uy][, G], p), inverse 1 fills
+ * U from out; the reference's arithmetic bit for bit.
+ * dgx_pos_write / dgx_pos_read replace simpleMesh::writeSolution / readSolution (simpleMesh.cc:322-459)
+ * with the printed fields in host memory (out[(e*N_F+f)*N_s+i]): one "<prefix><fname><step %010d>.pos"
+ * file per field (rho, ux[, uy][, g], p), Gmsh ElementNodeData with the mesh's element ids; reading checks
+ * the element count, ids and node counts and returns DGX_EINVAL on a mismatch (the reference exits). */
+dgx_status dgx_format(dgx_solver* s, double* U, double* out, int inverse, void* stream);
+dgx_status dgx_pos_write(const dgx_mesh* m, int physics, const double* out, const char* prefix, int step,
+                         double time);
+dgx_status dgx_pos_read(const dgx_mesh* m, int physics, const char* prefix, int step, double* time, double* out);
+
 /* SENSOR::sensing: flags (int[N_E], device) and Uavg (double[N_E*N_F], device; may be NULL). */
 dgx_status dgx_sensing(dgx_solver* s, const double* U, int* sensors, double* Uavg, void* stream);
 

@@ -74,6 +74,8 @@ def lib() -> ctypes.CDLL:
             getattr(L, n).restype = None
         L.or_solve.argtypes = [cp, ctypes.c_double, vp, vp, vp]
         L.or_hrl1D.argtypes = [ctypes.c_int] * 5 + [vp, ctypes.c_int, vp, vp]
+        L.or_format.argtypes = [cp, vp, vp, ctypes.c_int]
+        L.or_format.restype = None
         _lib = L
     return _lib
 
@@ -151,6 +153,20 @@ class Problem:
         du = np.zeros_like(f)
         lib().or_solve(ctypes.byref(self.cfg), dt, _dptr(self._keep["Minv"]), _dptr(np.ascontiguousarray(f)), _dptr(du))
         return du
+
+    def format(self, U: np.ndarray) -> np.ndarray:
+        """PRINTER formatter: the printed primitive fields of U, node by node."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        out = np.zeros_like(U)
+        lib().or_format(ctypes.byref(self.cfg), _dptr(U), _dptr(out), 0)
+        return out
+
+    def unformat(self, out: np.ndarray) -> np.ndarray:
+        """Inverse formatter (restart): the conserved state from printed fields."""
+        out = np.array(out, dtype=np.float64, copy=True)
+        U = np.zeros_like(out)
+        lib().or_format(ctypes.byref(self.cfg), _dptr(U), _dptr(out), 1)
+        return U
 
     def limit_hr(self, U: np.ndarray) -> np.ndarray:
         U = np.array(U, dtype=np.float64, copy=True)

@@ -922,4 +922,37 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
   return steps;
 }
 
+// PRINTER::print formatter (printer_twod_multifluid.cu:20-110,
+// printer_twod_singlefluid.cu:20-90, printer_oned_*.cu): node-wise
+// conserved <-> printed primitive fields, GAMNCON (gamma = 1 + 1/alpha).
+void or_format(const or_cfg* c, double* U, double* out, int inverse) {
+  const int NF = c->N_F, Ns = c->N_s, D = c->D, mf = c->physics == 1;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < Ns; i++) {
+      auto u = [&](int f) -> double& { return U[(e * NF + f) * Ns + i]; };
+      auto o = [&](int f) -> double& { return out[(e * NF + f) * Ns + i]; };
+      if (!inverse) {
+        const double rho = u(0), ux = u(1) / rho;
+        const double uy = D == 2 ? u(2) / rho : 0.0;
+        const double et = u(D + 1);
+        const double gamma = mf ? 1 + 1.0 / u(D + 2) : GAMMA_SF;
+        o(0) = rho;
+        o(1) = ux;
+        if (D == 2) o(2) = uy;
+        if (mf) o(D + 1) = gamma;
+        const double ke = D == 2 ? 0.5 * (ux * ux + uy * uy) * rho : 0.5 * ux * ux * rho;
+        o(D + 1 + mf) = (gamma - 1) * (et - ke);
+      } else {
+        const double rho = o(0), ux = o(1), uy = D == 2 ? o(2) : 0.0;
+        const double gamma = mf ? o(D + 1) : GAMMA_SF;
+        const double p = o(D + 1 + mf);
+        u(0) = rho;
+        u(1) = rho * ux;
+        if (D == 2) u(2) = rho * uy;
+        u(D + 1) = D == 2 ? p / (gamma - 1) + 0.5 * rho * (ux * ux + uy * uy) : p / (gamma - 1) + 0.5 * rho * ux * ux;
+        if (mf) u(D + 2) = 1.0 / (gamma - 1);
+      }
+    }
+}
+
 }  // extern "C"

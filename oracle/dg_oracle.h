@@ -70,6 +70,11 @@ void or_limit_flagged(const or_cfg* c, const or_tables* t, const int* flags, dou
 void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors); /* Limiting::HRIlimiting (sensors returned for inspection) */
 
+/* PRINTER formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110, GAMNCON):
+ * inverse 0: out = primitive output fields of U per node (rho, ux[, uy][, gamma], p);
+ * inverse 1: U = conserved fields from out (the restart direction). */
+void or_format(const or_cfg* c, double* U, double* out, int inverse);
+
 /* RK4 time integration (rk.cc:132-247).
  * mode 0: fixed Dt for nsteps steps; mode 1: CFL-based Dt until T reaches tf (Dt clamped),
  * or for at most nsteps steps when nsteps > 0 (a cap the reference does not have).

@@ -146,3 +146,23 @@ def test_sensors_and_hri_bitwise(c, order):
             U_or, s2 = prob.limit_hri(U, th)
             assert np.array_equal(s2, sens)
             assert np.array_equal(U_or, U_ref)
+
+
+@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+def test_printer_formatter_bitwise(c):
+    """PRINTER formatter (Lformater, printer_*.cu) both ways: conserved ->
+    printed fields, and the restart direction."""
+    R = oracle.ref_lib(*c)
+    R.Lformater.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_bool]
+    R.Lformater.restype = None
+    for seed, riem in ((5, False), (6, True)):
+        mesh, t, prob, U = _setup(c, 2, seed, riemann=riem)
+        out_ref = np.zeros_like(U)
+        Uc = U.copy()
+        R.Lformater(prob.cfg.N_s, prob.cfg.N_E, P(Uc), P(out_ref), False)
+        out = prob.format(U)
+        assert np.array_equal(out, out_ref)
+        back_ref = np.zeros_like(U)
+        R.Lformater(prob.cfg.N_s, prob.cfg.N_E, P(back_ref), P(out_ref), True)
+        assert np.array_equal(prob.unformat(out), back_ref)
+        assert np.allclose(back_ref, U, rtol=1e-13, atol=1e-13)
