@@ -148,8 +148,11 @@ __device__ inline void riemann<2, 0, 0>(const double* uL, const double* uR, doub
 }
 template <>
 __device__ inline void riemann<2, 0, 1>(const double* uL, const double* uR, double nx, double ny, double* F, double* nc) {
+  // divisions by a common divisor as products with its reciprocal (FP64
+  // division is a long sequence here; a few ulps against the reference)
   double rhoL = uL[0], rhoR = uR[0];
-  double vxL = uL[1] / uL[0], vxR = uR[1] / uR[0], vyL = uL[2] / uL[0], vyR = uR[2] / uR[0];
+  const double irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+  double vxL = uL[1] * irL, vxR = uR[1] * irR, vyL = uL[2] * irL, vyR = uR[2] * irR;
   double EtL = uL[3], EtR = uR[3];
   double tx = -ny, ty = nx;
   double vnL = vxL * nx + vyL * ny, vnR = vxR * nx + vyR * ny;
@@ -157,19 +160,21 @@ __device__ inline void riemann<2, 0, 1>(const double* uL, const double* uR, doub
   double gamma = kGammaSF;
   double pL = (gamma - 1) * (EtL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
   double pR = (gamma - 1) * (EtR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
-  double HL = (EtL + pL) / rhoL, HR = (EtR + pR) / rhoR;
-  double RT = sqrt(rhoR / rhoL);
+  double HL = (EtL + pL) * irL, HR = (EtR + pR) * irR;
+  double RT = sqrt(rhoR * irL);
+  const double i1RT = 1.0 / (1 + RT);
   double rho = RT * rhoL;
-  double vx = (vxL + RT * vxR) / (1 + RT);
-  double vy = (vyL + RT * vyR) / (1 + RT);
+  double vx = (vxL + RT * vxR) * i1RT;
+  double vy = (vyL + RT * vyR) * i1RT;
   double vn = vx * nx + vy * ny, vt = vx * tx + vy * ty;
-  double H = (HL + RT * HR) / (1 + RT);
+  double H = (HL + RT * HR) * i1RT;
   double a = sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
+  const double ia = 1.0 / a, ia2 = ia * ia;
   double drho = rhoR - rhoL, dp = pR - pL, dvn = vnR - vnL, dvt = vtR - vtL;
-  double dV0 = (dp - rho * a * dvn) / (2 * a * a);
-  double dV1 = rho * dvt / a;
-  double dV2 = drho - dp / (a * a);
-  double dV3 = (dp + rho * a * dvn) / (2 * a * a);
+  double dV0 = (dp - rho * a * dvn) * (0.5 * ia2);
+  double dV1 = rho * dvt * ia;
+  double dV2 = drho - dp * ia2;
+  double dV3 = (dp + rho * a * dvn) * (0.5 * ia2);
   double w0 = fabs(vn - a) * dV0, w1 = fabs(vn) * dV1, w2 = fabs(vn) * dV2, w3 = fabs(vn + a) * dV3;
   F[0] = 0.5 * (rhoL * vnL + rhoR * vnR) - 0.5 * (w0 * 1.0 + w1 * 0.0 + w2 * 1.0 + w3 * 1.0);
   F[1] = 0.5 * ((rhoL * vnL * vxL + pL * nx) + (rhoR * vnR * vxR + pR * nx)) -
@@ -204,8 +209,11 @@ __device__ inline void riemann<2, 1, 0>(const double* uL, const double* uR, doub
 }
 template <>
 __device__ inline void riemann<2, 1, 1>(const double* uL, const double* uR, double nx, double ny, double* F, double* nc) {
+  // divisions by a common divisor as products with its reciprocal (FP64
+  // division is a long sequence here; a few ulps against the reference)
   double rhoL = uL[0], rhoR = uR[0];
-  double vxL = uL[1] / uL[0], vxR = uR[1] / uR[0], vyL = uL[2] / uL[0], vyR = uR[2] / uR[0];
+  const double irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+  double vxL = uL[1] * irL, vxR = uR[1] * irR, vyL = uL[2] * irL, vyR = uR[2] * irR;
   double EtL = uL[3], EtR = uR[3], alphaL = uL[4], alphaR = uR[4];
   double tx = -ny, ty = nx;
   double vnL = vxL * nx + vyL * ny, vnR = vxR * nx + vyR * ny;
@@ -213,26 +221,28 @@ __device__ inline void riemann<2, 1, 1>(const double* uL, const double* uR, doub
   double gammaL = 1.0 + 1.0 / alphaL, gammaR = 1.0 + 1.0 / alphaR;
   double pL = (gammaL - 1) * (EtL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
   double pR = (gammaR - 1) * (EtR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
-  double HL = (EtL + pL) / rhoL, HR = (EtR + pR) / rhoR;
+  double HL = (EtL + pL) * irL, HR = (EtR + pR) * irR;
   double iL = pL * alphaL, iR = pR * alphaR;
-  double RT = sqrt(rhoR / rhoL);
+  double RT = sqrt(rhoR * irL);
+  const double i1RT = 1.0 / (1 + RT);
   double rho = RT * rhoL;
-  double vx = (vxL + RT * vxR) / (1 + RT);
-  double vy = (vyL + RT * vyR) / (1 + RT);
+  double vx = (vxL + RT * vxR) * i1RT;
+  double vy = (vyL + RT * vyR) * i1RT;
   double vn = vx * nx + vy * ny, vt = vx * tx + vy * ty;
-  double H = (HL + RT * HR) / (1 + RT);
-  double alpha = (alphaL + RT * alphaR) / (1 + RT);
+  double H = (HL + RT * HR) * i1RT;
+  double alpha = (alphaL + RT * alphaR) * i1RT;
   double gamma = 1 + 1.0 / alpha;
   double a = sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
-  double i = (iL + RT * iR) / (1 + RT);
+  const double ia = 1.0 / a, ia2 = ia * ia;
+  double i = (iL + RT * iR) * i1RT;
   double p = (gamma - 1) * i;
   double drho = rhoR - rhoL;
   double dp = (gamma - 1) * (gamma - 1) * (alpha * (iR - iL) - i * (alphaR - alphaL));
   double dvn = vnR - vnL, dvt = vtR - vtL;
-  double dV0 = (dp - rho * a * dvn) / (2 * a * a);
-  double dV1 = rho * dvt / a;
-  double dV2 = drho - dp / (a * a);
-  double dV3 = (dp + rho * a * dvn) / (2 * a * a);
+  double dV0 = (dp - rho * a * dvn) * (0.5 * ia2);
+  double dV1 = rho * dvt * ia;
+  double dV2 = drho - dp * ia2;
+  double dV3 = (dp + rho * a * dvn) * (0.5 * ia2);
   double dV4 = alphaR - alphaL;
   double w0 = fabs(vn - a) * dV0, w1 = fabs(vn) * dV1, w2 = fabs(vn) * dV2, w3 = fabs(vn + a) * dV3,
          w4 = fabs(vn) * dV4;
@@ -269,7 +279,9 @@ __device__ inline void volume_flux(const double* u, const double* dua, const dou
       salpha = 0;
     }
   } else {
-    double rho = u[0], vx = u[1] / rho, vy = u[2] / rho, Et = u[3];
+    double rho = u[0];
+    const double ir = 1.0 / rho;
+    double vx = u[1] * ir, vy = u[2] * ir, Et = u[3];
     double gamma = (PHYS == 0) ? kGammaSF : 1 + 1.0 / u[4];
     double p = (gamma - 1) * (Et - 0.5 * rho * (vx * vx + vy * vy));
     f[0 * 2 + 0] = rho * vx;

@@ -249,6 +249,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   __shared__ double sMinv[Ns * Ns];
   __shared__ double sPsi[MG * Ms], sPsiw[MG * Ms];
   __shared__ double sGeo[NT * NGEO];  // per element: J, invJac
+  __shared__ double sDte[NT];         // per element: Dt (J_0/J_e with the shared Minv)
   __shared__ double sFG[NFACE * 4];   // per tile face: n_x, n_y, JF side 0, JF side 1
   const VolTables<D, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
 
@@ -340,6 +341,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     };
     commit();
     __syncthreads();
+    // the output's Dt factor once per element (read after several barriers)
+    if (MODE == 1 && tid < NT) sDte[tid] = shared_minv ? dt * (a.J0 / sGeo[tid * NGEO]) : dt;
     // RK operands of this thread's output entries, then the next tile's
     // inputs: all in flight during the compute below
     double pre_un[R_U], pre_acc[R_U];
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
         a.fout[gi] = sU[el * SU + rr];
       } else {
         // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
-        const double dte = shared_minv ? dt * (a.J0 / sGeo[el * NGEO]) : dt;
+        const double dte = sDte[el];
         const double* M = shared_minv ? tb.minv
                                       : a.Minv + ((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * Ns * Ns;
         const int f = rr / Ns, i = rr % Ns;
