@@ -103,13 +103,13 @@ def main() -> int:
     solver.dt_from_cfl(U, cfl, dt_dev, stream=stream)
     torch.cuda.synchronize()
     dt = float(dt_dev.item())
-    lim = {"auto": w["limiter"], "none": 0, "hr": 1, "hri": 3, "learned": 4}[args.limiter]
+    lim = {"auto": w["limiter"], "none": 0, "hr": 1, "hri": 3, "learned": 8}[args.limiter]
     extra = {}
     net = None
-    if lim in (3, 4):
+    if lim in (3, 8):
         from dg_amd import indicator
         solver.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
-    if lim == 4:  # train the indicator on the GPU (own forward/backward kernels), outside the timed region
+    if lim == 8:  # train the indicator on the GPU (own forward/backward kernels), outside the timed region
         t_tr = time.perf_counter()
         data = indicator.merge(indicator.generate(), indicator.interface_data(), indicator.smooth_data())
         net, losses, thr = indicator.train(data, epochs=40)
@@ -138,11 +138,11 @@ def main() -> int:
     ms_per_step = 1e3 * elapsed / args.steps
     value = n_dof * args.steps / elapsed
 
-    if lim in (3, 4):  # the plain sensor judges the flags of the final state
+    if lim in (3, 8):  # the plain sensor judges the flags of the final state
         from dg_amd import indicator
         sens = torch.zeros(N_E, dtype=torch.int32, device=device)
         solver.sensing(U, sens)
-        if lim == 4:
+        if lim == 8:
             fl = torch.zeros_like(sens)
             solver.indicator_flags(U, fl)
             torch.cuda.synchronize()
@@ -151,7 +151,7 @@ def main() -> int:
         else:
             torch.cuda.synchronize()
             extra["sensor_flagged_final_state"] = int((sens.cpu().numpy() != 0).sum())
-    if args.precision == "fp32mfma" or lim in (3, 4):
+    if args.precision == "fp32mfma" or lim in (3, 8):
         # the plain FP64 GPU form (global HR) on the same workload and steps, timed here too
         sp = Solver(cfg, mesh)
         Up = torch.from_numpy(U0).to(device)
@@ -167,9 +167,9 @@ def main() -> int:
     if args.precision == "fp32mfma":
         # precision error: the same limiter in FP64 from the same state, warmup + K steps
         sf = Solver(cfg, mesh)
-        if lim in (3, 4):
+        if lim in (3, 8):
             sf.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
-        if lim == 4:
+        if lim == 8:
             sf.set_indicator(net, extra["indicator"]["threshold"])
         Uf = torch.from_numpy(U0).to(device)
         make_advance(sf, Uf, lim)(args.warmup + args.steps)
@@ -243,7 +243,7 @@ def main() -> int:
             "workload": w["name"],
             "elements": N_E, "dofs": n_dof, "dt": dt, "parallelism": "single GPU",
             "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
-            "limiter": {0: "none", 1: "global HR", 3: "sensor-gated HR", 4: "HR gated by the learned indicator"}[lim],
+            "limiter": {0: "none", 1: "global HR", 3: "sensor-gated HR", 8: "HR gated by the learned indicator"}[lim],
             "contractions": ("FP32 MFMA (FP64 fluxes, RK and limiter)" if args.precision == "fp32mfma" else
                              "FP64 MFMA element phases" if tiled and w["physics"] == 0 and w["order"] == 2 and w["dim"] == 2
                              else "FP64"),

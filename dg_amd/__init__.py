@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(_HERE, "libdgx.so")
 SINGLEFLUID, MULTIFLUID = 0, 1
 RUSANOV, ROE = 0, 1
 PERIODIC, FARFIELD, REFLECTIVE = 1, 2, 3
-LIMIT_NONE, LIMIT_HR, LIMIT_HRI, LIMIT_LEARNED = 0, 1, 3, 4
+# limiter codes of the reference (main.cc:167-175) and the learned indicator (no reference code)
+LIMIT_NONE, LIMIT_HR, LIMIT_M2, LIMIT_HRI, LIMIT_PR, LIMIT_LEARNED = 0, 1, 2, 3, 5, 8
 PATH_AUTO, PATH_GENERAL, PATH_TILED = 0, 1, 2
 
 
@@ -304,7 +305,7 @@ class Solver:
         return lib().dgx_solver_path(self._h)
 
     def set_indicator(self, mlp, threshold: float) -> None:
-        """Attach a learned indicator (Indicator) for limiter 4; None detaches."""
+        """Attach a learned indicator (Indicator) for limiter 8; None detaches."""
         self._mlp = mlp
         _check(lib().dgx_solver_set_indicator(self._h, mlp._h if mlp else None, threshold),
                "dgx_solver_set_indicator")

@@ -423,18 +423,25 @@ dgx_status dgx_set_sensor_thresholds(dgx_solver* h, int n, const double* th) {
 static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
   if (limiter == 0) return DGX_OK;
   if (limiter == 1) return hip_ok(launch_limit_hr(s, U, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
+  if (limiter == 2 || limiter == 5) {
+    if (s.cfg.physics != DGX_MULTIFLUID)
+      return fail(DGX_EUNSUPPORTED, "m2l and prl limit two-fluid states (the reference implements them for GAMNCON)");
+    if (limiter == 2) return hip_ok(launch_limit_m2(s, U, st), "m2 limiter") ? DGX_OK : DGX_EDEVICE;
+    return hip_ok(launch_limit_pr(s, U, st), "primitive HR limiter") ? DGX_OK : DGX_EDEVICE;
+  }
   if (limiter == 3) {
     if (s.cfg.physics != DGX_MULTIFLUID)
       return fail(DGX_EUNSUPPORTED, "hri limiting needs the multifluid sensor (single-fluid sensors read past Uavg in "
                                     "the reference, sensor_kernels.cu:55-60)");
     return hip_ok(launch_limit_hri(s, U, st), "HRI limiter") ? DGX_OK : DGX_EDEVICE;
   }
-  if (limiter == 4) {
-    if (!s.mlp) return fail(DGX_EUNSUPPORTED, "limiter 4 needs an indicator (dgx_solver_set_indicator)");
+  if (limiter == 8) {
+    if (!s.mlp) return fail(DGX_EUNSUPPORTED, "limiter 8 needs an indicator (dgx_solver_set_indicator)");
     if (!hip_ok(mlp_indicator_flags(s, U, st), "indicator")) return DGX_EDEVICE;
     return hip_ok(launch_limit_flagged(s, U, st), "flagged HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
-  return fail(DGX_EUNSUPPORTED, "limiter must be 0 (none), 1 (hrl), 3 (hri) or 4 (learned indicator + HR)");
+  return fail(DGX_EUNSUPPORTED,
+              "limiter must be 0 (none), 1 (hrl), 2 (m2l), 3 (hri), 5 (prl) or 8 (learned indicator + HR)");
 }
 
 dgx_status dgx_limit(dgx_solver* h, int limiter, double* U, void* stream) {

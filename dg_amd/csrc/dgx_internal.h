@@ -88,12 +88,13 @@ struct Solver {
   int* sens_act = nullptr;  // [e*2] (ping-pong)
   int* flag = nullptr;      // device flag / counters: [0] sensor relaxation, [2] flagged-element count
   int* flag_list = nullptr; // [N_E] flagged elements (launch_limit_flagged)
+  double* m2buf = nullptr;  // m2 limiting: 15 per-field arrays of N_E*N_s (allocated on first use)
   double* dscratch = nullptr;  // device scalars: [1] Dt
   double* dt_partial = nullptr;  // per-block maxima of |u|+a (ceil(N_E*N_s/256))
   int n_thresh = 0;
   double thresh[3] = {0, 0, 0};
   std::vector<void*> allocations;
-  // learned indicator (limiter 4): network (not owned), threshold, scratch
+  // learned indicator (limiter 8): network (not owned), threshold, scratch
   dgx_mlp* mlp = nullptr;
   float mlp_threshold = 0.0f;
   double* mlp_avg = nullptr;  // [e*3 + {rho, p, alpha}] cell means
@@ -122,6 +123,8 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st);
 hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st);
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st);
+hipError_t launch_limit_m2(Solver& s, double* U, hipStream_t st);  // m2l, two-fluid
+hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st);  // prl, two-fluid
 // HR sweeps on the cells flagged in s.sensors (the HRI sweeps without the sensor)
 hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st);
 // learned indicator (mlp.hip): features of U into X, and flags into s.sensors

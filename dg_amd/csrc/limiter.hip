@@ -241,6 +241,96 @@ __global__ __launch_bounds__(256) void k_hr2d(int nx, int NF, int N_N, const int
   }
 }
 
+// ------------------------------------------------- m2 and primitive limiting
+// Limiting::M2limiting (limiting.cc:123-594, two-fluid GAMNCON) and
+// PRlimiting (limiting.cc:102-121), launcher by launcher as the reference
+// runs them on per-field arrays x[e*Ns+i]: the per-field HR reuses k_apply
+// and k_hrl (one field at a time, as the reference calls hrl1D), and these
+// kernels restate the rest.
+__global__ __launch_bounds__(256) void k_field_get(long n, int Ns, int NF, int f, const double* __restrict__ U,
+                                                   double* __restrict__ x) {  // Lstridedcopy U -> field
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = U[((k / Ns) * NF + f) * Ns + k % Ns];
+}
+__global__ __launch_bounds__(256) void k_field_put(long n, int Ns, int NF, int f, const double* __restrict__ x,
+                                                   double* __restrict__ U) {  // field -> U
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) U[((k / Ns) * NF + f) * Ns + k % Ns] = x[k];
+}
+// pressure (physics.cu:891-960), GAMNCON
+template <int D>
+__global__ __launch_bounds__(256) void k_m2_pressure(long n, int Ns, const double* __restrict__ U, double* __restrict__ p) {
+  constexpr int NF = 2 + D + 1;
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* u = U + (k / Ns) * NF * Ns + k % Ns;
+  if (D == 1) {
+    const double rho = u[0], rhou = u[Ns], E = u[2 * Ns], gamma = 1.0 + 1.0 / u[3 * Ns];
+    p[k] = (gamma - 1) * (E - 0.5 * rhou * rhou / rho);
+  } else {
+    const double rho = u[0], rhou = u[Ns], rhov = u[2 * Ns], E = u[3 * Ns], gamma = 1.0 + 1.0 / u[4 * Ns];
+    p[k] = (gamma - 1) * (E - 0.5 * (rhou * rhou + rhov * rhov) / rho);
+  }
+}
+// kinetic_energy1D / kinetic_energy2D (physics.cu:814-890)
+template <int D>
+__global__ __launch_bounds__(256) void k_m2_kinetic(long n, const double* __restrict__ rho, const double* __restrict__ rhou,
+                                                    const double* __restrict__ rhov, double* __restrict__ K) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  K[k] = D == 1 ? 0.5 * rhou[k] * rhou[k] / rho[k] : 0.5 * (rhou[k] * rhou[k] + rhov[k] * rhov[k]) / rho[k];
+}
+__device__ inline int binomial_d(int n, int k) {
+  int r = 1;
+  for (int j = 1; j <= k; j++) r = r * (n - k + j) / j;
+  return r;
+}
+// internal_energy_multifluid (limiting_kernels.cu:160-200): one thread per (element, slice)
+__global__ __launch_bounds__(256) void k_m2_internal(int Ns, int N_E, int slicenum, const double* __restrict__ p,
+                                                     const double* __restrict__ g, double* __restrict__ rhoe) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)N_E * slicenum) return;
+  const long o = k * Ns;  // (e*slicenum + slice)*Ns
+  for (int i = 0; i < Ns; i++) {
+    double I = 0;
+    for (int q = 0; q < i + 1; q++) I += (double)binomial_d(i, q) * p[o + i - q] * g[o + q];
+    rhoe[o + i] = I;
+  }
+}
+// reconstruct_energy (limiting_kernels.cu:111-158)
+__global__ __launch_bounds__(256) void k_m2_reconstruct(int Ns, int N_E, int slicenum, const double* __restrict__ rhoeLim,
+                                                        const double* __restrict__ KLim, const double* __restrict__ EMono,
+                                                        double* __restrict__ ELim) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)N_E * slicenum) return;
+  const long o = k * Ns;
+  for (int i = 1; i < Ns; i++) ELim[o + i] = rhoeLim[o + i] + KLim[o + i];
+  double E0 = EMono[o];
+  for (int q = 2; q < Ns; q += 2) E0 -= 1.0 / ((double)ifact(q + 1)) * (ELim[o + q] - EMono[o + q]);
+  ELim[o] = E0;
+}
+// Cons2Prim / Prim2Cons (limiting_kernels.cu:957-1068), two-fluid GAMNCON
+template <int D>
+__global__ __launch_bounds__(256) void k_cons_prim(long n, int Ns, int to_prim, double* __restrict__ U) {
+  constexpr int NF = 2 + D + 1;
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double* u = U + (k / Ns) * NF * Ns + k % Ns;
+  const double beta = 0;
+  if (to_prim) {
+    const double rho = u[0], rhou = u[Ns], rhov = D == 2 ? u[2 * Ns] : 0.0, E = u[(D + 1) * Ns],
+                 alpha = u[(D + 2) * Ns];
+    u[Ns] = rhou / rho;
+    if (D == 2) u[2 * Ns] = rhov / rho;
+    u[(D + 1) * Ns] = (E - beta - 0.5 * (rhou * rhou + rhov * rhov) / rho) / alpha;
+  } else {
+    const double rho = u[0], uu = u[Ns], v = D == 2 ? u[2 * Ns] : 0.0, p = u[(D + 1) * Ns], alpha = u[(D + 2) * Ns];
+    u[Ns] = rho * uu;
+    if (D == 2) u[2 * Ns] = rho * v;
+    u[(D + 1) * Ns] = p * alpha + beta + 0.5 * rho * (uu * uu + v * v);
+  }
+}
+
 // ---------------------------------------------------------------- sensors
 // Uavg = sum_l (1/N_s * U_l), sequential (blasGemm M=1, sensor.cc:29)
 template <int Ns>
@@ -575,6 +665,94 @@ hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStre
     }
     return hipErrorInvalidValue;
   });
+}
+
+// Limiting::M2limiting: the reference's launch sequence on per-field arrays
+// (s.m2buf: 15 arrays of N_E*N_s doubles, allocated on first use).
+hipError_t launch_limit_m2(Solver& s, double* U, hipStream_t st) {
+  if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;
+  const long n = (long)s.N_E * s.N_s;
+  if (!s.m2buf) {
+    hipError_t e = hipMalloc(&s.m2buf, sizeof(double) * 15 * n);
+    if (e != hipSuccess) return e;
+    s.allocations.push_back(s.m2buf);
+  }
+  double* b = s.m2buf;
+  double *rho = b, *rhou = b + n, *rhov = b + 2 * n, *E = b + 3 * n, *gam = b + 4 * n, *pres = b + 5 * n,
+         *K = b + 6 * n, *mono = b + 7 * n, *lim = b + 8 * n, *gLim = b + 9 * n, *pLim = b + 10 * n,
+         *KLim = b + 11 * n, *EMono = b + 12 * n, *rhoeLim = b + 13 * n, *ELim = b + 14 * n;
+  const dim3 g(nblk(n, 256)), bl(256);
+  const int NF = s.NF, Ns = s.N_s, NE = s.N_E, NN = s.N_N, n1 = s.N1;
+  const int* nb = s.t.neighbors;
+  return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    constexpr int D = decltype(d)::value, P = decltype(p)::value, Nsc = Shape<D, P>::Ns;
+    auto get = [&](int f, double* x) { hipLaunchKernelGGL(k_field_get, g, bl, 0, st, n, Ns, NF, f, U, x); };
+    auto put = [&](int f, const double* x) { hipLaunchKernelGGL(k_field_put, g, bl, 0, st, n, Ns, NF, f, x, U); };
+    auto gemm = [&](const double* M, const double* X, double* Y) {
+      hipLaunchKernelGGL((k_apply<Nsc>), dim3(nblk(NE, 256)), bl, 0, st, M, X, Y, (long)NE);
+    };
+    const int sl = D == 1 ? 1 : n1, hs = D == 1 ? Ns : n1;  // slices and slice length
+    auto hrl = [&](int offxy, const double* A, double* Alim) {
+      hipLaunchKernelGGL((k_hrl<P>), dim3(nblk((long)NE * sl, 256)), bl, 0, st, NE, 1, NN, sl, nb, offxy, A, Alim);
+    };
+    auto hr = [&](const double* L2M, const double* M2L, int offxy, double* x, double* Lm, bool back) {
+      gemm(L2M, x, mono);
+      hrl(offxy, mono, Lm);
+      if (back) gemm(M2L, Lm, x);
+    };
+    auto energy = [&](const double* L2M, const double* M2L) {
+      hipLaunchKernelGGL((k_m2_kinetic<D>), g, bl, 0, st, n, rho, rhou, rhov, K);
+      gemm(L2M, K, KLim);
+      hipLaunchKernelGGL(k_m2_internal, dim3(nblk((long)NE * sl, 256)), bl, 0, st, hs, NE, sl, pLim, gLim, rhoeLim);
+      gemm(L2M, E, EMono);
+      hipLaunchKernelGGL(k_m2_reconstruct, dim3(nblk((long)NE * sl, 256)), bl, 0, st, hs, NE, sl, rhoeLim, KLim,
+                         EMono, ELim);
+      gemm(M2L, ELim, E);
+    };
+    if constexpr (D == 1) {
+      const double *L2M = s.t.Lag2Mono, *M2L = s.t.Mono2Lag;
+      get(0, rho); hr(L2M, M2L, 0, rho, lim, true);
+      get(1, rhou); hr(L2M, M2L, 0, rhou, lim, true);
+      get(2, E);
+      get(3, gam); hr(L2M, M2L, 0, gam, gLim, true);
+      hipLaunchKernelGGL((k_m2_pressure<1>), g, bl, 0, st, n, Ns, U, pres);
+      hr(L2M, M2L, 0, pres, pLim, false);
+      energy(L2M, M2L);
+      put(0, rho); put(1, rhou); put(2, E); put(3, gam);
+    } else {
+      const double *LX = s.t.Lag2MonoX, *XL = s.t.MonoX2Lag, *LY = s.t.Lag2MonoY, *YL = s.t.MonoY2Lag;
+      get(0, rho); hr(LX, XL, 0, rho, lim, true);
+      get(1, rhou); hr(LX, XL, 0, rhou, lim, true);
+      get(2, rhov); hr(LX, XL, 0, rhov, lim, true);
+      get(4, gam); hr(LX, XL, 0, gam, gLim, true);
+      hipLaunchKernelGGL((k_m2_pressure<2>), g, bl, 0, st, n, Ns, U, pres);
+      hr(LX, XL, 0, pres, pLim, true);
+      get(3, E);
+      energy(LX, XL);
+      hr(LY, YL, 2, rho, lim, true);
+      hr(LY, YL, 2, rhou, lim, true);
+      hr(LY, YL, 2, rhov, lim, true);
+      hr(LY, YL, 2, gam, gLim, true);
+      hr(LY, YL, 2, pres, pLim, false);
+      energy(LY, YL);
+      put(0, rho); put(1, rhou); put(2, rhov); put(3, E); put(4, gam);
+    }
+    return hipGetLastError();
+  });
+}
+
+// Limiting::PRlimiting: HR of the primitive variables.
+hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st) {
+  if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;
+  const long n = (long)s.N_E * s.N_s;
+  const dim3 g(nblk(n, 256)), bl(256);
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 1, U);
+  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 1, U);
+  hipError_t e = launch_limit_hr(s, U, st);
+  if (e != hipSuccess) return e;
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 0, U);
+  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 0, U);
+  return hipGetLastError();
 }
 
 // Resolve sequential-order flags from per-element first hits.

@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const uint16_t* __restrict__ w1
 }
 
 // ------------------------------------------------- fused indicator pass
-// Features and network in one pass (the solver's limiter 4): each wave
+// Features and network in one pass (the solver's limiter 8): each wave
 // stages 16 consecutive elements, forms their modes on the FP64 matrix
 // cores (wave_modes) and lane (column c, group q) picks up exactly the 16
 // features of element c that its MFMA B fragments hold (k = 16s + 4q + j);
@@ -683,7 +683,7 @@ uint64_t splitmix(uint64_t& s) {
 namespace dgx {
 
 // Indicator state of a solver: features, logits and the flags that gate the
-// sensor-driven HR sweeps (limiter 4).
+// sensor-driven HR sweeps (limiter 8).
 hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st) {
   dgx_mlp* m = s.mlp;
   const int N = s.N_E;

@@ -132,7 +132,11 @@ dgx_status dgx_rhs(dgx_solver* s, const double* U, double* f_rk, void* stream);
 /* Sensor thresholds (deck "#sensor thresholds", sensor.h:49-53); n = 0..3. */
 dgx_status dgx_set_sensor_thresholds(dgx_solver* s, int n, const double* thresholds);
 
-/* limiter: 0 none, 1 hrl (global HR), 3 hri (sensor-gated HR), 4 HR gated by the learned indicator */
+/* limiter (the reference's codes, main.cc:167-175): 0 none, 1 hrl (global HR, Limiting::HRlimiting),
+ * 2 m2l (two-fluid: HR of rho, momenta, 1/(gamma-1) and pressure, energy rebuilt from the limited internal and
+ * kinetic energies; Limiting::M2limiting, limiting.cc:123-594), 3 hri (sensor-gated HR), 5 prl (two-fluid: HR of
+ * the primitive variables; Limiting::PRlimiting, limiting.cc:102-121); and 8, HR gated by the learned
+ * indicator (no reference code).  m2i, pri and p0i (4, 6, 7) are not provided. */
 dgx_status dgx_limit(dgx_solver* s, int limiter, double* U, void* stream);
 /* Out = limited In (In unchanged; In != Out, both state-sized device arrays).
  * hrl on a 2D Cartesian tile mesh runs both sweeps in one launch (the same
@@ -243,7 +247,7 @@ dgx_status dgx_mlp_gradient(dgx_mlp* m, const float* X, const float* y, long n, 
  * (host float[epochs]) receives the mean loss of each epoch */
 dgx_status dgx_mlp_train(dgx_mlp* m, const float* X, const float* y, long n, int epochs, int batch, float lr,
                          float pos_weight, unsigned long long seed, float* epoch_loss, void* stream);
-/* Attach (not owned) an indicator to a solver; limiter 4 in dgx_limit and the
+/* Attach (not owned) an indicator to a solver; limiter 8 in dgx_limit and the
  * RK drivers then flags cells with z > threshold and applies the HR sweeps
  * of hri to them (limiting.cc:596-634 with the flags in place of the sensor). */
 dgx_status dgx_solver_set_indicator(dgx_solver* s, dgx_mlp* m, float threshold);

@@ -76,6 +76,10 @@ def lib() -> ctypes.CDLL:
         L.or_hrl1D.argtypes = [ctypes.c_int] * 5 + [vp, ctypes.c_int, vp, vp]
         L.or_format.argtypes = [cp, vp, vp, ctypes.c_int]
         L.or_format.restype = None
+        L.or_limit_m2.argtypes = [cp, tp, vp]
+        L.or_limit_m2.restype = None
+        L.or_limit_pr.argtypes = [cp, tp, vp]
+        L.or_limit_pr.restype = None
         _lib = L
     return _lib
 
@@ -153,6 +157,18 @@ class Problem:
         du = np.zeros_like(f)
         lib().or_solve(ctypes.byref(self.cfg), dt, _dptr(self._keep["Minv"]), _dptr(np.ascontiguousarray(f)), _dptr(du))
         return du
+
+    def limit_m2(self, U: np.ndarray) -> np.ndarray:
+        """Limiting::M2limiting (two-fluid)."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        lib().or_limit_m2(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(U))
+        return U
+
+    def limit_pr(self, U: np.ndarray) -> np.ndarray:
+        """Limiting::PRlimiting (two-fluid)."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        lib().or_limit_pr(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(U))
+        return U
 
     def format(self, U: np.ndarray) -> np.ndarray:
         """PRINTER formatter: the printed primitive fields of U, node by node."""

@@ -886,7 +886,9 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
   std::vector<int> sens(c->N_E, 0);
   auto limit = [&](double* X) {
     if (limiter == 1) or_limit_hr(c, t, X);
+    else if (limiter == 2) or_limit_m2(c, t, X);
     else if (limiter == 3) or_limit_hri(c, t, on, thresh, X, sens.data());
+    else if (limiter == 5) or_limit_pr(c, t, X);
   };
   double T = 0;
   int steps = 0;
@@ -953,6 +955,177 @@ void or_format(const or_cfg* c, double* U, double* out, int inverse) {
         if (mf) u(D + 2) = 1.0 / (gamma - 1);
       }
     }
+}
+
+// ---- m2 limiting pieces (reference launcher semantics)
+static void m2_field(int N_s, int N_E, int N_F, int f, const double* U, double* x) {  // Lstridedcopy
+  for (int e = 0; e < N_E; e++)
+    for (int i = 0; i < N_s; i++) x[(size_t)e * N_s + i] = U[((size_t)e * N_F + f) * N_s + i];
+}
+static void m2_store(int N_s, int N_E, int N_F, int f, const double* x, double* U) {
+  for (int e = 0; e < N_E; e++)
+    for (int i = 0; i < N_s; i++) U[((size_t)e * N_F + f) * N_s + i] = x[(size_t)e * N_s + i];
+}
+// pressure (physics.cu:891-960), GAMNCON
+static void m2_pressure(const or_cfg* c, const double* U, double* p) {
+  const int N_s = c->N_s, N_F = c->N_F, D = c->D;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < N_s; i++) {
+      auto u = [&](int f) { return U[((size_t)e * N_F + f) * N_s + i]; };
+      if (D == 1) {
+        double rho = u(0), rhou = u(1), E = u(2), gamma = 1.0 + 1.0 / u(3);
+        p[(size_t)e * N_s + i] = (gamma - 1) * (E - 0.5 * rhou * rhou / rho);
+      } else {
+        double rho = u(0), rhou = u(1), rhov = u(2), E = u(3), gamma = 1.0 + 1.0 / u(4);
+        p[(size_t)e * N_s + i] = (gamma - 1) * (E - 0.5 * (rhou * rhou + rhov * rhov) / rho);
+      }
+    }
+}
+// kinetic_energy1D / kinetic_energy2D (physics.cu:814-890)
+static void m2_kinetic(int D, size_t n, const double* rho, const double* rhou, const double* rhov, double* K) {
+  for (size_t k = 0; k < n; k++)
+    K[k] = D == 1 ? 0.5 * rhou[k] * rhou[k] / rho[k] : 0.5 * (rhou[k] * rhou[k] + rhov[k] * rhov[k]) / rho[k];
+}
+static int binomial(int n, int k) {
+  int r = 1;
+  for (int j = 1; j <= k; j++) r = r * (n - k + j) / j;
+  return r;
+}
+// internal_energy_multifluid (limiting_kernels.cu:160-200): Taylor product of p and 1/(gamma-1)
+static void m2_internal(int N_s, int N_E, int slicenum, const double* p, const double* g, double* rhoe) {
+  for (int e = 0; e < N_E; e++)
+    for (int sl = 0; sl < slicenum; sl++)
+      for (int i = 0; i < N_s; i++) {
+        double I = 0;
+        const size_t o = (size_t)e * N_s * slicenum + sl * N_s;
+        for (int k = 0; k < i + 1; k++) I += (double)binomial(i, k) * p[o + i - k] * g[o + k];
+        rhoe[o + i] = I;
+      }
+}
+// reconstruct_energy (limiting_kernels.cu:111-158)
+static void m2_reconstruct(int N_s, int N_E, int slicenum, const double* rhoeLim, const double* KLim,
+                           const double* EMono, double* ELim) {
+  for (int e = 0; e < N_E; e++)
+    for (int sl = 0; sl < slicenum; sl++) {
+      const size_t o = (size_t)e * N_s * slicenum + sl * N_s;
+      for (int i = 1; i < N_s; i++) ELim[o + i] = rhoeLim[o + i] + KLim[o + i];
+      double E0 = EMono[o];
+      for (int k = 2; k < N_s; k += 2) E0 -= 1.0 / ((double)lim_factorial(k + 1)) * (ELim[o + k] - EMono[o + k]);
+      ELim[o] = E0;
+    }
+}
+
+void or_limit_m2(const or_cfg* c, const or_tables* t, double* U) {
+  const int N_s = c->N_s, N_E = c->N_E, N_F = c->N_F, N_N = c->N_N, D = c->D;
+  const size_t n = (size_t)N_s * N_E;
+  std::vector<double> rho(n), rhou(n), rhov(n), E(n), gam(n), pres(n), K(n), mono(n), lim(n, 0.0), gLim(n, 0.0),
+      pLim(n, 0.0), KLim(n), EMono(n), rhoeLim(n), ELim(n, 0.0);
+  auto gemm = [&](const double* M, const double* X, double* Y) {
+    or_dgemm('N', 'N', N_s, N_E, N_s, 1, M, N_s, X, N_s, 0.0, Y, N_s);
+  };
+  if (D == 1) {
+    auto hr = [&](double* X, double* Lim, bool back) {
+      gemm(t->Lag2Mono, X, mono.data());
+      or_hrl1D(N_s, N_E, 1, N_N, 1, t->neighbors, 0, mono.data(), Lim);
+      if (back) gemm(t->Mono2Lag, Lim, X);
+    };
+    m2_field(N_s, N_E, N_F, 0, U, rho.data());
+    hr(rho.data(), lim.data(), true);
+    m2_field(N_s, N_E, N_F, 1, U, rhou.data());
+    hr(rhou.data(), lim.data(), true);
+    m2_field(N_s, N_E, N_F, 2, U, E.data());
+    gemm(t->Lag2Mono, E.data(), EMono.data());
+    m2_field(N_s, N_E, N_F, 3, U, gam.data());
+    hr(gam.data(), gLim.data(), true);
+    m2_pressure(c, U, pres.data());
+    hr(pres.data(), pLim.data(), false);
+    m2_kinetic(1, n, rho.data(), rhou.data(), nullptr, K.data());
+    gemm(t->Lag2Mono, K.data(), KLim.data());
+    m2_internal(N_s, N_E, 1, pLim.data(), gLim.data(), rhoeLim.data());
+    m2_reconstruct(N_s, N_E, 1, rhoeLim.data(), KLim.data(), EMono.data(), ELim.data());
+    gemm(t->Mono2Lag, ELim.data(), E.data());
+    m2_store(N_s, N_E, N_F, 0, rho.data(), U);
+    m2_store(N_s, N_E, N_F, 1, rhou.data(), U);
+    m2_store(N_s, N_E, N_F, 2, E.data(), U);
+    m2_store(N_s, N_E, N_F, 3, gam.data(), U);
+    return;
+  }
+  const int n1 = c->N_s1D;
+  auto hr = [&](const double* L2M, const double* M2L, int offxy, double* X, double* Lim, bool back) {
+    gemm(L2M, X, mono.data());
+    or_hrl1D(n1, N_E, 1, N_N, n1, t->neighbors, offxy, mono.data(), Lim);
+    if (back) gemm(M2L, Lim, X);
+  };
+  // x
+  m2_field(N_s, N_E, N_F, 0, U, rho.data());
+  hr(t->Lag2MonoX, t->MonoX2Lag, 0, rho.data(), lim.data(), true);
+  m2_field(N_s, N_E, N_F, 1, U, rhou.data());
+  hr(t->Lag2MonoX, t->MonoX2Lag, 0, rhou.data(), lim.data(), true);
+  m2_field(N_s, N_E, N_F, 2, U, rhov.data());
+  hr(t->Lag2MonoX, t->MonoX2Lag, 0, rhov.data(), lim.data(), true);
+  m2_field(N_s, N_E, N_F, 4, U, gam.data());
+  hr(t->Lag2MonoX, t->MonoX2Lag, 0, gam.data(), gLim.data(), true);
+  m2_pressure(c, U, pres.data());
+  hr(t->Lag2MonoX, t->MonoX2Lag, 0, pres.data(), pLim.data(), true);
+  m2_kinetic(2, n, rho.data(), rhou.data(), rhov.data(), K.data());
+  gemm(t->Lag2MonoX, K.data(), KLim.data());
+  m2_internal(n1, N_E, n1, pLim.data(), gLim.data(), rhoeLim.data());
+  m2_field(N_s, N_E, N_F, 3, U, E.data());
+  gemm(t->Lag2MonoX, E.data(), EMono.data());
+  m2_reconstruct(n1, N_E, n1, rhoeLim.data(), KLim.data(), EMono.data(), ELim.data());
+  gemm(t->MonoX2Lag, ELim.data(), E.data());
+  // y
+  hr(t->Lag2MonoY, t->MonoY2Lag, 2, rho.data(), lim.data(), true);
+  hr(t->Lag2MonoY, t->MonoY2Lag, 2, rhou.data(), lim.data(), true);
+  hr(t->Lag2MonoY, t->MonoY2Lag, 2, rhov.data(), lim.data(), true);
+  hr(t->Lag2MonoY, t->MonoY2Lag, 2, gam.data(), gLim.data(), true);
+  hr(t->Lag2MonoY, t->MonoY2Lag, 2, pres.data(), pLim.data(), false);
+  m2_kinetic(2, n, rho.data(), rhou.data(), rhov.data(), K.data());
+  gemm(t->Lag2MonoY, K.data(), KLim.data());
+  m2_internal(n1, N_E, n1, pLim.data(), gLim.data(), rhoeLim.data());
+  gemm(t->Lag2MonoY, E.data(), EMono.data());
+  m2_reconstruct(n1, N_E, n1, rhoeLim.data(), KLim.data(), EMono.data(), ELim.data());
+  gemm(t->MonoY2Lag, ELim.data(), E.data());
+  m2_store(N_s, N_E, N_F, 0, rho.data(), U);
+  m2_store(N_s, N_E, N_F, 1, rhou.data(), U);
+  m2_store(N_s, N_E, N_F, 2, rhov.data(), U);
+  m2_store(N_s, N_E, N_F, 3, E.data(), U);
+  m2_store(N_s, N_E, N_F, 4, gam.data(), U);
+}
+
+// Cons2Prim / Prim2Cons (limiting_kernels.cu:957-1068), two-fluid GAMNCON
+// (beta = 0, rhov = v = 0 in 1D, in the reference's expressions)
+static void pr_cons2prim(const or_cfg* c, double* U) {
+  const int N_s = c->N_s, N_F = c->N_F, D = c->D;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < N_s; i++) {
+      auto u = [&](int f) -> double& { return U[((size_t)e * N_F + f) * N_s + i]; };
+      const double beta = 0;
+      double rho = u(0), rhou = u(1), rhov = D == 2 ? u(2) : 0.0, E = u(D + 1), alpha = u(D + 2);
+      u(0) = rho;
+      u(1) = rhou / rho;
+      if (D == 2) u(2) = rhov / rho;
+      u(D + 1) = (E - beta - 0.5 * (rhou * rhou + rhov * rhov) / rho) / alpha;
+    }
+}
+static void pr_prim2cons(const or_cfg* c, double* U) {
+  const int N_s = c->N_s, N_F = c->N_F, D = c->D;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < N_s; i++) {
+      auto u = [&](int f) -> double& { return U[((size_t)e * N_F + f) * N_s + i]; };
+      const double beta = 0;
+      double rho = u(0), uu = u(1), v = D == 2 ? u(2) : 0.0, p = u(D + 1), alpha = u(D + 2);
+      u(0) = rho;
+      u(1) = rho * uu;
+      if (D == 2) u(2) = rho * v;
+      u(D + 1) = p * alpha + beta + 0.5 * rho * (uu * uu + v * v);
+    }
+}
+
+void or_limit_pr(const or_cfg* c, const or_tables* t, double* U) {
+  pr_cons2prim(c, U);
+  or_limit_hr(c, t, U);
+  pr_prim2cons(c, U);
 }
 
 }  // extern "C"

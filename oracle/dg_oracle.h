@@ -70,6 +70,15 @@ void or_limit_flagged(const or_cfg* c, const or_tables* t, const int* flags, dou
 void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors); /* Limiting::HRIlimiting (sensors returned for inspection) */
 
+/* Limiting::M2limiting (limiting.cc:123-594), two-fluid GAMNCON, 1D and 2D Cartesian: per-field HR of rho,
+ * rho u (rho v), 1/(gamma-1) and the pressure, then the energy rebuilt from the limited internal energy
+ * (pressure x 1/(gamma-1), internal_energy_multifluid, limiting_kernels.cu:160-200) and kinetic energy
+ * (kinetic_energy1D/2D, physics.cu:814-890), with the mean kept (reconstruct_energy,
+ * limiting_kernels.cu:111-158); in 2D an x sweep (back to Lagrange) then a y sweep. */
+void or_limit_m2(const or_cfg* c, const or_tables* t, double* U);
+/* Limiting::PRlimiting (limiting.cc:102-121): Cons2Prim, HRlimiting, Prim2Cons (limiting_kernels.cu:957-1068). */
+void or_limit_pr(const or_cfg* c, const or_tables* t, double* U);
+
 /* PRINTER formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110, GAMNCON):
  * inverse 0: out = primitive output fields of U per node (rho, ux[, uy][, gamma], p);
  * inverse 1: U = conserved fields from out (the restart direction). */
@@ -78,7 +87,7 @@ void or_format(const or_cfg* c, double* U, double* out, int inverse);
 /* RK4 time integration (rk.cc:132-247).
  * mode 0: fixed Dt for nsteps steps; mode 1: CFL-based Dt until T reaches tf (Dt clamped),
  * or for at most nsteps steps when nsteps > 0 (a cap the reference does not have).
- * limiter: 0 none, 1 hrl, 3 hri.  Returns number of steps taken, or -1 on Dt error.
+ * limiter: 0 none, 1 hrl, 2 m2l, 3 hri, 5 prl.  Returns number of steps taken, or -1 on Dt error.
  * *t_out receives the final time. dt_hist (may be NULL) receives up to max_hist Dt values. */
 int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
            int limiter, const int* on, const double* thresh, double* t_out, double* dt_hist, int max_hist);

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle
-from dg_amd import MULTIFLUID, problems
+from dg_amd import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, problems
 from tests.helpers import CONFIGS, cfg_id, small_mesh
 
 pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="reference sources not present: _ref not built")
@@ -166,3 +166,86 @@ def test_printer_formatter_bitwise(c):
         R.Lformater(prob.cfg.N_s, prob.cfg.N_E, P(back_ref), P(out_ref), True)
         assert np.array_equal(prob.unformat(out), back_ref)
         assert np.allclose(back_ref, U, rtol=1e-13, atol=1e-13)
+
+
+BCS_LIM = {1: [(FARFIELD, REFLECTIVE), (PERIODIC, PERIODIC)],
+           2: [(PERIODIC, PERIODIC, REFLECTIVE, FARFIELD), (REFLECTIVE, REFLECTIVE, REFLECTIVE, REFLECTIVE)]}
+
+
+def _ref_m2(R, prob, t, U):
+    """Limiting::M2limiting (limiting.cc:123-594, GAMNCON) composed from the
+    reference launchers and netlib dgemm, in the reference's call order."""
+    c, L = prob.cfg, oracle.lib()
+    D, Ns, NE, NF, NN, n1 = c.D, c.N_s, c.N_E, prob.nf, c.N_N, c.N_s1D
+    k = prob._keep
+    U = U.copy()
+    z = lambda: np.zeros(Ns * NE)
+    rho, rhou, rhov, E, gam, pres, K, mono, lim, gLim, pLim, KLim, EMono, rhoeLim, ELim = (z() for _ in range(15))
+    gemm = lambda M, X, Y: L.or_dgemm(b"N", b"N", Ns, NE, Ns, 1.0, P(M), Ns, P(X), Ns, 0.0, P(Y), Ns)
+    get = lambda f, x: R.Lstridedcopy(NE, Ns, Ns * NF, Ns, f * Ns, 0, P(U), P(x))
+    put = lambda f, x: R.Lstridedcopy(NE, Ns, Ns, Ns * NF, 0, f * Ns, P(x), P(U))
+    if D == 1:
+        def hr(x, Lm, back):
+            gemm(k["Lag2Mono"], x, mono)
+            R.Lhrl1D(Ns, NE, 1, NN, 1, P(k["neighbors"]), 0, P(mono), P(Lm))
+            if back:
+                gemm(k["Mono2Lag"], Lm, x)
+        get(0, rho); hr(rho, lim, True)
+        get(1, rhou); hr(rhou, lim, True)
+        get(2, E); gemm(k["Lag2Mono"], E, EMono)
+        get(3, gam); hr(gam, gLim, True)
+        R.Lpressure(Ns, NE, P(U), P(pres)); hr(pres, pLim, False)
+        R.Lkinetic_energy1D(Ns, NE, P(rho), P(rhou), P(K)); gemm(k["Lag2Mono"], K, KLim)
+        R.Linternal_energy_multifluid(Ns, NE, 1, P(pLim), P(gLim), P(rhoeLim))
+        R.Lreconstruct_energy(Ns, NE, 1, P(rhoeLim), P(KLim), P(EMono), P(ELim))
+        gemm(k["Mono2Lag"], ELim, E)
+        for f, x in enumerate((rho, rhou, E, gam)):
+            put(f, x)
+        return U
+
+    def hr(L2M, M2L, off, x, Lm, back):
+        gemm(k[L2M], x, mono)
+        R.Lhrl1D(n1, NE, 1, NN, n1, P(k["neighbors"]), off, P(mono), P(Lm))
+        if back:
+            gemm(k[M2L], Lm, x)
+    for f, x in ((0, rho), (1, rhou), (2, rhov)):
+        get(f, x); hr("Lag2MonoX", "MonoX2Lag", 0, x, lim, True)
+    get(4, gam); hr("Lag2MonoX", "MonoX2Lag", 0, gam, gLim, True)
+    R.Lpressure(Ns, NE, P(U), P(pres)); hr("Lag2MonoX", "MonoX2Lag", 0, pres, pLim, True)
+    R.Lkinetic_energy2D(Ns, NE, P(rho), P(rhou), P(rhov), P(K)); gemm(k["Lag2MonoX"], K, KLim)
+    R.Linternal_energy_multifluid(n1, NE, n1, P(pLim), P(gLim), P(rhoeLim))
+    get(3, E); gemm(k["Lag2MonoX"], E, EMono)
+    R.Lreconstruct_energy(n1, NE, n1, P(rhoeLim), P(KLim), P(EMono), P(ELim))
+    gemm(k["MonoX2Lag"], ELim, E)
+    for x in (rho, rhou, rhov):
+        hr("Lag2MonoY", "MonoY2Lag", 2, x, lim, True)
+    hr("Lag2MonoY", "MonoY2Lag", 2, gam, gLim, True)
+    hr("Lag2MonoY", "MonoY2Lag", 2, pres, pLim, False)
+    R.Lkinetic_energy2D(Ns, NE, P(rho), P(rhou), P(rhov), P(K)); gemm(k["Lag2MonoY"], K, KLim)
+    R.Linternal_energy_multifluid(n1, NE, n1, P(pLim), P(gLim), P(rhoeLim))
+    gemm(k["Lag2MonoY"], E, EMono)
+    R.Lreconstruct_energy(n1, NE, n1, P(rhoeLim), P(KLim), P(EMono), P(ELim))
+    gemm(k["MonoY2Lag"], ELim, E)
+    for f, x in enumerate((rho, rhou, rhov, E, gam)):
+        put(f, x)
+    return U
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_m2_and_pr_limiting_bitwise(dim, order):
+    """M2limiting and PRlimiting (two-fluid) against the reference launchers."""
+    c = (dim, MULTIFLUID, 1)
+    R = oracle.ref_lib(*c)
+    for k, bc in enumerate(BCS_LIM[dim]):
+        mesh, t, prob, U = _setup(c, order, 30 + k, riemann=True, bc=bc)
+        x = np.arange(U.size, dtype=np.float64)
+        U = U * (1.0 + 0.01 * np.sin(0.37 * x + k))
+        U_ref = _ref_m2(R, prob, t, U)
+        assert np.array_equal(prob.limit_m2(U), U_ref)
+        assert not np.array_equal(U_ref, U)
+        V = U.copy()
+        R.LCons2Prim(prob.cfg.N_s, prob.cfg.N_E, P(V))
+        V = prob.limit_hr(V)
+        R.LPrim2Cons(prob.cfg.N_s, prob.cfg.N_E, P(V))
+        assert np.array_equal(prob.limit_pr(U), V)
