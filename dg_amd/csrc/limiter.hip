@@ -741,6 +741,22 @@ hipError_t launch_limit_m2(Solver& s, double* U, hipStream_t st) {
   });
 }
 
+// Limiting::PRIlimiting (limiting.cc:636-662): the sensor on the conserved
+// state, then the flagged HR sweeps on the primitive variables.
+hipError_t launch_limit_pri(Solver& s, double* U, hipStream_t st) {
+  if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  if (e != hipSuccess) return e;
+  const long n = (long)s.N_E * s.N_s;
+  const dim3 g(nblk(n, 256)), bl(256);
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 1, U);
+  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 1, U);
+  if ((e = launch_limit_flagged(s, U, st)) != hipSuccess) return e;
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 0, U);
+  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 0, U);
+  return hipGetLastError();
+}
+
 // Limiting::PRlimiting: HR of the primitive variables.
 hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st) {
   if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;

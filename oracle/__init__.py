@@ -80,6 +80,8 @@ def lib() -> ctypes.CDLL:
         L.or_limit_m2.restype = None
         L.or_limit_pr.argtypes = [cp, tp, vp]
         L.or_limit_pr.restype = None
+        L.or_limit_pri.argtypes = [cp, tp, vp, vp, vp, vp]
+        L.or_limit_pri.restype = None
         _lib = L
     return _lib
 
@@ -169,6 +171,14 @@ class Problem:
         U = np.array(U, dtype=np.float64, copy=True)
         lib().or_limit_pr(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(U))
         return U
+
+    def limit_pri(self, U: np.ndarray, thresholds):
+        """Limiting::PRIlimiting (two-fluid); returns (U, sensors)."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        on, th = self._thresh(thresholds)
+        sens = np.zeros(self.cfg.N_E, dtype=np.int32)
+        lib().or_limit_pri(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
+        return U, sens
 
     def format(self, U: np.ndarray) -> np.ndarray:
         """PRINTER formatter: the printed primitive fields of U, node by node."""

@@ -889,6 +889,7 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
     else if (limiter == 2) or_limit_m2(c, t, X);
     else if (limiter == 3) or_limit_hri(c, t, on, thresh, X, sens.data());
     else if (limiter == 5) or_limit_pr(c, t, X);
+    else if (limiter == 6) or_limit_pri(c, t, on, thresh, X, sens.data());
   };
   double T = 0;
   int steps = 0;
@@ -1125,6 +1126,17 @@ static void pr_prim2cons(const or_cfg* c, double* U) {
 void or_limit_pr(const or_cfg* c, const or_tables* t, double* U) {
   pr_cons2prim(c, U);
   or_limit_hr(c, t, U);
+  pr_prim2cons(c, U);
+}
+
+// Limiting::PRIlimiting (limiting.cc:636-662): sensing on the conserved state,
+// then the hri sweeps on the primitive variables
+void or_limit_pri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors) {
+  std::vector<double> Uavg((size_t)c->N_E * c->N_F);
+  or_sensing(c, t->neighbors, U, on, thresh, sensors, Uavg.data());
+  pr_cons2prim(c, U);
+  or_limit_flagged(c, t, sensors, U);
   pr_prim2cons(c, U);
 }
 

@@ -78,6 +78,9 @@ void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const doub
 void or_limit_m2(const or_cfg* c, const or_tables* t, double* U);
 /* Limiting::PRlimiting (limiting.cc:102-121): Cons2Prim, HRlimiting, Prim2Cons (limiting_kernels.cu:957-1068). */
 void or_limit_pr(const or_cfg* c, const or_tables* t, double* U);
+/* Limiting::PRIlimiting (limiting.cc:636-662): sensing, Cons2Prim, hri sweeps, Prim2Cons. */
+void or_limit_pri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors);
 
 /* PRINTER formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110, GAMNCON):
  * inverse 0: out = primitive output fields of U per node (rho, ux[, uy][, gamma], p);
@@ -87,7 +90,7 @@ void or_format(const or_cfg* c, double* U, double* out, int inverse);
 /* RK4 time integration (rk.cc:132-247).
  * mode 0: fixed Dt for nsteps steps; mode 1: CFL-based Dt until T reaches tf (Dt clamped),
  * or for at most nsteps steps when nsteps > 0 (a cap the reference does not have).
- * limiter: 0 none, 1 hrl, 2 m2l, 3 hri, 5 prl.  Returns number of steps taken, or -1 on Dt error.
+ * limiter: 0 none, 1 hrl, 2 m2l, 3 hri, 5 prl, 6 pri.  Returns number of steps taken, or -1 on Dt error.
  * *t_out receives the final time. dt_hist (may be NULL) receives up to max_hist Dt values. */
 int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
            int limiter, const int* on, const double* thresh, double* t_out, double* dt_hist, int max_hist);

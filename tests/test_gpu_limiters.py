@@ -1,4 +1,4 @@
-"""The m2 and primitive-variable HR limiters (m2l, prl) on the GPU: bit for
+"""The m2 and primitive-variable HR limiters (m2l, prl, pri) on the GPU: bit for
 bit the oracle's (which equals the reference's launchers, see
 test_oracle_vs_reference.py) for 1D and 2D, p = 1..3, walls and periodic
 sides; limited RK4 trajectories within 1e-11 of the oracle's; single-fluid
@@ -8,7 +8,7 @@ import pytest
 import torch
 
 import oracle
-from dg_amd import (FARFIELD, LIMIT_M2, LIMIT_PR, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, Config,
+from dg_amd import (FARFIELD, LIMIT_M2, LIMIT_PR, LIMIT_PRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, Config,
                     DgxError, Mesh, Solver, problems)
 from tests.helpers import per_field_rel, small_mesh
 
@@ -28,7 +28,10 @@ def test_m2_and_pr_bitwise(device, dim, order):
         U0 = problems.riemann_state(mesh, MULTIFLUID, seed=40 + k)
         x = np.arange(U0.size, dtype=np.float64)
         U0 = U0 * (1.0 + 0.01 * np.sin(0.37 * x + k))
-        for lim, ref in ((LIMIT_M2, prob.limit_m2), (LIMIT_PR, prob.limit_pr)):
+        th = (0.01, 0.01, 0.01)
+        solver.set_sensor_thresholds(th)
+        for lim, ref in ((LIMIT_M2, prob.limit_m2), (LIMIT_PR, prob.limit_pr),
+                         (LIMIT_PRI, lambda u: prob.limit_pri(u, th)[0])):
             U = torch.from_numpy(U0).to(device)
             solver.limit(lim, U)
             torch.cuda.synchronize()
@@ -36,7 +39,7 @@ def test_m2_and_pr_bitwise(device, dim, order):
 
 
 @pytest.mark.parametrize("dim,n", [(1, 256), (2, (32, 16))])
-@pytest.mark.parametrize("lim", [LIMIT_M2, LIMIT_PR])
+@pytest.mark.parametrize("lim", [LIMIT_M2, LIMIT_PR, LIMIT_PRI])
 def test_limited_trajectories(device, dim, n, lim):
     if dim == 1:
         mesh = Mesh(1, 2, n, 1, -1.0, 1.0, bc=(FARFIELD, REFLECTIVE), nfields=4)
@@ -46,8 +49,10 @@ def test_limited_trajectories(device, dim, n, lim):
         U0 = problems.oblique_riemann_state(mesh, MULTIFLUID, seed=3)
     prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
     solver = Solver(Config(dim, 2, MULTIFLUID, ROE), mesh)
+    th = (0.01, 0.01, 0.01)
+    solver.set_sensor_thresholds(th)
     dt = 0.3 * prob.dt_from_cfl(U0, mesh.dx / 5)
-    U_ref, _, _, _ = prob.rk4(U0, 0, dt, nsteps=8, limiter=lim)
+    U_ref, _, _, _ = prob.rk4(U0, 0, dt, nsteps=8, limiter=lim, thresholds=th)
     U = torch.from_numpy(U0).to(device)
     solver.rk4_integrate(U, 0, dt, nsteps=8, limiter=lim)
     torch.cuda.synchronize()
@@ -59,6 +64,6 @@ def test_single_fluid_is_refused(device):
     mesh = small_mesh(2, 2, SINGLEFLUID)
     solver = Solver(Config(2, 2, SINGLEFLUID, ROE), mesh)
     U = torch.from_numpy(problems.riemann_state(mesh, SINGLEFLUID, seed=1)).to(device)
-    for lim in (LIMIT_M2, LIMIT_PR):
+    for lim in (LIMIT_M2, LIMIT_PR, LIMIT_PRI):
         with pytest.raises(DgxError):
             solver.limit(lim, U)

@@ -234,7 +234,7 @@ def _ref_m2(R, prob, t, U):
 @pytest.mark.parametrize("dim", [1, 2])
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_m2_and_pr_limiting_bitwise(dim, order):
-    """M2limiting and PRlimiting (two-fluid) against the reference launchers."""
+    """M2limiting, PRlimiting and PRIlimiting (two-fluid) against the reference launchers."""
     c = (dim, MULTIFLUID, 1)
     R = oracle.ref_lib(*c)
     for k, bc in enumerate(BCS_LIM[dim]):
@@ -249,3 +249,20 @@ def test_m2_and_pr_limiting_bitwise(dim, order):
         V = prob.limit_hr(V)
         R.LPrim2Cons(prob.cfg.N_s, prob.cfg.N_E, P(V))
         assert np.array_equal(prob.limit_pr(U), V)
+        # PRIlimiting (limiting.cc:636-662): sensing (pinned above), Cons2Prim, hri1D + copy_detected, Prim2Cons
+        th = (0.01, 0.01, 0.01)
+        sens, _ = prob.sensing(U, th)
+        assert sens.any()
+        cfg, kk = prob.cfg, prob._keep
+        V = U.copy()
+        R.LCons2Prim(cfg.N_s, cfg.N_E, P(V))
+        sweeps = ([("Lag2Mono", "Mono2Lag", cfg.N_s, 1, 0)] if dim == 1 else
+                  [("Lag2MonoX", "MonoX2Lag", cfg.N_s1D, cfg.N_s1D, 0), ("Lag2MonoY", "MonoY2Lag", cfg.N_s1D, cfg.N_s1D, 2)])
+        for l2m, m2l, n1, sl, off in sweeps:
+            Vn = np.zeros_like(V)
+            R.Lhri1D(cfg.N_s, cfg.N_E, cfg.N_N, P(kk["neighbors"]), n1, sl, off, P(kk[l2m]), P(kk[m2l]), P(sens), P(V),
+                     P(Vn))
+            R.Lcopy_detected(cfg.N_s, cfg.N_E, P(sens), P(Vn), P(V))
+        R.LPrim2Cons(cfg.N_s, cfg.N_E, P(V))
+        W, s2 = prob.limit_pri(U, th)
+        assert np.array_equal(s2, sens) and np.array_equal(W, V)
