@@ -423,11 +423,12 @@ dgx_status dgx_set_sensor_thresholds(dgx_solver* h, int n, const double* th) {
 static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
   if (limiter == 0) return DGX_OK;
   if (limiter == 1) return hip_ok(launch_limit_hr(s, U, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
-  if (limiter == 2 || limiter == 5 || limiter == 6) {
+  if (limiter == 2 || limiter == 4 || limiter == 5 || limiter == 6) {
     if (s.cfg.physics != DGX_MULTIFLUID)
       return fail(DGX_EUNSUPPORTED,
-                  "m2l, prl and pri limit two-fluid states (the reference implements them for GAMNCON)");
+                  "m2l, m2i, prl and pri limit two-fluid states (the reference implements them for GAMNCON)");
     if (limiter == 2) return hip_ok(launch_limit_m2(s, U, st), "m2 limiter") ? DGX_OK : DGX_EDEVICE;
+    if (limiter == 4) return hip_ok(launch_limit_m2i(s, U, st), "m2i limiter") ? DGX_OK : DGX_EDEVICE;
     if (limiter == 6) return hip_ok(launch_limit_pri(s, U, st), "primitive HRI limiter") ? DGX_OK : DGX_EDEVICE;
     return hip_ok(launch_limit_pr(s, U, st), "primitive HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
@@ -443,7 +444,8 @@ static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
     return hip_ok(launch_limit_flagged(s, U, st), "flagged HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
   return fail(DGX_EUNSUPPORTED,
-              "limiter must be 0 (none), 1 (hrl), 2 (m2l), 3 (hri), 5 (prl), 6 (pri) or 8 (learned indicator + HR)");
+              "limiter must be 0 (none), 1 (hrl), 2 (m2l), 3 (hri), 4 (m2i), 5 (prl), 6 (pri) or 8 (learned "
+              "indicator + HR)");
 }
 
 dgx_status dgx_limit(dgx_solver* h, int limiter, double* U, void* stream) {

@@ -459,6 +459,64 @@ __global__ __launch_bounds__(256) void k_sensor_final(int N_E, int N_N, const in
   sensors[x] = bval;
 }
 
+// set2average (limiting_kernels.cu:1792-1828) on one nodal column of Ns
+// values, SL slices of N+1: UC <- M2L * average(L2M * UC); Alim receives the
+// averaged monomial coefficients.
+template <int Ns, int N, int SL>
+__device__ inline void hr_avg_nodal(const double* __restrict__ L2M, const double* __restrict__ M2L, double* UC,
+                                    double* Alim) {
+  double A[Ns];
+#pragma unroll
+  for (int m = 0; m < Ns; ++m) {
+    double s = 0;
+#pragma unroll
+    for (int n = 0; n < Ns; ++n) s += L2M[n * Ns + m] * UC[n];
+    A[m] = s;
+  }
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl) average_monomial_d<N>(&A[sl * (N + 1)], &Alim[sl * (N + 1)]);
+#pragma unroll
+  for (int m = 0; m < Ns; ++m) {
+    double s = 0;
+#pragma unroll
+    for (int n = 0; n < Ns; ++n) s += M2L[n * Ns + m] * Alim[n];
+    UC[m] = s;
+  }
+}
+
+// HR (limiting_kernels.cu:1830-1868) on one nodal column with its left and
+// right neighbours' columns: UC <- M2L * limit(L2M * {UL, UC, UR}); Alim
+// receives the limited monomial coefficients.
+template <int Ns, int N, int SL>
+__device__ inline void hr_full_nodal(const double* __restrict__ L2M, const double* __restrict__ M2L, const double* UL,
+                                     double* UC, const double* UR, double* Alim) {
+  double AL[Ns], AC[Ns], AR[Ns];
+#pragma unroll
+  for (int m = 0; m < Ns; ++m) {
+    double s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+    for (int n = 0; n < Ns; ++n) {
+      double a = L2M[n * Ns + m];
+      s1 += a * UL[n];
+      s2 += a * UC[n];
+      s3 += a * UR[n];
+    }
+    AL[m] = s1; AC[m] = s2; AR[m] = s3;
+  }
+#pragma unroll
+  for (int i = 0; i < Ns; ++i) Alim[i] = 0;
+#pragma unroll
+  for (int sl = 0; sl < SL; ++sl)
+    limit_monomial_d<N>(&AL[sl * (N + 1)], &AC[sl * (N + 1)], &AR[sl * (N + 1)], &Alim[sl * (N + 1)]);
+#pragma unroll
+  for (int m = 0; m < Ns; ++m) {
+    double s = 0;
+#pragma unroll
+    for (int n = 0; n < Ns; ++n) s += M2L[n * Ns + m] * Alim[n];
+    UC[m] = s;
+  }
+}
+
 // hri1D (limiting_kernels.cu:322-427) for one column (element, field) of a
 // flagged element: HR in Lagrange space through Lag2Mono / Mono2Lag, or the
 // boundary rules; Unew receives the column.
@@ -466,13 +524,13 @@ template <int D, int P>
 __device__ inline void hri_column(int e, int fc, int NF, int N_N, const int* __restrict__ nb, int offxy,
                                   const double* __restrict__ L2M, const double* __restrict__ M2L,
                                   const double* __restrict__ U, double* __restrict__ Unew) {
-  constexpr int Ns = Shape<D, P>::Ns, N1 = P + 1, N = P;
-  constexpr int slicenum = (D == 1) ? 1 : N1;
+  constexpr int Ns = Shape<D, P>::Ns, N = P;
+  constexpr int slicenum = (D == 1) ? 1 : P + 1;
   const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
   int physical = 0;
   if (left < 0) physical = -left;
   if (right < 0) physical = -right;
-  double UC[Ns];
+  double UC[Ns], Alim[Ns];
 #pragma unroll
   for (int i = 0; i < Ns; ++i) UC[i] = U[((long)e * NF + fc) * Ns + i];
   bool avg = false, full = false;
@@ -486,53 +544,15 @@ __device__ inline void hri_column(int e, int fc, int NF, int N_N, const int* __r
     full = true;
   }
   if (avg) {
-    double A[Ns], Alim[Ns];
-#pragma unroll
-    for (int m = 0; m < Ns; ++m) {
-      double s = 0;
-#pragma unroll
-      for (int n = 0; n < Ns; ++n) s += L2M[n * Ns + m] * UC[n];
-      A[m] = s;
-    }
-#pragma unroll
-    for (int sl = 0; sl < slicenum; ++sl) average_monomial_d<N>(&A[sl * N1], &Alim[sl * N1]);
-#pragma unroll
-    for (int m = 0; m < Ns; ++m) {
-      double s = 0;
-#pragma unroll
-      for (int n = 0; n < Ns; ++n) s += M2L[n * Ns + m] * Alim[n];
-      UC[m] = s;
-    }
+    hr_avg_nodal<Ns, N, slicenum>(L2M, M2L, UC, Alim);
   } else if (full) {
-    double UL[Ns], UR[Ns], AL[Ns], AC[Ns], AR[Ns], Alim[Ns];
+    double UL[Ns], UR[Ns];
 #pragma unroll
     for (int i = 0; i < Ns; ++i) {
       UL[i] = U[((long)left * NF + fc) * Ns + i];
       UR[i] = U[((long)right * NF + fc) * Ns + i];
     }
-#pragma unroll
-    for (int m = 0; m < Ns; ++m) {
-      double s1 = 0, s2 = 0, s3 = 0;
-#pragma unroll
-      for (int n = 0; n < Ns; ++n) {
-        double a = L2M[n * Ns + m];
-        s1 += a * UL[n];
-        s2 += a * UC[n];
-        s3 += a * UR[n];
-      }
-      AL[m] = s1; AC[m] = s2; AR[m] = s3;
-    }
-#pragma unroll
-    for (int i = 0; i < Ns; ++i) Alim[i] = 0;
-#pragma unroll
-    for (int sl = 0; sl < slicenum; ++sl) limit_monomial_d<N>(&AL[sl * N1], &AC[sl * N1], &AR[sl * N1], &Alim[sl * N1]);
-#pragma unroll
-    for (int m = 0; m < Ns; ++m) {
-      double s = 0;
-#pragma unroll
-      for (int n = 0; n < Ns; ++n) s += M2L[n * Ns + m] * Alim[n];
-      UC[m] = s;
-    }
+    hr_full_nodal<Ns, N, slicenum>(L2M, M2L, UL, UC, UR, Alim);
   }
 #pragma unroll
   for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fc) * Ns + i] = UC[i];
@@ -561,6 +581,138 @@ __global__ __launch_bounds__(128) void k_hri_list(int NF, int N_N, const int* __
   const long n = (long)count[0] * NF;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
     hri_column<D, P>(list[t / NF], (int)(t % NF), NF, N_N, nb, offxy, L2M, M2L, U, Unew);
+}
+
+// m2i1D (limiting_kernels.cu:431-722) over the listed elements (two-fluid:
+// rho, rho u[, rho v], E, 1/(gamma-1)).  Eight lanes per element: lane c < D+3
+// limits one column (c <= D: rho and the momenta; D+1: 1/(gamma-1); D+2: the
+// pressure (:1870-1897), computed node by node from the five fields) through
+// HR or the boundary rules; then lane 0 rebuilds the energy from the limited
+// columns: kinetic_energy (:1899-1922), internal_energy (:1924-1953),
+// reconstruct_total_energy (:1955-2001).  A block of 64 lanes takes eight
+// elements per pass; the loop bound is uniform over the block.
+template <int D, int P>
+__global__ __launch_bounds__(64) void k_m2i_list(int N_N, const int* __restrict__ nb, int offxy,
+                                                 const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                                 const int* __restrict__ list, const int* __restrict__ count,
+                                                 const double* __restrict__ U, double* __restrict__ Unew) {
+  constexpr int Ns = Shape<D, P>::Ns, N = P, N1 = P + 1, NF = D + 3;
+  constexpr int SL = (D == 1) ? 1 : N1;
+  constexpr int fE = D + 1, fG = D + 2;
+  __shared__ double sNod[8][D + 1][Ns];  // limited rho, rho u[, rho v]
+  __shared__ double sG[8][Ns], sP[8][Ns];  // limited monomials of 1/(gamma-1) and p
+  const int g = threadIdx.x >> 3, c = threadIdx.x & 7;
+  const long n = count[0];
+  auto pressure = [&](long el, int i) -> double {
+    const double* u = U + (long)el * NF * Ns;
+    const double rho = u[i], ru = u[Ns + i], E = u[fE * Ns + i], ga = u[fG * Ns + i];
+    if (D == 1) return (E - 0.5 * ru * ru / rho) / ga;
+    const double rv = u[2 * Ns + i];
+    return (E - 0.5 * (ru * ru + rv * rv) / rho) / ga;
+  };
+  for (long base = (long)blockIdx.x * 8; base < n; base += (long)gridDim.x * 8) {
+    const bool live = base + g < n;
+    const int e = live ? list[base + g] : 0;
+    int physical = 0;
+    if (live) {
+      const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
+      if (left < 0) physical = -left;
+      if (right < 0) physical = -right;
+      if (c < NF) {
+        const int fld = c <= D ? c : fG;  // column c's field (the pressure column has none)
+        double UC[Ns], Alim[Ns] = {};
+#pragma unroll
+        for (int i = 0; i < Ns; ++i) UC[i] = c == D + 2 ? pressure(e, i) : U[((long)e * NF + fld) * Ns + i];
+        if (physical == 4) {
+        } else if (physical == 2 || physical == 3) {
+          const int un_idx = (offxy == 0) ? 1 : (offxy == 2 ? 2 : 0);  // wall-normal momentum
+          if (physical == 2 || c != un_idx) hr_avg_nodal<Ns, N, SL>(L2M, M2L, UC, Alim);
+        } else {
+          double UL[Ns], UR[Ns];
+#pragma unroll
+          for (int i = 0; i < Ns; ++i) {
+            UL[i] = c == D + 2 ? pressure(left, i) : U[((long)left * NF + fld) * Ns + i];
+            UR[i] = c == D + 2 ? pressure(right, i) : U[((long)right * NF + fld) * Ns + i];
+          }
+          hr_full_nodal<Ns, N, SL>(L2M, M2L, UL, UC, UR, Alim);
+        }
+        if (c <= D) {
+#pragma unroll
+          for (int i = 0; i < Ns; ++i) sNod[g][c][i] = UC[i];
+        } else if (c == D + 1) {
+#pragma unroll
+          for (int i = 0; i < Ns; ++i) sG[g][i] = Alim[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < Ns; ++i) sP[g][i] = Alim[i];
+        }
+        if (c <= D + 1) {
+#pragma unroll
+          for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fld) * Ns + i] = UC[i];
+        }
+      }
+    }
+    __syncthreads();
+    if (live && c == 0) {
+      double EC[Ns];
+#pragma unroll
+      for (int i = 0; i < Ns; ++i) EC[i] = U[((long)e * NF + fE) * Ns + i];
+      if (physical != 4) {
+        double tmp[Ns], KLim[Ns], rhoeLim[Ns], EMono[Ns], ELim[Ns];
+#pragma unroll
+        for (int i = 0; i < Ns; ++i) {
+          const double rho = sNod[g][0][i], ru = sNod[g][1][i];
+          if (D == 1) {
+            tmp[i] = 0.5 * (ru * ru) / rho;
+          } else {
+            const double rv = sNod[g][D][i];
+            tmp[i] = 0.5 * (ru * ru + rv * rv) / rho;
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < Ns; ++m) {
+          double s1 = 0, s2 = 0;
+#pragma unroll
+          for (int q = 0; q < Ns; ++q) {
+            s1 += L2M[q * Ns + m] * tmp[q];
+            s2 += L2M[q * Ns + m] * EC[q];
+          }
+          KLim[m] = s1;
+          EMono[m] = s2;
+        }
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl)
+#pragma unroll
+          for (int i = 0; i < N1; ++i) {
+            double I = 0;
+#pragma unroll
+            for (int q = 0; q < i + 1; ++q)
+              I += (double)binomial_d(i, q) * sP[g][sl * N1 + i - q] * sG[g][sl * N1 + q];
+            rhoeLim[sl * N1 + i] = I;
+          }
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+          const int i0 = sl * N1;
+#pragma unroll
+          for (int i = 1; i < N1; ++i) ELim[i0 + i] = rhoeLim[i0 + i] + KLim[i0 + i];
+          double E0 = EMono[i0];
+#pragma unroll
+          for (int q = 2; q < N1; q += 2) E0 -= 1.0 / ((double)ifact(q + 1)) * (ELim[i0 + q] - EMono[i0 + q]);
+          ELim[i0] = E0;
+        }
+#pragma unroll
+        for (int m = 0; m < Ns; ++m) {
+          double s = 0;
+#pragma unroll
+          for (int q = 0; q < Ns; ++q) s += M2L[q * Ns + m] * ELim[q];
+          EC[m] = s;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fE) * Ns + i] = EC[i];
+    }
+    __syncthreads();
+  }
 }
 
 // list[0 .. count) = the flagged elements (sensors[e] != 0), in no fixed
@@ -850,6 +1002,33 @@ hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st) {
   if (s.D == 1) return hri_list(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
   if ((e = hri_list(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
   return hri_list(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);
+}
+
+// Limiting::M2Ilimiting (limiting.cc:682-720), two-fluid: sensing, the
+// flagged elements listed once, then m2i1D + copy_detected per direction over
+// the list.
+hipError_t launch_limit_m2i(Solver& s, double* U, hipStream_t st) {
+  if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  if (e != hipSuccess) return e;
+  const int* count = s.flag + 2;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag + 2, 1L, 0);
+  hipLaunchKernelGGL(k_flag_list, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.sensors, s.flag_list,
+                     s.flag + 2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  auto sweep = [&](int offxy, const double* L2M, const double* M2L) -> hipError_t {
+    hipError_t r = by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+      hipLaunchKernelGGL((k_m2i_list<decltype(d)::value, decltype(p)::value>), dim3(1024), dim3(64), 0, st, s.N_N,
+                         s.t.neighbors, offxy, L2M, M2L, s.flag_list, count, U, s.work);
+      return hipGetLastError();
+    });
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_copy_list, dim3(1024), dim3(256), 0, st, s.NF * s.N_s, s.flag_list, count, s.work, U);
+    return hipGetLastError();
+  };
+  if (s.D == 1) return sweep(0, s.t.Lag2Mono, s.t.Mono2Lag);
+  if ((e = sweep(0, s.t.Lag2MonoX, s.t.MonoX2Lag)) != hipSuccess) return e;
+  return sweep(2, s.t.Lag2MonoY, s.t.MonoY2Lag);
 }
 
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {

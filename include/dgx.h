@@ -134,10 +134,11 @@ dgx_status dgx_set_sensor_thresholds(dgx_solver* s, int n, const double* thresho
 
 /* limiter (the reference's codes, main.cc:167-175): 0 none, 1 hrl (global HR, Limiting::HRlimiting),
  * 2 m2l (two-fluid: HR of rho, momenta, 1/(gamma-1) and pressure, energy rebuilt from the limited internal and
- * kinetic energies; Limiting::M2limiting, limiting.cc:123-594), 3 hri (sensor-gated HR), 5 prl (two-fluid: HR of
- * the primitive variables; Limiting::PRlimiting, limiting.cc:102-121), 6 pri (two-fluid: sensor on the
- * conserved state, hri sweeps on the primitive variables; Limiting::PRIlimiting, limiting.cc:636-662); and 8,
- * HR gated by the learned indicator (no reference code).  m2i and p0i (4, 7) are not provided. */
+ * kinetic energies; Limiting::M2limiting, limiting.cc:123-594), 3 hri (sensor-gated HR), 4 m2i (two-fluid: the
+ * m2 limiting on the sensor's cells only, element by element; Limiting::M2Ilimiting, limiting.cc:682-720), 5 prl
+ * (two-fluid: HR of the primitive variables; Limiting::PRlimiting, limiting.cc:102-121), 6 pri (two-fluid: sensor
+ * on the conserved state, hri sweeps on the primitive variables; Limiting::PRIlimiting, limiting.cc:636-662); and
+ * 8, HR gated by the learned indicator (no reference code).  p0i (7) is not provided. */
 dgx_status dgx_limit(dgx_solver* s, int limiter, double* U, void* stream);
 /* Out = limited In (In unchanged; In != Out, both state-sized device arrays).
  * hrl on a 2D Cartesian tile mesh runs both sweeps in one launch (the same

@@ -82,6 +82,10 @@ def lib() -> ctypes.CDLL:
         L.or_limit_pr.restype = None
         L.or_limit_pri.argtypes = [cp, tp, vp, vp, vp, vp]
         L.or_limit_pri.restype = None
+        L.or_limit_m2i.argtypes = [cp, tp, vp, vp, vp, vp]
+        L.or_limit_m2i.restype = None
+        L.or_m2i1D.argtypes = [cp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.or_m2i1D.restype = None
         _lib = L
     return _lib
 
@@ -178,6 +182,14 @@ class Problem:
         on, th = self._thresh(thresholds)
         sens = np.zeros(self.cfg.N_E, dtype=np.int32)
         lib().or_limit_pri(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
+        return U, sens
+
+    def limit_m2i(self, U: np.ndarray, thresholds):
+        """Limiting::M2Ilimiting (two-fluid); returns (U, sensors)."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        on, th = self._thresh(thresholds)
+        sens = np.zeros(self.cfg.N_E, dtype=np.int32)
+        lib().or_limit_m2i(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
         return U, sens
 
     def format(self, U: np.ndarray) -> np.ndarray:

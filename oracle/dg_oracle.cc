@@ -326,16 +326,18 @@ void gemv3_cm(int M, int N, const double* A, const double* B1, double* C1, const
     C1[m] = s1; C2[m] = s2; C3[m] = s3;
   }
 }
+// mono (optional) receives the limited monomial coefficients
 void set2average(int N_s, int N, int N_s1D, int slicenum, const double* L2M, const double* M2L, double* tmp,
-                 double* U) {  // :1792-1828
+                 double* U, double* mono = nullptr) {  // :1792-1828
   double* A = tmp;
   double* Alim = tmp + N_s;
   gemv_cm(N_s, N_s, L2M, U, A);
   for (int s = 0; s < slicenum; s++) average_monomial(N, &A[s * N_s1D], &Alim[s * N_s1D]);
   gemv_cm(N_s, N_s, M2L, Alim, U);
+  if (mono) for (int i = 0; i < N_s; i++) mono[i] = Alim[i];
 }
 void HRslices(int N_s, int N, int N_s1D, int slicenum, const double* L2M, const double* M2L, double* tmp,
-              const double* UL, double* UC, const double* UR) {  // :1830-1868
+              const double* UL, double* UC, const double* UR, double* mono = nullptr) {  // :1830-1868
   double* AL = tmp;
   double* AC = tmp + N_s;
   double* AR = tmp + 2 * N_s;
@@ -343,6 +345,7 @@ void HRslices(int N_s, int N, int N_s1D, int slicenum, const double* L2M, const 
   gemv3_cm(N_s, N_s, L2M, UL, AL, UC, AC, UR, AR);
   for (int s = 0; s < slicenum; s++) limit_monomial(N, &AL[s * N_s1D], &AC[s * N_s1D], &AR[s * N_s1D], &Alim[s * N_s1D]);
   gemv_cm(N_s, N_s, M2L, Alim, UC);
+  if (mono) for (int i = 0; i < N_s; i++) mono[i] = Alim[i];
 }
 
 }  // namespace
@@ -888,6 +891,7 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
     if (limiter == 1) or_limit_hr(c, t, X);
     else if (limiter == 2) or_limit_m2(c, t, X);
     else if (limiter == 3) or_limit_hri(c, t, on, thresh, X, sens.data());
+    else if (limiter == 4) or_limit_m2i(c, t, on, thresh, X, sens.data());
     else if (limiter == 5) or_limit_pr(c, t, X);
     else if (limiter == 6) or_limit_pri(c, t, on, thresh, X, sens.data());
   };
@@ -1127,6 +1131,123 @@ void or_limit_pr(const or_cfg* c, const or_tables* t, double* U) {
   pr_cons2prim(c, U);
   or_limit_hr(c, t, U);
   pr_prim2cons(c, U);
+}
+
+// ---- m2i: per-element m2 limiting on the flagged cells (two-fluid, GAMNCON)
+
+// pressure (limiting_kernels.cu:1870-1897), MULTIFLUID
+static void m2i_pressure(int D, int N_s, const double* rho, const double* rhou, const double* rhov, const double* E,
+                         const double* g, double* p) {
+  for (int i = 0; i < N_s; i++)
+    p[i] = D == 1 ? (E[i] - 0.5 * rhou[i] * rhou[i] / rho[i]) / g[i]
+                  : (E[i] - 0.5 * (rhou[i] * rhou[i] + rhov[i] * rhov[i]) / rho[i]) / g[i];
+}
+
+// m2i1D (limiting_kernels.cu:431-722) with kinetic_energy, internal_energy and
+// reconstruct_total_energy (:1899-2001): each flagged element is limited in
+// one direction from its own and its two neighbours' nodal values; rho, the
+// momenta, 1/(gamma-1) and the pressure go through HR (or the boundary
+// rules), and the energy is rebuilt from the limited pressure and
+// 1/(gamma-1) (internal energy, Taylor product) plus the kinetic energy of the
+// limited fields, with the zeroth coefficient corrected for conservation.
+void or_m2i1D(const or_cfg* c, const int* neighbors, int N_s1D, int slicenum, int offxy, const double* L2M,
+              const double* M2L, const int* sensors, const double* U, double* Unew) {
+  const int N_s = c->N_s, N_F = c->N_F, N_N = c->N_N, D = c->D;
+  const int N = N_s1D - 1;
+  const int fE = D + 1, fG = D + 2;
+  std::vector<double> L(N_F * N_s), C(N_F * N_s), R(N_F * N_s), pL(N_s), pC(N_s), pR(N_s), gLim(N_s), pLim(N_s),
+      KLim(N_s), rhoeLim(N_s), tmp(4 * N_s);
+  std::vector<double> zero(N_s, 0.0);
+  auto fld = [&](std::vector<double>& X, int f) { return &X[f * N_s]; };
+  for (int e = 0; e < c->N_E; e++) {
+    if (sensors[e] == 0) continue;
+    const int left = neighbors[e * N_N + offxy + 0], right = neighbors[e * N_N + offxy + 1];
+    int physical = 0;
+    if (left < 0) physical = -left;
+    if (right < 0) physical = -right;
+    for (int f = 0; f < N_F; f++)
+      for (int i = 0; i < N_s; i++) C[f * N_s + i] = U[((size_t)e * N_F + f) * N_s + i];
+    const double* vC = D == 2 ? fld(C, 2) : zero.data();
+    m2i_pressure(D, N_s, fld(C, 0), fld(C, 1), vC, fld(C, fE), fld(C, fG), pC.data());
+    bool energy = true;
+    if (physical == 4) {
+      energy = false;
+    } else if (physical == 2 || physical == 3) {
+      set2average(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(C, 0));
+      if (physical == 2) {
+        for (int f = 1; f <= D; f++) set2average(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(C, f));
+      } else if (D == 2) {
+        // the wall-normal momentum keeps its values
+        if (offxy == 0) set2average(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(C, 2));
+        else if (offxy == 2) set2average(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(C, 1));
+      }
+      set2average(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(C, fG), gLim.data());
+      set2average(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), pC.data(), pLim.data());
+    } else {
+      for (int f = 0; f < N_F; f++)
+        for (int i = 0; i < N_s; i++) {
+          L[f * N_s + i] = U[((size_t)left * N_F + f) * N_s + i];
+          R[f * N_s + i] = U[((size_t)right * N_F + f) * N_s + i];
+        }
+      m2i_pressure(D, N_s, fld(L, 0), fld(L, 1), D == 2 ? fld(L, 2) : zero.data(), fld(L, fE), fld(L, fG), pL.data());
+      m2i_pressure(D, N_s, fld(R, 0), fld(R, 1), D == 2 ? fld(R, 2) : zero.data(), fld(R, fE), fld(R, fG), pR.data());
+      for (int f = 0; f <= D; f++) HRslices(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(L, f), fld(C, f), fld(R, f));
+      HRslices(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), fld(L, fG), fld(C, fG), fld(R, fG), gLim.data());
+      HRslices(N_s, N, N_s1D, slicenum, L2M, M2L, tmp.data(), pL.data(), pC.data(), pR.data(), pLim.data());
+    }
+    if (energy) {
+      // kinetic_energy (:1899-1922): nodal K of the limited fields -> monomial
+      const double* rho = fld(C, 0);
+      const double* ru = fld(C, 1);
+      const double* rv = D == 2 ? fld(C, 2) : zero.data();
+      for (int i = 0; i < N_s; i++)
+        tmp[i] = D == 1 ? 0.5 * (ru[i] * ru[i]) / rho[i] : 0.5 * (ru[i] * ru[i] + rv[i] * rv[i]) / rho[i];
+      gemv_cm(N_s, N_s, L2M, tmp.data(), KLim.data());
+      // internal_energy (:1924-1953)
+      for (int sl = 0; sl < slicenum; sl++)
+        for (int i = 0; i < N_s1D; i++) {
+          double I = 0;
+          for (int k = 0; k < i + 1; k++)
+            I += (double)binomial(i, k) * pLim[sl * N_s1D + i - k] * gLim[sl * N_s1D + k];
+          rhoeLim[sl * N_s1D + i] = I;
+        }
+      // reconstruct_total_energy (:1955-2001)
+      double* EMono = tmp.data();
+      double* ELim = tmp.data() + N_s;
+      double* EC = fld(C, fE);
+      gemv_cm(N_s, N_s, L2M, EC, EMono);
+      for (int sl = 0; sl < slicenum; sl++) {
+        const int i0 = sl * N_s1D;
+        for (int i = 1; i < N_s1D; i++) ELim[i0 + i] = rhoeLim[i0 + i] + KLim[i0 + i];
+        double E0 = EMono[i0];
+        for (int k = 2; k < N_s1D; k += 2)
+          E0 -= 1.0 / ((double)lim_factorial(k + 1)) * (ELim[i0 + k] - EMono[i0 + k]);
+        ELim[i0] = E0;
+      }
+      gemv_cm(N_s, N_s, M2L, ELim, EC);
+    }
+    for (int f = 0; f < N_F; f++)
+      for (int i = 0; i < N_s; i++) Unew[((size_t)e * N_F + f) * N_s + i] = C[f * N_s + i];
+  }
+}
+
+// Limiting::M2Ilimiting (limiting.cc:682-720): sensing, then m2i1D +
+// copy_detected in x and in y (2D Cartesian) or once (1D)
+void or_limit_m2i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors) {
+  const int N_s = c->N_s, N_E = c->N_E, N_F = c->N_F;
+  std::vector<double> Uavg((size_t)N_E * N_F), Utmp((size_t)N_s * N_E * N_F, 0.0);
+  or_sensing(c, t->neighbors, U, on, thresh, sensors, Uavg.data());
+  if (c->D == 1) {
+    or_m2i1D(c, t->neighbors, N_s, 1, 0, t->Lag2Mono, t->Mono2Lag, sensors, U, Utmp.data());
+    or_copy_detected(c, sensors, Utmp.data(), U);
+  } else {
+    const int n1 = c->N_s1D;
+    or_m2i1D(c, t->neighbors, n1, n1, 0, t->Lag2MonoX, t->MonoX2Lag, sensors, U, Utmp.data());
+    or_copy_detected(c, sensors, Utmp.data(), U);
+    or_m2i1D(c, t->neighbors, n1, n1, 2, t->Lag2MonoY, t->MonoY2Lag, sensors, U, Utmp.data());
+    or_copy_detected(c, sensors, Utmp.data(), U);
+  }
 }
 
 // Limiting::PRIlimiting (limiting.cc:636-662): sensing on the conserved state,

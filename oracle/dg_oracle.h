@@ -82,6 +82,14 @@ void or_limit_pr(const or_cfg* c, const or_tables* t, double* U);
 void or_limit_pri(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors);
 
+/* m2i1D (limiting_kernels.cu:431-722), two-fluid GAMNCON: on the flagged elements, one direction (offxy 0 or 2)
+ * of the m2 limiting done element by element in Lagrange form; Unew receives the flagged elements. */
+void or_m2i1D(const or_cfg* c, const int* neighbors, int N_s1D, int slicenum, int offxy, const double* L2M,
+              const double* M2L, const int* sensors, const double* U, double* Unew);
+/* Limiting::M2Ilimiting (limiting.cc:682-720): sensing, then m2i1D + copy_detected per direction. */
+void or_limit_m2i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors);
+
 /* PRINTER formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110, GAMNCON):
  * inverse 0: out = primitive output fields of U per node (rho, ux[, uy][, gamma], p);
  * inverse 1: U = conserved fields from out (the restart direction). */

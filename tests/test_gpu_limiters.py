@@ -1,4 +1,4 @@
-"""The m2 and primitive-variable HR limiters (m2l, prl, pri) on the GPU: bit for
+"""The m2 and primitive-variable HR limiters (m2l, m2i, prl, pri) on the GPU: bit for
 bit the oracle's (which equals the reference's launchers, see
 test_oracle_vs_reference.py) for 1D and 2D, p = 1..3, walls and periodic
 sides; limited RK4 trajectories within 1e-11 of the oracle's; single-fluid
@@ -8,8 +8,8 @@ import pytest
 import torch
 
 import oracle
-from dg_amd import (FARFIELD, LIMIT_M2, LIMIT_PR, LIMIT_PRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, Config,
-                    DgxError, Mesh, Solver, problems)
+from dg_amd import (FARFIELD, LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE,
+                    SINGLEFLUID, Config, DgxError, Mesh, Solver, problems)
 from tests.helpers import per_field_rel, small_mesh
 
 pytestmark = pytest.mark.gpu
@@ -31,7 +31,8 @@ def test_m2_and_pr_bitwise(device, dim, order):
         th = (0.01, 0.01, 0.01)
         solver.set_sensor_thresholds(th)
         for lim, ref in ((LIMIT_M2, prob.limit_m2), (LIMIT_PR, prob.limit_pr),
-                         (LIMIT_PRI, lambda u: prob.limit_pri(u, th)[0])):
+                         (LIMIT_PRI, lambda u: prob.limit_pri(u, th)[0]),
+                         (LIMIT_M2I, lambda u: prob.limit_m2i(u, th)[0])):
             U = torch.from_numpy(U0).to(device)
             solver.limit(lim, U)
             torch.cuda.synchronize()
@@ -39,7 +40,7 @@ def test_m2_and_pr_bitwise(device, dim, order):
 
 
 @pytest.mark.parametrize("dim,n", [(1, 256), (2, (32, 16))])
-@pytest.mark.parametrize("lim", [LIMIT_M2, LIMIT_PR, LIMIT_PRI])
+@pytest.mark.parametrize("lim", [LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI])
 def test_limited_trajectories(device, dim, n, lim):
     if dim == 1:
         mesh = Mesh(1, 2, n, 1, -1.0, 1.0, bc=(FARFIELD, REFLECTIVE), nfields=4)
@@ -64,6 +65,25 @@ def test_single_fluid_is_refused(device):
     mesh = small_mesh(2, 2, SINGLEFLUID)
     solver = Solver(Config(2, 2, SINGLEFLUID, ROE), mesh)
     U = torch.from_numpy(problems.riemann_state(mesh, SINGLEFLUID, seed=1)).to(device)
-    for lim in (LIMIT_M2, LIMIT_PR, LIMIT_PRI):
+    for lim in (LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI):
         with pytest.raises(DgxError):
             solver.limit(lim, U)
+
+
+def test_m2i_many_flagged_bitwise(device):
+    """More flagged cells than one pass of the m2i grid covers (8192): the
+    block loop over the list, bit for bit the oracle."""
+    mesh = Mesh(2, 2, 128, 96, 0.0, 1.6, 0.0, 1.2, bc=(PERIODIC, PERIODIC, REFLECTIVE, FARFIELD), nfields=5)
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    U0 = problems.oblique_riemann_state(mesh, MULTIFLUID, seed=5)
+    rng = np.random.default_rng(5)
+    U0 = U0 * (1.0 + 0.05 * rng.standard_normal(U0.shape))
+    th = (0.001, 0.001, 0.001)
+    solver.set_sensor_thresholds(th)
+    ref, sens = prob.limit_m2i(U0, th)
+    assert (sens != 0).sum() > 8192
+    U = torch.from_numpy(U0).to(device)
+    solver.limit(LIMIT_M2I, U)
+    torch.cuda.synchronize()
+    assert np.array_equal(U.cpu().numpy(), ref)

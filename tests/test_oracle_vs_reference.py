@@ -266,3 +266,35 @@ def test_m2_and_pr_limiting_bitwise(dim, order):
         R.LPrim2Cons(cfg.N_s, cfg.N_E, P(V))
         W, s2 = prob.limit_pri(U, th)
         assert np.array_equal(s2, sens) and np.array_equal(W, V)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_m2i_limiting_bitwise(dim, order):
+    """M2Ilimiting (two-fluid): sensing, then Lm2i1D + Lcopy_detected per direction, against the oracle."""
+    c = (dim, MULTIFLUID, 1)
+    R = oracle.ref_lib(*c)
+    for k, bc in enumerate(BCS_LIM[dim]):
+        mesh, t, prob, U = _setup(c, order, 50 + k, riemann=True, bc=bc)
+        x = np.arange(U.size, dtype=np.float64)
+        U = U * (1.0 + 0.01 * np.sin(0.41 * x + k))
+        th = (0.01, 0.01, 0.01)
+        sens, _ = prob.sensing(U, th)
+        assert sens.any()
+        cfg, kk = prob.cfg, prob._keep
+        V = U.copy()
+        sweeps = ([("Lag2Mono", "Mono2Lag", cfg.N_s, 1, 0)] if dim == 1 else
+                  [("Lag2MonoX", "MonoX2Lag", cfg.N_s1D, cfg.N_s1D, 0), ("Lag2MonoY", "MonoY2Lag", cfg.N_s1D, cfg.N_s1D, 2)])
+        for l2m, m2l, n1, sl, off in sweeps:
+            Vn = np.zeros_like(V)
+            R.Lm2i1D(cfg.N_s, cfg.N_E, cfg.N_N, P(kk["neighbors"]), n1, sl, off, P(kk[l2m]), P(kk[m2l]), P(sens), P(V),
+                     P(Vn))
+            # one sweep of the oracle on the same input
+            Wn = np.zeros_like(V)
+            oracle.lib().or_m2i1D(ctypes.byref(cfg), P(kk["neighbors"]), n1, sl, off, P(kk[l2m]), P(kk[m2l]), P(sens),
+                                  P(V), P(Wn))
+            assert np.array_equal(Wn, Vn)
+            R.Lcopy_detected(cfg.N_s, cfg.N_E, P(sens), P(Vn), P(V))
+        W, s2 = prob.limit_m2i(U, th)
+        assert np.array_equal(s2, sens) and np.array_equal(W, V)
+        assert not np.array_equal(W, U)
