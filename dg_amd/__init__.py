@@ -20,7 +20,7 @@ SINGLEFLUID, MULTIFLUID = 0, 1
 RUSANOV, ROE = 0, 1
 PERIODIC, FARFIELD, REFLECTIVE = 1, 2, 3
 # limiter codes of the reference (main.cc:167-175) and the learned indicator (no reference code)
-LIMIT_NONE, LIMIT_HR, LIMIT_M2, LIMIT_HRI, LIMIT_M2I, LIMIT_PR, LIMIT_PRI, LIMIT_LEARNED = 0, 1, 2, 3, 4, 5, 6, 8
+LIMIT_NONE, LIMIT_HR, LIMIT_M2, LIMIT_HRI, LIMIT_M2I, LIMIT_PR, LIMIT_PRI, LIMIT_P0I, LIMIT_LEARNED = range(9)
 PATH_AUTO, PATH_GENERAL, PATH_TILED = 0, 1, 2
 
 

@@ -432,10 +432,11 @@ static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
     if (limiter == 6) return hip_ok(launch_limit_pri(s, U, st), "primitive HRI limiter") ? DGX_OK : DGX_EDEVICE;
     return hip_ok(launch_limit_pr(s, U, st), "primitive HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
-  if (limiter == 3) {
+  if (limiter == 3 || limiter == 7) {
     if (s.cfg.physics != DGX_MULTIFLUID)
-      return fail(DGX_EUNSUPPORTED, "hri limiting needs the multifluid sensor (single-fluid sensors read past Uavg in "
-                                    "the reference, sensor_kernels.cu:55-60)");
+      return fail(DGX_EUNSUPPORTED, "hri and p0i limiting need the multifluid sensor (single-fluid sensors read past "
+                                    "Uavg in the reference, sensor_kernels.cu:55-60)");
+    if (limiter == 7) return hip_ok(launch_limit_p0i(s, U, st), "p0i limiter") ? DGX_OK : DGX_EDEVICE;
     return hip_ok(launch_limit_hri(s, U, st), "HRI limiter") ? DGX_OK : DGX_EDEVICE;
   }
   if (limiter == 8) {
@@ -444,8 +445,8 @@ static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
     return hip_ok(launch_limit_flagged(s, U, st), "flagged HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
   return fail(DGX_EUNSUPPORTED,
-              "limiter must be 0 (none), 1 (hrl), 2 (m2l), 3 (hri), 4 (m2i), 5 (prl), 6 (pri) or 8 (learned "
-              "indicator + HR)");
+              "limiter must be 0 (none), 1 (hrl), 2 (m2l), 3 (hri), 4 (m2i), 5 (prl), 6 (pri), 7 (p0i) or 8 "
+              "(learned indicator + HR)");
 }
 
 dgx_status dgx_limit(dgx_solver* h, int limiter, double* U, void* stream) {

@@ -715,6 +715,25 @@ __global__ __launch_bounds__(64) void k_m2i_list(int N_N, const int* __restrict_
   }
 }
 
+// p0i (limiting_kernels.cu:723-768) over the listed elements: one thread per
+// (element, field) sets the column to its cell average
+// sum_g sum_i phi_w[i*N_G+g] U_i / refArea.  In place: a thread reads and
+// writes only its own column, which equals the reference's Unew +
+// copy_detected.
+__global__ __launch_bounds__(256) void k_p0i_list(int NF, int Ns, int NG, double refArea,
+                                                  const double* __restrict__ phi_w, const int* __restrict__ list,
+                                                  const int* __restrict__ count, double* __restrict__ U) {
+  const long n = (long)count[0] * NF;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    double* u = U + ((long)list[t / NF] * NF + t % NF) * Ns;
+    double avg = 0;
+    for (int g = 0; g < NG; ++g)
+      for (int i = 0; i < Ns; ++i) avg += phi_w[i * NG + g] * u[i];
+    avg = avg / refArea;
+    for (int i = 0; i < Ns; ++i) u[i] = avg;
+  }
+}
+
 // list[0 .. count) = the flagged elements (sensors[e] != 0), in no fixed
 // order (each is limited independently); count must be zero on entry.  One
 // atomic add per block, on a global counter.
@@ -1029,6 +1048,20 @@ hipError_t launch_limit_m2i(Solver& s, double* U, hipStream_t st) {
   if (s.D == 1) return sweep(0, s.t.Lag2Mono, s.t.Mono2Lag);
   if ((e = sweep(0, s.t.Lag2MonoX, s.t.MonoX2Lag)) != hipSuccess) return e;
   return sweep(2, s.t.Lag2MonoY, s.t.MonoY2Lag);
+}
+
+// Limiting::P0Ilimiting (limiting.cc:723-744): sensing, then the flagged
+// elements set to their cell averages.  refArea is 4 for the quadrilateral
+// (main.cc:286); the reference leaves it unset for segments, here 2.
+hipError_t launch_limit_p0i(Solver& s, double* U, hipStream_t st) {
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag + 2, 1L, 0);
+  hipLaunchKernelGGL(k_flag_list, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.sensors, s.flag_list,
+                     s.flag + 2);
+  hipLaunchKernelGGL(k_p0i_list, dim3(1024), dim3(256), 0, st, s.NF, s.N_s, s.N_G, s.D == 1 ? 2.0 : 4.0, s.t.phi_w,
+                     s.flag_list, s.flag + 2, U);
+  return hipGetLastError();
 }
 
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {

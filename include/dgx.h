@@ -137,8 +137,9 @@ dgx_status dgx_set_sensor_thresholds(dgx_solver* s, int n, const double* thresho
  * kinetic energies; Limiting::M2limiting, limiting.cc:123-594), 3 hri (sensor-gated HR), 4 m2i (two-fluid: the
  * m2 limiting on the sensor's cells only, element by element; Limiting::M2Ilimiting, limiting.cc:682-720), 5 prl
  * (two-fluid: HR of the primitive variables; Limiting::PRlimiting, limiting.cc:102-121), 6 pri (two-fluid: sensor
- * on the conserved state, hri sweeps on the primitive variables; Limiting::PRIlimiting, limiting.cc:636-662); and
- * 8, HR gated by the learned indicator (no reference code).  p0i (7) is not provided. */
+ * on the conserved state, hri sweeps on the primitive variables; Limiting::PRIlimiting, limiting.cc:636-662), 7 p0i
+ * (two-fluid sensor: the flagged elements set to their cell averages; Limiting::P0Ilimiting, limiting.cc:723-744);
+ * and 8, HR gated by the learned indicator (no reference code). */
 dgx_status dgx_limit(dgx_solver* s, int limiter, double* U, void* stream);
 /* Out = limited In (In unchanged; In != Out, both state-sized device arrays).
  * hrl on a 2D Cartesian tile mesh runs both sweeps in one launch (the same

@@ -86,6 +86,12 @@ def lib() -> ctypes.CDLL:
         L.or_limit_m2i.restype = None
         L.or_m2i1D.argtypes = [cp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp]
         L.or_m2i1D.restype = None
+        L.or_limit_p0i.argtypes = [cp, tp, vp, vp, vp, vp]
+        L.or_limit_p0i.restype = None
+        L.or_p0i.argtypes = [cp, vp, ctypes.c_double, vp, vp, vp]
+        L.or_p0i.restype = None
+        L.or_ref_area.argtypes = [cp]
+        L.or_ref_area.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -190,6 +196,14 @@ class Problem:
         on, th = self._thresh(thresholds)
         sens = np.zeros(self.cfg.N_E, dtype=np.int32)
         lib().or_limit_m2i(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
+        return U, sens
+
+    def limit_p0i(self, U: np.ndarray, thresholds):
+        """Limiting::P0Ilimiting (two-fluid sensor); returns (U, sensors)."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        on, th = self._thresh(thresholds)
+        sens = np.zeros(self.cfg.N_E, dtype=np.int32)
+        lib().or_limit_p0i(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
         return U, sens
 
     def format(self, U: np.ndarray) -> np.ndarray:

@@ -894,6 +894,7 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
     else if (limiter == 4) or_limit_m2i(c, t, on, thresh, X, sens.data());
     else if (limiter == 5) or_limit_pr(c, t, X);
     else if (limiter == 6) or_limit_pri(c, t, on, thresh, X, sens.data());
+    else if (limiter == 7) or_limit_p0i(c, t, on, thresh, X, sens.data());
   };
   double T = 0;
   int steps = 0;
@@ -1248,6 +1249,36 @@ void or_limit_m2i(const or_cfg* c, const or_tables* t, const int* on, const doub
     or_m2i1D(c, t->neighbors, n1, n1, 2, t->Lag2MonoY, t->MonoY2Lag, sensors, U, Utmp.data());
     or_copy_detected(c, sensors, Utmp.data(), U);
   }
+}
+
+// p0i (limiting_kernels.cu:723-768): each flagged element's fields set to
+// their cell average, sum_g sum_i phi_w[i][g] U_i / refArea
+void or_p0i(const or_cfg* c, const double* phi_w, double refArea, const int* sensors, const double* U,
+            double* Unew) {
+  const int N_s = c->N_s, N_F = c->N_F, N_G = c->N_G;
+  for (int e = 0; e < c->N_E; e++)
+    for (int fc = 0; fc < N_F; fc++) {
+      if (sensors[e] == 0) continue;
+      double Uavg = 0;
+      for (int g = 0; g < N_G; g++)
+        for (int i = 0; i < N_s; i++) Uavg += phi_w[i * N_G + g] * U[((size_t)e * N_F + fc) * N_s + i];
+      Uavg = Uavg / refArea;
+      for (int i = 0; i < N_s; i++) Unew[((size_t)e * N_F + fc) * N_s + i] = Uavg;
+    }
+}
+
+// reference element measure: 4 for the quadrilateral [-1,1]^2 (main.cc:286);
+// the reference leaves it unset for segments (main.cc:284), here 2 = |[-1,1]|
+double or_ref_area(const or_cfg* c) { return c->D == 1 ? 2.0 : 4.0; }
+
+// Limiting::P0Ilimiting (limiting.cc:723-744): sensing, p0i, copy_detected
+void or_limit_p0i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors) {
+  const int N_s = c->N_s, N_E = c->N_E, N_F = c->N_F;
+  std::vector<double> Uavg((size_t)N_E * N_F), Utmp((size_t)N_s * N_E * N_F, 0.0);
+  or_sensing(c, t->neighbors, U, on, thresh, sensors, Uavg.data());
+  or_p0i(c, t->phi_w, or_ref_area(c), sensors, U, Utmp.data());
+  or_copy_detected(c, sensors, Utmp.data(), U);
 }
 
 // Limiting::PRIlimiting (limiting.cc:636-662): sensing on the conserved state,

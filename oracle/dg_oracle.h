@@ -90,6 +90,14 @@ void or_m2i1D(const or_cfg* c, const int* neighbors, int N_s1D, int slicenum, in
 void or_limit_m2i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors);
 
+/* p0i (limiting_kernels.cu:723-768): flagged elements set to their cell averages (phi_w quadrature / refArea). */
+void or_p0i(const or_cfg* c, const double* phi_w, double refArea, const int* sensors, const double* U,
+            double* Unew);
+double or_ref_area(const or_cfg* c); /* 4 for quadrilaterals (main.cc:286), 2 for segments (unset in the reference) */
+/* Limiting::P0Ilimiting (limiting.cc:723-744): sensing, p0i, copy_detected. */
+void or_limit_p0i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
+                  int* sensors);
+
 /* PRINTER formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110, GAMNCON):
  * inverse 0: out = primitive output fields of U per node (rho, ux[, uy][, gamma], p);
  * inverse 1: U = conserved fields from out (the restart direction). */

@@ -1,4 +1,4 @@
-"""The m2 and primitive-variable HR limiters (m2l, m2i, prl, pri) on the GPU: bit for
+"""The m2, primitive-variable HR and p=0 limiters (m2l, m2i, prl, pri, p0i) on the GPU: bit for
 bit the oracle's (which equals the reference's launchers, see
 test_oracle_vs_reference.py) for 1D and 2D, p = 1..3, walls and periodic
 sides; limited RK4 trajectories within 1e-11 of the oracle's; single-fluid
@@ -8,7 +8,7 @@ import pytest
 import torch
 
 import oracle
-from dg_amd import (FARFIELD, LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE,
+from dg_amd import (FARFIELD, LIMIT_M2, LIMIT_M2I, LIMIT_P0I, LIMIT_PR, LIMIT_PRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE,
                     SINGLEFLUID, Config, DgxError, Mesh, Solver, problems)
 from tests.helpers import per_field_rel, small_mesh
 
@@ -32,7 +32,8 @@ def test_m2_and_pr_bitwise(device, dim, order):
         solver.set_sensor_thresholds(th)
         for lim, ref in ((LIMIT_M2, prob.limit_m2), (LIMIT_PR, prob.limit_pr),
                          (LIMIT_PRI, lambda u: prob.limit_pri(u, th)[0]),
-                         (LIMIT_M2I, lambda u: prob.limit_m2i(u, th)[0])):
+                         (LIMIT_M2I, lambda u: prob.limit_m2i(u, th)[0]),
+                         (LIMIT_P0I, lambda u: prob.limit_p0i(u, th)[0])):
             U = torch.from_numpy(U0).to(device)
             solver.limit(lim, U)
             torch.cuda.synchronize()
@@ -40,7 +41,7 @@ def test_m2_and_pr_bitwise(device, dim, order):
 
 
 @pytest.mark.parametrize("dim,n", [(1, 256), (2, (32, 16))])
-@pytest.mark.parametrize("lim", [LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI])
+@pytest.mark.parametrize("lim", [LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI, LIMIT_P0I])
 def test_limited_trajectories(device, dim, n, lim):
     if dim == 1:
         mesh = Mesh(1, 2, n, 1, -1.0, 1.0, bc=(FARFIELD, REFLECTIVE), nfields=4)
@@ -65,7 +66,7 @@ def test_single_fluid_is_refused(device):
     mesh = small_mesh(2, 2, SINGLEFLUID)
     solver = Solver(Config(2, 2, SINGLEFLUID, ROE), mesh)
     U = torch.from_numpy(problems.riemann_state(mesh, SINGLEFLUID, seed=1)).to(device)
-    for lim in (LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI):
+    for lim in (LIMIT_M2, LIMIT_M2I, LIMIT_PR, LIMIT_PRI, LIMIT_P0I):
         with pytest.raises(DgxError):
             solver.limit(lim, U)
 

@@ -298,3 +298,35 @@ def test_m2i_limiting_bitwise(dim, order):
         W, s2 = prob.limit_m2i(U, th)
         assert np.array_equal(s2, sens) and np.array_equal(W, V)
         assert not np.array_equal(W, U)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_p0i_limiting_bitwise(dim, order):
+    """P0Ilimiting: sensing, Lp0i + Lcopy_detected against the oracle.  The
+    reference sets refArea = 4 for quadrilaterals (main.cc:286) and leaves it
+    unset for segments; the launcher takes it as an argument, so 1D is pinned
+    with the segment's length 2 that the oracle uses."""
+    c = (dim, MULTIFLUID, 1)
+    R = oracle.ref_lib(*c)
+    R.Lp0i.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp, vp, vp, vp]
+    for k, bc in enumerate(BCS_LIM[dim]):
+        mesh, t, prob, U = _setup(c, order, 60 + k, riemann=True, bc=bc)
+        cfg, kk = prob.cfg, prob._keep
+        th = (0.01, 0.01, 0.01)
+        sens, _ = prob.sensing(U, th)
+        assert sens.any() and not sens.all()
+        area = oracle.lib().or_ref_area(ctypes.byref(cfg))
+        assert area == (4.0 if dim == 2 else 2.0)
+        Vn = np.zeros_like(U)
+        R.Lp0i(cfg.N_s, cfg.N_E, cfg.N_G, area, P(sens), P(kk["phi_w"]), P(U), P(Vn))
+        V = U.copy()
+        R.Lcopy_detected(cfg.N_s, cfg.N_E, P(sens), P(Vn), P(V))
+        W, s2 = prob.limit_p0i(U, th)
+        assert np.array_equal(s2, sens) and np.array_equal(W, V)
+        # each flagged element's fields are constant; the others are untouched
+        W3 = W.reshape(cfg.N_E, prob.nf, cfg.N_s)
+        f = sens != 0
+        assert np.array_equal(W3[f], np.repeat(W3[f][:, :, :1], cfg.N_s, axis=2))
+        assert np.array_equal(W3[~f], U.reshape(cfg.N_E, prob.nf, cfg.N_s)[~f])
+
