@@ -16,7 +16,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdgx.so")
 
-SINGLEFLUID, MULTIFLUID = 0, 1
+SINGLEFLUID, MULTIFLUID, PASSIVE, STIFFENED = 0, 1, 2, 3
+
+
+def nfields(dim: int, physics: int) -> int:
+    """N_F: rho, momenta, E, then alpha (two-fluid), rho*phic and phinc
+    (passive) or alpha and beta (stiffened)."""
+    return 2 + dim + (0, 1, 2, 2)[physics]
 RUSANOV, ROE = 0, 1
 PERIODIC, FARFIELD, REFLECTIVE = 1, 2, 3
 # limiter codes of the reference (main.cc:167-175) and the learned indicator (no reference code)
@@ -29,7 +35,7 @@ class Config(ctypes.Structure):
 
     @property
     def nfields(self) -> int:
-        return 2 + self.dim + self.physics
+        return nfields(self.dim, self.physics)
 
 
 _lib = None
@@ -163,7 +169,7 @@ class Mesh:
 
     def read_pos(self, physics: int, prefix: str, step: int):
         """(printed fields, time) read from .pos files."""
-        nf = 2 + self.sizes["D"] + physics
+        nf = nfields(self.sizes["D"], physics)
         out = np.zeros(self.sizes["N_E"] * nf * self.sizes["N_s"])
         t = ctypes.c_double()
         _check(lib().dgx_pos_read(self._h, physics, str(prefix).encode(), step, ctypes.byref(t), out.ctypes.data),

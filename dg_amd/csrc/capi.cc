@@ -178,7 +178,7 @@ extern "C" {
 
 const char* dgx_last_error(void) { return g_err.c_str(); }
 
-int dgx_nfields(const dgx_config* c) { return 2 + c->dim + c->physics; }
+int dgx_nfields(const dgx_config* c) { return 2 + c->dim + (c->physics == 0 ? 0 : (c->physics == 1 ? 1 : 2)); }
 
 // ------------------------------------------------------------------ mesh
 dgx_status dgx_mesh_build(int dim, int order, int nx, int ny, double x0, double x1, double y0, double y1,
@@ -271,7 +271,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   const Tables& t = mesh->t;
   if (cfg->dim != t.D || cfg->order != t.order)
     return fail(DGX_EINVAL, "dgx_solver_create: configuration dim/order do not match the mesh");
-  if (cfg->physics < 0 || cfg->physics > 1 || cfg->flux < 0 || cfg->flux > 1)
+  if (cfg->physics < 0 || cfg->physics > 3 || cfg->flux < 0 || cfg->flux > 1)
     return fail(DGX_EINVAL, "dgx_solver_create: unknown physics or flux");
   dgx_solver* h = new dgx_solver;
   h->ref_nodes = mesh->t.ref_nodes;
@@ -313,6 +313,9 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   s.t.minv_stride = uniform_minv ? 0 : 1;
   s.t.J0 = t.J[0];
   cartesian_layout(t, s.cart);
+  // the tile kernel covers single-fluid and two-fluid states; passive and
+  // stiffened states run on the face + element kernels
+  if (cfg->physics > 1) s.cart.on = false;
 
   // every element edge must receive exactly one contributing face side
   std::vector<int> ef((size_t)NE * NN, -1);
@@ -433,9 +436,9 @@ static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
     return hip_ok(launch_limit_pr(s, U, st), "primitive HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
   if (limiter == 3 || limiter == 7) {
-    if (s.cfg.physics != DGX_MULTIFLUID)
-      return fail(DGX_EUNSUPPORTED, "hri and p0i limiting need the multifluid sensor (single-fluid sensors read past "
-                                    "Uavg in the reference, sensor_kernels.cu:55-60)");
+    if (s.cfg.physics == DGX_SINGLEFLUID)
+      return fail(DGX_EUNSUPPORTED, "hri and p0i limiting need the sensor, which reads past Uavg for single-fluid "
+                                    "states in the reference (sensor_kernels.cu:55-60)");
     if (limiter == 7) return hip_ok(launch_limit_p0i(s, U, st), "p0i limiter") ? DGX_OK : DGX_EDEVICE;
     return hip_ok(launch_limit_hri(s, U, st), "HRI limiter") ? DGX_OK : DGX_EDEVICE;
   }
@@ -473,14 +476,14 @@ dgx_status dgx_format(dgx_solver* h, double* U, double* out, int inverse, void* 
 dgx_status dgx_pos_write(const dgx_mesh* m, int physics, const double* out, const char* prefix, int step,
                          double time) {
   if (!m || !out || !prefix) return fail(DGX_EINVAL, "dgx_pos_write: null argument");
-  if (physics != DGX_SINGLEFLUID && physics != DGX_MULTIFLUID) return fail(DGX_EINVAL, "dgx_pos_write: physics");
+  if (physics < DGX_SINGLEFLUID || physics > DGX_STIFFENED) return fail(DGX_EINVAL, "dgx_pos_write: physics");
   const std::string err = pos_write(m->t, physics, out, prefix, step, time);
   return err.empty() ? DGX_OK : fail(DGX_EINVAL, "dgx_pos_write: " + err);
 }
 
 dgx_status dgx_pos_read(const dgx_mesh* m, int physics, const char* prefix, int step, double* time, double* out) {
   if (!m || !out || !prefix) return fail(DGX_EINVAL, "dgx_pos_read: null argument");
-  if (physics != DGX_SINGLEFLUID && physics != DGX_MULTIFLUID) return fail(DGX_EINVAL, "dgx_pos_read: physics");
+  if (physics < DGX_SINGLEFLUID || physics > DGX_STIFFENED) return fail(DGX_EINVAL, "dgx_pos_read: physics");
   const std::string err = pos_read(m->t, physics, prefix, step, time, out);
   return err.empty() ? DGX_OK : fail(DGX_EINVAL, "dgx_pos_read: " + err);
 }

@@ -35,7 +35,15 @@ static hipError_t dispatch_phys(Fn&& fn) {
   using I2 = std::integral_constant<int, 2>;
   const dgx_config& c = g_cfg;
   auto fl = [&](auto d, auto ph) { return c.flux == 0 ? fn(d, ph, I0{}) : fn(d, ph, I1{}); };
-  auto ph = [&](auto d) { return c.physics == 0 ? fl(d, I0{}) : fl(d, I1{}); };
+  using I3 = std::integral_constant<int, 3>;
+  auto ph = [&](auto d) {
+    switch (c.physics) {
+      case 0: return fl(d, I0{});
+      case 1: return fl(d, I1{});
+      case 2: return fl(d, I2{});
+      default: return fl(d, I3{});
+    }
+  };
   return c.dim == 1 ? ph(I1{}) : ph(I2{});
 }
 
@@ -94,17 +102,18 @@ __global__ void kc_add_sfq(long n, double* A, const double* S, const double* F, 
 template <int D, int PH>
 __global__ void kc_evaluate_sf(int N_G, int N_E, double* s, double* f, const double* Ug, const double* dUg,
                                const double* invJac) {
-  constexpr int NF = Fields<D, PH>::NF;
+  constexpr int NF = Fields<D, PH>::NF, NNC = Fields<D, PH>::NNC, NFX = NF - NNC;
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)N_E * N_G) return;
   int e = (int)(k / N_G), g = (int)(k % N_G);
-  double u[NF], dua[D], ij[D * D], fx[NF * D], sa;
+  double u[NF], dua[(NNC ? NNC : 1) * D], ij[D * D], fx[NF * D], sa[NNC ? NNC : 1];
   for (int fc = 0; fc < NF; ++fc) u[fc] = Ug[((long)e * NF + fc) * N_G + g];
-  for (int al = 0; al < D; ++al) dua[al] = PH ? dUg[(((long)e * NF + (NF - 1)) * N_G + g) * D + al] : 0.0;
+  for (int m = 0; m < NNC; ++m)
+    for (int al = 0; al < D; ++al) dua[m * D + al] = dUg[(((long)e * NF + (NFX + m)) * N_G + g) * D + al];
   for (int q = 0; q < D * D; ++q) ij[q] = invJac[((long)e * N_G + g) * D * D + q];
   volume_flux<D, PH>(u, dua, ij, fx, sa);
   for (int fc = 0; fc < NF; ++fc) {
-    s[((long)e * NF + fc) * N_G + g] = (PH && fc == NF - 1) ? sa : 0.0;
+    s[((long)e * NF + fc) * N_G + g] = fc >= NFX ? sa[fc - NFX] : 0.0;
     for (int a = 0; a < D; ++a) f[(((long)e * NF + fc) * N_G + g) * D + a] = fx[fc * D + a];
   }
 }
@@ -215,7 +224,7 @@ extern "C" {
 
 dgx_status dgx_set_config(const dgx_config* cfg) {
   if (!cfg || (cfg->dim != 1 && cfg->dim != 2) || cfg->order < 1 || cfg->order > 3 || cfg->physics < 0 ||
-      cfg->physics > 1 || cfg->flux < 0 || cfg->flux > 1) {
+      cfg->physics > 3 || cfg->flux < 0 || cfg->flux > 1) {
     set_error("dgx_set_config: invalid configuration");
     return DGX_EINVAL;
   }

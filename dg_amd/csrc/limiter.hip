@@ -364,7 +364,10 @@ __global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int*
   double vxL = Uavg[e * NF + fc] / rhoL; fc++;
   if (D == 2) { vyL = Uavg[e * NF + fc] / rhoL; fc++; }
   double EtL = Uavg[e * NF + fc]; fc++;
-  double alphaL = (PH == 0) ? 1 / (kGammaSF - 1) : Uavg[e * NF + fc];
+  // passive states use the single-fluid gamma; stiffened states read beta
+  // after alpha (sensor_kernels.cu:57-64)
+  double alphaL = (PH == 0 || PH == 2) ? 1 / (kGammaSF - 1) : Uavg[e * NF + fc];
+  if (PH == 3) betaL = Uavg[e * NF + fc + 1];
   double gammaL = 1.0 + 1.0 / alphaL;
   double pinfL = (1 - 1.0 / gammaL) * betaL;
   double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
@@ -380,7 +383,8 @@ __global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int*
     double vxR = Uavg[right * NF + fc] / rhoR; fc++;
     if (D == 2) { vyR = Uavg[right * NF + fc] / rhoR; fc++; }
     double EtR = Uavg[right * NF + fc]; fc++;
-    double alphaR = (PH == 0) ? 1 / (kGammaSF - 1) : Uavg[right * NF + fc];
+    double alphaR = (PH == 0 || PH == 2) ? 1 / (kGammaSF - 1) : Uavg[right * NF + fc];
+    if (PH == 3) betaR = Uavg[right * NF + fc + 1];
     double gammaR = 1.0 + 1.0 / alphaR;
     double pinfR = (1 - 1.0 / gammaR) * betaR;
     double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
@@ -782,6 +786,20 @@ __global__ __launch_bounds__(256) void k_copy_detected(int N_E, int per_elem, co
 // ------------------------------------------------------------ host side
 static inline unsigned nblk(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// fn(dim, physics) with both as integral constants (physics 0..3)
+template <typename Fn>
+static hipError_t by_dim_phys(int D, int physics, Fn&& fn) {
+  auto ph = [&](auto d) -> hipError_t {
+    switch (physics) {
+      case 0: return fn(d, std::integral_constant<int, 0>{});
+      case 1: return fn(d, std::integral_constant<int, 1>{});
+      case 2: return fn(d, std::integral_constant<int, 2>{});
+      default: return fn(d, std::integral_constant<int, 3>{});
+    }
+  };
+  return D == 1 ? ph(std::integral_constant<int, 1>{}) : ph(std::integral_constant<int, 2>{});
+}
+
 template <typename Fn>
 static hipError_t by_order_ns(int D, int P, Fn&& fn) {
   using I1 = std::integral_constant<int, 1>;
@@ -983,11 +1001,7 @@ hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg
                        st, s.N_E, s.N_N, s.t.neighbors, Uavg, th, tgt, val);
     return hipGetLastError();
   };
-  using I0 = std::integral_constant<int, 0>;
-  using I1 = std::integral_constant<int, 1>;
-  using I2 = std::integral_constant<int, 2>;
-  if (s.D == 1) e = s.cfg.physics ? hit(I1{}, I1{}) : hit(I1{}, I0{});
-  else e = s.cfg.physics ? hit(I2{}, I1{}) : hit(I2{}, I0{});
+  e = by_dim_phys(s.D, s.cfg.physics, hit);
   if (e != hipSuccess) return e;
   // sensors start at zero (sensor.cc:17-20)
   int* init = s.sens_act + 2 * (long)s.N_E;
@@ -1100,12 +1114,7 @@ hipError_t compat_sensors(Solver& tmp, const int* nb, const double* Uavg, int* s
                        0, 0, tmp.N_E, tmp.N_N, nb, Uavg, th, tgt, val);
     return hipGetLastError();
   };
-  using I0 = std::integral_constant<int, 0>;
-  using I1 = std::integral_constant<int, 1>;
-  using I2 = std::integral_constant<int, 2>;
-  hipError_t e;
-  if (tmp.D == 1) e = tmp.cfg.physics ? hit(I1{}, I1{}) : hit(I1{}, I0{});
-  else e = tmp.cfg.physics ? hit(I2{}, I1{}) : hit(I2{}, I0{});
+  hipError_t e = by_dim_phys(tmp.D, tmp.cfg.physics, hit);
   if (e != hipSuccess) return e;
   tmp.sens_act = scratch + 2 * tmp.N_E;
   int* init = scratch + 4 * tmp.N_E;

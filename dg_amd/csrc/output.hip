@@ -23,63 +23,95 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <type_traits>
 
 #include "dgx_internal.h"
 #include "physics.cuh"
 
 namespace dgx {
 
+// Lformater (printer_{oned,twod}_{singlefluid,multifluid,passive,stiffened}.cu:20-110)
+// node by node.  Printed fields: rho, ux[, uy], then p (single fluid);
+// G, p (two-fluid); p, phic, phinc (passive); G, Pinf, p (stiffened).
 template <int D, int PH>
 __global__ __launch_bounds__(256) void k_format(int N_s, int N_E, double* __restrict__ U, double* __restrict__ out,
                                                 int inverse) {
-  constexpr int NF = 2 + D + PH;
+  constexpr int NF = Fields<D, PH>::NF;
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)N_E * N_s) return;
   const long e = k / N_s, i = k % N_s;
   double* u = U + e * NF * N_s + i;
   double* o = out + e * NF * N_s + i;
+  constexpr int x = D + 1;  // first field after the velocities
   if (!inverse) {
     const double rho = u[0];
     const double ux = u[1 * N_s] / rho;
     const double uy = D == 2 ? u[2 * N_s] / rho : 0.0;
-    const double et = u[(D + 1) * N_s];
-    const double gamma = PH ? 1 + 1.0 / u[(D + 2) * N_s] : kGammaSF;
+    const double et = u[x * N_s];
+    const double gamma = (PH == 1 || PH == 3) ? 1 + 1.0 / u[(x + 1) * N_s] : kGammaSF;
     o[0] = rho;
     o[1 * N_s] = ux;
     if (D == 2) o[2 * N_s] = uy;
-    if (PH) o[(D + 1) * N_s] = gamma;
-    if (D == 2)
-      o[(D + 1 + PH) * N_s] = (gamma - 1) * (et - 0.5 * (ux * ux + uy * uy) * rho);
-    else
-      o[(D + 1 + PH) * N_s] = (gamma - 1) * (et - 0.5 * ux * ux * rho);
+    if constexpr (PH == 3) {
+      const double pinf = (1 - 1.0 / gamma) * u[(x + 2) * N_s];
+      o[x * N_s] = gamma;
+      o[(x + 1) * N_s] = pinf;
+      if (D == 2)
+        o[(x + 2) * N_s] = (gamma - 1) * (et - gamma * pinf / (gamma - 1) - 0.5 * (ux * ux + uy * uy) * rho);
+      else
+        o[(x + 2) * N_s] = (gamma - 1) * (et - gamma * pinf / (gamma - 1) - 0.5 * ux * ux * rho);
+    } else {
+      constexpr int ip = PH == 1 ? x + 1 : x;  // printed pressure
+      if (PH == 1) o[x * N_s] = gamma;
+      if (D == 2)
+        o[ip * N_s] = (gamma - 1) * (et - 0.5 * (ux * ux + uy * uy) * rho);
+      else
+        o[ip * N_s] = (gamma - 1) * (et - 0.5 * ux * ux * rho);
+      if constexpr (PH == 2) {
+        o[(x + 1) * N_s] = u[(x + 1) * N_s] / rho;
+        o[(x + 2) * N_s] = u[(x + 2) * N_s];
+      }
+    }
   } else {
     const double rho = o[0], ux = o[1 * N_s];
     const double uy = D == 2 ? o[2 * N_s] : 0.0;
-    const double gamma = PH ? o[(D + 1) * N_s] : kGammaSF;
-    const double p = o[(D + 1 + PH) * N_s];
+    const double ke = D == 2 ? 0.5 * rho * (ux * ux + uy * uy) : 0.5 * rho * ux * ux;
     u[0] = rho;
     u[1 * N_s] = rho * ux;
-    if (D == 2) {
-      u[2 * N_s] = rho * uy;
-      u[(D + 1) * N_s] = p / (gamma - 1) + 0.5 * rho * (ux * ux + uy * uy);
+    if (D == 2) u[2 * N_s] = rho * uy;
+    if constexpr (PH == 3) {
+      const double gamma = o[x * N_s], pinf = o[(x + 1) * N_s], p = o[(x + 2) * N_s];
+      u[x * N_s] = p / (gamma - 1) + gamma * pinf / (gamma - 1) + ke;
+      u[(x + 1) * N_s] = 1.0 / (gamma - 1);
+      u[(x + 2) * N_s] = gamma * pinf / (gamma - 1);
     } else {
-      u[(D + 1) * N_s] = p / (gamma - 1) + 0.5 * rho * ux * ux;
+      const double gamma = PH == 1 ? o[x * N_s] : kGammaSF;
+      const double p = o[(PH == 1 ? x + 1 : x) * N_s];
+      u[x * N_s] = p / (gamma - 1) + ke;
+      if (PH == 1) u[(x + 1) * N_s] = 1.0 / (gamma - 1);
+      if constexpr (PH == 2) {
+        const double phic = o[(x + 1) * N_s], phinc = o[(x + 2) * N_s];
+        u[(x + 1) * N_s] = rho * phic;
+        u[(x + 2) * N_s] = phinc;
+      }
     }
-    if (PH) u[(D + 2) * N_s] = 1.0 / (gamma - 1);
   }
 }
 
 hipError_t launch_format(const Solver& s, double* U, double* out, int inverse, hipStream_t st) {
   const long n = (long)s.N_E * s.N_s;
   const dim3 g((unsigned)((n + 255) / 256)), b(256);
-  const bool mf = s.cfg.physics == DGX_MULTIFLUID;
-  if (s.D == 1) {
-    if (mf) hipLaunchKernelGGL((k_format<1, 1>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
-    else hipLaunchKernelGGL((k_format<1, 0>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
-  } else {
-    if (mf) hipLaunchKernelGGL((k_format<2, 1>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
-    else hipLaunchKernelGGL((k_format<2, 0>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse);
-  }
+  auto go = [&](auto d) {
+    constexpr int D = decltype(d)::value;
+    switch (s.cfg.physics) {
+      case 0: hipLaunchKernelGGL((k_format<D, 0>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse); break;
+      case 1: hipLaunchKernelGGL((k_format<D, 1>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse); break;
+      case 2: hipLaunchKernelGGL((k_format<D, 2>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse); break;
+      default: hipLaunchKernelGGL((k_format<D, 3>), g, b, 0, st, s.N_s, s.N_E, U, out, inverse); break;
+    }
+  };
+  if (s.D == 1) go(std::integral_constant<int, 1>{});
+  else go(std::integral_constant<int, 2>{});
   return hipGetLastError();
 }
 
@@ -88,9 +120,14 @@ void pos_field_names(int D, int physics, std::vector<std::string>& names, std::v
   names = {"Rho", "Ux"};
   fnames = {"rho", "ux"};
   if (D == 2) { names.push_back("Uy"); fnames.push_back("uy"); }
-  if (physics == DGX_MULTIFLUID) { names.push_back("G"); fnames.push_back("g"); }
+  if (physics == DGX_MULTIFLUID || physics == DGX_STIFFENED) { names.push_back("G"); fnames.push_back("g"); }
+  if (physics == DGX_STIFFENED) { names.push_back("Pinf"); fnames.push_back("pinf"); }
   names.push_back("P");
   fnames.push_back("p");
+  if (physics == DGX_PASSIVE) {
+    names.push_back("PhiC"); fnames.push_back("phic");
+    names.push_back("PhiNC"); fnames.push_back("phinc");
+  }
 }
 
 static std::string pos_path(const std::string& prefix, const std::string& fname, int step) {

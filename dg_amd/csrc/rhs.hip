@@ -286,7 +286,12 @@ static hipError_t dispatch(const dgx_config& c, Fn&& fn) {
     return c.flux == 0 ? fn(d, p, ph, I0{}) : fn(d, p, ph, I1{});
   };
   auto by_phys = [&](auto d, auto p) -> hipError_t {
-    return c.physics == 0 ? by_flux(d, p, I0{}) : by_flux(d, p, I1{});
+    switch (c.physics) {
+      case 0: return by_flux(d, p, I0{});
+      case 1: return by_flux(d, p, I1{});
+      case 2: return by_flux(d, p, I2{});
+      default: return by_flux(d, p, I3{});
+    }
   };
   auto by_order = [&](auto d) -> hipError_t {
     switch (c.order) {

@@ -19,10 +19,11 @@ template <int D, int P, int PH>
 struct Vol {
   using S = Shape<D, P>;
   static constexpr int NF = Fields<D, PH>::NF, Ns = S::Ns, NG = S::NG, Ms = S::Ms, NN = S::NN;
-  static constexpr int NFX = NF - PH;       // fields that carry a volume flux (alpha has none)
+  static constexpr int NNC = Fields<D, PH>::NNC;  // non-conservative fields (the last NNC; no volume flux)
+  static constexpr int NFX = NF - NNC;            // fields that carry a volume flux
   static constexpr int NU = NF * Ns;        // state doubles per element
   static constexpr int NQE = NN * NF * Ms;  // face-contribution doubles per element
-  static constexpr int NQ = NFX * D + PH;   // per-point values handed from phase 1 to phase 2
+  static constexpr int NQ = NFX * D + NNC;  // per-point values handed from phase 1 to phase 2
   static constexpr int TPE = pow2ceil(NG);  // lanes per element in phase 1
   static constexpr int EPB = 256 / TPE;     // elements per 256-thread batch
   static constexpr int NTAB_PHI = NG * Ns, NTAB_D = NG * D * Ns;
@@ -48,9 +49,9 @@ __device__ inline void load_vol_tables(const VolTables<D, P, PH>& t, const doubl
   for (int k = tid; k < V::NTAB_D; k += nthr) {
     const int i = k / (V::NG * D), r = k % (V::NG * D);
     if (with_dw) t.dw[r * V::Ns + i] = dphi_w[k];
-    if constexpr (PH == 1) t.dphi[k] = dphi[k];
+    if constexpr (V::NNC > 0) t.dphi[k] = dphi[k];
   }
-  if constexpr (PH == 1)
+  if constexpr (V::NNC > 0)
     for (int k = tid; k < V::NTAB_PHI; k += nthr) t.pw[(k % V::NG) * V::Ns + k / V::NG] = phi_w[k];
   if (uniform_minv && Minv)
     for (int k = tid; k < V::Ns * V::Ns; k += nthr) t.minv[k] = Minv[k];
@@ -61,7 +62,7 @@ template <int D, int P, int PH>
 __device__ inline void vol_phase1(const VolTables<D, P, PH>& t, int g, const double* u, double Je, const double* ij,
                                   double* out) {
   using V = Vol<D, P, PH>;
-  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, NFX = V::NFX;
+  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, NFX = V::NFX, NNC = V::NNC;
   double ug[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
@@ -70,19 +71,19 @@ __device__ inline void vol_phase1(const VolTables<D, P, PH>& t, int g, const dou
     for (int i = 0; i < Ns; ++i) s += u[f * Ns + i] * t.phi[g + i * NG];
     ug[f] = s;
   }
-  double dua[D];
+  double dua[(NNC ? NNC : 1) * D];
 #pragma unroll
-  for (int al = 0; al < D; ++al) {
-    double s = 0.0;
-    if constexpr (PH == 1) {
+  for (int m = 0; m < NNC; ++m)
 #pragma unroll
-      for (int i = 0; i < Ns; ++i) s += u[(NF - 1) * Ns + i] * t.dphi[(g * D + al) + i * NG * D];
+    for (int al = 0; al < D; ++al) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < Ns; ++i) s += u[(NFX + m) * Ns + i] * t.dphi[(g * D + al) + i * NG * D];
+      dua[m * D + al] = s;
     }
-    dua[al] = s;
-  }
   double fx[NF * D];
-  double salpha;
-  volume_flux<D, PH>(ug, dua, ij, fx, salpha);
+  double snc[NNC ? NNC : 1];
+  volume_flux<D, PH>(ug, dua, ij, fx, snc);
 #pragma unroll
   for (int f = 0; f < NFX; ++f)
 #pragma unroll
@@ -92,7 +93,8 @@ __device__ inline void vol_phase1(const VolTables<D, P, PH>& t, int g, const dou
       for (int k = 0; k < D; ++k) sol += ij[al * D + k] * fx[f * D + k] * Je;
       out[f * D + al] = sol;
     }
-  if constexpr (PH == 1) out[NFX * D] = salpha * Je;
+#pragma unroll
+  for (int m = 0; m < NNC; ++m) out[NFX * D + m] = snc[m] * Je;
 }
 
 // Phase 2 for lane (element, node i): S+F from the per-point values fe
@@ -103,7 +105,7 @@ template <int D, int P, int PH>
 __device__ inline void vol_phase2(const VolTables<D, P, PH>& t, int i, const double* fe, const double* qe,
                                   double* rhs) {
   using V = Vol<D, P, PH>;
-  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, NFX = V::NFX, NQ = V::NQ, NN = V::NN, Ms = V::Ms;
+  constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, NFX = V::NFX, NQ = V::NQ, NN = V::NN, Ms = V::Ms, NNC = V::NNC;
   double acc[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) acc[f] = 0.0;
@@ -115,7 +117,8 @@ __device__ inline void vol_phase2(const VolTables<D, P, PH>& t, int i, const dou
 #pragma unroll
       for (int f = 0; f < NFX; ++f) acc[f] += w * fe[g * NQ + f * D + al];
     }
-    if constexpr (PH == 1) acc[NF - 1] += t.pw[g * Ns + i] * fe[g * NQ + NFX * D];
+#pragma unroll
+    for (int m = 0; m < NNC; ++m) acc[NFX + m] += t.pw[g * Ns + i] * fe[g * NQ + NFX * D + m];
   }
   double q[NF];
 #pragma unroll
@@ -243,6 +246,7 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
   using MM = Mfma16<TM>;
   constexpr int NF = 4 + PH, Ns = 9, NG = 16, D = 2, KP = NG + 1, Ms = 3;
   static_assert(NT % 16 == 0, "groups of 16 elements, four per wave");
+  static_assert(PH <= 1, "single-fluid and two-fluid states only");
   const int w = tid >> 6, l = tid & 63, c = l & 15, q = l >> 4;
   double* S = sWork + w * vol_wave_lds<T, PH>();
   T* F = reinterpret_cast<T*>(S + (PH ? 4 * NG : 0));
@@ -293,7 +297,7 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
       const double Je = sGeo[el * NGEO];
       const double* ij = sGeo + el * NGEO + 1;
       double fx[NF * D], salpha;
-      volume_flux<2, PH>(ug, dua, ij, fx, salpha);
+      volume_flux<2, PH>(ug, dua, ij, fx, &salpha);
 #pragma unroll
       for (int f = 0; f < 4; ++f)
 #pragma unroll

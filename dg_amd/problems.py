@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-from . import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID, Config, Mesh
+from . import FARFIELD, MULTIFLUID, PASSIVE, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID, STIFFENED, Config, Mesh, nfields
 
 GAMMA = 1.4
 
@@ -27,6 +27,21 @@ def node_xy(mesh: Mesh):
     x = xyz[..., 0]
     y = xyz[..., 1] if s["D"] == 2 else None
     return x, y
+
+
+def _closure(physics, rho, p, gam, ke, pinf=None, phic=None, phinc=None):
+    """[E, extra fields...] for the physics: E = p/(gamma-1) + ke (+ gamma
+    pinf/(gamma-1), stiffened); two-fluid adds 1/(gamma-1); passive adds
+    rho*phic and phinc; stiffened adds 1/(gamma-1) and gamma pinf/(gamma-1)."""
+    if physics == STIFFENED:
+        beta = gam * pinf / (gam - 1.0)
+        return [p / (gam - 1.0) + beta + ke, 1.0 / (gam - 1.0), beta]
+    f = [p / (gam - 1.0) + ke]
+    if physics == MULTIFLUID:
+        f.append(1.0 / (gam - 1.0))
+    elif physics == PASSIVE:
+        f += [rho * phic, phinc]
+    return f
 
 
 def conserved_1d(rho, u, p, gamma, physics):
@@ -47,7 +62,7 @@ def conserved_2d(rho, u, v, p, gamma, physics):
 
 # ----------------------------------------------------------------- 1D Sod
 def sod_mesh(nx: int = 1000, order: int = 2, physics: int = MULTIFLUID) -> Mesh:
-    return Mesh(1, order, nx, 1, -1.0, 1.0, bc=(FARFIELD, FARFIELD), nfields=2 + 1 + physics)
+    return Mesh(1, order, nx, 1, -1.0, 1.0, bc=(FARFIELD, FARFIELD), nfields=nfields(1, physics))
 
 
 def sod_state(mesh: Mesh, physics: int = MULTIFLUID) -> np.ndarray:
@@ -137,17 +152,30 @@ def smooth_random_state(mesh: Mesh, physics: int, seed: int = 0, amp: float = 0.
     rho = 1.0 + amp * wave()
     p = 1.0 + amp * wave()
     u = 0.5 * amp * wave() + rng.uniform(-0.3, 0.3)
-    gam = 1.4 + 0.2 * (0.5 + 0.5 * np.tanh(wave())) if physics == MULTIFLUID else np.full_like(x, GAMMA)
+    gam = (1.4 + 0.2 * (0.5 + 0.5 * np.tanh(wave())) if physics in (MULTIFLUID, STIFFENED)
+           else np.full_like(x, GAMMA))
+    extra = {}
+    if physics == STIFFENED:
+        extra["pinf"] = 0.5 + 0.5 * amp * wave()
+    elif physics == PASSIVE:
+        extra = {"phic": 0.5 + amp * wave(), "phinc": 0.5 + amp * wave()}
     if D == 1:
-        E = p / (gam - 1.0) + 0.5 * rho * u * u
-        f = [rho, rho * u, E]
+        f = [rho, rho * u] + _closure(physics, rho, p, gam, 0.5 * rho * u * u, **extra)
     else:
         v = 0.5 * amp * wave() + rng.uniform(-0.3, 0.3)
-        E = p / (gam - 1.0) + 0.5 * rho * (u * u + v * v)
-        f = [rho, rho * u, rho * v, E]
-    if physics == MULTIFLUID:
-        f.append(1.0 / (gam - 1.0))
+        f = [rho, rho * u, rho * v] + _closure(physics, rho, p, gam, 0.5 * rho * (u * u + v * v), **extra)
     return _pack(mesh, f)
+
+
+def _side_extras(physics, rng, left):
+    """Per-side pinf (stiffened) or phic/phinc (passive) for the Riemann states."""
+    if physics == STIFFENED:
+        a, b = rng.uniform(0.0, 2.0, 2)
+        return {"pinf": np.where(left, a, b)}
+    if physics == PASSIVE:
+        c = rng.uniform(0.0, 1.0, 4)
+        return {"phic": np.where(left, c[0], c[1]), "phinc": np.where(left, c[2], c[3])}
+    return {}
 
 
 def riemann_state(mesh: Mesh, physics: int, seed: int = 0, x_split: float | None = None) -> np.ndarray:
@@ -162,20 +190,14 @@ def riemann_state(mesh: Mesh, physics: int, seed: int = 0, x_split: float | None
     rl, rr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
     pl, pr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
     ul, ur = rng.uniform(-2, 2, 2)
-    gl, gr = (rng.uniform(1.1, 1.7, 2) if physics == MULTIFLUID else (GAMMA, GAMMA))
+    gl, gr = (rng.uniform(1.1, 1.7, 2) if physics in (MULTIFLUID, STIFFENED) else (GAMMA, GAMMA))
     rho = np.where(left, rl, rr)
     p = np.where(left, pl, pr)
     u = np.where(left, ul, ur)
     gam = np.where(left, gl, gr)
-    if D == 1:
-        E = p / (gam - 1.0) + 0.5 * rho * u * u
-        f = [rho, rho * u, E]
-    else:
-        v = np.zeros_like(x)
-        E = p / (gam - 1.0) + 0.5 * rho * u * u
-        f = [rho, rho * u, v, E]
-    if physics == MULTIFLUID:
-        f.append(1.0 / (gam - 1.0))
+    extra = _side_extras(physics, rng, left)
+    tail = _closure(physics, rho, p, gam, 0.5 * rho * u * u, **extra)
+    f = [rho, rho * u] + ([] if D == 1 else [np.zeros_like(x)]) + tail
     return _pack(mesh, f)
 
 
@@ -195,16 +217,14 @@ def oblique_riemann_state(mesh: Mesh, physics: int, seed: int = 0) -> np.ndarray
     pl, pr = np.exp(rng.uniform(np.log(0.5), np.log(2.0), 2))
     ul, ur = rng.uniform(-0.5, 0.5, 2)
     vl, vr = rng.uniform(-0.5, 0.5, 2)
-    gl, gr = (rng.uniform(1.1, 1.7, 2) if physics == MULTIFLUID else (GAMMA, GAMMA))
+    gl, gr = (rng.uniform(1.1, 1.7, 2) if physics in (MULTIFLUID, STIFFENED) else (GAMMA, GAMMA))
     rho = np.where(left, rl, rr)
     p = np.where(left, pl, pr)
     u = np.where(left, ul, ur)
     v = np.where(left, vl, vr)
     gam = np.where(left, gl, gr)
-    E = p / (gam - 1.0) + 0.5 * rho * (u * u + v * v)
-    f = [rho, rho * u, rho * v, E]
-    if physics == MULTIFLUID:
-        f.append(1.0 / (gam - 1.0))
+    extra = _side_extras(physics, rng, left)
+    f = [rho, rho * u, rho * v] + _closure(physics, rho, p, gam, 0.5 * rho * (u * u + v * v), **extra)
     return _pack(mesh, f)
 
 

@@ -39,13 +39,15 @@ typedef _Bool dgx_bool;
 
 #define DGX_SINGLEFLUID 0
 #define DGX_MULTIFLUID 1
+#define DGX_PASSIVE 2   /* rho, momenta, E, rho*phic (conservative), phinc (non-conservative); gamma = 1.4 */
+#define DGX_STIFFENED 3 /* rho, momenta, E, alpha = 1/(gamma-1), beta = gamma*pinf/(gamma-1) */
 #define DGX_RUSANOV 0
 #define DGX_ROE 1
 
 typedef struct {
   int dim;     /* D: 1 or 2 */
   int order;   /* p: 1..3 */
-  int physics; /* DGX_SINGLEFLUID | DGX_MULTIFLUID (GAMNCON, no mass fractions) */
+  int physics; /* DGX_SINGLEFLUID | DGX_MULTIFLUID (GAMNCON) | DGX_PASSIVE | DGX_STIFFENED; no mass fractions */
   int flux;    /* DGX_RUSANOV | DGX_ROE */
 } dgx_config;
 

@@ -101,7 +101,7 @@ def ref_available() -> bool:
 
 
 def ref_name(dim: int, physics: int, flux: int) -> str:
-    return f"{dim}d_{'mf' if physics else 'sf'}_{'roe' if flux else 'rus'}"
+    return f"{dim}d_{('sf', 'mf', 'pa', 'st')[physics]}_{'roe' if flux else 'rus'}"
 
 
 _refs: dict = {}
@@ -126,7 +126,7 @@ class Problem:
     def __init__(self, tables: dict, physics: int, flux: int):
         self.t = tables
         D = tables["D"]
-        self.nf = 2 + D + physics
+        self.nf = 2 + D + (0, 1, 2, 2)[physics]
         c = OrCfg()
         c.D, c.N_F, c.physics, c.flux = D, self.nf, physics, flux
         for n in ("N_s", "N_G", "M_s", "M_G", "N_N", "N_E", "M_T", "N_s1D"):

@@ -119,6 +119,273 @@ void roe_2d_sf(double rhoL, double rhoR, double vxL, double vxR, double vyL, dou
   F[3] = 0.5 * (rhoL * vnL * HL + rhoR * vnR * HR) - 0.5 * (ws0 * dV0 * R03 + ws1 * dV1 * R13 + ws2 * dV2 * R23 + ws3 * dV3 * R33);
 }
 
+// ---- passive scalars: rho, momenta, E, rho*phic (conservative), phinc
+// (non-conservative), single-fluid gamma
+
+// oned_passive_fluxes.h:26-65
+void rus_1d_pa(double rhoL, double rhoR, double vxL, double vxR, double EtL, double EtR, double phicL, double phicR,
+               double phincL, double phincR, double nx, double* F, double* nc) {
+  double gamma = GAMMA_SF;
+  double pL = (gamma - 1) * (EtL - 0.5 * rhoL * vxL * vxL);
+  double pR = (gamma - 1) * (EtR - 0.5 * rhoR * vxR * vxR);
+  double aL = std::sqrt((gamma * pL) / rhoL);
+  double aR = std::sqrt((gamma * pR) / rhoR);
+  double maxvap = dmax(std::fabs(vxL) + aL, std::fabs(vxR) + aR);
+  F[0] = 0.5 * ((rhoL * vxL + rhoR * vxR) * nx - maxvap * (rhoR - rhoL));
+  F[1] = 0.5 * ((rhoL * vxL * vxL + pL + (rhoR * vxR * vxR + pR)) * nx - maxvap * (rhoR * vxR - rhoL * vxL));
+  F[2] = 0.5 * (((EtL + pL) * vxL + (EtR + pR) * vxR) * nx - maxvap * (EtR - EtL));
+  F[3] = 0.5 * ((rhoL * vxL * phicL + rhoR * vxR * phicR) * nx - maxvap * (rhoR * phicR - rhoL * phicL));
+  F[4] = 0.5 * (-maxvap * (phincR - phincL));
+  nc[4] = 0.5 * 0.5 * (vxL + vxR) * (phincL - phincR) * nx;
+}
+
+// oned_passive_fluxes.h:175-266
+void roe_1d_pa(double rhoL, double rhoR, double vxL, double vxR, double EtL, double EtR, double phicL, double phicR,
+               double phincL, double phincR, double nx, double* F, double* nc) {
+  double gamma = GAMMA_SF;
+  double pL = (gamma - 1) * (EtL - 0.5 * rhoL * vxL * vxL);
+  double pR = (gamma - 1) * (EtR - 0.5 * rhoR * vxR * vxR);
+  double HL = (EtL + pL) / rhoL;
+  double HR = (EtR + pR) / rhoR;
+  double RT = std::sqrt(rhoR / rhoL);
+  double rho = RT * rhoL;
+  double v = (vxL + RT * vxR) / (1 + RT);
+  double H = (HL + RT * HR) / (1 + RT);
+  double a = std::sqrt((gamma - 1) * (H - 0.5 * v * v));
+  double Dp = pR - pL;
+  double dV0 = (Dp - rho * a * (vxR - vxL)) / (2 * a * a);
+  double dV1 = (rhoR - rhoL) - Dp / (a * a);
+  double dV2 = (Dp + rho * a * (vxR - vxL)) / (2 * a * a);
+  double ws0 = std::fabs(v - a), ws1 = std::fabs(v), ws2 = std::fabs(v + a);
+  double R00 = 1, R01 = v - a, R02 = H - v * a;
+  double R10 = 1, R11 = v, R12 = 0.5 * v * v;
+  double R20 = 1, R21 = v + a, R22 = H + v * a;
+  F[0] = 0.5 * (rhoL * vxL + rhoR * vxR) * nx - 0.5 * (ws0 * dV0 * R00 + ws1 * dV1 * R10 + ws2 * dV2 * R20);
+  F[1] = 0.5 * (rhoL * vxL * vxL + pL + (rhoR * vxR * vxR + pR)) * nx - 0.5 * (ws0 * dV0 * R01 + ws1 * dV1 * R11 + ws2 * dV2 * R21);
+  F[2] = 0.5 * ((EtL + pL) * vxL + (EtR + pR) * vxR) * nx - 0.5 * (ws0 * dV0 * R02 + ws1 * dV1 * R12 + ws2 * dV2 * R22);
+  F[3] = 0.5 * (rhoL * vxL * phicL + rhoR * vxR * phicR) * nx - 0.5 * (ws0 * dV0 * R00 + ws1 * dV1 * R10 + ws2 * dV2 * R20);
+  F[4] = 0.0 - 0.5 * (ws0 * dV0 * R00 + ws1 * dV1 * R10 + ws2 * dV2 * R20);
+  nc[4] = -0.5 * v * (phincR - phincL) * nx;
+}
+
+// twod_passive_fluxes.h:27-75
+void rus_2d_pa(double rhoL, double rhoR, double vxL, double vxR, double vyL, double vyR, double EtL, double EtR,
+               double phicL, double phicR, double phincL, double phincR, double nx, double ny, double* F, double* nc) {
+  double vnL = vxL * nx + vyL * ny;
+  double vnR = vxR * nx + vyR * ny;
+  double gamma = GAMMA_SF;
+  double pL = (gamma - 1) * (EtL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
+  double pR = (gamma - 1) * (EtR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
+  double aL = std::sqrt((gamma * pL) / rhoL);
+  double aR = std::sqrt((gamma * pR) / rhoR);
+  double HL = (EtL + pL) / rhoL;
+  double HR = (EtR + pR) / rhoR;
+  double maxvap = dmax(std::fabs(vnL) + aL, std::fabs(vnR) + aR);
+  F[0] = 0.5 * (rhoL * vnL + rhoR * vnR - maxvap * (rhoR - rhoL));
+  F[1] = 0.5 * ((rhoL * vnL * vxL + pL * nx) + (rhoR * vnR * vxR + pR * nx) - maxvap * (rhoR * vxR - rhoL * vxL));
+  F[2] = 0.5 * ((rhoL * vnL * vyL + pL * ny) + (rhoR * vnR * vyR + pR * ny) - maxvap * (rhoR * vyR - rhoL * vyL));
+  F[3] = 0.5 * (rhoL * vnL * HL + rhoR * vnR * HR - maxvap * (EtR - EtL));
+  F[4] = 0.5 * (rhoL * vnL * phicL + rhoR * vnR * phicR - maxvap * (rhoR * phicR - rhoL * phicL));
+  F[5] = 0.5 * (-maxvap * (phincR - phincL));
+  nc[5] = 0.5 * 0.5 * (vnL + vnR) * (phincL - phincR);
+}
+
+// twod_passive_fluxes.h:197-323
+void roe_2d_pa(double rhoL, double rhoR, double vxL, double vxR, double vyL, double vyR, double EtL, double EtR,
+               double phicL, double phicR, double phincL, double phincR, double nx, double ny, double* F, double* nc) {
+  double tx = -ny, ty = nx;
+  double vnL = vxL * nx + vyL * ny;
+  double vnR = vxR * nx + vyR * ny;
+  double vtL = vxL * tx + vyL * ty;
+  double vtR = vxR * tx + vyR * ty;
+  double gamma = GAMMA_SF;
+  double pL = (gamma - 1) * (EtL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
+  double pR = (gamma - 1) * (EtR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
+  double HL = (EtL + pL) / rhoL;
+  double HR = (EtR + pR) / rhoR;
+  double RT = std::sqrt(rhoR / rhoL);
+  double rho = RT * rhoL;
+  double vx = (vxL + RT * vxR) / (1 + RT);
+  double vy = (vyL + RT * vyR) / (1 + RT);
+  double vn = vx * nx + vy * ny;
+  double vt = vx * tx + vy * ty;
+  double H = (HL + RT * HR) / (1 + RT);
+  double a = std::sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
+  double drho = rhoR - rhoL, dp = pR - pL, dvn = vnR - vnL, dvt = vtR - vtL;
+  double dV0 = (dp - rho * a * dvn) / (2 * a * a);
+  double dV1 = rho * dvt / a;
+  double dV2 = drho - dp / (a * a);
+  double dV3 = (dp + rho * a * dvn) / (2 * a * a);
+  double ws0 = std::fabs(vn - a), ws1 = std::fabs(vn), ws2 = std::fabs(vn), ws3 = std::fabs(vn + a);
+  double R00 = 1, R01 = vx - a * nx, R02 = vy - a * ny, R03 = H - vn * a;
+  double R10 = 0, R11 = a * tx, R12 = a * ty, R13 = vt * a;
+  double R20 = 1, R21 = vx, R22 = vy, R23 = 0.5 * (vx * vx + vy * vy);
+  double R30 = 1, R31 = vx + a * nx, R32 = vy + a * ny, R33 = H + vn * a;
+  double w0 = ws0 * dV0 * R00 + ws1 * dV1 * R10 + ws2 * dV2 * R20 + ws3 * dV3 * R30;
+  F[0] = 0.5 * (rhoL * vnL + rhoR * vnR) - 0.5 * w0;
+  F[1] = 0.5 * ((rhoL * vnL * vxL + pL * nx) + (rhoR * vnR * vxR + pR * nx)) -
+         0.5 * (ws0 * dV0 * R01 + ws1 * dV1 * R11 + ws2 * dV2 * R21 + ws3 * dV3 * R31);
+  F[2] = 0.5 * ((rhoL * vnL * vyL + pL * ny) + (rhoR * vnR * vyR + pR * ny)) -
+         0.5 * (ws0 * dV0 * R02 + ws1 * dV1 * R12 + ws2 * dV2 * R22 + ws3 * dV3 * R32);
+  F[3] = 0.5 * (rhoL * vnL * HL + rhoR * vnR * HR) - 0.5 * (ws0 * dV0 * R03 + ws1 * dV1 * R13 + ws2 * dV2 * R23 + ws3 * dV3 * R33);
+  F[4] = 0.5 * (rhoL * vnL * phicL + rhoR * vnR * phicR) - 0.5 * w0;
+  F[5] = -0.5 * w0;
+  nc[5] = 0.5 * vn * (phincL - phincR);
+}
+
+// ---- stiffened gas: rho, momenta, E, alpha = 1/(gamma-1), beta =
+// gamma*pinf/(gamma-1), both advected non-conservatively
+
+// oned_stiffened_fluxes.h:29-88 (no mass fractions)
+void rus_1d_st(double rhoL, double rhoR, double vxL, double vxR, double EtL, double EtR, double alphaL, double alphaR,
+               double betaL, double betaR, double nx, double* F, double* nc) {
+  double gammaL = 1.0 + 1.0 / alphaL;
+  double gammaR = 1.0 + 1.0 / alphaR;
+  double pinfL = (1 - 1.0 / gammaL) * betaL;
+  double pinfR = (1 - 1.0 / gammaR) * betaR;
+  double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * vxL * vxL);
+  double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * vxR * vxR);
+  double aL = std::sqrt((gammaL * (pL + pinfL)) / rhoL);
+  double aR = std::sqrt((gammaR * (pR + pinfR)) / rhoR);
+  double maxvap = dmax(std::fabs(vxL) + aL, std::fabs(vxR) + aR);
+  F[0] = 0.5 * ((rhoL * vxL + rhoR * vxR) * nx - maxvap * (rhoR - rhoL));
+  F[1] = 0.5 * ((rhoL * vxL * vxL + pL + (rhoR * vxR * vxR + pR)) * nx - maxvap * (rhoR * vxR - rhoL * vxL));
+  F[2] = 0.5 * (((EtL + pL) * vxL + (EtR + pR) * vxR) * nx - maxvap * (EtR - EtL));
+  F[3] = 0.5 * (-maxvap * (alphaR - alphaL));
+  nc[3] = -0.5 * 0.5 * (vxL + vxR) * (alphaR - alphaL) * nx;
+  F[4] = 0.5 * (-maxvap * (betaR - betaL));
+  nc[4] = -0.5 * 0.5 * (vxL + vxR) * (betaR - betaL) * nx;
+}
+
+// oned_stiffened_fluxes.h:142-295 (no mass fractions); F and nc zeroed by the caller
+void roe_1d_st(double rhoL, double rhoR, double vxL, double vxR, double EtL, double EtR, double alphaL, double alphaR,
+               double betaL, double betaR, double nx, double* F, double* nc) {
+  double gammaL = 1.0 + 1.0 / alphaL;
+  double gammaR = 1.0 + 1.0 / alphaR;
+  double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * vxL * vxL);
+  double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * vxR * vxR);
+  double RT = std::sqrt(rhoR / rhoL);
+  double rho = RT * rhoL;
+  double v = (vxL + RT * vxR) / (1 + RT);
+  double HL = (EtL + pL) / rhoL;
+  double HR = (EtR + pR) / rhoR;
+  double H = (HL + RT * HR) / (1 + RT);
+  double alpha = (alphaL + RT * alphaR) / (1 + RT);
+  double gamma = 1 + 1.0 / alpha;
+  double a = std::sqrt((gamma - 1) * (H - 0.5 * v * v));
+  double iL = pL * alphaL;
+  double iR = pR * alphaR;
+  double i = (std::sqrt(rhoL) * iL + std::sqrt(rhoR) * iR) / (std::sqrt(rhoL) + std::sqrt(rhoR));
+  double Dp = (gamma - 1) * (gamma - 1) * (alpha * (iR - iL) - i * (alphaR - alphaL));
+  double p = (gamma - 1) * i;
+  double dV0 = (Dp - rho * a * (vxR - vxL)) / (2 * a * a);
+  double dV1 = (rhoR - rhoL) - Dp / (a * a);
+  double dV2 = (Dp + rho * a * (vxR - vxL)) / (2 * a * a);
+  double dV3 = alphaR - alphaL;
+  double dV4 = betaR - betaL;
+  double ws0 = std::fabs(v - a), ws1 = std::fabs(v), ws2 = std::fabs(v + a), ws3 = std::fabs(v), ws4 = std::fabs(v);
+  double R00 = 1, R01 = v - a, R02 = H - v * a, R03 = 0, R04 = 0;
+  double R10 = 1, R11 = v, R12 = 0.5 * v * v, R13 = 0, R14 = 0;
+  double R20 = 1, R21 = v + a, R22 = H + v * a, R23 = 0, R24 = 0;
+  double R30 = 0, R31 = 0, R32 = p, R33 = 1, R34 = 0;
+  double R40 = 0, R41 = 0, R42 = 1, R43 = 0, R44 = 1;
+  F[0] = 0.5 * (rhoL * vxL + rhoR * vxR) * nx -
+         0.5 * (ws0 * dV0 * R00 + ws1 * dV1 * R10 + ws2 * dV2 * R20 + ws3 * dV3 * R30 + ws4 * dV4 * R40);
+  F[1] = 0.5 * (rhoL * vxL * vxL + pL + (rhoR * vxR * vxR + pR)) * nx -
+         0.5 * (ws0 * dV0 * R01 + ws1 * dV1 * R11 + ws2 * dV2 * R21 + ws3 * dV3 * R31 + ws4 * dV4 * R41);
+  F[2] = 0.5 * ((EtL + pL) * vxL + (EtR + pR) * vxR) * nx -
+         0.5 * (ws0 * dV0 * R02 + ws1 * dV1 * R12 + ws2 * dV2 * R22 + ws3 * dV3 * R32 + ws4 * dV4 * R42);
+  nc[3] = -0.5 * v * (alphaR - alphaL) * nx;
+  F[3] += -0.5 * (ws0 * dV0 * R03 + ws1 * dV1 * R13 + ws2 * dV2 * R23 + ws3 * dV3 * R33 + ws4 * dV4 * R43);
+  nc[4] = -0.5 * v * (betaR - betaL) * nx;
+  F[4] += -0.5 * (ws0 * dV0 * R04 + ws1 * dV1 * R14 + ws2 * dV2 * R24 + ws3 * dV3 * R34 + ws4 * dV4 * R44);
+}
+
+// twod_stiffened_fluxes.h:30-104 (no mass fractions)
+void rus_2d_st(double rhoL, double rhoR, double vxL, double vxR, double vyL, double vyR, double EtL, double EtR,
+               double alphaL, double alphaR, double betaL, double betaR, double nx, double ny, double* F, double* nc) {
+  double vnL = vxL * nx + vyL * ny;
+  double vnR = vxR * nx + vyR * ny;
+  double gammaL = 1.0 + 1.0 / alphaL;
+  double gammaR = 1.0 + 1.0 / alphaR;
+  double pinfL = (1 - 1.0 / gammaL) * betaL;
+  double pinfR = (1 - 1.0 / gammaR) * betaR;
+  double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
+  double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
+  double aL = std::sqrt((gammaL * (pL + pinfL)) / rhoL);
+  double aR = std::sqrt((gammaR * (pR + pinfR)) / rhoR);
+  double HL = (EtL + pL) / rhoL;
+  double HR = (EtR + pR) / rhoR;
+  double maxvap = dmax(std::fabs(vnL) + aL, std::fabs(vnR) + aR);
+  F[0] = 0.5 * (rhoL * vnL + rhoR * vnR - maxvap * (rhoR - rhoL));
+  F[1] = 0.5 * ((rhoL * vnL * vxL + pL * nx) + (rhoR * vnR * vxR + pR * nx) - maxvap * (rhoR * vxR - rhoL * vxL));
+  F[2] = 0.5 * ((rhoL * vnL * vyL + pL * ny) + (rhoR * vnR * vyR + pR * ny) - maxvap * (rhoR * vyR - rhoL * vyL));
+  F[3] = 0.5 * (rhoL * vnL * HL + rhoR * vnR * HR - maxvap * (EtR - EtL));
+  F[4] = 0.5 * (-maxvap * (alphaR - alphaL));
+  nc[4] = -0.5 * 0.5 * (vnL + vnR) * (alphaR - alphaL);
+  F[5] = 0.5 * (-maxvap * (betaR - betaL));
+  nc[5] = -0.5 * 0.5 * (vnL + vnR) * (betaR - betaL);
+}
+
+// twod_stiffened_fluxes.h:153-362 (no mass fractions); F and nc zeroed by the caller
+void roe_2d_st(double rhoL, double rhoR, double vxL, double vxR, double vyL, double vyR, double EtL, double EtR,
+               double alphaL, double alphaR, double betaL, double betaR, double nx, double ny, double* F, double* nc) {
+  double tx = -ny, ty = nx;
+  double vnL = vxL * nx + vyL * ny;
+  double vnR = vxR * nx + vyR * ny;
+  double vtL = vxL * tx + vyL * ty;
+  double vtR = vxR * tx + vyR * ty;
+  double gammaL = 1.0 + 1.0 / alphaL;
+  double gammaR = 1.0 + 1.0 / alphaR;
+  double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
+  double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
+  double HL = (EtL + pL) / rhoL;
+  double HR = (EtR + pR) / rhoR;
+  double iL = pL * alphaL;
+  double iR = pR * alphaR;
+  double RT = std::sqrt(rhoR / rhoL);
+  double rho = RT * rhoL;
+  double vx = (vxL + RT * vxR) / (1 + RT);
+  double vy = (vyL + RT * vyR) / (1 + RT);
+  double vn = vx * nx + vy * ny;
+  double vt = vx * tx + vy * ty;
+  double H = (HL + RT * HR) / (1 + RT);
+  double alpha = (alphaL + RT * alphaR) / (1 + RT);
+  double gamma = 1 + 1.0 / alpha;
+  double a = std::sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
+  double i = (iL + RT * iR) / (1 + RT);
+  double p = (gamma - 1) * i;
+  double drho = rhoR - rhoL;
+  double dp = (gamma - 1) * (gamma - 1) * (alpha * (iR - iL) - i * (alphaR - alphaL));
+  double dvn = vnR - vnL, dvt = vtR - vtL;
+  double dV0 = (dp - rho * a * dvn) / (2 * a * a);
+  double dV1 = rho * dvt / a;
+  double dV2 = drho - dp / (a * a);
+  double dV3 = (dp + rho * a * dvn) / (2 * a * a);
+  double dV4 = alphaR - alphaL;
+  double dV5 = betaR - betaL;
+  double ws0 = std::fabs(vn - a), ws1 = std::fabs(vn), ws2 = std::fabs(vn), ws3 = std::fabs(vn + a);
+  double ws4 = std::fabs(vn), ws5 = std::fabs(vn);
+  double R00 = 1, R01 = vx - a * nx, R02 = vy - a * ny, R03 = H - vn * a, R04 = 0, R05 = 0;
+  double R10 = 0, R11 = a * tx, R12 = a * ty, R13 = vt * a, R14 = 0, R15 = 0;
+  double R20 = 1, R21 = vx, R22 = vy, R23 = 0.5 * (vx * vx + vy * vy), R24 = 0, R25 = 0;
+  double R30 = 1, R31 = vx + a * nx, R32 = vy + a * ny, R33 = H + vn * a, R34 = 0, R35 = 0;
+  double R40 = 0, R41 = 0, R42 = 0, R43 = p, R44 = 1, R45 = 0;
+  double R50 = 0, R51 = 0, R52 = 0, R53 = 1, R54 = 0, R55 = 1;
+  F[0] = 0.5 * (rhoL * vnL + rhoR * vnR) -
+         0.5 * (ws0 * dV0 * R00 + ws1 * dV1 * R10 + ws2 * dV2 * R20 + ws3 * dV3 * R30 + ws4 * dV4 * R40 + ws5 * dV5 * R50);
+  F[1] = 0.5 * ((rhoL * vnL * vxL + pL * nx) + (rhoR * vnR * vxR + pR * nx)) -
+         0.5 * (ws0 * dV0 * R01 + ws1 * dV1 * R11 + ws2 * dV2 * R21 + ws3 * dV3 * R31 + ws4 * dV4 * R41 + ws5 * dV5 * R51);
+  F[2] = 0.5 * ((rhoL * vnL * vyL + pL * ny) + (rhoR * vnR * vyR + pR * ny)) -
+         0.5 * (ws0 * dV0 * R02 + ws1 * dV1 * R12 + ws2 * dV2 * R22 + ws3 * dV3 * R32 + ws4 * dV4 * R42 + ws5 * dV5 * R52);
+  F[3] = 0.5 * (rhoL * vnL * HL + rhoR * vnR * HR) -
+         0.5 * (ws0 * dV0 * R03 + ws1 * dV1 * R13 + ws2 * dV2 * R23 + ws3 * dV3 * R33 + ws4 * dV4 * R43 + ws5 * dV5 * R53);
+  nc[4] = -0.5 * vn * (alphaR - alphaL);
+  F[4] += -0.5 * (ws0 * dV0 * R04 + ws1 * dV1 * R14 + ws2 * dV2 * R24 + ws3 * dV3 * R34 + ws4 * dV4 * R44 + ws5 * dV5 * R54);
+  nc[5] = -0.5 * vn * (betaR - betaL);
+  F[5] += -0.5 * (ws0 * dV0 * R05 + ws1 * dV1 * R15 + ws2 * dV2 * R25 + ws3 * dV3 * R35 + ws4 * dV4 * R45 + ws5 * dV5 * R55);
+}
+
 // oned_multifluid_fluxes.h:30-94 (GAMNCON, no mass fractions)
 void rus_1d_mf(double rhoL, double rhoR, double vxL, double vxR, double EtL, double EtR, double alphaL,
                double alphaR, double nx, double* F, double* nc) {
@@ -439,7 +706,7 @@ void or_evaluate_sf(const or_cfg* c, double* s, double* f, const double* Ug, con
           FX(0, 0) = rho * u;
           FX(1, 0) = rho * u * u + p;
           FX(2, 0) = (Et + p) * u;
-        } else {
+        } else if (c->physics == 1) {
           double gamma = 1 + 1.0 / UG(3);
           double p = (gamma - 1) * (Et - 0.5 * rho * u * u);
           S(0) = 0;
@@ -450,6 +717,30 @@ void or_evaluate_sf(const or_cfg* c, double* s, double* f, const double* Ug, con
           FX(2, 0) = (Et + p) * u;
           S(3) = -u * dUg[(e * N_F + 3) * N_G + g] * invJac[e * N_G + g];
           FX(3, 0) = 0;
+        } else if (c->physics == 2) {  // physics.cu:57-78
+          double gamma = GAMMA_SF;
+          double p = (gamma - 1) * (Et - 0.5 * rho * u * u);
+          S(0) = 0; S(1) = 0; S(2) = 0; S(3) = 0;
+          S(4) = -u * dUg[(e * N_F + 4) * N_G + g] * invJac[e * N_G + g];
+          FX(0, 0) = rho * u;
+          FX(1, 0) = rho * u * u + p;
+          FX(2, 0) = (Et + p) * u;
+          FX(3, 0) = rho * u * (UG(3) / rho);
+          FX(4, 0) = 0;
+        } else {  // physics.cu:144-170
+          double gamma = 1 + 1.0 / UG(3);
+          double beta = UG(4);
+          double p = (gamma - 1) * (Et - beta - 0.5 * rho * u * u);
+          S(0) = 0;
+          S(1) = rho * GX;
+          S(2) = rho * u * GX;
+          S(3) = -u * dUg[(e * N_F + 3) * N_G + g] * invJac[e * N_G + g];
+          S(4) = -u * dUg[(e * N_F + 4) * N_G + g] * invJac[e * N_G + g];
+          FX(0, 0) = rho * u;
+          FX(1, 0) = rho * u * u + p;
+          FX(2, 0) = (Et + p) * u;
+          FX(3, 0) = 0;
+          FX(4, 0) = 0;
         }
       } else {
         double rho = UG(0);
@@ -462,6 +753,41 @@ void or_evaluate_sf(const or_cfg* c, double* s, double* f, const double* Ug, con
           S(0) = 0; S(1) = 0; S(2) = 0; S(3) = 0;
           FX(0, 0) = rho * u; FX(1, 0) = rho * u * u + p; FX(2, 0) = rho * u * v; FX(3, 0) = (Et + p) * u;
           FX(0, 1) = rho * v; FX(1, 1) = rho * u * v; FX(2, 1) = rho * v * v + p; FX(3, 1) = (Et + p) * v;
+        } else if (c->physics == 2) {  // physics.cu:183-220
+          double phic = UG(4) / rho;
+          double vdotgradphinc = 0;
+          for (int al = 0; al < D; al++)
+            vdotgradphinc += -u * dUg[((e * N_F + 5) * N_G + g) * D + al] * invJac[((e * N_G + g) * D + 0) * D + al] -
+                             v * dUg[((e * N_F + 5) * N_G + g) * D + al] * invJac[((e * N_G + g) * D + 1) * D + al];
+          double gamma = GAMMA_SF;
+          double p = (gamma - 1) * (Et - 0.5 * rho * (u * u + v * v));
+          S(0) = 0; S(1) = 0; S(2) = 0; S(3) = 0; S(4) = 0;
+          S(5) = vdotgradphinc;
+          FX(0, 0) = rho * u; FX(1, 0) = rho * u * u + p; FX(2, 0) = rho * u * v; FX(3, 0) = (Et + p) * u;
+          FX(4, 0) = rho * u * phic; FX(5, 0) = 0;
+          FX(0, 1) = rho * v; FX(1, 1) = rho * u * v; FX(2, 1) = rho * v * v + p; FX(3, 1) = (Et + p) * v;
+          FX(4, 1) = rho * v * phic; FX(5, 1) = 0;
+        } else if (c->physics == 3) {  // physics.cu:302-352
+          double gamma = 1 + 1.0 / UG(4);
+          double beta = UG(5);
+          double p = (gamma - 1) * (Et - beta - 0.5 * rho * (u * u + v * v));
+          double vdotgradalpha = 0, vdotgradbeta = 0;
+          for (int al = 0; al < D; al++) {
+            vdotgradalpha += -u * dUg[((e * N_F + 4) * N_G + g) * D + al] * invJac[((e * N_G + g) * D + 0) * D + al] -
+                             v * dUg[((e * N_F + 4) * N_G + g) * D + al] * invJac[((e * N_G + g) * D + 1) * D + al];
+            vdotgradbeta += -u * dUg[((e * N_F + 5) * N_G + g) * D + al] * invJac[((e * N_G + g) * D + 0) * D + al] -
+                            v * dUg[((e * N_F + 5) * N_G + g) * D + al] * invJac[((e * N_G + g) * D + 1) * D + al];
+          }
+          S(0) = 0;
+          S(1) = rho * GX;
+          S(2) = rho * GY;
+          S(3) = rho * (u * GX + v * GY);
+          S(4) = vdotgradalpha;
+          S(5) = vdotgradbeta;
+          FX(0, 0) = rho * u; FX(1, 0) = rho * u * u + p; FX(2, 0) = rho * u * v; FX(3, 0) = (Et + p) * u;
+          FX(4, 0) = 0; FX(5, 0) = 0;
+          FX(0, 1) = rho * v; FX(1, 1) = rho * u * v; FX(2, 1) = rho * v * v + p; FX(3, 1) = (Et + p) * v;
+          FX(4, 1) = 0; FX(5, 1) = 0;
         } else {
           double gamma = 1 + 1.0 / UG(4);
           double p = (gamma - 1) * (Et - 0.5 * rho * (u * u + v * v));
@@ -505,6 +831,14 @@ void or_evaluate_q(const or_cfg* c, double* q, const double* UgF, const double* 
             rus_1d_sf(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0), UF(2, 1), nx, F);
           else
             roe_1d_sf(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0), UF(2, 1), nx, F);
+        } else if (c->physics == 2) {
+          auto fn = c->flux == 0 ? rus_1d_pa : roe_1d_pa;
+          fn(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0), UF(2, 1), UF(3, 0) / UF(0, 0),
+             UF(3, 1) / UF(0, 1), UF(4, 0), UF(4, 1), nx, F, nc);
+        } else if (c->physics == 3) {
+          auto fn = c->flux == 0 ? rus_1d_st : roe_1d_st;
+          fn(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0), UF(2, 1), UF(3, 0), UF(3, 1),
+             UF(4, 0), UF(4, 1), nx, F, nc);
         } else {
           if (c->flux == 0)
             rus_1d_mf(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0), UF(2, 1), UF(3, 0),
@@ -522,6 +856,14 @@ void or_evaluate_q(const or_cfg* c, double* q, const double* UgF, const double* 
           else
             roe_2d_sf(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0) / UF(0, 0),
                       UF(2, 1) / UF(0, 1), UF(3, 0), UF(3, 1), nx, ny, F);
+        } else if (c->physics == 2) {
+          auto fn = c->flux == 0 ? rus_2d_pa : roe_2d_pa;
+          fn(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0) / UF(0, 0), UF(2, 1) / UF(0, 1),
+             UF(3, 0), UF(3, 1), UF(4, 0) / UF(0, 0), UF(4, 1) / UF(0, 1), UF(5, 0), UF(5, 1), nx, ny, F, nc);
+        } else if (c->physics == 3) {
+          auto fn = c->flux == 0 ? rus_2d_st : roe_2d_st;
+          fn(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0) / UF(0, 0), UF(2, 1) / UF(0, 1),
+             UF(3, 0), UF(3, 1), UF(4, 0), UF(4, 1), UF(5, 0), UF(5, 1), nx, ny, F, nc);
         } else {
           if (c->flux == 0)
             rus_2d_mf(UF(0, 0), UF(0, 1), UF(1, 0) / UF(0, 0), UF(1, 1) / UF(0, 1), UF(2, 0) / UF(0, 0),
@@ -609,14 +951,29 @@ void or_findUPA(const or_cfg* c, const double* U, double* UPA) {
     for (int i = 0; i < N_s; i++) {
 #define UU(k) U[(e * N_F + (k)) * N_s + i]
       double r;
-      if (c->D == 1) {
+      if (c->physics == 3) {  // oned_/twod_stiffened_upa (upa.h:98-130)
+        if (c->D == 1) {
+          double rho = UU(0), u = UU(1) / UU(0), E = UU(2), alpha = UU(3), beta = UU(4);
+          double gamma = 1.0 + 1.0 / alpha;
+          double pinf = (1 - 1.0 / gamma) * beta;
+          double p = (gamma - 1) * (E - beta - 0.5 * rho * u * u);
+          r = std::fabs(u) + std::sqrt(gamma * (p + pinf) / rho);
+        } else {
+          double rho = UU(0), u = UU(1) / UU(0), v = UU(2) / UU(0), E = UU(3), alpha = UU(4), beta = UU(5);
+          double gamma = 1.0 + 1.0 / alpha;
+          double pinf = (1 - 1.0 / gamma) * beta;
+          double p = (gamma - 1) * (E - beta - 0.5 * rho * (u * u + v * v));
+          double a = std::sqrt(gamma * (p + pinf) / rho);
+          r = dmax(std::fabs(u) + a, std::fabs(v) + a);
+        }
+      } else if (c->D == 1) {  // passive: the single-fluid form (upa.h:20-40)
         double rho = UU(0), u = UU(1) / UU(0), E = UU(2);
-        double gamma = c->physics == 0 ? GAMMA_SF : 1.0 + 1.0 / UU(3);
+        double gamma = c->physics != 1 ? GAMMA_SF : 1.0 + 1.0 / UU(3);
         double p = (gamma - 1) * (E - 0.5 * rho * u * u);
         r = std::fabs(u) + std::sqrt(gamma * p / rho);
       } else {
         double rho = UU(0), u = UU(1) / UU(0), v = UU(2) / UU(0), E = UU(3);
-        double gamma = c->physics == 0 ? GAMMA_SF : 1.0 + 1.0 / UU(4);
+        double gamma = c->physics != 1 ? GAMMA_SF : 1.0 + 1.0 / UU(4);
         double p = (gamma - 1) * (E - 0.5 * rho * (u * u + v * v));
         double a = std::sqrt(gamma * p / rho);
         r = dmax(std::fabs(u) + a, std::fabs(v) + a);
@@ -725,8 +1082,9 @@ void or_calc_sensors(const or_cfg* c, const int* on, const double* thresh, const
     if (D == 2) { vyL = Uavg[e * N_F + fc] / rhoL; fc++; }
     double EtL = Uavg[e * N_F + fc]; fc++;
     double alphaL;
-    if (c->physics == 0) alphaL = 1 / (GAMMA_SF - 1);
+    if (c->physics == 0 || c->physics == 2) alphaL = 1 / (GAMMA_SF - 1);
     else { alphaL = Uavg[e * N_F + fc]; fc++; }
+    if (c->physics == 3) { betaL = Uavg[e * N_F + fc]; fc++; }
     double gammaL = 1.0 + 1.0 / alphaL;
     pinfL = (1 - 1.0 / gammaL) * betaL;
     double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
@@ -743,8 +1101,9 @@ void or_calc_sensors(const or_cfg* c, const int* on, const double* thresh, const
         if (D == 2) { vyR = Uavg[right * N_F + fc] / rhoR; fc++; }
         double EtR = Uavg[right * N_F + fc]; fc++;
         double alphaR;
-        if (c->physics == 0) alphaR = 1 / (GAMMA_SF - 1);
+        if (c->physics == 0 || c->physics == 2) alphaR = 1 / (GAMMA_SF - 1);
         else { alphaR = Uavg[right * N_F + fc]; fc++; }
+        if (c->physics == 3) { betaR = Uavg[right * N_F + fc]; fc++; }
         double gammaR = 1.0 + 1.0 / alphaR;
         pinfR = (1 - 1.0 / gammaR) * betaR;
         double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
@@ -934,7 +1293,8 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
 // printer_twod_singlefluid.cu:20-90, printer_oned_*.cu): node-wise
 // conserved <-> printed primitive fields, GAMNCON (gamma = 1 + 1/alpha).
 void or_format(const or_cfg* c, double* U, double* out, int inverse) {
-  const int NF = c->N_F, Ns = c->N_s, D = c->D, mf = c->physics == 1;
+  const int NF = c->N_F, Ns = c->N_s, D = c->D, ph = c->physics;
+  const int x = D + 1;  // first field after the velocities
   for (int e = 0; e < c->N_E; e++)
     for (int i = 0; i < Ns; i++) {
       auto u = [&](int f) -> double& { return U[(e * NF + f) * Ns + i]; };
@@ -942,23 +1302,48 @@ void or_format(const or_cfg* c, double* U, double* out, int inverse) {
       if (!inverse) {
         const double rho = u(0), ux = u(1) / rho;
         const double uy = D == 2 ? u(2) / rho : 0.0;
-        const double et = u(D + 1);
-        const double gamma = mf ? 1 + 1.0 / u(D + 2) : GAMMA_SF;
+        const double et = u(x);
         o(0) = rho;
         o(1) = ux;
         if (D == 2) o(2) = uy;
-        if (mf) o(D + 1) = gamma;
         const double ke = D == 2 ? 0.5 * (ux * ux + uy * uy) * rho : 0.5 * ux * ux * rho;
-        o(D + 1 + mf) = (gamma - 1) * (et - ke);
+        if (ph == 3) {  // printer_{oned,twod}_stiffened.cu
+          const double gamma = 1 + 1.0 / u(x + 1);
+          const double pinf = (1 - 1.0 / gamma) * u(x + 2);
+          o(x) = gamma;
+          o(x + 1) = pinf;
+          o(x + 2) = (gamma - 1) * (et - gamma * pinf / (gamma - 1) - ke);
+        } else {
+          const double gamma = ph == 1 ? 1 + 1.0 / u(x + 1) : GAMMA_SF;
+          if (ph == 1) o(x) = gamma;
+          o(ph == 1 ? x + 1 : x) = (gamma - 1) * (et - ke);
+          if (ph == 2) {  // printer_{oned,twod}_passive.cu
+            o(x + 1) = u(x + 1) / rho;
+            o(x + 2) = u(x + 2);
+          }
+        }
       } else {
         const double rho = o(0), ux = o(1), uy = D == 2 ? o(2) : 0.0;
-        const double gamma = mf ? o(D + 1) : GAMMA_SF;
-        const double p = o(D + 1 + mf);
+        const double ke = D == 2 ? 0.5 * rho * (ux * ux + uy * uy) : 0.5 * rho * ux * ux;
         u(0) = rho;
         u(1) = rho * ux;
         if (D == 2) u(2) = rho * uy;
-        u(D + 1) = D == 2 ? p / (gamma - 1) + 0.5 * rho * (ux * ux + uy * uy) : p / (gamma - 1) + 0.5 * rho * ux * ux;
-        if (mf) u(D + 2) = 1.0 / (gamma - 1);
+        if (ph == 3) {
+          const double gamma = o(x), pinf = o(x + 1), p = o(x + 2);
+          u(x) = p / (gamma - 1) + gamma * pinf / (gamma - 1) + ke;
+          u(x + 1) = 1.0 / (gamma - 1);
+          u(x + 2) = gamma * pinf / (gamma - 1);
+        } else {
+          const double gamma = ph == 1 ? o(x) : GAMMA_SF;
+          const double p = o(ph == 1 ? x + 1 : x);
+          u(x) = p / (gamma - 1) + ke;
+          if (ph == 1) u(x + 1) = 1.0 / (gamma - 1);
+          if (ph == 2) {
+            const double phic = o(x + 1), phinc = o(x + 2);
+            u(x + 1) = rho * phic;
+            u(x + 2) = phinc;
+          }
+        }
       }
     }
 }

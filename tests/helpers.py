@@ -4,11 +4,14 @@ from __future__ import annotations
 import numpy as np
 
 import dg_amd
-from dg_amd import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID, Config, Mesh
+from dg_amd import (FARFIELD, MULTIFLUID, PASSIVE, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID, STIFFENED, Config,
+                    Mesh, nfields)
 from dg_amd import problems
 import oracle
 
 CONFIGS = [(d, ph, fl) for d in (1, 2) for ph in (SINGLEFLUID, MULTIFLUID) for fl in (RUSANOV, ROE)]
+# passive-scalar and stiffened-gas configurations (face + element kernels)
+CONFIGS_PS = [(d, ph, fl) for d in (1, 2) for ph in (PASSIVE, STIFFENED) for fl in (RUSANOV, ROE)]
 
 
 def cfg_id(c):
@@ -17,7 +20,7 @@ def cfg_id(c):
 
 
 def small_mesh(dim, order, physics, bc=None, n=(12, 7)):
-    nf = 2 + dim + physics
+    nf = nfields(dim, physics)
     if dim == 1:
         bc = bc or (FARFIELD, REFLECTIVE)
         return Mesh(1, order, n[0], 1, -1.0, 1.0, bc=bc, nfields=nf)

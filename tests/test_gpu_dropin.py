@@ -13,7 +13,7 @@ import torch
 
 import dg_amd
 import oracle
-from dg_amd import MULTIFLUID, ROE, RUSANOV, SINGLEFLUID, Config, problems
+from dg_amd import MULTIFLUID, PASSIVE, ROE, RUSANOV, SINGLEFLUID, STIFFENED, Config, problems
 from tests.helpers import cfg_id, small_mesh
 
 pytestmark = pytest.mark.gpu
@@ -52,7 +52,8 @@ def H(t):
     return t.cpu().numpy()
 
 
-CASES = [(1, MULTIFLUID, RUSANOV), (1, SINGLEFLUID, ROE), (2, SINGLEFLUID, ROE), (2, MULTIFLUID, ROE)]
+CASES = [(1, MULTIFLUID, RUSANOV), (1, SINGLEFLUID, ROE), (2, SINGLEFLUID, ROE), (2, MULTIFLUID, ROE),
+         (1, PASSIVE, ROE), (2, PASSIVE, RUSANOV), (1, STIFFENED, RUSANOV), (2, STIFFENED, ROE)]
 
 
 @pytest.mark.parametrize("c", CASES, ids=cfg_id)

@@ -13,7 +13,7 @@ import torch
 import oracle
 from dg_amd import (FARFIELD, LIMIT_HR, LIMIT_HRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID,
                     Config, Solver, problems)
-from tests.helpers import CONFIGS, cfg_id, per_field_rel, rel_err, small_mesh
+from tests.helpers import CONFIGS, CONFIGS_PS, cfg_id, per_field_rel, rel_err, small_mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ BCS_2D = [(PERIODIC, PERIODIC, REFLECTIVE, FARFIELD), (FARFIELD, REFLECTIVE, PER
           (REFLECTIVE, REFLECTIVE, REFLECTIVE, REFLECTIVE)]
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + CONFIGS_PS, ids=cfg_id)
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_rhs_matches_oracle(device, c, order):
     d, ph, fl = c
@@ -50,7 +50,7 @@ def test_rhs_matches_oracle(device, c, order):
             assert np.abs(f - f_ref).max() <= 1e-12 * scale, (bc, np.abs(f - f_ref).max() / scale)
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + CONFIGS_PS, ids=cfg_id)
 def test_rk4_fixed_dt_trajectory(device, c):
     d, ph, fl = c
     order = 2
@@ -155,7 +155,7 @@ def test_sensors_and_hri_bitwise(device, dim, order):
 
 
 def test_dt_from_cfl(device):
-    for c in CONFIGS:
+    for c in CONFIGS + CONFIGS_PS:
         d, ph, fl = c
         mesh = small_mesh(d, 2, ph)
         prob = oracle.Problem(mesh.tables(), ph, fl)
@@ -201,3 +201,35 @@ def test_vortex_full_size_invariants(device):
     rho_ex, _, _, _ = problems.vortex_fields(x, y, t=20 * dt)
     rho = U1.reshape(s["N_E"], 4, s["N_s"])[:, 0]
     assert np.abs(rho - rho_ex).max() < 1e-5
+
+
+@pytest.mark.parametrize("c", CONFIGS_PS, ids=cfg_id)
+def test_passive_stiffened_limited_trajectories(device, c):
+    """Passive-scalar and stiffened-gas Riemann problems with the global HR
+    and the sensor-gated HR limiters: 10 CFL-sized steps within 1e-11 of the
+    oracle; the sensor flags and one HRI application bit for bit."""
+    d, ph, fl = c
+    order = 2
+    mesh = small_mesh(d, order, ph, n=(24, 10))
+    prob = oracle.Problem(mesh.tables(), ph, fl)
+    solver = Solver(Config(d, order, ph, fl), mesh)
+    U0 = problems.riemann_state(mesh, ph, seed=11) if d == 1 else problems.oblique_riemann_state(mesh, ph, seed=11)
+    th = (0.01, 0.01, 0.01)
+    solver.set_sensor_thresholds(th)
+    Us = _stepped_riemann(mesh, prob, ph, seed=13) if d == 1 else U0 * (1.0 + 0.01 * np.sin(np.arange(U0.size)))
+    sens_ref, _ = prob.sensing(Us, th)
+    assert sens_ref.any()
+    sd = torch.zeros(prob.cfg.N_E, dtype=torch.int32, device=device)
+    ua = torch.zeros(prob.cfg.N_E * prob.nf, dtype=torch.float64, device=device)
+    Ud = dev(Us, device)
+    solver.sensing(Ud, sd, ua)
+    assert np.array_equal(host(sd), sens_ref)
+    solver.limit(LIMIT_HRI, Ud)
+    assert np.array_equal(host(Ud), prob.limit_hri(Us, th)[0])
+    dt = 0.3 * prob.dt_from_cfl(U0, mesh.dx / (2 * order + 1))
+    for lim in (LIMIT_HR, LIMIT_HRI):
+        U_ref, _, _, _ = prob.rk4(U0, 0, dt, nsteps=10, limiter=lim, thresholds=th)
+        Ud = dev(U0, device)
+        solver.rk4_integrate(Ud, 0, dt, nsteps=10, limiter=lim)
+        for l2, li in per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s):
+            assert l2 < 1e-11 and li < 1e-11, (lim, l2, li)

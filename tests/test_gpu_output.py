@@ -8,15 +8,15 @@ import pytest
 import torch
 
 import oracle
-from dg_amd import (FARFIELD, LIMIT_HR, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, Config, Mesh, Solver,
-                    problems)
+from dg_amd import (FARFIELD, LIMIT_HR, MULTIFLUID, PASSIVE, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, STIFFENED, Config,
+                    Mesh, Solver, nfields, problems)
 from tests.helpers import per_field_rel
 
 pytestmark = pytest.mark.gpu
 
 
 def _setup(dim, ph):
-    nf = 2 + dim + ph
+    nf = nfields(dim, ph)
     if dim == 1:
         m = Mesh(1, 2, 64, 1, -1.0, 1.0, bc=(FARFIELD, REFLECTIVE), nfields=nf)
     else:
@@ -24,7 +24,8 @@ def _setup(dim, ph):
     return m, oracle.Problem(m.tables(), ph, ROE), Solver(Config(dim, 2, ph, ROE), m)
 
 
-@pytest.mark.parametrize("dim,ph", [(1, SINGLEFLUID), (1, MULTIFLUID), (2, SINGLEFLUID), (2, MULTIFLUID)])
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("ph", [SINGLEFLUID, MULTIFLUID, PASSIVE, STIFFENED])
 def test_formatter_and_checkpoint(device, tmp_path, dim, ph):
     m, prob, solver = _setup(dim, ph)
     U0 = problems.riemann_state(m, ph, seed=8)

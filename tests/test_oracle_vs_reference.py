@@ -15,8 +15,8 @@ import numpy as np
 import pytest
 
 import oracle
-from dg_amd import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, problems
-from tests.helpers import CONFIGS, cfg_id, small_mesh
+from dg_amd import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, STIFFENED, problems
+from tests.helpers import CONFIGS, CONFIGS_PS, cfg_id, small_mesh
 
 pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="reference sources not present: _ref not built")
 
@@ -63,7 +63,7 @@ def _ref_rhs(R, prob, t, U):
     return frk
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + CONFIGS_PS, ids=cfg_id)
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_rhs_bitwise(c, order):
     R = oracle.ref_lib(*c)
@@ -75,7 +75,7 @@ def test_rhs_bitwise(c, order):
         assert np.abs(f_or).max() > 0
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + CONFIGS_PS, ids=cfg_id)
 def test_solve_and_upa_bitwise(c):
     R = oracle.ref_lib(*c)
     mesh, t, prob, U = _setup(c, 2, 3)
@@ -111,7 +111,8 @@ def test_hrl1D_bitwise(dim, order):
         assert np.abs(a1).max() > 0
 
 
-@pytest.mark.parametrize("c", [x for x in CONFIGS if x[1] == MULTIFLUID], ids=cfg_id)
+@pytest.mark.parametrize("c", [x for x in CONFIGS if x[1] == MULTIFLUID] + [x for x in CONFIGS_PS if x[2] == 1],
+                         ids=cfg_id)
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_sensors_and_hri_bitwise(c, order):
     R = oracle.ref_lib(*c)
@@ -148,7 +149,7 @@ def test_sensors_and_hri_bitwise(c, order):
             assert np.array_equal(U_or, U_ref)
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + [x for x in CONFIGS_PS if x[1] == STIFFENED], ids=cfg_id)
 def test_printer_formatter_bitwise(c):
     """PRINTER formatter (Lformater, printer_*.cu) both ways: conserved ->
     printed fields, and the restart direction."""

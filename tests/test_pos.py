@@ -11,12 +11,14 @@ import numpy as np
 import pytest
 
 import oracle
-from dg_amd import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, DgxError, Mesh, problems
+from dg_amd import (FARFIELD, MULTIFLUID, PASSIVE, PERIODIC, REFLECTIVE, ROE, SINGLEFLUID, STIFFENED, DgxError, Mesh,
+                    nfields, problems)
 
 
-@pytest.mark.parametrize("dim,ph", [(1, SINGLEFLUID), (1, MULTIFLUID), (2, SINGLEFLUID), (2, MULTIFLUID)])
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("ph", [SINGLEFLUID, MULTIFLUID, PASSIVE, STIFFENED])
 def test_pos_files_round_trip(tmp_path, dim, ph):
-    nf = 2 + dim + ph
+    nf = nfields(dim, ph)
     if dim == 1:
         m = Mesh(1, 2, 12, 1, -1.0, 1.0, bc=(FARFIELD, REFLECTIVE), nfields=nf)
     else:
@@ -25,7 +27,8 @@ def test_pos_files_round_trip(tmp_path, dim, ph):
     out = oracle.Problem(t, ph, ROE).format(problems.riemann_state(m, ph, seed=3))
     prefix = str(tmp_path) + "/"
     m.write_pos(ph, out, prefix, 7, 0.1)
-    names = ["rho", "ux"] + (["uy"] if dim == 2 else []) + (["g"] if ph else []) + ["p"]
+    names = (["rho", "ux"] + (["uy"] if dim == 2 else []) + (["g"] if ph in (MULTIFLUID, STIFFENED) else []) +
+             (["pinf"] if ph == STIFFENED else []) + ["p"] + (["phic", "phinc"] if ph == PASSIVE else []))
     for n in names:
         assert (tmp_path / f"{n}0000000007.pos").exists()
     lines = (tmp_path / "rho0000000007.pos").read_text().split("\n")
