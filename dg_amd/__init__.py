@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
         L.dgx_solver_set_path.argtypes = [vp, i]
         L.dgx_solver_set_precision.argtypes = [vp, i]
         L.dgx_solver_path.argtypes = [vp]
+        L.dgx_solver_hr_one_launch.argtypes = [vp]
         L.dgx_profile_enable.argtypes = [vp, i]
         L.dgx_profile_read.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_long)]
         _lib = L
@@ -309,6 +310,11 @@ class Solver:
     def path(self) -> int:
         """Kernel path in use: PATH_GENERAL or PATH_TILED."""
         return lib().dgx_solver_path(self._h)
+
+    @property
+    def hr_one_launch(self) -> bool:
+        """True when the 2D global HR limiter runs as one launch."""
+        return bool(lib().dgx_solver_hr_one_launch(self._h))
 
     def set_indicator(self, mlp, threshold: float) -> None:
         """Attach a learned indicator (Indicator) for limiter 8; None detaches."""

@@ -365,15 +365,25 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
          upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
          upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
   if (s.cart.on && D == 2 && s.P <= 2) {
-    // the one-launch HR limiter takes the rows below and above a tile from
-    // the table and assumes e -/+ nx for the rows inside it (p = 3 would
-    // spill registers: two sweeps)
+    // the one-launch HR limiter assumes e -/+ 1 and e -/+ nx inside a tile,
+    // takes the rows below and above it and the ends of every row from the
+    // table, and takes the x neighbours of a halo-row element to be the halo
+    // elements of the x neighbours (p = 3 would not fit its LDS: two sweeps)
     const int nx = s.cart.nx;
+    auto nbr = [&](int e, int k) { return t.neighbors[(size_t)e * NN + k]; };
     s.hr2d = true;
     for (int e = 0; e < NE && s.hr2d; ++e) {
-      const int ey = e / nx;
-      if (ey > 0 && t.neighbors[(size_t)e * NN + 2] != e - nx) s.hr2d = false;
-      if (ey < s.cart.ny - 1 && t.neighbors[(size_t)e * NN + 3] != e + nx) s.hr2d = false;
+      const int ex = e % nx, ey = e / nx;
+      if (ex > 0 && nbr(e, 0) != e - 1) s.hr2d = false;
+      if (ex < nx - 1 && nbr(e, 1) != e + 1) s.hr2d = false;
+      if (ey > 0 && nbr(e, 2) != e - nx) s.hr2d = false;
+      if (ey < s.cart.ny - 1 && nbr(e, 3) != e + nx) s.hr2d = false;
+      for (int d = 2; d < 4 && s.hr2d; ++d) {
+        const int h = nbr(e, d);
+        if (h < 0) continue;
+        if (ex > 0 && nbr(h, 0) != nbr(e - 1, d)) s.hr2d = false;
+        if (ex < nx - 1 && nbr(h, 1) != nbr(e + 1, d)) s.hr2d = false;
+      }
     }
   }
   const size_t S = (size_t)s.state_size();
@@ -505,12 +515,12 @@ static dgx_status timed_limit(Solver& s, int limiter, double* U, hipStream_t st)
 }
 
 #ifndef DGX_RK_HR2D
-// 0: the RK driver limits with the two sweep launches in place.  The
-// one-launch kernel halves the limiter's HBM passes but measured slower on
-// the 1024x512 p=2 two-fluid grid (337 us against 218 us per application):
-// the limiter is bound by its FP64 arithmetic, which the halo rows raise by
-// half.  It serves dgx_limit_into.
-#define DGX_RK_HR2D 0
+// 1: on grids where the one-launch HR kernel applies (Solver::hr2d) the RK
+// driver limits out of place with it; 0: the two sweep launches in place.
+// On the 1024x512 p=2 two-fluid grid: 198 us against 218 us per application
+// (the kernel computes each element's Taylor-x coefficients once per tile
+// and keeps the intermediate state in LDS).
+#define DGX_RK_HR2D 1
 #endif
 
 static dgx_status timed_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st) {
@@ -709,6 +719,7 @@ dgx_status dgx_solver_set_precision(dgx_solver* h, int precision) {
 }
 
 int dgx_solver_path(const dgx_solver* h) { return (h && h->s.tiled()) ? PATH_TILED : PATH_GENERAL; }
+int dgx_solver_hr_one_launch(const dgx_solver* h) { return (h && h->s.hr2d) ? 1 : 0; }
 
 dgx_status dgx_profile_enable(dgx_solver* h, int enable) {
   if (!h) return fail(DGX_EINVAL, "dgx_profile_enable: null solver");

@@ -127,7 +127,7 @@ hipError_t launch_limit_m2(Solver& s, double* U, hipStream_t st);  // m2l, two-f
 hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st);  // prl, two-fluid
 hipError_t launch_limit_pri(Solver& s, double* U, hipStream_t st); // pri, two-fluid
 hipError_t launch_limit_m2i(Solver& s, double* U, hipStream_t st); // m2i, two-fluid
-hipError_t launch_limit_p0i(Solver& s, double* U, hipStream_t st); // p0i (needs the two-fluid sensor)
+hipError_t launch_limit_p0i(Solver& s, double* U, hipStream_t st); // p0i (sensor: not single fluid)
 // HR sweeps on the cells flagged in s.sensors (the HRI sweeps without the sensor)
 hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st);
 // learned indicator (mlp.hip): features of U into X, and flags into s.sensors

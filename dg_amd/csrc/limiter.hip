@@ -169,53 +169,85 @@ __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, cons
 }
 
 // Both sweeps of the 2D global HR limiter in one pass, out of place, on a
-// Cartesian grid (element e = ey*nx + ex): a block owns a tile of TXE x TYE
-// elements, runs the x sweep (Lag2MonoX, hrl along x, MonoX2MonoY) for the
-// tile rows and for the rows just below and above it (found through the
-// neighbour table, so periodic and wall rows need no special case) into
-// LDS, then the y sweep (hrl along y, MonoY2Lag) of the tile rows from LDS.
+// Cartesian grid (element e = ey*nx + ex) for a tile of TXE x TYE elements:
+//   1. the Taylor-x coefficients (Lag2MonoX) of every element the tile needs,
+//      once each, into LDS: the tile, the rows just below and above it, and
+//      the element left and right of each of these rows (all found through
+//      the neighbour table, so periodic ends and walls need no special case);
+//   2. the x sweep (hrl along x, MonoX2MonoY) of the tile rows and of the two
+//      halo rows, from LDS into LDS;
+//   3. the y sweep (hrl along y, MonoY2Lag) of the tile rows, to Out.
 // Every column sees exactly the arithmetic of the two k_hr_sweep launches,
-// so the result is bitwise the same; the intermediate state never reaches
-// HBM.  In and Out must differ: neighbouring tiles read In's rows.
+// so the result is bitwise the same, and the intermediate state never
+// reaches HBM.  In and Out must differ: neighbouring tiles read In's rows.
 template <int Ns, int N, int SL, int TXE, int TYE>
 __global__ __launch_bounds__(256) void k_hr2d(int nx, int NF, int N_N, const int* __restrict__ nb,
                                               const double* __restrict__ L2MX, const double* __restrict__ MX2MY,
                                               const double* __restrict__ MY2L, const double* __restrict__ In,
                                               double* __restrict__ Out) {
-  constexpr int ROWS = TYE + 2, NFMAX = 5;
-  __shared__ double sB[ROWS * TXE * NFMAX * Ns];
+  constexpr int ROWS = TYE + 2, W = TXE + 2, NFMAX = 5;
+  // one x-sweep column per thread, so that its result can wait in registers
+  // while the Taylor-x coefficients are still read, then take their place
+  static_assert(ROWS * TXE * NFMAX <= 256, "one x-sweep column per thread");
+  __shared__ double sA[ROWS * W * NFMAX * Ns];  // Taylor-x coefficients [row][col][f], then Taylor-y [row][i][f]
+  __shared__ int sE[ROWS * W];                  // element of each slot, -1 if absent
+  double* sB = sA;
   const int tiles_x = nx / TXE;
   const int ex0 = (blockIdx.x % tiles_x) * TXE, ey0 = (blockIdx.x / tiles_x) * TYE;
-  // x sweep: rows 0 (below the tile) .. ROWS-1 (above it)
-  for (int t = threadIdx.x; t < ROWS * TXE * NF; t += blockDim.x) {
-    const int f = t % NF, i = (t / NF) % TXE, r = t / (NF * TXE);
+  // slot elements: row 0 below the tile, rows 1..TYE the tile, row TYE+1 above
+  for (int t = threadIdx.x; t < ROWS * TXE; t += blockDim.x) {
+    const int i = t % TXE, r = t / TXE;
     int e;
     if (r == 0) e = nb[(ey0 * nx + ex0 + i) * N_N + 2];
     else if (r == ROWS - 1) e = nb[((ey0 + TYE - 1) * nx + ex0 + i) * N_N + 3];
     else e = (ey0 + r - 1) * nx + ex0 + i;
-    if (e < 0) continue;  // a wall below or above: the y sweep does not read this row
-    const long col = (long)e * NF + f;
-    const int left = nb[e * N_N], right = nb[e * N_N + 1];
-    const int physical = hr_physical(left, right);
-    double x[Ns], AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
+    sE[r * W + i + 1] = e;
+    if (i == 0) sE[r * W] = e >= 0 ? nb[e * N_N + 0] : -1;
+    if (i == TXE - 1) sE[r * W + W - 1] = e >= 0 ? nb[e * N_N + 1] : -1;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ROWS * W * NF; t += blockDim.x) {
+    const int f = t % NF, slot = t / NF;
+    const int e = sE[slot];
+    if (e < 0) continue;
+    double x[Ns], A[Ns];
+    const double* src = In + ((long)e * NF + f) * Ns;
 #pragma unroll
-    for (int l = 0; l < Ns; ++l) x[l] = In[col * Ns + l];
-    col_apply<Ns>(L2MX, x, AC);
+    for (int l = 0; l < Ns; ++l) x[l] = src[l];
+    col_apply<Ns>(L2MX, x, A);
+    double* a = sA + (slot * NF + f) * Ns;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) a[l] = A[l];
+  }
+  __syncthreads();
+  // x sweep of every row present (a wall below or above leaves its halo row out:
+  // the y sweep does not read it)
+  const int t2 = threadIdx.x;
+  const int f2 = t2 % NF, i2 = (t2 / NF) % TXE, r2 = t2 / (NF * TXE);
+  const bool on2 = t2 < ROWS * TXE * NF && sE[r2 * W + i2 + 1] >= 0;
+  double xs[Ns];
+  if (on2) {
+    const int slot = r2 * W + i2 + 1;
+    const int e = sE[slot];
+    const int physical = hr_physical(nb[e * N_N + 0], nb[e * N_N + 1]);
+    double AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
+    const double* ac = sA + (slot * NF + f2) * Ns;
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) AC[l] = ac[l];
     if (physical == 0) {
-      const double* pl = In + ((long)left * NF + f) * Ns;
-      const double* pr = In + ((long)right * NF + f) * Ns;
+      const double* al = sA + ((slot - 1) * NF + f2) * Ns;
+      const double* ar = sA + ((slot + 1) * NF + f2) * Ns;
 #pragma unroll
-      for (int l = 0; l < Ns; ++l) x[l] = pl[l];
-      col_apply<Ns>(L2MX, x, AL);
-#pragma unroll
-      for (int l = 0; l < Ns; ++l) x[l] = pr[l];
-      col_apply<Ns>(L2MX, x, AR);
+      for (int l = 0; l < Ns; ++l) { AL[l] = al[l]; AR[l] = ar[l]; }
     }
-    hrl_column<N, SL>(physical, f, 0, AL, AC, AR, Alim);
-    double* b = sB + ((r * TXE + i) * NF + f) * Ns;
-    col_apply<Ns>(MX2MY, Alim, x);
+    hrl_column<N, SL>(physical, f2, 0, AL, AC, AR, Alim);
+    col_apply<Ns>(MX2MY, Alim, xs);
+  }
+  __syncthreads();
+  if (on2) {
+    double* b = sB + ((r2 * TXE + i2) * NF + f2) * Ns;
 #pragma unroll
-    for (int l = 0; l < Ns; ++l) b[l] = x[l];
+    for (int l = 0; l < Ns; ++l) b[l] = xs[l];
   }
   __syncthreads();
   // y sweep of the tile rows

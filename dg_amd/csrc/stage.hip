@@ -255,8 +255,12 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 
   const int tid = threadIdx.x;
   const bool shared_minv = a.minv_stride == 0;
-  load_vol_tables(tb, a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid, 256, !MFW);
-  for (int k = tid; k < MG * Ms; k += 256) { sPsi[k] = a.psi[k]; sPsiw[k] = a.psi_w[k]; }
+  // reference tables and the first tile's inputs are fetched together (one wait)
+  static_assert(MG * Ms <= 256, "face tables fit one pass");
+  VolTableFetch<D, P, PH, !MFW> tf;
+  tf.fetch(a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && shared_minv, tid);
+  double psi_r = 0.0, psiw_r = 0.0;
+  if (tid < MG * Ms) { psi_r = a.psi[tid]; psiw_r = a.psi_w[tid]; }
   if constexpr (MFW) node_q_table<NF>(sNodeQ, tid);
   const double dt = (MODE == 1) ? a.dt[0] : 0.0;
 
@@ -333,6 +337,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 
   int tile = tile_of(a, 0);
   if (tile >= 0) issue(tile);
+  tf.store(tb, MODE == 1 && shared_minv, tid);
+  if (tid < MG * Ms) { sPsi[tid] = psi_r; sPsiw[tid] = psiw_r; }
   for (int it = 0; tile >= 0; ++it) {
     const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
     auto gidx = [&](int k) -> size_t {

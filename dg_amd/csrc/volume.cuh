@@ -57,6 +57,66 @@ __device__ inline void load_vol_tables(const VolTables<D, P, PH>& t, const doubl
     for (int k = tid; k < V::Ns * V::Ns; k += nthr) t.minv[k] = Minv[k];
 }
 
+// The same tables for 256-thread blocks in two steps: fetch() issues every
+// global load into registers and store() writes them to LDS, so that a
+// caller can issue its own loads in between and wait once for all of them.
+template <int D, int P, int PH, bool DW>
+struct VolTableFetch {
+  using V = Vol<D, P, PH>;
+  static constexpr int RP = (V::NTAB_PHI + 255) / 256, RD = (V::NTAB_D + 255) / 256;
+  static constexpr int RM = (V::Ns * V::Ns + 255) / 256;
+  static constexpr bool NC = V::NNC > 0;
+  double phi[RP], pw[NC ? RP : 1], dw[DW ? RD : 1], dphi[NC ? RD : 1], minv[RM];
+  __device__ inline void fetch(const double* phi_g, const double* dphi_g, const double* phi_w, const double* dphi_w,
+                               const double* Minv, bool with_minv, int tid) {
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+      const int k = tid + r * 256;
+      if (k < V::NTAB_PHI) {
+        phi[r] = phi_g[k];
+        if constexpr (NC) pw[r] = phi_w[k];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int k = tid + r * 256;
+      if (k < V::NTAB_D) {
+        if constexpr (DW) dw[r] = dphi_w[k];
+        if constexpr (NC) dphi[r] = dphi_g[k];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      const int k = tid + r * 256;
+      if (with_minv && k < V::Ns * V::Ns) minv[r] = Minv[k];
+    }
+  }
+  __device__ inline void store(const VolTables<D, P, PH>& t, bool with_minv, int tid) const {
+#pragma unroll
+    for (int r = 0; r < RP; ++r) {
+      const int k = tid + r * 256;
+      if (k < V::NTAB_PHI) {
+        t.phi[k] = phi[r];
+        if constexpr (NC) t.pw[(k % V::NG) * V::Ns + k / V::NG] = pw[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int k = tid + r * 256;
+      if (k < V::NTAB_D) {
+        const int i = k / (V::NG * D), q = k % (V::NG * D);
+        if constexpr (DW) t.dw[q * V::Ns + i] = dw[r];
+        if constexpr (NC) t.dphi[k] = dphi[r];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      const int k = tid + r * 256;
+      if (with_minv && k < V::Ns * V::Ns) t.minv[k] = minv[r];
+    }
+  }
+};
+
 // Phase 1 for lane (element u, Gauss point g): fluxes with metric terms.
 template <int D, int P, int PH>
 __device__ inline void vol_phase1(const VolTables<D, P, PH>& t, int g, const double* u, double Je, const double* ij,

@@ -196,6 +196,10 @@ dgx_status dgx_solver_set_path(dgx_solver* s, int path);
  * it to 0). */
 dgx_status dgx_solver_set_precision(dgx_solver* s, int precision);
 int dgx_solver_path(const dgx_solver* s);
+/* 1 when the 2D global HR limiter runs as one launch (dgx_limit_into, and
+ * the RK driver's limiter): 2D p <= 2 Cartesian grids of single-fluid or
+ * two-fluid states whose neighbour table follows the grid; else 0. */
+int dgx_solver_hr_one_launch(const dgx_solver* s);
 
 /* ------------------------------------------------- batched 1D ensemble */
 /* Many independent 1D problems advanced together: each runs the loop of

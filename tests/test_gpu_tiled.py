@@ -106,6 +106,7 @@ def test_one_launch_hr_limiter_bitwise(device, ph, order):
         mesh = tiled_mesh(2, order, ph, bc, SIZES_2D[k % 2])
         prob = oracle.Problem(mesh.tables(), ph, ROE)
         solver = Solver(Config(2, order, ph, ROE), mesh)
+        assert solver.hr_one_launch == (order <= 2)
         U0 = problems.oblique_riemann_state(mesh, ph, seed=30 + k)
         x = np.arange(U0.size, dtype=np.float64)
         U0 = U0 * (1.0 + 0.01 * np.sin(0.37 * x + k))  # every mode non-trivial
