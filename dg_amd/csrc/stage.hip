@@ -451,7 +451,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 
     // ---- element phases
     if constexpr (MFW) {
-      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO>(tb, a.dphi_w, sU, sGeo, sQ, sW, sNodeQ, tid);
+      // with the shared Minv the wave also applies Dt * Minv on the matrix cores
+      const bool wave_minv = MODE == 1 && shared_minv;
+      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO>(tb, a.dphi_w, sU, sGeo, sQ, sW, sNodeQ, tid,
+                                              wave_minv ? tb.minv : nullptr, sDte);
       __syncthreads();
     } else {
     for (int b = 0; b < NT; b += EPB) {
@@ -485,6 +488,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       const size_t gi = gidx(k);
       if constexpr (MODE == 0) {
         a.fout[gi] = sU[el * SU + rr];
+      } else if (MFW && shared_minv) {
+        rk_combine(a.stage, sU[el * SU + rr], pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
       } else {
         // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
         const double dte = sDte[el];

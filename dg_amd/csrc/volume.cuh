@@ -293,11 +293,16 @@ constexpr int vol_wave_lds() { return (PH ? 4 * 16 : 0) + (16 * 17 * (int)sizeof
 // The integration operand (dphi_w, [i*NG*D + g*D + al]) is read from global
 // memory into registers once.  Multifluid adds alpha and its reference
 // gradient at the point (FP64 sums over the 9 nodes) and the alpha source
-// integrated against phi_w (FP64).  No block barrier: every exchange stays
-// inside the wave.  sWork needs 4 * vol_wave_lds<T, PH>() doubles.
+// integrated against phi_w (FP64).  With minv (the shared mass-matrix
+// inverse in LDS, [ii*Ns + i]) and dte (Dt per element), the right-hand side
+// is then replaced by du = dte * Minv * R, again on the FP64 matrix cores
+// (A = R rows read back from sU, B = Minv; alpha by FP64 sums).  No block
+// barrier: every exchange stays inside the wave.  sWork needs
+// 4 * vol_wave_lds<T, PH>() doubles.
 template <typename T, int PH, int NT, int SU, int SQ, int NGEO, typename TM = T>
 __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __restrict__ dphi_w, double* sU,
-                              const double* sGeo, const double* sQ, double* sWork, const int* sNodeQ, int tid) {
+                              const double* sGeo, const double* sQ, double* sWork, const int* sNodeQ, int tid,
+                              const double* minv = nullptr, const double* dte = nullptr) {
   // TM: type of the map (T unless set), T: type of the integration.  On gfx950
   // the FP32 16x16x4 integration measured slower than the FP64 one (volume
   // phase of a 2D p=2 tile: 11.0k against 7.3k cycles), the FP32 map about
@@ -323,6 +328,13 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
     bdw[s] = c < Ns ? (T)dphi_w[c * NG * D + g * D + al] : T(0);
   }
   const int a_el = MM::el(c), a_f = MM::field(c);
+  // Minv operand: column c (output node), rows k = 4s + q (input node)
+  double bminv[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int k = 4 * s + q;
+    bminv[s] = (minv && k < Ns && c < Ns) ? minv[k * Ns + c] : 0.0;
+  }
 #pragma unroll 1
   for (int b = 0; b < NT; b += 16) {
     const int e_base = b + 4 * w, el = e_base + q;
@@ -405,6 +417,36 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
 #pragma unroll
       for (int v = 0; v < 4; ++v) sU[el * SU + v * Ns + c] = (double)r[v] + face(v);
       if constexpr (PH == 1) sU[el * SU + 4 * Ns + c] = sa + face(4);
+    }
+    if (minv) {
+      // du = dte * Minv R: rows (el, f) = el + 4f of the four elements, columns = nodes
+      using M64 = Mfma16<double>;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();  // the right-hand side of the four elements is in sU
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      M64::V4 du = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int k = 4 * s + q;
+        const double a = k < Ns ? sU[(e_base + M64::el(c)) * SU + M64::field(c) * Ns + k] : 0.0;
+        du = M64::mma(a, bminv[s], du);
+      }
+      double dua5 = 0.0;
+      if constexpr (PH == 1) {
+        if (c < Ns) {
+#pragma unroll
+          for (int k = 0; k < Ns; ++k) dua5 += minv[k * Ns + c] * sU[el * SU + 4 * Ns + k];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();  // every lane has read the right-hand side
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (c < Ns) {
+        const double d = dte[el];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sU[el * SU + v * Ns + c] = d * du[v];
+        if constexpr (PH == 1) sU[el * SU + 4 * Ns + c] = d * dua5;
+      }
     }
   }
 }
