@@ -363,7 +363,6 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       }
     }
     const int next = DGX_STAGE_PERSIST ? tile_of(a, it + 1) : -1;
-    if (next >= 0) issue(next);
 
     // ---- faces: one lane per (face, Gauss point)
     for (int itf = tid; itf < NFACE * MG; itf += 256) {
@@ -422,6 +421,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       if constexpr (PH == 1) w[NF] = nc[NF - 1];
     }
     __syncthreads();
+    // the next tile's inputs (persistent blocks): in flight from here to the
+    // next commit, past the face phase, whose registers peak
+    if (next >= 0) issue(next);
     // ---- face integration against psi_w into the receiving elements' slots
     for (int itf = tid; itf < NFACE * 2 * Ms; itf += 256) {
       const int fi = itf / (2 * Ms), d = (itf / Ms) % 2, j = itf % Ms;
