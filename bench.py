@@ -33,6 +33,7 @@ import sys
 import time
 
 HBM_PEAK_GBPS = 8000.0  # MI355X published HBM3E bandwidth
+PRECISION_MODE = {"fp64": 0, "fp32mfma": 1, "splitbf16": 2}  # dgx_solver_set_precision
 
 
 def main() -> int:
@@ -43,8 +44,9 @@ def main() -> int:
     ap.add_argument("--config", choices=["vortex", "sod", "bubble", "ensemble"], default="vortex")
     ap.add_argument("--n", type=int, default=0, help="vortex: elements per direction (default 512)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "fp32mfma"], default="fp64",
-                    help="tile-kernel volume contractions: FP64, or FP32 on MFMA (2D p=2; the rest stays FP64)")
+    ap.add_argument("--precision", choices=["fp64", "fp32mfma", "splitbf16"], default="fp64",
+                    help="tile-kernel volume contractions: FP64, FP32 on MFMA, or split-BF16 products with FP32 "
+                         "accumulation (2D p=2; the rest stays FP64)")
     ap.add_argument("--path", choices=["auto", "general", "tiled"], default="auto",
                     help="kernel path: fused tile kernel (auto on this mesh) or face + element kernels")
     ap.add_argument("--limiter", choices=["auto", "none", "hr", "hri", "learned"], default="auto",
@@ -90,8 +92,8 @@ def main() -> int:
     solver = Solver(cfg, mesh)
     solver.set_path({"auto": PATH_AUTO, "general": PATH_GENERAL, "tiled": PATH_TILED}[args.path])
     tiled = solver.path == PATH_TILED
-    if args.precision == "fp32mfma":
-        solver.set_precision(1)
+    if args.precision != "fp64":
+        solver.set_precision(PRECISION_MODE[args.precision])
     s = mesh.sizes
     N_E, N_s, NF = s["N_E"], s["N_s"], cfg.nfields
     n_dof = N_E * N_s * NF
@@ -151,7 +153,7 @@ def main() -> int:
         else:
             torch.cuda.synchronize()
             extra["sensor_flagged_final_state"] = int((sens.cpu().numpy() != 0).sum())
-    if args.precision == "fp32mfma" or lim in (3, 8):
+    if args.precision != "fp64" or lim in (3, 8):
         # the plain FP64 GPU form (global HR) on the same workload and steps, timed here too
         sp = Solver(cfg, mesh)
         Up = torch.from_numpy(U0).to(device)
@@ -164,7 +166,7 @@ def main() -> int:
         ms_plain = 1e3 * (time.perf_counter() - tp) / args.steps
         extra["plain_fp64_form"] = {"limiter": {0: "none", 1: "hr"}.get(w["limiter"], w["limiter"]),
                                     "ms_per_step": ms_plain, "speedup_of_this_form": ms_plain / ms_per_step}
-    if args.precision == "fp32mfma":
+    if args.precision != "fp64":
         # precision error: the same limiter in FP64 from the same state, warmup + K steps
         sf = Solver(cfg, mesh)
         if lim in (3, 8):
@@ -237,7 +239,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32 (mfma contractions)",
+        "dtype": {"fp64": "f64", "fp32mfma": "f64+f32 (mfma contractions)",
+                  "splitbf16": "f64+split-bf16 (mfma contractions, f32 accumulation)"}[args.precision],
         "data": "synthetic",
         "config": {
             "workload": w["name"],
@@ -245,6 +248,8 @@ def main() -> int:
             "kernel_path": "tiled (one fused kernel per RK stage)" if tiled else "face + element kernels",
             "limiter": {0: "none", 1: "global HR", 3: "sensor-gated HR", 8: "HR gated by the learned indicator"}[lim],
             "contractions": ("FP32 MFMA (FP64 fluxes, RK and limiter)" if args.precision == "fp32mfma" else
+                             "split-BF16 MFMA, FP32 accumulation (FP64 fluxes, RK and limiter)"
+                             if args.precision == "splitbf16" else
                              "FP64 MFMA element phases" if tiled and w["physics"] == 0 and w["order"] == 2 and w["dim"] == 2
                              else "FP64"),
         },
@@ -400,8 +405,8 @@ def cpu_baseline(args, device, w):
         U_ref, steps, _, _ = prob.rk4(U0, 0, dt, nsteps=nst, limiter=w["limiter"])
     el = time.perf_counter() - t0
     solver = Solver(Config(w["dim"], order, w["physics"], w["flux"]), mesh)
-    if args.precision == "fp32mfma":
-        solver.set_precision(1)
+    if args.precision != "fp64":
+        solver.set_precision(PRECISION_MODE[args.precision])
     Ud = torch.from_numpy(U0).to(device)
     if w["cfl_steps"]:
         solver.rk4_integrate(Ud, 1, cfl, nsteps=nst, tf=1e30, limiter=w["limiter"])

@@ -303,7 +303,8 @@ class Solver:
         _check(lib().dgx_solver_set_path(self._h, path), "dgx_solver_set_path")
 
     def set_precision(self, precision: int) -> None:
-        """0: FP64; 1: FP32-MFMA map/integrate contractions in the tile kernel (2D p=2)."""
+        """0: FP64; 1: FP32-MFMA map/integrate contractions in the tile kernel (2D p=2);
+        2: the same contractions as split-BF16 products with FP32 accumulation."""
         _check(lib().dgx_solver_set_precision(self._h, precision), "dgx_solver_set_precision")
 
     @property

@@ -710,10 +710,12 @@ dgx_status dgx_solver_set_path(dgx_solver* h, int path) {
 }
 
 dgx_status dgx_solver_set_precision(dgx_solver* h, int precision) {
-  if (!h || precision < 0 || precision > 1) return fail(DGX_EINVAL, "dgx_solver_set_precision: precision must be 0 or 1");
+  if (!h || precision < 0 || precision > 2)
+    return fail(DGX_EINVAL, "dgx_solver_set_precision: precision must be 0, 1 or 2");
   Solver& s = h->s;
-  if (precision == 1 && !(s.tiled() && s.D == 2 && s.P == 2))
-    return fail(DGX_EUNSUPPORTED, "dgx_solver_set_precision: FP32-MFMA contractions need the tile kernel, 2D, p=2");
+  if (precision > 0 && !(s.tiled() && s.D == 2 && s.P == 2))
+    return fail(DGX_EUNSUPPORTED,
+                "dgx_solver_set_precision: FP32 and split-BF16 contractions need the tile kernel, 2D, p=2");
   s.precision = precision;
   return DGX_OK;
 }

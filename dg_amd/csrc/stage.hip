@@ -224,8 +224,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int NT = T::NT, NFACE = T::NFACE, NH = T::NH, HT = NF * Ms;
   constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
   // element phases on the matrix cores (FP64, or FP32 in the mixed-precision mode) or as VALU batches
-  constexpr bool MFW = D == 2 && P == 2 && NT % 16 == 0 && (PREC == 1 || DGX_STAGE_MFMA64);
-  using TW = typename std::conditional<PREC == 1, float, double>::type;
+  // PREC 0: FP64; 1: FP32 contractions; 2: split-BF16 contractions with FP32 accumulation
+  constexpr bool MFW = D == 2 && P == 2 && NT % 16 == 0 && (PREC >= 1 || DGX_STAGE_MFMA64);
+  using TW = typename std::conditional<PREC >= 1, float, double>::type;
   constexpr int NWP = NF + PH;  // per face point: F, and the alpha non-conservative term
   constexpr int NW_FACE = NFACE * MG * NWP, NW_VOL = MFW ? 0 : EPB * SW;
   constexpr int NW_MF = MFW ? 4 * vol_wave_lds<TW, PH>() : 0;
@@ -455,8 +456,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     if constexpr (MFW) {
       // with the shared Minv the wave also applies Dt * Minv on the matrix cores
       const bool wave_minv = MODE == 1 && shared_minv;
-      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO>(tb, a.dphi_w, sU, sGeo, sQ, sW, sNodeQ, tid,
-                                              wave_minv ? tb.minv : nullptr, sDte);
+      vol_mfma_wave<TW, PH, NT, SU, SQ, NGEO, TW, PREC == 2>(tb, a.dphi_w, sU, sGeo, sQ, sW, sNodeQ, tid,
+                                                             wave_minv ? tb.minv : nullptr, sDte);
       __syncthreads();
     } else {
     for (int b = 0; b < NT; b += EPB) {
@@ -573,6 +574,8 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
     if constexpr (D == 2 && P == 2 && TX * TY == 32) {
       if (s.precision == 1)
         kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
+      if (s.precision == 2)
+        kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2> : k_stage<D, P, PH, FL, 1, TX, TY, 2>;
     }
     // persistent grid: as many blocks as fit on the device at once
     int nb = ntiles;

@@ -6,6 +6,8 @@
 // (rk_kernels.cu:22-56, rk.cc:164-198).  All data is in LDS; one 256-thread
 // block processes batches of EPB elements.
 #pragma once
+#include <type_traits>
+
 #include "physics.cuh"
 
 namespace dgx {
@@ -270,6 +272,24 @@ struct Mfma16<float> {
   }
 };
 
+// Split-BF16 operands for v_mfma_f32_16x16x16_bf16: x ~ hi + lo, both BF16
+// (round to nearest even); a product is taken as hi*hi + hi*lo + lo*hi with
+// FP32 accumulation (the lo*lo term, ~2^-16 relative, is dropped).
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+__device__ inline unsigned bf16_rne(unsigned u) { return (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u; }
+__device__ inline void bf16_split(float x, short& hi, short& lo) {
+  const unsigned h = bf16_rne(__float_as_uint(x));
+  const unsigned l = bf16_rne(__float_as_uint(x - __uint_as_float(h)));
+  hi = (short)(h >> 16);
+  lo = (short)(l >> 16);
+}
+__device__ inline f32x4_t mma_bf16x3(const bf16x4_t& ah, const bf16x4_t& al, const bf16x4_t& bh, const bf16x4_t& bl,
+                                     f32x4_t c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
+}
+
 // Doubles of per-wave LDS that vol_mfma_wave needs: the alpha source of four
 // elements (multifluid, FP64), then 16 flux rows of one direction in T
 // (16 points, padded).
@@ -299,10 +319,16 @@ constexpr int vol_wave_lds() { return (PH ? 4 * 16 : 0) + (16 * 17 * (int)sizeof
 // (A = R rows read back from sU, B = Minv; alpha by FP64 sums).  No block
 // barrier: every exchange stays inside the wave.  sWork needs
 // 4 * vol_wave_lds<T, PH>() doubles.
-template <typename T, int PH, int NT, int SU, int SQ, int NGEO, typename TM = T>
+// SPLIT (T = TM = float): the map and the integration as split-BF16
+// products on v_mfma_f32_16x16x16_bf16 (K = 16 per instruction: A lane l =
+// row l%16, k = 4(l/16)+0..3; B lane l = column l%16, same k; D as the FP32
+// 16x16x4 layout), three instructions per contraction block.
+template <typename T, int PH, int NT, int SU, int SQ, int NGEO, typename TM = T, bool SPLIT = false>
 __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __restrict__ dphi_w, double* sU,
                               const double* sGeo, const double* sQ, double* sWork, const int* sNodeQ, int tid,
                               const double* minv = nullptr, const double* dte = nullptr) {
+  static_assert(!SPLIT || (std::is_same<T, float>::value && std::is_same<TM, float>::value),
+                "split-BF16 accumulates in FP32");
   // TM: type of the map (T unless set), T: type of the integration.  On gfx950
   // the FP32 16x16x4 integration measured slower than the FP64 one (volume
   // phase of a 2D p=2 tile: 11.0k against 7.3k cycles), the FP32 map about
@@ -328,6 +354,24 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
     bdw[s] = c < Ns ? (T)dphi_w[c * NG * D + g * D + al] : T(0);
   }
   const int a_el = MM::el(c), a_f = MM::field(c);
+  // split-BF16 B operands: phi[g = c][i = 4q+j] for the map, dphi_w[i = c][g = 4q+j][al] for the integration
+  bf16x4_t sphi_h = {0, 0, 0, 0}, sphi_l = {0, 0, 0, 0}, sdw_h[2], sdw_l[2];
+  if constexpr (SPLIT) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 4 * q + j;
+      short h, lo;
+      bf16_split(k < Ns ? (float)t.phi[c + k * NG] : 0.0f, h, lo);
+      sphi_h[j] = h;
+      sphi_l[j] = lo;
+#pragma unroll
+      for (int al = 0; al < D; ++al) {
+        bf16_split(c < Ns ? (float)dphi_w[c * NG * D + k * D + al] : 0.0f, h, lo);
+        sdw_h[al][j] = h;
+        sdw_l[al][j] = lo;
+      }
+    }
+  }
   // Minv operand: column c (output node), rows k = 4s + q (input node)
   double bminv[3];
 #pragma unroll
@@ -340,11 +384,24 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
     const int e_base = b + 4 * w, el = e_base + q;
     // ---- map
     typename MM::V4 acc = {TM(0), TM(0), TM(0), TM(0)};
+    if constexpr (SPLIT) {
+      bf16x4_t ah, alo;
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      const int k = 4 * s + q;
-      const TM a = k < Ns ? (TM)sU[(e_base + a_el) * SU + a_f * Ns + k] : TM(0);
-      acc = MM::mma(a, bphi[s], acc);
+      for (int j = 0; j < 4; ++j) {
+        const int k = 4 * q + j;
+        short h, lo;
+        bf16_split(k < Ns ? (float)sU[(e_base + a_el) * SU + a_f * Ns + k] : 0.0f, h, lo);
+        ah[j] = h;
+        alo[j] = lo;
+      }
+      acc = mma_bf16x3(ah, alo, sphi_h, sphi_l, acc);
+    } else {
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int k = 4 * s + q;
+        const TM a = k < Ns ? (TM)sU[(e_base + a_el) * SU + a_f * Ns + k] : TM(0);
+        acc = MM::mma(a, bphi[s], acc);
+      }
     }
     double ug[NF], dua[D] = {0.0, 0.0};
 #pragma unroll
@@ -390,8 +447,20 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if constexpr (SPLIT) {
+        bf16x4_t ah, alo;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) r = M::mma(F[c * KP + 4 * s + q], bdw[4 * al + s], r);
+        for (int j = 0; j < 4; ++j) {
+          short h, lo;
+          bf16_split((float)F[c * KP + 4 * q + j], h, lo);
+          ah[j] = h;
+          alo[j] = lo;
+        }
+        r = mma_bf16x3(ah, alo, sdw_h[al], sdw_l[al], r);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) r = M::mma(F[c * KP + 4 * s + q], bdw[4 * al + s], r);
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();  // the rows are rewritten next
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -190,10 +190,12 @@ dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cf
 dgx_status dgx_solver_set_path(dgx_solver* s, int path);
 /* Precision of the tile kernel's volume contractions: 0 FP64 (default);
  * 1 the map to the Gauss points and the integration against the test
- * functions in FP32 on MFMA with FP32 accumulation, while the fluxes, the
- * Riemann solver, Dt*Minv, the RK4 accumulation and the limiter stay FP64
- * (2D p=2 on the tile path only; choosing the face + element path resets
- * it to 0). */
+ * functions in FP32 on MFMA with FP32 accumulation; 2 the same two
+ * contractions as split-BF16 products (x = hi + lo in BF16, hi*hi + hi*lo +
+ * lo*hi on v_mfma_f32_16x16x16_bf16) with FP32 accumulation.  The fluxes,
+ * the Riemann solver, Dt*Minv, the RK4 accumulation and the limiter stay
+ * FP64 (2D p=2 on the tile path only; choosing the face + element path
+ * resets it to 0). */
 dgx_status dgx_solver_set_precision(dgx_solver* s, int precision);
 int dgx_solver_path(const dgx_solver* s);
 /* 1 when the 2D global HR limiter runs as one launch (dgx_limit_into, and
