@@ -454,8 +454,9 @@ static dgx_status do_limit(Solver& s, int limiter, double* U, hipStream_t st) {
   }
   if (limiter == 8) {
     if (!s.mlp) return fail(DGX_EUNSUPPORTED, "limiter 8 needs an indicator (dgx_solver_set_indicator)");
-    if (!hip_ok(mlp_indicator_flags(s, U, st), "indicator")) return DGX_EDEVICE;
-    return hip_ok(launch_limit_flagged(s, U, st), "flagged HR limiter") ? DGX_OK : DGX_EDEVICE;
+    // the indicator lists the flagged elements itself (no separate compaction pass)
+    if (!hip_ok(mlp_indicator_flags(s, U, st, true), "indicator")) return DGX_EDEVICE;
+    return hip_ok(launch_limit_listed(s, U, st), "flagged HR limiter") ? DGX_OK : DGX_EDEVICE;
   }
   return fail(DGX_EUNSUPPORTED,
               "limiter must be 0 (none), 1 (hrl), 2 (m2l), 3 (hri), 4 (m2i), 5 (prl), 6 (pri), 7 (p0i) or 8 "

@@ -130,9 +130,12 @@ hipError_t launch_limit_m2i(Solver& s, double* U, hipStream_t st); // m2i, two-f
 hipError_t launch_limit_p0i(Solver& s, double* U, hipStream_t st); // p0i (sensor: not single fluid)
 // HR sweeps on the cells flagged in s.sensors (the HRI sweeps without the sensor)
 hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st);
+// the same sweeps over a list already in s.flag_list / s.flag[2]
+hipError_t launch_limit_listed(Solver& s, double* U, hipStream_t st);
 // learned indicator (mlp.hip): features of U into X, and flags into s.sensors
 hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st);
-hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st);
+// with_list: also list the flagged elements in s.flag_list, their count in s.flag[2]
+hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool with_list = false);
 // solution output and restart (output.hip): PRINTER formatter, .pos files
 hipError_t launch_format(const Solver& s, double* U, double* out, int inverse, hipStream_t st);
 std::string pos_write(const Tables& t, int physics, const double* out, const std::string& prefix, int step,

@@ -1064,6 +1064,11 @@ hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st) {
                      s.flag + 2);
   hipError_t e;
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_limit_listed(s, U, st);
+}
+
+hipError_t launch_limit_listed(Solver& s, double* U, hipStream_t st) {
+  hipError_t e;
   if (s.D == 1) return hri_list(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
   if ((e = hri_list(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
   return hri_list(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);

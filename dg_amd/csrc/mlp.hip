@@ -157,8 +157,10 @@ __device__ inline void wave_modes(double* u, const double* Vinv, double* inv, in
 // as the float the network input takes; -1 marks a domain boundary (the
 // feature is 0 there and the cell is flagged by rule).
 __global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ avg, const int* __restrict__ nb, int N,
-                                                   float4* __restrict__ jumps) {
+                                                   float4* __restrict__ jumps, int* __restrict__ count_reset) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  // the flagged-element count that k_mlp_indicator appends to starts at zero
+  if (count_reset && e == 0) *count_reset = 0;
   if (e >= N) return;
   int o[4];
 #pragma unroll
@@ -355,7 +357,8 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
                                                        const uint16_t* __restrict__ w2b,
                                                        const uint16_t* __restrict__ w3b, const float* __restrict__ P,
                                                        float threshold, float* __restrict__ z,
-                                                       int* __restrict__ flags) {
+                                                       int* __restrict__ flags, int* __restrict__ list,
+                                                       int* __restrict__ count) {
   __shared__ double sU[4][16 * 45];
   __shared__ double sV[81];
   __shared__ double sInv[4][16 * 5];
@@ -451,12 +454,24 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
     const float s01 = (q & 1) ? p1 + part : part + p1;
     const float p2 = __shfl_xor(s01, 32);
     const float tot = (q & 2) ? p2 + s01 : s01 + p2;
+    int fv = 0;
     if (valid && q == 0) {
       const float zz = tot + b3;
       if (z) z[e] = zz;
       // cells next to a domain boundary are flagged by rule, as by the
       // sensor (a boundary neighbour flags the cell, sensor_kernels.cu:19-163)
-      flags[e] = (at_wall(jmp) || zz > threshold) ? 1 : 0;
+      fv = (at_wall(jmp) || zz > threshold) ? 1 : 0;
+      flags[e] = fv;
+    }
+    if (list) {
+      // append the group's flagged elements to list[0 .. *count): one atomic per group with flags
+      const unsigned long long bal = __ballot(fv != 0);
+      if (bal) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(count, (int)__popcll(bal));
+        base = __shfl(base, 0);
+        if (fv) list[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = e;
+      }
     }
   }
 }
@@ -684,16 +699,17 @@ namespace dgx {
 
 // Indicator state of a solver: features, logits and the flags that gate the
 // sensor-driven HR sweeps (limiter 8).
-hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st) {
+hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool with_list) {
   dgx_mlp* m = s.mlp;
   const int N = s.N_E;
+  int* count = with_list ? s.flag + 2 : nullptr;
   hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
-                     reinterpret_cast<float4*>(s.mlp_jumps));
+                     reinterpret_cast<float4*>(s.mlp_jumps), count);
   const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
   hipLaunchKernelGGL(k_mlp_indicator, dim3(nb), dim3(256), 0, st, U, s.t.mlp_vinv,
                      reinterpret_cast<const float4*>(s.mlp_jumps), N, m->w1b, m->w2b, m->w3b, m->P, s.mlp_threshold,
-                     s.mlp_z, s.sensors);
+                     s.mlp_z, s.sensors, with_list ? s.flag_list : nullptr, count);
   return hipGetLastError();
 }
 
@@ -701,7 +717,7 @@ hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st) {
   const int N = s.N_E;
   hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
-                     reinterpret_cast<float4*>(s.mlp_jumps));
+                     reinterpret_cast<float4*>(s.mlp_jumps), nullptr);
   const unsigned nbf = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
   hipLaunchKernelGGL(k_mlp_features, dim3(nbf), dim3(256), 0, st, U, s.t.mlp_vinv,
                      reinterpret_cast<const float4*>(s.mlp_jumps), N, X);
