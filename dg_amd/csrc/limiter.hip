@@ -553,6 +553,50 @@ __device__ inline void hr_full_nodal(const double* __restrict__ L2M, const doubl
   }
 }
 
+// hri1D (limiting_kernels.cu:322-427) on one column of a flagged element
+// whose nodal values are in UC (in/out), with the boundary rules of the
+// direction offxy; UL and UR (the neighbours' columns) are read only when
+// both neighbours exist (physical == 0).
+template <int D, int P>
+__device__ inline void hri_limit(int physical, int fc, int offxy, const double* __restrict__ L2M,
+                                 const double* __restrict__ M2L, const double* UL, double* UC, const double* UR) {
+  constexpr int Ns = Shape<D, P>::Ns, N = P;
+  constexpr int slicenum = (D == 1) ? 1 : P + 1;
+  double Alim[Ns];
+  if (physical == 4) {
+  } else if (physical == 2) {
+    hr_avg_nodal<Ns, N, slicenum>(L2M, M2L, UC, Alim);
+  } else if (physical == 3) {
+    const int un_idx = (offxy == 0) ? 1 : (offxy == 2 ? 2 : 0);
+    if (fc != un_idx) hr_avg_nodal<Ns, N, slicenum>(L2M, M2L, UC, Alim);
+  } else {
+    hr_full_nodal<Ns, N, slicenum>(L2M, M2L, UL, UC, UR, Alim);
+  }
+}
+
+// The limited column (element e, field fc) of one sweep, from U, into UC.
+template <int D, int P>
+__device__ inline void hri_sweep_column(int e, int fc, int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                        const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                        const double* __restrict__ U, double* UC) {
+  constexpr int Ns = Shape<D, P>::Ns;
+  const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
+  int physical = 0;
+  if (left < 0) physical = -left;
+  if (right < 0) physical = -right;
+#pragma unroll
+  for (int i = 0; i < Ns; ++i) UC[i] = U[((long)e * NF + fc) * Ns + i];
+  double UL[Ns], UR[Ns];
+  if (physical == 0) {
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) {
+      UL[i] = U[((long)left * NF + fc) * Ns + i];
+      UR[i] = U[((long)right * NF + fc) * Ns + i];
+    }
+  }
+  hri_limit<D, P>(physical, fc, offxy, L2M, M2L, UL, UC, UR);
+}
+
 // hri1D (limiting_kernels.cu:322-427) for one column (element, field) of a
 // flagged element: HR in Lagrange space through Lag2Mono / Mono2Lag, or the
 // boundary rules; Unew receives the column.
@@ -560,36 +604,9 @@ template <int D, int P>
 __device__ inline void hri_column(int e, int fc, int NF, int N_N, const int* __restrict__ nb, int offxy,
                                   const double* __restrict__ L2M, const double* __restrict__ M2L,
                                   const double* __restrict__ U, double* __restrict__ Unew) {
-  constexpr int Ns = Shape<D, P>::Ns, N = P;
-  constexpr int slicenum = (D == 1) ? 1 : P + 1;
-  const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
-  int physical = 0;
-  if (left < 0) physical = -left;
-  if (right < 0) physical = -right;
-  double UC[Ns], Alim[Ns];
-#pragma unroll
-  for (int i = 0; i < Ns; ++i) UC[i] = U[((long)e * NF + fc) * Ns + i];
-  bool avg = false, full = false;
-  if (physical == 4) {
-  } else if (physical == 2) {
-    avg = true;
-  } else if (physical == 3) {
-    int un_idx = (offxy == 0) ? 1 : (offxy == 2 ? 2 : 0);
-    avg = (fc != un_idx);
-  } else {
-    full = true;
-  }
-  if (avg) {
-    hr_avg_nodal<Ns, N, slicenum>(L2M, M2L, UC, Alim);
-  } else if (full) {
-    double UL[Ns], UR[Ns];
-#pragma unroll
-    for (int i = 0; i < Ns; ++i) {
-      UL[i] = U[((long)left * NF + fc) * Ns + i];
-      UR[i] = U[((long)right * NF + fc) * Ns + i];
-    }
-    hr_full_nodal<Ns, N, slicenum>(L2M, M2L, UL, UC, UR, Alim);
-  }
+  constexpr int Ns = Shape<D, P>::Ns;
+  double UC[Ns];
+  hri_sweep_column<D, P>(e, fc, NF, N_N, nb, offxy, L2M, M2L, U, UC);
 #pragma unroll
   for (int i = 0; i < Ns; ++i) Unew[((long)e * NF + fc) * Ns + i] = UC[i];
 }
