@@ -36,6 +36,19 @@ constexpr double kGammaSF = 1.4;  // constants::GLOBAL_GAMMA
 
 __host__ __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 
+// 1/x for the positive, normal operands of the flux kernels (densities,
+// 1 + sqrt ratios, sound speeds, 1/(gamma-1)): the hardware reciprocal
+// refined by two Newton steps, within an ulp or two of the correctly
+// rounded quotient, without the scaling and special-case fix-up of IEEE
+// division.
+__device__ inline double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 // ---------------------------------------------------------------------------
 // Riemann solvers.  uL/uR: conserved face states; n: unit normal (side 0
 // outward).  Outputs F (flux) and nc (non-conservative term), both NF long.
@@ -158,7 +171,7 @@ __device__ inline void riemann<2, 0, 1>(const double* uL, const double* uR, doub
   // divisions by a common divisor as products with its reciprocal (FP64
   // division is a long sequence here; a few ulps against the reference)
   double rhoL = uL[0], rhoR = uR[0];
-  const double irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+  const double irL = rcp_nr(rhoL), irR = rcp_nr(rhoR);
   double vxL = uL[1] * irL, vxR = uR[1] * irR, vyL = uL[2] * irL, vyR = uR[2] * irR;
   double EtL = uL[3], EtR = uR[3];
   double tx = -ny, ty = nx;
@@ -169,14 +182,14 @@ __device__ inline void riemann<2, 0, 1>(const double* uL, const double* uR, doub
   double pR = (gamma - 1) * (EtR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
   double HL = (EtL + pL) * irL, HR = (EtR + pR) * irR;
   double RT = sqrt(rhoR * irL);
-  const double i1RT = 1.0 / (1 + RT);
+  const double i1RT = rcp_nr(1 + RT);
   double rho = RT * rhoL;
   double vx = (vxL + RT * vxR) * i1RT;
   double vy = (vyL + RT * vyR) * i1RT;
   double vn = vx * nx + vy * ny, vt = vx * tx + vy * ty;
   double H = (HL + RT * HR) * i1RT;
   double a = sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
-  const double ia = 1.0 / a, ia2 = ia * ia;
+  const double ia = rcp_nr(a), ia2 = ia * ia;
   double drho = rhoR - rhoL, dp = pR - pL, dvn = vnR - vnL, dvt = vtR - vtL;
   double dV0 = (dp - rho * a * dvn) * (0.5 * ia2);
   double dV1 = rho * dvt * ia;
@@ -219,28 +232,28 @@ __device__ inline void riemann<2, 1, 1>(const double* uL, const double* uR, doub
   // divisions by a common divisor as products with its reciprocal (FP64
   // division is a long sequence here; a few ulps against the reference)
   double rhoL = uL[0], rhoR = uR[0];
-  const double irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+  const double irL = rcp_nr(rhoL), irR = rcp_nr(rhoR);
   double vxL = uL[1] * irL, vxR = uR[1] * irR, vyL = uL[2] * irL, vyR = uR[2] * irR;
   double EtL = uL[3], EtR = uR[3], alphaL = uL[4], alphaR = uR[4];
   double tx = -ny, ty = nx;
   double vnL = vxL * nx + vyL * ny, vnR = vxR * nx + vyR * ny;
   double vtL = vxL * tx + vyL * ty, vtR = vxR * tx + vyR * ty;
-  double gammaL = 1.0 + 1.0 / alphaL, gammaR = 1.0 + 1.0 / alphaR;
+  double gammaL = 1.0 + rcp_nr(alphaL), gammaR = 1.0 + rcp_nr(alphaR);
   double pL = (gammaL - 1) * (EtL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
   double pR = (gammaR - 1) * (EtR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
   double HL = (EtL + pL) * irL, HR = (EtR + pR) * irR;
   double iL = pL * alphaL, iR = pR * alphaR;
   double RT = sqrt(rhoR * irL);
-  const double i1RT = 1.0 / (1 + RT);
+  const double i1RT = rcp_nr(1 + RT);
   double rho = RT * rhoL;
   double vx = (vxL + RT * vxR) * i1RT;
   double vy = (vyL + RT * vyR) * i1RT;
   double vn = vx * nx + vy * ny, vt = vx * tx + vy * ty;
   double H = (HL + RT * HR) * i1RT;
   double alpha = (alphaL + RT * alphaR) * i1RT;
-  double gamma = 1 + 1.0 / alpha;
+  double gamma = 1 + rcp_nr(alpha);
   double a = sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
-  const double ia = 1.0 / a, ia2 = ia * ia;
+  const double ia = rcp_nr(a), ia2 = ia * ia;
   double i = (iL + RT * iR) * i1RT;
   double p = (gamma - 1) * i;
   double drho = rhoR - rhoL;
@@ -526,9 +539,9 @@ __device__ inline void volume_flux(const double* u, const double* dua, const dou
     }
   } else {
     double rho = u[0];
-    const double ir = 1.0 / rho;
+    const double ir = rcp_nr(rho);
     double vx = u[1] * ir, vy = u[2] * ir, Et = u[3];
-    double gamma = (PHYS == 0) ? kGammaSF : 1 + 1.0 / u[4];
+    double gamma = (PHYS == 0) ? kGammaSF : 1 + rcp_nr(u[4]);
     double p = (gamma - 1) * (Et - 0.5 * rho * (vx * vx + vy * vy));
     f[0 * 2 + 0] = rho * vx;
     f[1 * 2 + 0] = rho * vx * vx + p;
