@@ -252,6 +252,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   __shared__ double sGeo[NT * NGEO];  // per element: J, invJac
   __shared__ double sDte[NT];         // per element: Dt (J_0/J_e with the shared Minv)
   __shared__ double sFG[NFACE * 4];   // per tile face: n_x, n_y, JF side 0, JF side 1
+  __shared__ int sFD[NFACE];          // per tile face: receiving slot, edge and closure of each side (packed)
   const VolTables<D, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
 
   const int tid = threadIdx.x;
@@ -369,6 +370,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     for (int itf = tid; itf < NFACE * MG; itf += 256) {
       const int fi = itf / MG, g = itf % MG;
       const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
+      // what the integration below needs of the descriptor, per side: (dst + 1) | edge << 8 | cl << 10
+      if (g == 0)
+        sFD[fi] = ((fd.dst[0] + 1) | (fd.edge[0] << 8) | (fd.cl[0] << 10)) |
+                  (((fd.dst[1] + 1) | (fd.edge[1] << 8) | (fd.cl[1] << 10)) << 16);
       const double* fg = sFG + fi * 4;
       const double nxv = fg[0], nyv = (D == 2) ? fg[1] : 0.0, jf0 = fg[2], jf1 = fg[3];
       double tr[2][NF][Ms];
@@ -428,11 +433,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     // ---- face integration against psi_w into the receiving elements' slots
     for (int itf = tid; itf < NFACE * 2 * Ms; itf += 256) {
       const int fi = itf / (2 * Ms), d = (itf / Ms) % 2, j = itf % Ms;
-      const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
-      // selects, not fd.x[d]: a runtime index into the descriptor puts it on the stack
-      const int dst = d ? fd.dst[1] : fd.dst[0];
+      const int side = (sFD[fi] >> (16 * d)) & 0xffff;
+      const int dst = (side & 0xff) - 1;
       if (dst < 0) continue;
-      const int cl = d ? fd.cl[1] : fd.cl[0], edge = d ? fd.edge[1] : fd.edge[0];
+      const int cl = side >> 10, edge = (side >> 8) & 3;
       const int jf = (D == 2 && cl >= 4) ? reversed_pos<P>(j) : j;
       double* q = sQ + dst * SQ + edge * NF * Ms;
       const double jfd = sFG[fi * 4 + 2 + d];
