@@ -313,9 +313,6 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   s.t.minv_stride = uniform_minv ? 0 : 1;
   s.t.J0 = t.J[0];
   cartesian_layout(t, s.cart);
-  // the tile kernel covers single-fluid and two-fluid states; passive and
-  // stiffened states run on the face + element kernels
-  if (cfg->physics > 1) s.cart.on = false;
 
   // every element edge must receive exactly one contributing face side
   std::vector<int> ef((size_t)NE * NN, -1);
@@ -364,7 +361,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
          upload(s, &s.t.MonoY2Lag, t.MonoY2Lag.data(), t.MonoY2Lag.size()) &&
          upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
          upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
-  if (s.cart.on && D == 2 && s.P <= 2) {
+  if (s.cart.on && D == 2 && s.P <= 2 && s.NF <= 5) {
     // the one-launch HR limiter assumes e -/+ 1 and e -/+ nx inside a tile,
     // takes the rows below and above it and the ends of every row from the
     // table, and takes the x neighbours of a halo-row element to be the halo
@@ -714,9 +711,9 @@ dgx_status dgx_solver_set_precision(dgx_solver* h, int precision) {
   if (!h || precision < 0 || precision > 2)
     return fail(DGX_EINVAL, "dgx_solver_set_precision: precision must be 0, 1 or 2");
   Solver& s = h->s;
-  if (precision > 0 && !(s.tiled() && s.D == 2 && s.P == 2))
-    return fail(DGX_EUNSUPPORTED,
-                "dgx_solver_set_precision: FP32 and split-BF16 contractions need the tile kernel, 2D, p=2");
+  if (precision > 0 && !(s.tiled() && s.D == 2 && s.P == 2 && s.cfg.physics <= DGX_MULTIFLUID))
+    return fail(DGX_EUNSUPPORTED, "dgx_solver_set_precision: FP32 and split-BF16 contractions need the tile "
+                                  "kernel, 2D, p=2, single-fluid or two-fluid states");
   s.precision = precision;
   return DGX_OK;
 }

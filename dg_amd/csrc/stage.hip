@@ -225,9 +225,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int SU = lds_stride(NU), SQ = lds_stride(NQE), SW = lds_stride(NG * NQ);
   // element phases on the matrix cores (FP64, or FP32 in the mixed-precision mode) or as VALU batches
   // PREC 0: FP64; 1: FP32 contractions; 2: split-BF16 contractions with FP32 accumulation
-  constexpr bool MFW = D == 2 && P == 2 && NT % 16 == 0 && (PREC >= 1 || DGX_STAGE_MFMA64);
+  constexpr bool MFW = D == 2 && P == 2 && NT % 16 == 0 && PH <= 1 && (PREC >= 1 || DGX_STAGE_MFMA64);
   using TW = typename std::conditional<PREC >= 1, float, double>::type;
-  constexpr int NWP = NF + PH;  // per face point: F, and the alpha non-conservative term
+  constexpr int NNC = V::NNC;    // non-conservative fields: the last NNC
+  constexpr int NWP = NF + NNC;  // per face point: F, and the non-conservative terms
   constexpr int NW_FACE = NFACE * MG * NWP, NW_VOL = MFW ? 0 : EPB * SW;
   constexpr int NW_MF = MFW ? 4 * vol_wave_lds<TW, PH>() : 0;
   constexpr int NW0 = NW_FACE > NW_VOL ? NW_FACE : NW_VOL;
@@ -419,12 +420,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
         ur[f] = sr;
       }
       riemann<D, PH, FL>(ul, ur, nxv, nyv, F, nc);
-      // the non-conservative term is zero except for alpha (multifluid):
-      // keep F and that one term; sides and face Jacobians at integration
+      // the non-conservative terms are zero except for the last NNC fields
+      // (alpha; phinc; alpha and beta): keep F and those; sides and face
+      // Jacobians at integration
       double* w = sW + (fi * MG + g) * NWP;
 #pragma unroll
       for (int f = 0; f < NF; ++f) w[f] = F[f];
-      if constexpr (PH == 1) w[NF] = nc[NF - 1];
+#pragma unroll
+      for (int m = 0; m < NNC; ++m) w[NF + m] = nc[NF - NNC + m];
     }
     __syncthreads();
     // the next tile's inputs (persistent blocks): in flight from here to the
@@ -446,7 +449,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 #pragma unroll
         for (int g = 0; g < MG; ++g) {
           const double* w = sW + (fi * MG + g) * NWP;
-          const double ncf = (PH == 1 && f == NF - 1) ? w[NF] : 0.0;
+          const double ncf = f >= NF - NNC ? w[NF + f - (NF - NNC)] : 0.0;
           // q_L = (-F + nc) JF_0, q_R = (F + nc) JF_1 (physics.cu:800-804, kernels.cu:199-228)
           const double qv = (d ? (w[f] + ncf) : (-w[f] + ncf)) * jfd;
           s += sPsiw[g + j * MG] * qv;
@@ -545,7 +548,12 @@ static hipError_t dispatch_stage(const dgx_config& c, Fn&& fn) {
     return c.flux == 0 ? fn(d, p, ph, I0{}) : fn(d, p, ph, I1{});
   };
   auto by_phys = [&](auto d, auto p) -> hipError_t {
-    return c.physics == 0 ? by_flux(d, p, I0{}) : by_flux(d, p, I1{});
+    switch (c.physics) {
+      case 0: return by_flux(d, p, I0{});
+      case 1: return by_flux(d, p, I1{});
+      case 2: return by_flux(d, p, I2{});
+      default: return by_flux(d, p, I3{});
+    }
   };
   auto by_order = [&](auto d) -> hipError_t {
     switch (c.order) {
@@ -575,7 +583,7 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
     auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0> : k_stage<D, P, PH, FL, 1, TX, TY, 0>;
-    if constexpr (D == 2 && P == 2 && TX * TY == 32) {
+    if constexpr (D == 2 && P == 2 && TX * TY == 32 && PH <= 1) {
       if (s.precision == 1)
         kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
       if (s.precision == 2)

@@ -12,8 +12,8 @@ import torch
 
 import oracle
 from dg_amd import (FARFIELD, LIMIT_HR, LIMIT_HRI, MULTIFLUID, PATH_GENERAL, PATH_TILED, PERIODIC, REFLECTIVE, ROE,
-                    Config, DgxError, Mesh, Solver, problems)
-from tests.helpers import CONFIGS, cfg_id, per_field_rel
+                    Config, DgxError, Mesh, Solver, nfields, problems)
+from tests.helpers import CONFIGS, CONFIGS_PS, cfg_id, per_field_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -36,13 +36,13 @@ SIZES_2D = [(16, 8), (32, 16)]
 
 
 def tiled_mesh(d, order, ph, bc, n):
-    nf = 2 + d + ph
+    nf = nfields(d, ph)
     if d == 1:
         return Mesh(1, order, n, 1, -1.0, 1.0, bc=bc, nfields=nf)
     return Mesh(2, order, n[0], n[1], 0.0, 1.6, 0.0, 0.6, bc=bc, nfields=nf)
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + CONFIGS_PS, ids=cfg_id)
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_tiled_rhs_matches_oracle_and_general_path(device, c, order):
     d, ph, fl = c
@@ -74,7 +74,7 @@ def test_tiled_rhs_matches_oracle_and_general_path(device, c, order):
             assert err_g <= 1e-12, (bc, n, err_g)
 
 
-@pytest.mark.parametrize("c", CONFIGS, ids=cfg_id)
+@pytest.mark.parametrize("c", CONFIGS + CONFIGS_PS, ids=cfg_id)
 def test_tiled_rk4_trajectory_with_limiter(device, c):
     d, ph, fl = c
     order = 2
