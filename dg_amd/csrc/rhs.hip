@@ -228,6 +228,117 @@ __global__ __launch_bounds__(256) void k_elem(ElemArgs a) {
   }
 }
 
+// The same for 2D p=2 single-fluid and two-fluid states with the element
+// phases on the FP64 matrix cores (vol_mfma_wave): a block of 256 threads
+// handles 32 consecutive elements; with the shared mass-matrix inverse the
+// wave also forms Dt*Minv*R.  Padded per-element LDS strides (a multiple of
+// 4 doubles apart for neighbouring elements: distinct bank groups).
+constexpr bool elem_bank_ok(int n) { return (2 * n) % 32 >= 4 && (2 * n) % 32 <= 28; }
+constexpr int elem_stride(int n) { return elem_bank_ok(n) ? n : elem_stride(n + 1); }
+
+template <int PH, int FL, int MODE>
+__global__ __launch_bounds__(256) void k_elem_mfma(ElemArgs a) {
+  constexpr int D = 2, P = 2, NT = 32, NGEO = 1 + D * D;
+  using V = Vol<D, P, PH>;
+  constexpr int NF = V::NF, Ns = V::Ns, NU = V::NU, NQE = V::NQE;
+  constexpr int SU = elem_stride(NU), SQ = elem_stride(NQE);
+  __shared__ double sU[NT * SU];
+  __shared__ double sQ[NT * SQ];
+  __shared__ double sW[4 * vol_wave_lds<double, PH>()];
+  __shared__ double sPhi[V::NTAB_PHI];
+  __shared__ double sDphi[PH ? V::NTAB_D : 1];
+  __shared__ double sPw[PH ? V::NTAB_PHI : 1];
+  __shared__ double sMinv[Ns * Ns];
+  __shared__ double sGeo[NT * NGEO];
+  __shared__ double sDte[NT];
+  __shared__ int sNodeQ[18];
+  const VolTables<D, P, PH> tb{sPhi, sDphi, nullptr, sPw, sMinv};
+
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * NT;
+  const int n_el = min(NT, a.N_E - e0);
+  const bool uniform_minv = a.minv_stride == 0;
+  VolTableFetch<D, P, PH, false> tf;
+  tf.fetch(a.phi, a.dphi, a.phi_w, a.dphi_w, a.Minv, MODE == 1 && uniform_minv, tid);
+  constexpr int RU = (NT * NU + 255) / 256, RQ = (NT * NQE + 255) / 256, RG = (NT * NGEO + 255) / 256;
+  double ru[RU], rq[RQ], rg[RG];
+  // elements past the end of the mesh (last block) repeat the block's first element
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int k = tid + 256 * r, el = k / NU;
+    if (k < NT * NU) ru[r] = a.Uin[(size_t)(e0 + (el < n_el ? el : 0)) * NU + k % NU];
+  }
+#pragma unroll
+  for (int r = 0; r < RQ; ++r) {
+    const int k = tid + 256 * r, el = k / NQE;
+    if (k < NT * NQE) rq[r] = a.Qe[(size_t)(e0 + (el < n_el ? el : 0)) * NQE + k % NQE];
+  }
+#pragma unroll
+  for (int r = 0; r < RG; ++r) {
+    const int k = tid + 256 * r, el = k / NGEO, q = k % NGEO;
+    if (k < NT * NGEO) {
+      const int eg = (e0 + (el < n_el ? el : 0)) * a.geo_stride;
+      rg[r] = q == 0 ? a.J[eg] : a.invJac[eg * D * D + q - 1];
+    }
+  }
+  tf.store(tb, MODE == 1 && uniform_minv, tid);
+  node_q_table<NF>(sNodeQ, tid);
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int k = tid + 256 * r;
+    if (k < NT * NU) sU[(k / NU) * SU + k % NU] = ru[r];
+  }
+#pragma unroll
+  for (int r = 0; r < RQ; ++r) {
+    const int k = tid + 256 * r;
+    if (k < NT * NQE) sQ[(k / NQE) * SQ + k % NQE] = rq[r];
+  }
+#pragma unroll
+  for (int r = 0; r < RG; ++r) {
+    const int k = tid + 256 * r;
+    if (k < NT * NGEO) sGeo[k] = rg[r];
+  }
+  __syncthreads();
+  const double dt = MODE == 1 ? a.dt[0] : 0.0;
+  if (MODE == 1 && tid < NT) sDte[tid] = uniform_minv ? dt * (a.J0 / sGeo[tid * NGEO]) : dt;
+  // RK operands of this thread's output entries, in flight during the element phases
+  double pre_un[RU], pre_acc[RU];
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    pre_un[r] = pre_acc[r] = 0.0;
+    const int k = tid + 256 * r;
+    if (MODE == 1 && k < n_el * NU) {
+      if (a.stage < 3) pre_un[r] = a.Un[(size_t)e0 * NU + k];
+      if (a.stage > 0) pre_acc[r] = a.Uacc[(size_t)e0 * NU + k];
+    }
+  }
+  __syncthreads();
+  const bool wave_minv = MODE == 1 && uniform_minv;
+  vol_mfma_wave<double, PH, NT, SU, SQ, NGEO>(tb, a.dphi_w, sU, sGeo, sQ, sW, sNodeQ, tid,
+                                              wave_minv ? tb.minv : nullptr, sDte);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RU; ++r) {
+    const int k = tid + 256 * r;
+    if (k >= n_el * NU) break;
+    const int el = k / NU, rr = k % NU;
+    const size_t gi = (size_t)e0 * NU + k;
+    if constexpr (MODE == 0) {
+      a.fout[gi] = sU[el * SU + rr];
+    } else if (wave_minv) {
+      rk_combine(a.stage, sU[el * SU + rr], pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+    } else {
+      const double* M = a.Minv + (size_t)(e0 + el) * Ns * Ns;
+      const int f = rr / Ns, i = rr % Ns;
+      const double* R = sU + el * SU + f * Ns;
+      double sol = 0.0;
+#pragma unroll
+      for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
+      rk_combine(a.stage, sDte[el] * sol, pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Dt from CFL: max over nodes of |u|+a, then Dt = cfl / max (rk.cc:250-272).
 // Two levels: per-block maxima, then one block reduces them (no atomics).
@@ -320,6 +431,14 @@ hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, 
              s.t.minv_stride, s.t.J0, s.Qe, s.N_E, stage, Uin, Un, Uacc, Uout, dt_dev, f_out};
   return dispatch(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
+    if constexpr (D == 2 && P == 2 && PH <= 1) {
+      const unsigned nbm = blocks_for(s.N_E, 32);
+      if (mode == 0)
+        hipLaunchKernelGGL((k_elem_mfma<PH, FL, 0>), dim3(nbm), dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((k_elem_mfma<PH, FL, 1>), dim3(nbm), dim3(256), 0, st, a);
+      return hipGetLastError();
+    }
     constexpr int EPB = Vol<D, P, PH>::EPB;
     const unsigned nb = blocks_for(s.N_E, EPB);
     if (mode == 0)
