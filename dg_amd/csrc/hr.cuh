@@ -18,6 +18,12 @@ __device__ inline double minmod_d(double a, double b) {
   return sa * fb;
 }
 
+// skip_zero_terms: the loops below leave out the products with a zero
+// integral (known once the loops are unrolled).  Each left-out product of a
+// finite value is +-0, and adding +-0 to a sum that started at +0.0 does not
+// change it: a non-zero sum stays as it is, and in round-to-nearest the sum
+// is never -0 (x + (-x) = +0 and +0 + (-0) = +0).  The results are therefore
+// the bits of the full sums the reference forms.
 __host__ __device__ constexpr int ifact(int n) { return n <= 1 ? 1 : n * ifact(n - 1); }
 __host__ __device__ constexpr long ipow(long b, int n) { return n == 0 ? 1 : b * ipow(b, n - 1); }
 
@@ -44,14 +50,17 @@ __device__ inline void limit_monomial_d(const double* AL, const double* AC, cons
     double avgdUL = 0, avgdUC = 0, avgdUR = 0, avgRL = 0, avgRC = 0, avgRR = 0;
 #pragma unroll
     for (int n = m - 1; n <= N; n++) {
+      // the zero integrals (n - m even) are left out: see skip_zero_terms
       double integral = integ_deriv(m - 1, n);
-      avgdUL += AL[n] * integral;
-      avgdUC += AC[n] * integral;
-      avgdUR += AR[n] * integral;
+      if (integral != 0.0) {
+        avgdUL += AL[n] * integral;
+        avgdUC += AC[n] * integral;
+        avgdUR += AR[n] * integral;
+      }
       if (n >= m + 1) {
         double alim = Alim[n];
         avgRL += alim * integ_deriv_bounds(m - 1, n, -3, -1);
-        avgRC += alim * integral;
+        if (integral != 0.0) avgRC += alim * integral;
         avgRR += alim * integ_deriv_bounds(m - 1, n, 1, 3);
       }
     }
@@ -71,7 +80,7 @@ __device__ inline void average_monomial_d(const double* A, double* Alim) {
   double avg = 0;
 #pragma unroll
   for (int n = 0; n <= N; n++) {
-    avg += A[n] * integ_deriv(0, n);
+    if (integ_deriv(0, n) != 0.0) avg += A[n] * integ_deriv(0, n);
     Alim[n] = 0;
   }
   Alim[0] = 0.5 * avg;

@@ -24,9 +24,11 @@
 //   limiting_kernels.cu:322-427; copy_detected, sensor_kernels.cu:167-200).
 #pragma clang fp contract(off)
 #include <type_traits>
+#include <utility>
 
 #include "dgx_internal.h"
 #include "hr.cuh"
+#include "hr_tables.h"
 #include "physics.cuh"
 
 namespace dgx {
@@ -106,6 +108,41 @@ __device__ inline void col_apply(const double* __restrict__ M, const double* x, 
     for (int r = 0; r < Ns; ++r) y[r] = y[r] + x[l] * M[r + l * Ns];
 }
 
+// y = M x for one of the constant tables of hr_tables.h (WHICH: 0 Lag2MonoX,
+// 1 MonoX2MonoY, 2 MonoY2Lag) in col_apply's order.  The loops run over
+// integer sequences so that every entry is a constant expression: the zero
+// entries are left out, which keeps col_apply's bits (see skip_zero_terms in
+// hr.cuh), and the multiplications by +-1 fold away.
+template <int P, int WHICH>
+__host__ __device__ constexpr double hr_entry(int i) {
+  return WHICH == 0 ? HrTables2D<P>::L2MX[i] : (WHICH == 1 ? HrTables2D<P>::MX2MY[i] : HrTables2D<P>::MY2L[i]);
+}
+
+template <int P, int WHICH, int L, int... R>
+__device__ inline void col_apply_c_col(const double* x, double* y, std::integer_sequence<int, R...>) {
+  constexpr int Ns = HrTables2D<P>::Ns;
+  auto term = [&](auto r) {
+    constexpr int rr = decltype(r)::value;
+    constexpr double m = hr_entry<P, WHICH>(rr + L * Ns);
+    if constexpr (m != 0.0) y[rr] = y[rr] + x[L] * m;
+  };
+  (term(std::integral_constant<int, R>{}), ...);
+}
+
+template <int P, int WHICH, int... L>
+__device__ inline void col_apply_c_all(const double* x, double* y, std::integer_sequence<int, L...>) {
+  constexpr int Ns = HrTables2D<P>::Ns;
+  (col_apply_c_col<P, WHICH, L>(x, y, std::make_integer_sequence<int, Ns>{}), ...);
+}
+
+template <int P, int WHICH>
+__device__ inline void col_apply_c(const double* x, double* y) {
+  constexpr int Ns = HrTables2D<P>::Ns;
+#pragma unroll
+  for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+  col_apply_c_all<P, WHICH>(x, y, std::make_integer_sequence<int, Ns>{});
+}
+
 // hrl1D for every slice of one column (limiting_kernels.cu:245-319)
 template <int N, int SL>
 __device__ inline void hrl_column(int physical, int fc, int offxy, const double* AL, const double* AC,
@@ -170,32 +207,36 @@ __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, cons
 
 // Both sweeps of the 2D global HR limiter in one pass, out of place, on a
 // Cartesian grid (element e = ey*nx + ex) for a tile of TXE x TYE elements:
-//   1. the Taylor-x coefficients (Lag2MonoX) of every element the tile needs,
-//      once each, into LDS: the tile, the rows just below and above it, and
-//      the element left and right of each of these rows (all found through
-//      the neighbour table, so periodic ends and walls need no special case);
-//   2. the x sweep (hrl along x, MonoX2MonoY) of the tile rows and of the two
+//   1. the nodal values of every element the tile needs, with coalesced loads
+//      (an element's NF*Ns values are contiguous), into LDS: the tile, the
+//      rows just below and above it, and the element left and right of each
+//      of these rows (all found through the neighbour table, so periodic ends
+//      and walls need no special case);
+//   2. their Taylor-x coefficients (Lag2MonoX), in place, one column per thread;
+//      the three transforms are the compile-time tables of hr_tables.h, which
+//      the host has found equal to the solver's (Solver::hr2d);
+//   3. the x sweep (hrl along x, MonoX2MonoY) of the tile rows and of the two
 //      halo rows, from LDS into LDS;
-//   3. the y sweep (hrl along y, MonoY2Lag) of the tile rows, to Out.
+//   4. the y sweep (hrl along y, MonoY2Lag) of the tile rows, staged in LDS
+//      and stored row by row (a tile row is TXE*NF*Ns contiguous values).
 // Every column sees exactly the arithmetic of the two k_hr_sweep launches,
 // so the result is bitwise the same, and the intermediate state never
 // reaches HBM.  In and Out must differ: neighbouring tiles read In's rows.
-template <int Ns, int N, int SL, int TXE, int TYE>
-__global__ __launch_bounds__(256) void k_hr2d(int nx, int NF, int N_N, const int* __restrict__ nb,
-                                              const double* __restrict__ L2MX, const double* __restrict__ MX2MY,
-                                              const double* __restrict__ MY2L, const double* __restrict__ In,
-                                              double* __restrict__ Out) {
-  constexpr int ROWS = TYE + 2, W = TXE + 2, NFMAX = 5;
+template <int Ns, int N, int SL, int NF, int TXE, int TYE, int BS>
+__global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restrict__ nb,
+                                             const double* __restrict__ In, double* __restrict__ Out) {
+  constexpr int ROWS = TYE + 2, W = TXE + 2, EL = NF * Ns, ROW_OUT = TXE * EL;
   // one x-sweep column per thread, so that its result can wait in registers
   // while the Taylor-x coefficients are still read, then take their place
-  static_assert(ROWS * TXE * NFMAX <= 256, "one x-sweep column per thread");
-  __shared__ double sA[ROWS * W * NFMAX * Ns];  // Taylor-x coefficients [row][col][f], then Taylor-y [row][i][f]
-  __shared__ int sE[ROWS * W];                  // element of each slot, -1 if absent
-  double* sB = sA;
+  static_assert(ROWS * TXE * NF <= BS, "one x-sweep column per thread");
+  static_assert(ROW_OUT % 2 == 0, "rows stored as pairs of doubles");
+  // nodal values, then Taylor-x [row][col][f]; then Taylor-y [row][i][f]; then the output [j][i][f]
+  __shared__ __align__(16) double sA[ROWS * W * EL];
+  __shared__ int sE[ROWS * W];  // element of each slot, -1 if absent
   const int tiles_x = nx / TXE;
   const int ex0 = (blockIdx.x % tiles_x) * TXE, ey0 = (blockIdx.x / tiles_x) * TYE;
   // slot elements: row 0 below the tile, rows 1..TYE the tile, row TYE+1 above
-  for (int t = threadIdx.x; t < ROWS * TXE; t += blockDim.x) {
+  for (int t = threadIdx.x; t < ROWS * TXE; t += BS) {
     const int i = t % TXE, r = t / TXE;
     int e;
     if (r == 0) e = nb[(ey0 * nx + ex0 + i) * N_N + 2];
@@ -206,16 +247,19 @@ __global__ __launch_bounds__(256) void k_hr2d(int nx, int NF, int N_N, const int
     if (i == TXE - 1) sE[r * W + W - 1] = e >= 0 ? nb[e * N_N + 1] : -1;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < ROWS * W * NF; t += blockDim.x) {
-    const int f = t % NF, slot = t / NF;
+  for (int q = threadIdx.x; q < ROWS * W * EL; q += BS) {
+    const int slot = q / EL;
     const int e = sE[slot];
-    if (e < 0) continue;
+    if (e >= 0) sA[q] = In[(long)e * EL + (q - slot * EL)];
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < ROWS * W * NF; c += BS) {
+    if (sE[c / NF] < 0) continue;
     double x[Ns], A[Ns];
-    const double* src = In + ((long)e * NF + f) * Ns;
+    double* a = sA + c * Ns;
 #pragma unroll
-    for (int l = 0; l < Ns; ++l) x[l] = src[l];
-    col_apply<Ns>(L2MX, x, A);
-    double* a = sA + (slot * NF + f) * Ns;
+    for (int l = 0; l < Ns; ++l) x[l] = a[l];
+    col_apply_c<N, 0>(x, A);
 #pragma unroll
     for (int l = 0; l < Ns; ++l) a[l] = A[l];
   }
@@ -241,35 +285,45 @@ __global__ __launch_bounds__(256) void k_hr2d(int nx, int NF, int N_N, const int
       for (int l = 0; l < Ns; ++l) { AL[l] = al[l]; AR[l] = ar[l]; }
     }
     hrl_column<N, SL>(physical, f2, 0, AL, AC, AR, Alim);
-    col_apply<Ns>(MX2MY, Alim, xs);
+    col_apply_c<N, 1>(Alim, xs);
   }
   __syncthreads();
   if (on2) {
-    double* b = sB + ((r2 * TXE + i2) * NF + f2) * Ns;
+    double* b = sA + ((r2 * TXE + i2) * NF + f2) * Ns;
 #pragma unroll
     for (int l = 0; l < Ns; ++l) b[l] = xs[l];
   }
   __syncthreads();
-  // y sweep of the tile rows
-  for (int t = threadIdx.x; t < TYE * TXE * NF; t += blockDim.x) {
-    const int f = t % NF, i = (t / NF) % TXE, j = t / (NF * TXE);
+  // y sweep of the tile rows (TYE*TXE*NF <= BS columns: one pass)
+  const bool on3 = t2 < TYE * TXE * NF;
+  if (on3) {
+    const int f = f2, i = i2, j = r2;
     const int e = (ey0 + j) * nx + ex0 + i;
     const int physical = hr_physical(nb[e * N_N + 2], nb[e * N_N + 3]);
-    double AC[Ns], AL[Ns], AR[Ns], Alim[Ns], x[Ns];
-    const double* bc = sB + (((j + 1) * TXE + i) * NF + f) * Ns;
+    double AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
+    const double* bc = sA + (((j + 1) * TXE + i) * NF + f) * Ns;
 #pragma unroll
     for (int l = 0; l < Ns; ++l) AC[l] = bc[l];
     if (physical == 0) {
-      const double* bl = sB + ((j * TXE + i) * NF + f) * Ns;
-      const double* br = sB + (((j + 2) * TXE + i) * NF + f) * Ns;
+      const double* bl = sA + ((j * TXE + i) * NF + f) * Ns;
+      const double* br = sA + (((j + 2) * TXE + i) * NF + f) * Ns;
 #pragma unroll
       for (int l = 0; l < Ns; ++l) { AL[l] = bl[l]; AR[l] = br[l]; }
     }
     hrl_column<N, SL>(physical, f, 2, AL, AC, AR, Alim);
-    col_apply<Ns>(MY2L, Alim, x);
-    double* o = Out + ((long)e * NF + f) * Ns;
+    col_apply_c<N, 2>(Alim, xs);
+  }
+  __syncthreads();
+  if (on3) {
+    double* o = sA + t2 * Ns;  // [j][i][f]: the element-major order of the tile rows
 #pragma unroll
-    for (int l = 0; l < Ns; ++l) o[l] = x[l];
+    for (int l = 0; l < Ns; ++l) o[l] = xs[l];
+  }
+  __syncthreads();
+  const double2* so = reinterpret_cast<const double2*>(sA);
+  for (int q = threadIdx.x; q < TYE * ROW_OUT / 2; q += BS) {
+    const int j = q / (ROW_OUT / 2), r = q - j * (ROW_OUT / 2);
+    reinterpret_cast<double2*>(Out + ((long)(ey0 + j) * nx + ex0) * EL)[r] = so[q];
   }
 }
 
@@ -890,16 +944,29 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
 // Out = HR(In) with both 2D sweeps in one launch (k_hr2d), p <= 2; needs the
 // tile layout and a neighbour table that follows it (Solver::hr2d).
 hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st) {
-  if (s.D != 2 || !s.hr2d || In == Out) return hipErrorInvalidValue;
-  constexpr int TXE = 8, TYE = 4;
-  const int tiles = (s.cart.nx / TXE) * (s.cart.ny / TYE);
+  if (s.D != 2 || !s.hr2d || In == Out || (s.NF != 4 && s.NF != 5)) return hipErrorInvalidValue;
+  // 8 x 8 tiles (512 threads) where the rows allow, else 8 x 4: the taller
+  // tile recomputes fewer halo columns (1.56 against 1.88 Taylor-x columns
+  // per output column) and ran 167 against 177 us on the 1024 x 512 p = 2
+  // two-fluid state
+  constexpr int TXE = 8;
+  const int tye = s.cart.ny % 8 == 0 ? 8 : 4;
+  const int tiles = (s.cart.nx / TXE) * (s.cart.ny / tye);
   return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value;
     if constexpr (D == 2 && P <= 2) {
       constexpr int Ns = Shape<D, P>::Ns, SL = P + 1;
-      hipLaunchKernelGGL((k_hr2d<Ns, P, SL, TXE, TYE>), dim3(tiles), dim3(256), 0, st, s.cart.nx, s.NF, s.N_N,
-                         s.t.neighbors, s.t.Lag2MonoX, s.t.MonoX2MonoY, s.t.MonoY2Lag, In, Out);
-      return hipGetLastError();
+      auto go = [&](auto nf, auto ty) -> hipError_t {
+        constexpr int NF = decltype(nf)::value, TYE = decltype(ty)::value, BS = TYE == 4 ? 256 : 512;
+        hipLaunchKernelGGL((k_hr2d<Ns, P, SL, NF, TXE, TYE, BS>), dim3(tiles), dim3(BS), 0, st, s.cart.nx, s.N_N,
+                           s.t.neighbors, In, Out);
+        return hipGetLastError();
+      };
+      using I4 = std::integral_constant<int, 4>;
+      using I5 = std::integral_constant<int, 5>;
+      using I8 = std::integral_constant<int, 8>;
+      if (s.NF == 4) return tye == 8 ? go(I4{}, I8{}) : go(I4{}, I4{});
+      return tye == 8 ? go(I5{}, I8{}) : go(I5{}, I4{});
     }
     return hipErrorInvalidValue;
   });

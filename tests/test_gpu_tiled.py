@@ -101,9 +101,11 @@ def test_tiled_rk4_trajectory_with_limiter(device, c):
 def test_one_launch_hr_limiter_bitwise(device, ph, order):
     """dgx_limit_into on a tile mesh (both 2D sweeps in one launch, out of
     place) equals the oracle's two sweeps and the in-place limiter bit for
-    bit, for every boundary combination, and leaves its input unchanged."""
-    for k, bc in enumerate(BCS_2D):
-        mesh = tiled_mesh(2, order, ph, bc, SIZES_2D[k % 2])
+    bit, for every boundary combination, and leaves its input unchanged.
+    16 x 12 elements take the 8 x 4 tiles, the other sizes the 8 x 8 ones."""
+    cases = [(bc, SIZES_2D[k % 2]) for k, bc in enumerate(BCS_2D)] + [(BCS_2D[0], (16, 12)), (BCS_2D[1], (16, 12))]
+    for k, (bc, n) in enumerate(cases):
+        mesh = tiled_mesh(2, order, ph, bc, n)
         prob = oracle.Problem(mesh.tables(), ph, ROE)
         solver = Solver(Config(2, order, ph, ROE), mesh)
         assert solver.hr_one_launch == (order <= 2)
@@ -115,7 +117,7 @@ def test_one_launch_hr_limiter_bitwise(device, ph, order):
         Ud = dev(U0, device)
         out = torch.full_like(Ud, float("nan"))
         solver.limit_into(LIMIT_HR, Ud, out)
-        assert np.array_equal(host(out), U_ref), (bc, order)
+        assert np.array_equal(host(out), U_ref), (bc, n, order)
         assert np.array_equal(host(Ud), U0)
         solver.limit(LIMIT_HR, Ud)
         assert np.array_equal(host(Ud), U_ref)
