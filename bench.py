@@ -113,7 +113,7 @@ def main() -> int:
         solver.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
     if lim == 8:  # train the indicator on the GPU (own forward/backward kernels), outside the timed region
         t_tr = time.perf_counter()
-        data = indicator.merge(indicator.generate(), indicator.interface_data(), indicator.smooth_data())
+        data = indicator.training_set()
         net, losses, thr = indicator.train(data, epochs=40)
         solver.set_indicator(net, thr)
         extra["indicator"] = {"training_s": round(time.perf_counter() - t_tr, 2), "samples": int(data.y.size),

@@ -42,12 +42,20 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4m __attribute__((ext_vector_type(4)));
 
+// float -> BF16, round to nearest even.  Device code uses gfx950's
+// conversion instruction (v_cvt_pk_bf16_f32, one instruction for the
+// bit arithmetic below): the same bits for every value that is not a NaN
+// (a NaN stays a NaN; features and activations never are one).
 __host__ __device__ inline uint16_t f2bf(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
+#else
   uint32_t u;
   std::memcpy(&u, &f, 4);
   if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));  // inf / nan
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+#endif
 }
 __host__ __device__ inline float bf2f(uint16_t b) {
   uint32_t u = (uint32_t)b << 16;
@@ -194,6 +202,36 @@ __device__ inline float feature_of(int k, const double* m, const double* inv, fl
   }
   const float j = k == 45 ? jmp.x : k == 46 ? jmp.y : k == 47 ? jmp.z : k == 48 ? jmp.w : 0.0f;
   return j < 0.0f ? 0.0f : j;
+}
+
+// The 16 features k = 16s + 4q + j (s, j = 0..3) of one element as the
+// B fragments of the first layer: the values of feature_of, without a branch
+// on the lane group q.  k < 32 are modes; k = 32 + 4q + j are modes up to 44
+// and the jumps L, R, D at 45..47 (q = 3, j = 1..3); k = 48 is the jump U
+// (q = 0, j = 0); the rest are zero.
+__device__ inline double clamp_feature(double v) { return v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0)); }
+__device__ inline float jump_feature(float j) { return j < 0.0f ? 0.0f : j; }
+__device__ inline void element_features(int q, const double* m, const double* inv, float4 jmp, bf16x4 (&bx)[4]) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 16 * s + 4 * q + j;
+      bx[s][j] = (short)f2bf((float)clamp_feature(m[k] * inv[k / 9]));
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int k = min(32 + 4 * q + j, 44);
+    const float mf = (float)clamp_feature(m[k] * inv[k / 9]);
+    if (j == 0) {
+      bx[2][j] = (short)f2bf(mf);
+    } else {
+      const float jv = jump_feature(j == 1 ? jmp.x : (j == 2 ? jmp.y : jmp.z));
+      bx[2][j] = (short)f2bf(q == 3 ? jv : mf);
+    }
+  }
+  bx[3][0] = (short)f2bf(q == 0 ? jump_feature(jmp.w) : 0.0f);
+  bx[3][1] = bx[3][2] = bx[3][3] = 0;
 }
 
 // Staging of 16 elements for a wave: the loads of elements e0 .. e0+ne-1
@@ -418,11 +456,7 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
     wave_modes(sU[wv], sV, sInv[wv], lane);
     bf16x4 bx[4];
     if (valid) {
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          bx[s][j] = (short)f2bf(feature_of(16 * s + 4 * q + j, &sU[wv][col * 45], &sInv[wv][col * 5], jmp));
+      element_features(q, &sU[wv][col * 45], &sInv[wv][col * 5], jmp, bx);
     } else {
 #pragma unroll
       for (int s = 0; s < 4; ++s)

@@ -169,6 +169,58 @@ def interface_data(n_problems: int = 32, nx: int = 128, ny: int = 64, snapshots=
     return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
 
 
+def bubble_data(n_problems: int = 24, nx: int = 256, ny: int = 128, snapshots=(0, 10, 20, 40, 60, 80),
+                neg_ratio: int = 10, seed: int = SEED + 3, cfl: float = 0.3, device="cuda") -> Dataset:
+    """Random shocked bubbles on the shock-bubble box [0,2]x[0,1] (walls at
+    y = 0 and 1): a planar shock of random Mach number in [1.05, 2] hits a
+    circular bubble of random radius, centre, density (log-uniform in
+    [0.05, 5]) and gamma ([1.1, 1.7]) in pressure equilibrium, evolved with
+    the global HR limiter through the passage of the shock; flagged cells are
+    kept with a seeded sample of unflagged ones, as in interface_data.  These
+    teach the reflected and transmitted shocks a curved interface makes."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    mesh = problems.shock_bubble_mesh(nx, ny, 2)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_sensor_thresholds(SENSOR_THRESHOLDS)
+    solver.set_indicator(Indicator(seed=0), 0.0)
+    NE = mesh.sizes["N_E"]
+    X = torch.zeros((NE, 64), dtype=torch.float32, device=device)
+    sens = torch.zeros(NE, dtype=torch.int32, device=device)
+    interior = ~(mesh.array("neighbors").reshape(-1, 4) < 0).any(axis=1)
+    out_X, out_s = [], []
+    for _ in range(n_problems):
+        rb = float(rng.uniform(0.1, 0.3))
+        U0 = problems.shock_bubble_state(
+            mesh, mach=float(rng.uniform(1.05, 2.0)), xs=0.1, xb=float(rng.uniform(0.2 + rb, 1.2)),
+            yb=float(rng.uniform(0.05 + rb, 0.95 - rb)), rb=rb,
+            rho_b=float(np.exp(rng.uniform(np.log(0.05), np.log(5.0)))), gamma_b=float(rng.uniform(1.1, 1.7)))
+        U = torch.from_numpy(U0).to(device)
+        done = 0
+        for snap in snapshots:
+            if snap > done:
+                try:
+                    solver.rk4_integrate(U, 1, cfl * mesh.dx / 5, nsteps=snap - done, tf=1e30, limiter=LIMIT_HR)
+                except DgxError:
+                    break
+                done = snap
+            solver.features(U, X)
+            solver.sensing(U, sens)
+            torch.cuda.synchronize()
+            Xh, sh = X.cpu().numpy(), sens.cpu().numpy()
+            fin = np.isfinite(Xh).all(axis=1) & interior
+            pos = np.nonzero(fin & (sh != 0))[0]
+            neg = np.nonzero(fin & (sh == 0))[0]
+            neg = rng.choice(neg, size=min(neg.size, max(neg_ratio * pos.size, 64)), replace=False)
+            sel = np.concatenate([pos, neg])
+            out_X.append(Xh[sel])
+            out_s.append(sh[sel])
+    Xa = np.concatenate(out_X)
+    sa = np.concatenate(out_s).astype(np.int32)
+    return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
+
+
 def smooth_data(n_problems: int = 16, nx: int = 64, ny: int = 32, snapshots=(0, 5), amp_range=(0.005, 0.1),
                 seed: int = SEED + 2, cfl: float = 0.3, device="cuda") -> Dataset:
     """Smooth random two-fluid flows on a periodic box (seeded waves of random
@@ -208,6 +260,13 @@ def smooth_data(n_problems: int = 16, nx: int = 64, ny: int = 32, snapshots=(0, 
     Xa = np.concatenate(out_X)
     sa = np.concatenate(out_s).astype(np.int32)
     return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
+
+
+def training_set(device="cuda") -> Dataset:
+    """The default training data: Riemann mosaics, random interfaces, random
+    shocked bubbles and smooth flows, all labelled by the sensor."""
+    return merge(generate(device=device), interface_data(device=device), bubble_data(device=device),
+                 smooth_data(device=device))
 
 
 def merge(*parts: Dataset) -> Dataset:
