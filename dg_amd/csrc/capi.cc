@@ -362,8 +362,8 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
          upload(s, &s.t.MonoY2Lag, t.MonoY2Lag.data(), t.MonoY2Lag.size()) &&
          upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
          upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
-  if (s.cart.on && D == 2 && s.P <= 2 && s.NF <= 5 &&
-      hr_tables_match(s.P, t.Lag2MonoX.data(), t.MonoX2MonoY.data(), t.MonoY2Lag.data())) {
+  s.hr_const = D == 2 && s.P <= 2 && hr_tables_match(s.P, t.Lag2MonoX.data(), t.MonoX2MonoY.data(), t.MonoY2Lag.data());
+  if (s.cart.on && s.hr_const && s.NF <= 5) {
     // the one-launch HR limiter assumes e -/+ 1 and e -/+ nx inside a tile,
     // takes the rows below and above it and the ends of every row from the
     // table, and takes the x neighbours of a halo-row element to be the halo

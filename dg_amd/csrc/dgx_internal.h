@@ -75,6 +75,7 @@ struct Solver {
   int precision = 0;  // tile kernel: 0 FP64, 1 FP32-MFMA map/integrate (2D p=2)
   bool tiled() const { return cart.on && path != PATH_GENERAL; }
   bool hr2d = false;  // 2D tile layout whose neighbour table follows it: one-launch HR limiter (k_hr2d)
+  bool hr_const = false;  // 2D p <= 2 with the HR transforms of hr_tables.h: constant-table sweeps
   // work buffers
   double* Qe = nullptr;     // face contributions, element-major [((e*N_N+k)*NF+f)*M_s+j]
   double* Uacc = nullptr;   // RK accumulator

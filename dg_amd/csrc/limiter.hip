@@ -205,6 +205,42 @@ __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, cons
   for (int r = 0; r < Ns; ++r) Out[col * Ns + r] = x[r];
 }
 
+// k_hr_sweep in 2D for p <= 2 with the transforms of hr_tables.h as
+// compile-time constants (the host has found the solver's tables equal to
+// them, Solver::hr_const): YS 0 is the x sweep (Lag2MonoX, then MonoX2MonoY),
+// YS 1 the y sweep (MonoY2Lag).  The same bits as k_hr_sweep.
+template <int P, int YS>
+__global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, const int* __restrict__ nb,
+                                                    const double* __restrict__ In, double* __restrict__ Out) {
+  constexpr int Ns = HrTables2D<P>::Ns, SL = P + 1, offxy = YS ? 2 : 0;
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= (long)N_E * NF) return;
+  const int e = (int)(col / NF), fc = (int)(col % NF);
+  const int left = nb[e * N_N + offxy], right = nb[e * N_N + offxy + 1];
+  const int physical = hr_physical(left, right);
+  auto first = [&](const double* src, double* A) {
+    double x[Ns];
+#pragma unroll
+    for (int l = 0; l < Ns; ++l) x[l] = src[l];
+    if constexpr (YS) {
+#pragma unroll
+      for (int l = 0; l < Ns; ++l) A[l] = x[l];
+    } else {
+      col_apply_c<P, 0>(x, A);
+    }
+  };
+  double AC[Ns], AL[Ns], AR[Ns], Alim[Ns], y[Ns];
+  first(In + col * Ns, AC);
+  if (physical == 0) {
+    first(In + ((long)left * NF + fc) * Ns, AL);
+    first(In + ((long)right * NF + fc) * Ns, AR);
+  }
+  hrl_column<P, SL>(physical, fc, offxy, AL, AC, AR, Alim);
+  col_apply_c<P, YS ? 2 : 1>(Alim, y);
+#pragma unroll
+  for (int r = 0; r < Ns; ++r) Out[col * Ns + r] = y[r];
+}
+
 // Both sweeps of the 2D global HR limiter in one pass, out of place, on a
 // Cartesian grid (element e = ey*nx + ex) for a tile of TXE x TYE elements:
 //   1. the nodal values of every element the tile needs, with coalesced loads
@@ -931,6 +967,11 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
       hipLaunchKernelGGL((k_apply<Ns>), g, b, 0, st, s.t.Lag2Mono, U, s.work, ncol);
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, nullptr,
                          s.t.Mono2Lag, s.work, U);
+    } else if (P <= 2 && s.hr_const) {
+      hipLaunchKernelGGL((k_hr_sweep_c<(P <= 2 ? P : 2), 0>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, U,
+                         s.work);
+      hipLaunchKernelGGL((k_hr_sweep_c<(P <= 2 ? P : 2), 1>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors,
+                         s.work, U);
     } else {
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, s.t.Lag2MonoX,
                          s.t.MonoX2MonoY, U, s.work);
