@@ -263,3 +263,37 @@ def test_limiter_transforms(order):
                 assert np.allclose(ay[sl], ey, atol=1e-13)
             assert np.allclose(MY2L @ (L2MY @ u), u, atol=1e-13)
             assert np.allclose(MX2L @ (L2MX @ u), u, atol=1e-13)
+
+
+def _header_tables():
+    """The constant transforms compiled into the HR kernels (hr_tables.h),
+    read from the header's initialisers."""
+    import os
+    import re
+
+    path = os.path.join(os.path.dirname(__file__), "..", "dg_amd", "csrc", "hr_tables.h")
+    text = open(path).read()
+    out = {}
+    for p, body in re.findall(r"struct HrTables2D<(\d)> \{(.*?)\n\};", text, re.S):
+        for name, vals in re.findall(r"static constexpr double (\w+)\[\d+\] = \{(.*?)\};", body, re.S):
+            out[(int(p), name)] = np.array([float(v) for v in vals.replace("\n", " ").split(",")])
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_constant_hr_transforms_equal_the_built_tables(order):
+    """The HR kernels carry Lag2MonoX, MonoX2MonoY and MonoY2Lag as constants;
+    the solver compares the built tables with them before it uses those
+    kernels.  They must be the built tables exactly (the mesh-independent
+    reference-element transforms), or the constant-table kernels would go
+    unused."""
+    tabs = _header_tables()
+    for n, ext in (((5, 4), (0.0, 1.5, -0.5, 0.3)), ((3, 7), (-2.0, 1.0, 0.0, 4.0))):
+        t = mesh_for(2, order, n=n, ext=ext).tables()
+        for key, name in (("Lag2MonoX", "L2MX"), ("MonoX2MonoY", "MX2MY"), ("MonoY2Lag", "MY2L")):
+            built = np.asarray(t[key], dtype=np.float64).ravel()
+            const = tabs[(order, name)]
+            assert built.shape == const.shape
+            assert np.array_equal(built, const), (key, order)
+            # zero entries are exact zeros (the kernels leave those terms out)
+            assert np.array_equal(built == 0.0, const == 0.0)
