@@ -967,11 +967,11 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
       hipLaunchKernelGGL((k_apply<Ns>), g, b, 0, st, s.t.Lag2Mono, U, s.work, ncol);
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, nullptr,
                          s.t.Mono2Lag, s.work, U);
-    } else if (P <= 2 && s.hr_const) {
-      hipLaunchKernelGGL((k_hr_sweep_c<(P <= 2 ? P : 2), 0>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, U,
-                         s.work);
-      hipLaunchKernelGGL((k_hr_sweep_c<(P <= 2 ? P : 2), 1>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors,
-                         s.work, U);
+    } else if (s.hr_const) {
+      if constexpr (D == 2 && P <= 2) {
+        hipLaunchKernelGGL((k_hr_sweep_c<P, 0>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, U, s.work);
+        hipLaunchKernelGGL((k_hr_sweep_c<P, 1>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, s.work, U);
+      }
     } else {
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, s.t.Lag2MonoX,
                          s.t.MonoX2MonoY, U, s.work);
