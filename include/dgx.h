@@ -200,7 +200,10 @@ dgx_status dgx_solver_set_precision(dgx_solver* s, int precision);
 int dgx_solver_path(const dgx_solver* s);
 /* 1 when the 2D global HR limiter runs as one launch (dgx_limit_into, and
  * the RK driver's limiter): 2D p <= 2 Cartesian grids of single-fluid or
- * two-fluid states whose neighbour table follows the grid; else 0. */
+ * two-fluid states whose neighbour table follows the grid and whose
+ * Lagrange/Taylor transforms are the standard ones of the Gmsh element (the
+ * kernel carries them as constants; every mesh the library builds or reads
+ * has them); else 0. */
 int dgx_solver_hr_one_launch(const dgx_solver* s);
 
 /* ------------------------------------------------- batched 1D ensemble */
