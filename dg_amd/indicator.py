@@ -347,3 +347,24 @@ def multimode_interface_state(mesh: Mesh, modes: int = 8, mach: float = 1.22, se
     E = p / (gam - 1.0) + 0.5 * rho * u * u
     f = np.stack([rho, rho * u, np.zeros_like(rho), E, 1.0 / (gam - 1.0)], axis=1)
     return np.ascontiguousarray(f.reshape(-1))
+
+
+def _profile_pass(nx: int = 1024, ny: int = 512, passes: int = 4) -> None:
+    """The flagging pass alone on the shock-bubble state, for profilers: a
+    seeded untrained network (the pass costs the same whatever the weights)
+    and no training, so that a counter run sees a handful of dispatches."""
+    import torch
+
+    mesh = problems.shock_bubble_mesh(nx, ny, 2)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_indicator(Indicator(seed=0), 0.0)
+    U = torch.from_numpy(problems.shock_bubble_state(mesh)).to("cuda")
+    flags = torch.zeros(mesh.sizes["N_E"], dtype=torch.int32, device="cuda")
+    for _ in range(passes):
+        solver.indicator_flags(U, flags)
+    torch.cuda.synchronize()
+    print(f"indicator pass x{passes} on {nx}x{ny}: {int(flags.sum().item())} cells flagged", flush=True)
+
+
+if __name__ == "__main__":
+    _profile_pass()
