@@ -12,7 +12,7 @@ import torch
 
 import oracle
 from dg_amd import (FARFIELD, LIMIT_HR, LIMIT_HRI, MULTIFLUID, PERIODIC, REFLECTIVE, ROE, RUSANOV, SINGLEFLUID,
-                    Config, Solver, problems)
+                    Config, Mesh, Solver, problems)
 from tests.helpers import CONFIGS, CONFIGS_PS, cfg_id, per_field_rel, rel_err, small_mesh
 
 pytestmark = pytest.mark.gpu
@@ -233,3 +233,38 @@ def test_passive_stiffened_limited_trajectories(device, c):
         solver.rk4_integrate(Ud, 0, dt, nsteps=10, limiter=lim)
         for l2, li in per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s):
             assert l2 < 1e-11 and li < 1e-11, (lim, l2, li)
+
+
+@pytest.mark.parametrize("limiter", [LIMIT_HR, LIMIT_HRI])
+@pytest.mark.parametrize("path", ["tiled", "general"])
+def test_limited_two_fluid_conserves_totals(device, limiter, path):
+    """A periodic two-fluid oblique Riemann problem over 20 limited RK4 steps
+    (global HR: the one-launch kernel on the tile path, two sweeps on the
+    face + element path; sensor-gated HR): the totals of rho, rho u, rho v and
+    E stay constant to round-off, since the HR limiter keeps every element's
+    mean and periodic faces cancel.  (1/(gamma-1) is advected in
+    non-conservative form and is not a conserved total.)"""
+    from dg_amd import PATH_GENERAL, PATH_TILED
+
+    mesh = Mesh(2, 2, 64, 32, 0.0, 2.0, 0.0, 1.0, bc=(PERIODIC,) * 4, nfields=5)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_path(PATH_TILED if path == "tiled" else PATH_GENERAL)
+    solver.set_sensor_thresholds((0.01, 0.01, 0.01))
+    if path == "tiled" and limiter == LIMIT_HR:
+        assert solver.hr_one_launch
+    s = mesh.sizes
+    w = mesh.array("weight")
+    phi = mesh.array("phi").reshape(s["N_s"], s["N_G"]).T
+    J = mesh.array("J")
+
+    def totals(U):
+        return np.einsum("g,gi,efi,e->f", w, phi, U.reshape(s["N_E"], 5, s["N_s"]), J)[:4]
+
+    U0 = problems.oblique_riemann_state(mesh, MULTIFLUID, seed=11)
+    Ud = dev(U0, device)
+    solver.rk4_integrate(Ud, 0, 0.2 * mesh.dx / 5, nsteps=20, limiter=limiter)
+    U1 = host(Ud)
+    assert not np.array_equal(U1, U0)
+    t0, t1 = totals(U0), totals(U1)
+    scale = np.abs(np.einsum("g,gi,efi,e->f", w, np.abs(phi), np.abs(U0.reshape(s["N_E"], 5, s["N_s"])), J)[:4])
+    assert np.all(np.abs(t1 - t0) <= 1e-12 * scale), (t0, t1)
