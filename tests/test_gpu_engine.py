@@ -235,21 +235,24 @@ def test_passive_stiffened_limited_trajectories(device, c):
             assert l2 < 1e-11 and li < 1e-11, (lim, l2, li)
 
 
-@pytest.mark.parametrize("limiter", [LIMIT_HR, LIMIT_HRI])
+@pytest.mark.parametrize("limiter", [LIMIT_HR, LIMIT_HRI, 8])
 @pytest.mark.parametrize("path", ["tiled", "general"])
 def test_limited_two_fluid_conserves_totals(device, limiter, path):
     """A periodic two-fluid oblique Riemann problem over 20 limited RK4 steps
     (global HR: the one-launch kernel on the tile path, two sweeps on the
-    face + element path; sensor-gated HR): the totals of rho, rho u, rho v and
-    E stay constant to round-off, since the HR limiter keeps every element's
-    mean and periodic faces cancel.  (1/(gamma-1) is advected in
-    non-conservative form and is not a conserved total.)"""
-    from dg_amd import PATH_GENERAL, PATH_TILED
+    face + element path; sensor-gated HR; HR on the cells an untrained
+    network flags, limiter 8): the totals of rho, rho u, rho v and E stay
+    constant to round-off, since the HR limiter keeps every element's mean
+    and periodic faces cancel.  (1/(gamma-1) is advected in non-conservative
+    form and is not a conserved total.)"""
+    from dg_amd import PATH_GENERAL, PATH_TILED, Indicator
 
     mesh = Mesh(2, 2, 64, 32, 0.0, 2.0, 0.0, 1.0, bc=(PERIODIC,) * 4, nfields=5)
     solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
     solver.set_path(PATH_TILED if path == "tiled" else PATH_GENERAL)
     solver.set_sensor_thresholds((0.01, 0.01, 0.01))
+    if limiter == 8:
+        solver.set_indicator(Indicator(seed=3), 0.0)
     if path == "tiled" and limiter == LIMIT_HR:
         assert solver.hr_one_launch
     s = mesh.sizes
