@@ -159,3 +159,20 @@ def test_path_selection(device):
     assert solver.path == PATH_GENERAL
     mesh = tiled_mesh(1, 2, ph, BCS_1D[0], 200)
     assert Solver(Config(1, 2, ph, ROE), mesh).path == PATH_GENERAL
+
+
+def test_one_launch_hr_full_size_equals_two_sweeps(device):
+    """At the shock-bubble's full size (1024 x 512, p = 2, two-fluid, walls
+    above and below), the one-launch limiter (dgx_limit_into) and the
+    in-place two-sweep limiter (dgx_limit) give the same bits."""
+    mesh = problems.shock_bubble_mesh(1024, 512, 2)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    assert solver.hr_one_launch
+    U = dev(problems.shock_bubble_state(mesh), device)
+    # a few unlimited steps, so that every mode of the state is non-trivial
+    solver.rk4_integrate(U, 0, 2e-4, nsteps=3)
+    out = torch.full_like(U, float("nan"))
+    solver.limit_into(LIMIT_HR, U, out)
+    solver.limit(LIMIT_HR, U)
+    torch.cuda.synchronize()
+    assert torch.equal(out, U)
