@@ -519,7 +519,7 @@ static dgx_status timed_limit(Solver& s, int limiter, double* U, hipStream_t st)
 #ifndef DGX_RK_HR2D
 // 1: on grids where the one-launch HR kernel applies (Solver::hr2d) the RK
 // driver limits out of place with it; 0: the two sweep launches in place.
-// On the 1024x512 p=2 two-fluid grid: 198 us against 218 us per application
+// On the 1024x512 p=2 two-fluid grid: 166 us against 212 us per application
 // (the kernel computes each element's Taylor-x coefficients once per tile
 // and keeps the intermediate state in LDS).
 #define DGX_RK_HR2D 1

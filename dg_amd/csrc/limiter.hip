@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, co
 // Every column sees exactly the arithmetic of the two k_hr_sweep launches,
 // so the result is bitwise the same, and the intermediate state never
 // reaches HBM.  In and Out must differ: neighbouring tiles read In's rows.
-template <int Ns, int N, int SL, int NF, int TXE, int TYE, int BS>
+template <int Ns, int N, int SL, int NF, int TXE, int TYE, int BS, bool OUT_VEC2>
 __global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restrict__ nb,
                                              const double* __restrict__ In, double* __restrict__ Out) {
   constexpr int ROWS = TYE + 2, W = TXE + 2, EL = NF * Ns, ROW_OUT = TXE * EL;
@@ -356,10 +356,21 @@ __global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restr
     for (int l = 0; l < Ns; ++l) o[l] = xs[l];
   }
   __syncthreads();
-  const double2* so = reinterpret_cast<const double2*>(sA);
-  for (int q = threadIdx.x; q < TYE * ROW_OUT / 2; q += BS) {
-    const int j = q / (ROW_OUT / 2), r = q - j * (ROW_OUT / 2);
-    reinterpret_cast<double2*>(Out + ((long)(ey0 + j) * nx + ex0) * EL)[r] = so[q];
+  // pairs of doubles when Out is 16-byte aligned (every row start then is:
+  // nx and ex0 are multiples of TXE), else single doubles.  A template
+  // parameter, not a run-time branch: the branch cost two VGPRs, one wave
+  // per SIMD and 25 % of the time.
+  if constexpr (OUT_VEC2) {
+    const double2* so = reinterpret_cast<const double2*>(sA);
+    for (int q = threadIdx.x; q < TYE * ROW_OUT / 2; q += BS) {
+      const int j = q / (ROW_OUT / 2), r = q - j * (ROW_OUT / 2);
+      reinterpret_cast<double2*>(Out + ((long)(ey0 + j) * nx + ex0) * EL)[r] = so[q];
+    }
+  } else {
+    for (int q = threadIdx.x; q < TYE * ROW_OUT; q += BS) {
+      const int j = q / ROW_OUT, r = q - j * ROW_OUT;
+      Out[((long)(ey0 + j) * nx + ex0) * EL + r] = sA[q];
+    }
   }
 }
 
@@ -999,8 +1010,12 @@ hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStre
       constexpr int Ns = Shape<D, P>::Ns, SL = P + 1;
       auto go = [&](auto nf, auto ty) -> hipError_t {
         constexpr int NF = decltype(nf)::value, TYE = decltype(ty)::value, BS = TYE == 4 ? 256 : 512;
-        hipLaunchKernelGGL((k_hr2d<Ns, P, SL, NF, TXE, TYE, BS>), dim3(tiles), dim3(BS), 0, st, s.cart.nx, s.N_N,
-                           s.t.neighbors, In, Out);
+        if ((reinterpret_cast<uintptr_t>(Out) & 15) == 0)
+          hipLaunchKernelGGL((k_hr2d<Ns, P, SL, NF, TXE, TYE, BS, true>), dim3(tiles), dim3(BS), 0, st, s.cart.nx,
+                             s.N_N, s.t.neighbors, In, Out);
+        else
+          hipLaunchKernelGGL((k_hr2d<Ns, P, SL, NF, TXE, TYE, BS, false>), dim3(tiles), dim3(BS), 0, st, s.cart.nx,
+                             s.N_N, s.t.neighbors, In, Out);
         return hipGetLastError();
       };
       using I4 = std::integral_constant<int, 4>;

@@ -176,3 +176,30 @@ def test_one_launch_hr_full_size_equals_two_sweeps(device):
     solver.limit(LIMIT_HR, U)
     torch.cuda.synchronize()
     assert torch.equal(out, U)
+
+
+def test_one_launch_hr_output_not_16_byte_aligned(device):
+    """The one-launch limiter stores pairs of doubles only into a 16-byte
+    aligned output; an output (or an RK state) that starts 8 bytes past such a
+    boundary takes single stores and the same bits."""
+    mesh = tiled_mesh(2, 2, MULTIFLUID, BCS_2D[0], (16, 8))
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    assert solver.hr_one_launch
+    U0 = problems.oblique_riemann_state(mesh, MULTIFLUID, seed=7)
+    U_ref = prob.limit_hr(U0)
+    Ud = dev(U0, device)
+    big = torch.full((U0.size + 1,), float("nan"), dtype=torch.float64, device=device)
+    out = big[1:]
+    assert out.data_ptr() % 16 == 8
+    solver.limit_into(LIMIT_HR, Ud, out)
+    assert np.array_equal(host(out), U_ref)
+    # the RK driver's last limiter application writes the caller's state
+    dt = 0.3 * prob.dt_from_cfl(U0, mesh.dx / 5)
+    Ua = dev(U0, device)
+    solver.rk4_integrate(Ua, 0, dt, nsteps=3, limiter=LIMIT_HR)
+    big2 = torch.zeros(U0.size + 1, dtype=torch.float64, device=device)
+    Ub = big2[1:]
+    Ub.copy_(dev(U0, device))
+    solver.rk4_integrate(Ub, 0, dt, nsteps=3, limiter=LIMIT_HR)
+    assert torch.equal(Ua, Ub)
