@@ -255,9 +255,10 @@ __global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, co
 //      halo rows, from LDS into LDS;
 //   4. the y sweep (hrl along y, MonoY2Lag) of the tile rows, staged in LDS
 //      and stored row by row (a tile row is TXE*NF*Ns contiguous values).
-// Every column sees exactly the arithmetic of the two k_hr_sweep launches,
-// so the result is bitwise the same, and the intermediate state never
-// reaches HBM.  In and Out must differ: neighbouring tiles read In's rows.
+// Every column gets the bits of the two k_hr_sweep launches (the products
+// with zero table entries that it leaves out change no sum), and the
+// intermediate state never reaches HBM.  In and Out must differ:
+// neighbouring tiles read In's rows.
 template <int Ns, int N, int SL, int NF, int TXE, int TYE, int BS, bool OUT_VEC2>
 __global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restrict__ nb,
                                              const double* __restrict__ In, double* __restrict__ Out) {
