@@ -39,6 +39,9 @@ class Config(ctypes.Structure):
 
 
 _lib = None
+# dgx_output_fn: (user, count, step, time, U, stream) -> status
+OUTPUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                             ctypes.c_void_p, ctypes.c_void_p)
 
 
 def lib() -> ctypes.CDLL:
@@ -78,6 +81,9 @@ def lib() -> ctypes.CDLL:
         L.dgx_dt_from_cfl.argtypes = [vp, vp, dp, vp, vp]
         L.dgx_rk4_step.argtypes = [vp, vp, vp, i, vp]
         L.dgx_rk4_integrate.argtypes = [vp, vp, i, dp, i, dp, i, ip, ctypes.POINTER(dp), vp]
+        L.dgx_rk4_integrate_out.argtypes = [vp, vp, dp, ctypes.POINTER(dp), i, i, OUTPUT_FN, vp, ip,
+                                            ctypes.POINTER(dp), vp]
+        L.dgx_conservation.argtypes = [vp, vp, vp, vp]
         L.dgx_set_config.argtypes = [ctypes.POINTER(Config)]
         L.dgx_batch_create.argtypes = [ctypes.POINTER(Config), i, ip, ip, dp, dp, ip, ctypes.POINTER(vp)]
         L.dgx_batch_free.argtypes = [vp]
@@ -297,6 +303,32 @@ class Solver:
         _check(lib().dgx_rk4_integrate(self._h, _ptr(U), mode, dt_or_cfl, nsteps, tf, limiter, ctypes.byref(steps),
                                        ctypes.byref(tfin), _stream(stream)), "dgx_rk4_integrate")
         return steps.value, tfin.value
+
+    def rk4_integrate_out(self, U, cfl: float, out_times, limiter: int = 0, on_output=None, stream=None):
+        """CFL-mode RK4 with the reference's output times (the last is the final
+        time): Dt is clamped to each of them; on_output(count, step, time) is
+        called after each output step (the stream synchronised; U holds the
+        state).  Returns (steps, final time)."""
+        outs = (ctypes.c_double * len(out_times))(*out_times)
+
+        def cb(user, count, step, time, Uptr, st):
+            try:
+                r = on_output(count, step, time) if on_output else 0
+                return int(r or 0)
+            except Exception:
+                return -1
+
+        fn = OUTPUT_FN(cb)
+        steps = ctypes.c_int()
+        tfin = ctypes.c_double()
+        _check(lib().dgx_rk4_integrate_out(self._h, _ptr(U), cfl, outs, len(out_times), limiter, fn, None,
+                                           ctypes.byref(steps), ctypes.byref(tfin), _stream(stream)),
+               "dgx_rk4_integrate_out")
+        return steps.value, tfin.value
+
+    def conservation(self, U, totals, stream=None) -> None:
+        """DG_SOLVER::conservation: the integral of each field, into totals (device float64[N_F])."""
+        _check(lib().dgx_conservation(self._h, _ptr(U), _ptr(totals), _stream(stream)), "dgx_conservation")
 
     def set_path(self, path: int) -> None:
         """PATH_AUTO, PATH_GENERAL (face + element kernels) or PATH_TILED (one fused kernel per stage)."""

@@ -349,7 +349,8 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
             upload(s, &s.t.J, J.data(), uniform_geo ? 1 : J.size()) &&
             upload(s, &s.t.invJac, ij.data(), uniform_geo ? (size_t)D * D : ij.size()) &&
             upload(s, &s.t.Minv, t.Minv.data(), uniform_minv ? (size_t)Ns * Ns : t.Minv.size()) &&
-            upload(s, &s.t.neighbors, t.neighbors.data(), t.neighbors.size());
+            upload(s, &s.t.neighbors, t.neighbors.data(), t.neighbors.size()) &&
+            upload(s, &s.t.weight, t.weight.data(), t.weight.size());
   if (ok && s.cart.on)
     ok = upload(s, &s.t.face_geo_x, s.cart.gx.data(), s.cart.gx.size()) &&
          upload(s, &s.t.face_geo_y, s.cart.gy.data(), s.cart.gy.size());
@@ -392,8 +393,8 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
        alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
        alloc(s, &s.sensors, (size_t)NE) && alloc(s, &s.sens_tgt, (size_t)2 * NE) &&
        alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.flag_list, (size_t)NE) &&
-       alloc(s, &s.dscratch, 8) &&
-       alloc(s, &s.dt_partial, ((size_t)NE * Ns + 255) / 256) &&
+       alloc(s, &s.dscratch, 8) && alloc(s, &s.cons_partial, conservation_scratch_doubles()) &&
+       alloc(s, &s.dt_partial, ((size_t)NE * Ns + 255) / 256 + 1) &&
        (!s.cart.on || alloc(s, &s.Ustar2, S));
   if (ok) ok = hip_ok(hipMemset(s.sensors, 0, sizeof(int) * NE), "hipMemset");
   if (!ok) {
@@ -500,6 +501,14 @@ dgx_status dgx_pos_read(const dgx_mesh* m, int physics, const char* prefix, int 
   return err.empty() ? DGX_OK : fail(DGX_EINVAL, "dgx_pos_read: " + err);
 }
 
+dgx_status dgx_conservation(dgx_solver* h, const double* U, double* totals, void* stream) {
+  if (!h || !U || !totals) return fail(DGX_EINVAL, "dgx_conservation: null argument");
+  const Solver& s = h->s;
+  if (s.N_G > 25 || s.N_s > 16) return fail(DGX_EUNSUPPORTED, "dgx_conservation: element larger than p=3");
+  return hip_ok(launch_conservation(s, U, totals, s.cons_partial, as_stream(stream)), "conservation") ? DGX_OK
+                                                                                                      : DGX_EDEVICE;
+}
+
 dgx_status dgx_sensing(dgx_solver* h, const double* U, int* sensors, double* Uavg, void* stream) {
   if (!h || !U || !sensors) return fail(DGX_EINVAL, "dgx_sensing: null argument");
   return hip_ok(launch_sensing(h->s, U, sensors, Uavg, as_stream(stream)), "sensing") ? DGX_OK : DGX_EDEVICE;
@@ -580,6 +589,69 @@ dgx_status dgx_rk4_step(dgx_solver* h, double* U, const double* dt_dev, int limi
   return rk4_step(h->s, U, dt_dev, limiter, as_stream(stream));
 }
 
+// RK::RK_integration's CFL branch (rk.cc:100-233): the initial limit, then
+// per step Dt from CFL (one 8-byte read-back, as DtFromCFL's, rk.cc:265-268),
+// the stop on a tiny or NaN Dt (rk.cc:147-148), the clamps to the final time
+// and to the next output time (rk.cc:149-150), the step, and the output call
+// after a clamped step (rk.cc:215-233).  out_times are the reference's
+// _output_time_array[1..]; the last one is the final time.
+static dgx_status integrate_cfl(Solver& s, double* U, double cfl, int nsteps, const double* out_times, int n_out,
+                                int limiter, dgx_output_fn on_output, void* user, int* n_out_steps, double* T_out,
+                                hipStream_t st) {
+  double* dt_dev = s.dscratch + 1;
+  dgx_status r;
+  int n = 0, count = 0;
+  double T = 0;
+  const double tf = out_times[n_out - 1];
+  auto finish = [&]() {
+    *n_out_steps = n;
+    *T_out = T;
+  };
+  if ((r = do_limit(s, limiter, U, st)) != DGX_OK) return r;
+  bool done = false;
+  while (!done && (nsteps <= 0 || n < nsteps)) {
+    {
+      KernelTimer kt(s, 3, st);
+      if (!hip_ok(launch_upa_max(s, U, cfl, dt_dev, st), "dt kernel")) return DGX_EDEVICE;
+    }
+    double dt = 0;
+    if (!hip_ok(hipMemcpyAsync(&dt, dt_dev, sizeof(double), hipMemcpyDeviceToHost, st), "dt readback") ||
+        !hip_ok(hipStreamSynchronize(st), "dt readback"))
+      return DGX_EDEVICE;
+    if (dt < 1e-14 || dt != dt) {
+      finish();
+      return fail(DGX_EDT, "time step too small or NaN (rk.cc:147-148)");
+    }
+    bool output = false;
+    const double tout = out_times[count < n_out ? count : n_out - 1];
+    if (dt > (tf - T)) {
+      dt = tf - T;
+      output = done = true;
+    } else if (dt > (tout - T)) {
+      dt = tout - T;
+      output = true;
+    }
+    if (output && (!hip_ok(hipMemcpyAsync(dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, st), "dt clamp") ||
+                   !hip_ok(hipStreamSynchronize(st), "dt clamp")))
+      return DGX_EDEVICE;
+    if ((r = rk4_step(s, U, dt_dev, limiter, st)) != DGX_OK) return r;
+    T = T + dt;
+    ++n;
+    if (output) {
+      ++count;
+      if (on_output && count <= n_out) {
+        if (!hip_ok(hipStreamSynchronize(st), "rk4 output")) return DGX_EDEVICE;
+        if ((r = on_output(user, count, n, T, U, st)) != DGX_OK) {
+          finish();
+          return r;
+        }
+      }
+    }
+  }
+  finish();
+  return DGX_OK;
+}
+
 dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cfl, int nsteps, double tf, int limiter,
                              int* steps, double* t_final, void* stream) {
   if (!h || !U) return fail(DGX_EINVAL, "dgx_rk4_integrate: null argument");
@@ -598,32 +670,11 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
       T = T + dt_or_cfl;
     }
   } else if (mode == 1) {
-    if ((r = do_limit(s, limiter, U, st)) != DGX_OK) return r;
-    bool done = false;
-    while (!done && (nsteps <= 0 || n < nsteps)) {
-      {
-        KernelTimer kt(s, 3, st);
-        if (!hip_ok(launch_upa_max(s, U, dt_or_cfl, dt_dev, st), "dt kernel")) return DGX_EDEVICE;
-      }
-      double dt = 0;
-      if (!hip_ok(hipMemcpyAsync(&dt, dt_dev, sizeof(double), hipMemcpyDeviceToHost, st), "dt readback") ||
-          !hip_ok(hipStreamSynchronize(st), "dt readback"))
-        return DGX_EDEVICE;
-      if (dt < 1e-14 || dt != dt) {
-        if (steps) *steps = n;
-        if (t_final) *t_final = T;
-        return fail(DGX_EDT, "time step too small or NaN (rk.cc:147-148)");
-      }
-      if (dt > (tf - T)) {
-        dt = tf - T;
-        done = true;
-        if (!hip_ok(hipMemcpyAsync(dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, st), "dt clamp") ||
-            !hip_ok(hipStreamSynchronize(st), "dt clamp"))
-          return DGX_EDEVICE;
-      }
-      if ((r = rk4_step(s, U, dt_dev, limiter, st)) != DGX_OK) return r;
-      T = T + dt;
-      ++n;
+    r = integrate_cfl(s, U, dt_or_cfl, nsteps, &tf, 1, limiter, nullptr, nullptr, &n, &T, st);
+    if (r != DGX_OK) {
+      if (steps) *steps = n;
+      if (t_final) *t_final = T;
+      return r;
     }
   } else {
     return fail(DGX_EINVAL, "dgx_rk4_integrate: mode must be 0 (fixed Dt) or 1 (CFL)");
@@ -632,6 +683,23 @@ dgx_status dgx_rk4_integrate(dgx_solver* h, double* U, int mode, double dt_or_cf
   if (steps) *steps = n;
   if (t_final) *t_final = T;
   return DGX_OK;
+}
+
+dgx_status dgx_rk4_integrate_out(dgx_solver* h, double* U, double cfl, const double* out_times, int n_out,
+                                 int limiter, dgx_output_fn on_output, void* user, int* steps, double* t_final,
+                                 void* stream) {
+  if (!h || !U || !out_times || n_out < 1) return fail(DGX_EINVAL, "dgx_rk4_integrate_out: bad arguments");
+  for (int k = 0; k < n_out; ++k)
+    if (!(out_times[k] > (k ? out_times[k - 1] : 0.0)))
+      return fail(DGX_EINVAL, "dgx_rk4_integrate_out: output times must increase from above 0");
+  hipStream_t st = as_stream(stream);
+  int n = 0;
+  double T = 0;
+  dgx_status r = integrate_cfl(h->s, U, cfl, 0, out_times, n_out, limiter, on_output, user, &n, &T, st);
+  if (r == DGX_OK && !hip_ok(hipStreamSynchronize(st), "rk4 integrate")) r = DGX_EDEVICE;
+  if (steps) *steps = n;
+  if (t_final) *t_final = T;
+  return r;
 }
 
 // Tensor-Legendre transform of the 2D p=2 element: V[i][a*3+b] =

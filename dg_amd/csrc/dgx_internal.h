@@ -45,6 +45,7 @@ struct DevTables {
   // learned indicator (2D p=2): tensor-Legendre modes from nodal values,
   // Vinv[q*N_s + i]; row 0 gives the cell mean
   double* mlp_vinv = nullptr;
+  double* weight = nullptr;     // volume quadrature weights [g] (conservation totals)
   // limiter / sensor
   int* neighbors = nullptr;     // [e*N_N + {L,R,D,U}]
   double *Lag2Mono = nullptr, *Mono2Lag = nullptr;
@@ -91,7 +92,8 @@ struct Solver {
   int* flag_list = nullptr; // [N_E] flagged elements (launch_limit_flagged)
   double* m2buf = nullptr;  // m2 limiting: 15 per-field arrays of N_E*N_s (allocated on first use)
   double* dscratch = nullptr;  // device scalars: [1] Dt
-  double* dt_partial = nullptr;  // per-block maxima of |u|+a (ceil(N_E*N_s/256))
+  double* cons_partial = nullptr;  // conservation totals: per-block double-double partial sums
+  double* dt_partial = nullptr;  // per-block maxima of |u|+a (ceil(N_E*N_s/256)), then node 0's |u|+a
   int n_thresh = 0;
   double thresh[3] = {0, 0, 0};
   std::vector<void*> allocations;
@@ -139,6 +141,10 @@ hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st);
 hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool with_list = false);
 // solution output and restart (output.hip): PRINTER formatter, .pos files
 hipError_t launch_format(const Solver& s, double* U, double* out, int inverse, hipStream_t st);
+// DG_SOLVER::conservation totals of U into totals (device double[NF]); scratch holds
+// conservation_scratch_doubles() doubles
+hipError_t launch_conservation(const Solver& s, const double* U, double* totals, double* scratch, hipStream_t st);
+size_t conservation_scratch_doubles();
 std::string pos_write(const Tables& t, int physics, const double* out, const std::string& prefix, int step,
                       double time);
 std::string pos_read(const Tables& t, int physics, const std::string& prefix, int step, double* time, double* out);

@@ -115,6 +115,129 @@ hipError_t launch_format(const Solver& s, double* U, double* out, int inverse, h
   return hipGetLastError();
 }
 
+// DG_SOLVER::conservation (dg_solver.h:367-389): I[fc] = sum over elements e
+// and Gauss points g of Ug[(e,fc),g]*J_e*w_g, with Ug = phi*U as a netlib
+// gemm (each point's sum over nodes in node order, no contraction).  The
+// reference adds the N_E*N_G terms one after another into one double; here
+// every term has the reference's bits and the terms are summed as
+// double-double values (TwoSum, then the low parts) in a fixed tree: each
+// thread its elements in order, the block in LDS, the blocks in one final
+// block.  The result is the exact sum of the terms to within about one
+// rounding, deterministic for a given N_E.
+struct DD {
+  double hi, lo;
+};
+__device__ inline DD dd_add(DD a, double x) {
+  const double s = a.hi + x, bp = s - a.hi;
+  const double err = (a.hi - (s - bp)) + (x - bp);
+  return DD{s, a.lo + err};
+}
+__device__ inline DD dd_add(DD a, DD b) {
+  DD r = dd_add(a, b.hi);
+  r.lo = r.lo + b.lo;
+  return r;
+}
+
+constexpr int kConsThreads = 256;
+constexpr int kConsMaxBlocks = 1024;
+constexpr int kConsMaxNF = 6;
+
+template <int NF>
+__global__ __launch_bounds__(kConsThreads) void k_conservation(int N_s, int N_G, int N_E,
+                                                               const double* __restrict__ U,
+                                                               const double* __restrict__ phi,
+                                                               const double* __restrict__ J, int geo_stride,
+                                                               const double* __restrict__ weight,
+                                                               double* __restrict__ partial) {
+  __shared__ double sPhi[25 * 16], sW[25];
+  __shared__ double sHi[kConsThreads], sLo[kConsThreads];
+  for (int k = threadIdx.x; k < N_G * N_s; k += blockDim.x) sPhi[k] = phi[k];
+  for (int k = threadIdx.x; k < N_G; k += blockDim.x) sW[k] = weight[k];
+  __syncthreads();
+  DD acc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] = DD{0.0, 0.0};
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < N_E; e += gridDim.x * blockDim.x) {
+    const double Je = J[(long)e * geo_stride];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double* u = U + ((long)e * NF + f) * N_s;
+      for (int g = 0; g < N_G; ++g) {
+        double ug = 0.0;
+        for (int l = 0; l < N_s; ++l) ug = ug + u[l] * sPhi[g + l * N_G];
+        acc[f] = dd_add(acc[f], ug * Je * sW[g]);
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    sHi[threadIdx.x] = acc[f].hi;
+    sLo[threadIdx.x] = acc[f].lo;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        const DD r = dd_add(DD{sHi[threadIdx.x], sLo[threadIdx.x]}, DD{sHi[threadIdx.x + w], sLo[threadIdx.x + w]});
+        sHi[threadIdx.x] = r.hi;
+        sLo[threadIdx.x] = r.lo;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partial[((long)blockIdx.x * NF + f) * 2] = sHi[0];
+      partial[((long)blockIdx.x * NF + f) * 2 + 1] = sLo[0];
+    }
+    __syncthreads();
+  }
+}
+
+template <int NF>
+__global__ __launch_bounds__(kConsThreads) void k_conservation_final(int nblocks, const double* __restrict__ partial,
+                                                                     double* __restrict__ totals) {
+  __shared__ double sHi[kConsThreads], sLo[kConsThreads];
+  for (int f = 0; f < NF; ++f) {
+    DD acc{0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+      acc = dd_add(acc, DD{partial[((long)b * NF + f) * 2], partial[((long)b * NF + f) * 2 + 1]});
+    sHi[threadIdx.x] = acc.hi;
+    sLo[threadIdx.x] = acc.lo;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        const DD r = dd_add(DD{sHi[threadIdx.x], sLo[threadIdx.x]}, DD{sHi[threadIdx.x + w], sLo[threadIdx.x + w]});
+        sHi[threadIdx.x] = r.hi;
+        sLo[threadIdx.x] = r.lo;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[f] = sHi[0] + sLo[0];
+    __syncthreads();
+  }
+}
+
+int conservation_blocks(int N_E) {
+  const int nb = (N_E + kConsThreads - 1) / kConsThreads;
+  return nb < kConsMaxBlocks ? nb : kConsMaxBlocks;
+}
+size_t conservation_scratch_doubles() { return (size_t)kConsMaxBlocks * kConsMaxNF * 2; }
+
+hipError_t launch_conservation(const Solver& s, const double* U, double* totals, double* scratch, hipStream_t st) {
+  if (s.N_G > 25 || s.N_s > 16 || s.NF > kConsMaxNF) return hipErrorInvalidValue;
+  const int nb = conservation_blocks(s.N_E);
+  auto go = [&](auto nf) {
+    constexpr int NF = decltype(nf)::value;
+    hipLaunchKernelGGL((k_conservation<NF>), dim3(nb), dim3(kConsThreads), 0, st, s.N_s, s.N_G, s.N_E, U, s.t.phi,
+                       s.t.J, s.t.geo_stride, s.t.weight, scratch);
+    hipLaunchKernelGGL((k_conservation_final<NF>), dim3(1), dim3(kConsThreads), 0, st, nb, scratch, totals);
+  };
+  switch (s.NF) {
+    case 3: go(std::integral_constant<int, 3>{}); break;
+    case 4: go(std::integral_constant<int, 4>{}); break;
+    case 5: go(std::integral_constant<int, 5>{}); break;
+    default: go(std::integral_constant<int, 6>{}); break;
+  }
+  return hipGetLastError();
+}
+
 // PRINTER::set_names (printer.cc:11-80): printed field names and file names
 void pos_field_names(int D, int physics, std::vector<std::string>& names, std::vector<std::string>& fnames) {
   names = {"Rho", "Ux"};

@@ -342,18 +342,23 @@ __global__ __launch_bounds__(256) void k_elem_mfma(ElemArgs a) {
 // ---------------------------------------------------------------------------
 // Dt from CFL: max over nodes of |u|+a, then Dt = cfl / max (rk.cc:250-272).
 // Two levels: per-block maxima, then one block reduces them (no atomics).
-// The max keeps NaNs, so a NaN state gives a NaN Dt and the driver stops.
+// The reference takes the node with BLAS idamax, which starts from node 0 and
+// moves on only to a strictly larger |x|: a NaN (negative pressure, upa.h)
+// after node 0 is passed over, a NaN at node 0 is kept.  Here the maxima pass
+// over NaNs (b > a is false for a NaN b; every |u|+a is >= 0, so starting
+// from 0 changes nothing) and node 0's own value, stored after the per-block
+// maxima, turns Dt into NaN when it is NaN: the same Dt as the reference's.
 // ---------------------------------------------------------------------------
-__device__ inline double nan_max(double a, double b) { return (b != b || b > a) ? b : a; }
+__device__ inline double skip_nan_max(double a, double b) { return b > a ? b : a; }
 
 __device__ inline double block_max(double v, double* red) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off));
+  for (int off = 32; off > 0; off >>= 1) v = skip_nan_max(v, __shfl_xor(v, off));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) red[w] = v;
   __syncthreads();
   double m = red[0];
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = nan_max(m, red[k]);
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = skip_nan_max(m, red[k]);
   return m;
 }
 
@@ -371,17 +376,20 @@ __global__ __launch_bounds__(256) void k_upa_max(const double* __restrict__ U, i
     for (int f = 0; f < NF; ++f) u[f] = U[(e * NF + f) * Ns + i];
     v = upa<D, PH>(u);
   }
+  if (n == 0) partial[gridDim.x] = v;  // node 0, where idamax starts
   const double m = block_max(v, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
+// partial[0..n-1]: per-block maxima; partial[n]: |u|+a of node 0.
 __global__ __launch_bounds__(256) void k_dt_finalize(const double* __restrict__ partial, int n, double cfl,
                                                      double* dt) {
   __shared__ double red[4];
   double v = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) v = nan_max(v, partial[k]);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v = skip_nan_max(v, partial[k]);
   const double m = block_max(v, red);
-  if (threadIdx.x == 0) dt[0] = cfl / m;
+  const double first = partial[n];
+  if (threadIdx.x == 0) dt[0] = cfl / (first != first ? first : m);
 }
 
 // ---------------------------------------------------------------------------

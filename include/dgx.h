@@ -161,6 +161,12 @@ dgx_status dgx_pos_write(const dgx_mesh* m, int physics, const double* out, cons
                          double time);
 dgx_status dgx_pos_read(const dgx_mesh* m, int physics, const char* prefix, int step, double* time, double* out);
 
+/* DG_SOLVER::conservation (dg_solver.h:367-389): totals (device double[N_F]) = the integral of each conserved
+ * field over the mesh, sum over elements and Gauss points of (phi*U)_g * J_e * w_g.  Each term has the
+ * reference's bits; the terms are summed in double-double in a fixed order (deterministic, within about one
+ * rounding of their exact sum; the reference adds them one after another). */
+dgx_status dgx_conservation(dgx_solver* s, const double* U, double* totals, void* stream);
+
 /* SENSOR::sensing: flags (int[N_E], device) and Uavg (double[N_E*N_F], device; may be NULL). */
 dgx_status dgx_sensing(dgx_solver* s, const double* U, int* sensors, double* Uavg, void* stream);
 
@@ -179,6 +185,17 @@ dgx_status dgx_rk4_step(dgx_solver* s, double* U, const double* dt_dev, int limi
  * Returns the step count in *steps and the final time in *t_final. */
 dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
                              int limiter, int* steps, double* t_final, void* stream);
+/* RK::RK_integration's CFL branch with its output times (rk.cc:100-233, rk.h:58-83): out_times[0..n_out-1]
+ * are the reference's _output_time_array[1..] (entry 0 is the initial time), strictly increasing, the last
+ * being the final time.  Dt = CFL/max(|u|+a) is clamped to the final time (rk.cc:149) and to the next output
+ * time (rk.cc:150); after a clamped step on_output (may be NULL) is called with the output count (1-based, as
+ * the reference's count), the step count, the time and U, with the stream synchronised: it stands for
+ * PRINTER::print (rk.cc:215-233) and may call dgx_format / dgx_pos_write.  A non-zero return stops the run
+ * with that status. */
+typedef dgx_status (*dgx_output_fn)(void* user, int count, int step, double time, double* U, void* stream);
+dgx_status dgx_rk4_integrate_out(dgx_solver* s, double* U, double cfl, const double* out_times, int n_out,
+                                 int limiter, dgx_output_fn on_output, void* user, int* steps, double* t_final,
+                                 void* stream);
 
 /* Kernel path.  0 auto: the fused tile kernel (one launch per RK stage)
  * on uniform Cartesian meshes whose element counts are multiples of the tile

@@ -66,6 +66,8 @@ def lib() -> ctypes.CDLL:
         L.or_limit_flagged.argtypes = [cp, tp, vp, vp]
         L.or_rk4.argtypes = [cp, tp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                              vp, vp, vp, vp, ctypes.c_int]
+        L.or_rk4_out.argtypes = [cp, tp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, vp, ctypes.c_int,
+                                 ctypes.c_int, vp, vp, vp, vp, ctypes.c_int, vp]
         L.or_dgemm.argtypes = [ctypes.c_char, ctypes.c_char, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_double, vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int]
         for n in ("or_mapToFace", "or_evaluate_sf", "or_evaluate_q", "or_redistribute_sf", "or_redistribute_q",
@@ -90,6 +92,8 @@ def lib() -> ctypes.CDLL:
         L.or_limit_p0i.restype = None
         L.or_p0i.argtypes = [cp, vp, ctypes.c_double, vp, vp, vp]
         L.or_p0i.restype = None
+        L.or_conservation.argtypes = [cp, vp, vp, vp, vp, vp]
+        L.or_conservation.restype = None
         L.or_ref_area.argtypes = [cp]
         L.or_ref_area.restype = ctypes.c_double
         _lib = L
@@ -206,6 +210,15 @@ class Problem:
         lib().or_limit_p0i(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(on), _dptr(th), _dptr(U), _dptr(sens))
         return U, sens
 
+    def conservation(self, U: np.ndarray) -> np.ndarray:
+        """DG_SOLVER::conservation: the integral of each field over the mesh."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        w = np.ascontiguousarray(self.t["weight"], dtype=np.float64)
+        out = np.zeros(self.nf)
+        lib().or_conservation(ctypes.byref(self.cfg), _dptr(self._keep["phi"]), _dptr(self._keep["J"]), _dptr(w),
+                              _dptr(U), _dptr(out))
+        return out
+
     def format(self, U: np.ndarray) -> np.ndarray:
         """PRINTER formatter: the printed primitive fields of U, node by node."""
         U = np.array(U, dtype=np.float64, copy=True)
@@ -254,6 +267,21 @@ class Problem:
         lib().or_limit_flagged(ctypes.byref(self.cfg), ctypes.byref(self.tab), f.ctypes.data_as(ctypes.c_void_p),
                                _dptr(U))
         return U
+
+    def rk4_out(self, U: np.ndarray, cfl: float, out_times, limiter: int = 0, thresholds=(), nsteps: int = 0,
+                max_hist: int = 100000):
+        """CFL-mode RK4 with the reference's output times (the last one is the
+        final time); returns (U, steps, T, dt history, step count at each output)."""
+        U = np.array(U, dtype=np.float64, copy=True)
+        on, th = self._thresh(thresholds)
+        outs = np.ascontiguousarray(out_times, dtype=np.float64)
+        tout = ctypes.c_double()
+        hist = np.zeros(max_hist)
+        osteps = np.full(outs.size, -1, dtype=np.int32)
+        n = lib().or_rk4_out(ctypes.byref(self.cfg), ctypes.byref(self.tab), _dptr(U), 1, cfl, nsteps, _dptr(outs),
+                             outs.size, limiter, _dptr(on), _dptr(th), ctypes.byref(tout), _dptr(hist), max_hist,
+                             _dptr(osteps))
+        return U, n, tout.value, hist[:max(n, 0)], osteps
 
     def rk4(self, U: np.ndarray, mode: int, dt_or_cfl: float, nsteps: int = 0, tf: float = 0.0, limiter: int = 0,
             thresholds=(), max_hist: int = 100000):

@@ -1238,9 +1238,13 @@ void or_limit_hri(const or_cfg* c, const or_tables* t, const int* on, const doub
   }
 }
 
-// rk.cc:132-247 (no output, no particles, single rank)
-int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
-           int limiter, const int* on, const double* thresh, double* t_out, double* dt_hist, int max_hist) {
+// rk.cc:132-247 (no particles, single rank).  Output times: out_times[0..n_out-1] are the reference's
+// _output_time_array[1..] (rk.h:58-83; the initial time 0 is entry 0 there), the last one the final time;
+// out_steps[k] receives the step count after which output k happened (CFL mode; the printing itself is left
+// to the caller).
+int or_rk4_out(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps,
+               const double* out_times, int n_out, int limiter, const int* on, const double* thresh, double* t_out,
+               double* dt_hist, int max_hist, int* out_steps) {
   const double beta[4] = {0.0, 0.5, 0.5, 1.0};
   const double gam[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
   const size_t n = (size_t)c->N_s * c->N_E * c->N_F;
@@ -1258,9 +1262,12 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
   double T = 0;
   int steps = 0;
   bool done = false;
+  const double tf = n_out > 0 ? out_times[n_out - 1] : 0.0;
+  int count = 0;  // next output: out_times[count] (rk.cc:127-128)
   if (mode == 1) limit(U);  // CFL runs limit the initial condition first (rk.cc:100-114)
   while (!done) {
     double Dt;
+    bool output = false;
     if (mode == 0) {
       if (steps >= nsteps) break;
       Dt = dt_or_cfl;
@@ -1268,7 +1275,9 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
       if (nsteps > 0 && steps >= nsteps) break;  // optional step cap (not in the reference)
       Dt = or_dt_from_cfl(c, U, dt_or_cfl);
       if (Dt < 1e-14 || Dt != Dt) { if (t_out) *t_out = T; return -1; }
-      if (Dt > (tf - T)) { Dt = tf - T; done = true; }
+      const double Tout = out_times[count < n_out ? count : n_out - 1];
+      if (Dt > (tf - T)) { Dt = tf - T; output = true; done = true; }  // rk.cc:149
+      else if (Dt > (Tout - T)) { Dt = Tout - T; output = true; }      // rk.cc:150
     }
     if (dt_hist && steps < max_hist) dt_hist[steps] = Dt;
     std::memcpy(Us.data(), U, n * sizeof(double));
@@ -1284,9 +1293,31 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
     limit(U);
     T = T + Dt;
     steps++;
+    if (output) {  // rk.cc:215-233
+      if (out_steps && count < n_out) out_steps[count] = steps;
+      count++;
+    }
   }
   if (t_out) *t_out = T;
   return steps;
+}
+
+int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
+           int limiter, const int* on, const double* thresh, double* t_out, double* dt_hist, int max_hist) {
+  return or_rk4_out(c, t, U, mode, dt_or_cfl, nsteps, &tf, 1, limiter, on, thresh, t_out, dt_hist, max_hist, nullptr);
+}
+
+// DG_SOLVER::conservation (dg_solver.h:367-389): UgC = phi*U (netlib gemm), then
+// I[fc] += UgC*J_e*w_g over elements and points, one running sum per field.
+void or_conservation(const or_cfg* c, const double* phi, const double* J, const double* weight, const double* U,
+                     double* I) {
+  const int N_G = c->N_G, N_s = c->N_s, N_E = c->N_E, N_F = c->N_F;
+  std::vector<double> UgC((size_t)N_G * N_E * N_F);
+  or_dgemm('N', 'N', N_G, N_E * N_F, N_s, 1, phi, N_G, U, N_s, 0.0, UgC.data(), N_G);
+  for (int fc = 0; fc < N_F; fc++) I[fc] = 0.0;
+  for (int fc = 0; fc < N_F; fc++)
+    for (int e = 0; e < N_E; e++)
+      for (int g = 0; g < N_G; g++) I[fc] += UgC[((size_t)e * N_F + fc) * N_G + g] * J[e] * weight[g];
 }
 
 // PRINTER::print formatter (printer_twod_multifluid.cu:20-110,

@@ -98,6 +98,11 @@ double or_ref_area(const or_cfg* c); /* 4 for quadrilaterals (main.cc:286), 2 fo
 void or_limit_p0i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors);
 
+/* DG_SOLVER::conservation (dg_solver.h:367-389): I[fc] = sum_e sum_g (phi*U)[(e,fc),g] * J[e] * weight[g],
+ * accumulated one term after another in that order. */
+void or_conservation(const or_cfg* c, const double* phi, const double* J, const double* weight, const double* U,
+                     double* I);
+
 /* PRINTER formatter (Lformater, printer_{oned,twod}_{singlefluid,multifluid}.cu:20-110, GAMNCON):
  * inverse 0: out = primitive output fields of U per node (rho, ux[, uy][, gamma], p);
  * inverse 1: U = conserved fields from out (the restart direction). */
@@ -110,6 +115,12 @@ void or_format(const or_cfg* c, double* U, double* out, int inverse);
  * *t_out receives the final time. dt_hist (may be NULL) receives up to max_hist Dt values. */
 int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
            int limiter, const int* on, const double* thresh, double* t_out, double* dt_hist, int max_hist);
+/* Same with the reference's output times (rk.cc:149-150, 215-233): in CFL mode Dt is clamped to the next output
+ * time as well as to the final time out_times[n_out-1]; out_steps[k] (may be NULL) receives the step count after
+ * which output k happened.  out_times are _output_time_array[1..] (entry 0, the initial time, left out). */
+int or_rk4_out(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_or_cfl, int nsteps,
+               const double* out_times, int n_out, int limiter, const int* on, const double* thresh, double* t_out,
+               double* dt_hist, int max_hist, int* out_steps);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,185 @@
+"""GPU parity on the benchmark workloads themselves, against the CPU oracle.
+
+- 2D two-fluid shock-bubble (Mach 1.22 shock, light bubble r=0.25, walls top
+  and bottom, farfield left and right, Roe, p=2, 5 fields) at 64x32 and
+  128x64, 24 HR-limited RK4 steps on both kernel paths: 1e-11 relative L2 and
+  Linf per conserved field against or_rk4.  The shock starts at x=0.24, so it
+  crosses the bubble's edge within the run and every field (rho v included)
+  is non-trivial.
+- The same runs with the FP32-MFMA and split-BF16 contractions: 1e-6
+  relative L2 per field against the FP64 oracle.
+- 1024x512 (the benchmark size) with periodic sides: mass, momenta and energy
+  totals (dgx_conservation) are kept to round-off over 10 HR-limited steps.
+- 1D Sod (1000 elements, t = 0.4, CFL 0.4, Rusanov, HR limiter): the whole run
+  in CFL mode, with the same step count, final time and Dt sequence as the
+  oracle, and 1e-11 per field at the end.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from dg_amd import (LIMIT_HR, MULTIFLUID, PATH_GENERAL, PATH_TILED, PERIODIC, ROE, RUSANOV, Config, Mesh, Solver,
+                    problems)
+from tests.helpers import per_field_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x, device):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def _bubble(nx, ny):
+    mesh = problems.shock_bubble_mesh(nx, ny)
+    return mesh, problems.shock_bubble_state(mesh, xs=0.24)
+
+
+@pytest.mark.parametrize("path", [PATH_TILED, PATH_GENERAL])
+@pytest.mark.parametrize("nx,ny", [(64, 32), (128, 64)])
+def test_shock_bubble_limited_trajectory(device, nx, ny, path):
+    mesh, U0 = _bubble(nx, ny)
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
+    dt = prob.dt_from_cfl(U0, 0.4 * mesh.dx / 5)
+    nsteps = 24
+    U_ref, n, T, _ = prob.rk4(U0, 0, dt, nsteps=nsteps, limiter=LIMIT_HR)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_path(path)
+    Ud = dev(U0, device)
+    steps, _ = solver.rk4_integrate(Ud, 0, dt, nsteps=nsteps, limiter=LIMIT_HR)
+    assert steps == nsteps
+    errs = per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s)
+    print(f"shock-bubble {nx}x{ny} path {path}: per-field (L2, Linf) {errs}")
+    for l2, li in errs:
+        assert l2 < 1e-11 and li < 1e-11, errs
+    # the run is not trivial: the limiter and the shock changed every field
+    V = U_ref.reshape(-1, prob.nf, prob.cfg.N_s)
+    assert np.abs(V[:, 2]).max() > 1e-3  # rho v, zero at the start
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_shock_bubble_reduced_precision(device, prec):
+    """FP32-MFMA (1) and split-BF16 (2) map/integrate contractions, FP64
+    elsewhere: 1e-6 relative L2 per field against the FP64 oracle after 24
+    HR-limited steps at 128x64."""
+    mesh, U0 = _bubble(128, 64)
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
+    dt = prob.dt_from_cfl(U0, 0.4 * mesh.dx / 5)
+    U_ref, _, _, _ = prob.rk4(U0, 0, dt, nsteps=24, limiter=LIMIT_HR)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_precision(prec)
+    Ud = dev(U0, device)
+    solver.rk4_integrate(Ud, 0, dt, nsteps=24, limiter=LIMIT_HR)
+    errs = per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s)
+    print(f"shock-bubble 128x64 precision {prec}: per-field (L2, Linf) {errs}")
+    l2s = [e[0] for e in errs]
+    assert max(l2s) < 1e-6, errs
+    assert max(l2s) > 1e-13, errs  # the reduced-precision contractions really ran
+
+
+def test_shock_bubble_full_size_conservation(device):
+    """1024x512, periodic in x and y (no boundary fluxes): the integrals of
+    rho, rho u, rho v and E stay constant over 10 HR-limited steps to
+    round-off (the HR limiter keeps every cell mean).  Tolerance per field:
+    1e-13 of the integral of |field| over the mesh, per step."""
+    mesh = Mesh(2, 2, 1024, 512, 0.0, 2.0, 0.0, 1.0, bc=(PERIODIC, PERIODIC, PERIODIC, PERIODIC), nfields=5)
+    U0 = problems.shock_bubble_state(mesh)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    assert solver.path == PATH_TILED and solver.hr_one_launch
+    Ud = dev(U0, device)
+    tot0 = torch.zeros(5, dtype=torch.float64, device=device)
+    solver.conservation(Ud, tot0)
+    absU = dev(np.abs(U0), device)
+    scale = torch.zeros(5, dtype=torch.float64, device=device)
+    solver.conservation(absU, scale)  # integral of |nodal values| (interpolated), a size for each field
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
+    dt = prob.dt_from_cfl(U0, 0.4 * mesh.dx / 5)
+    nsteps = 10
+    solver.rk4_integrate(Ud, 0, dt, nsteps=nsteps, limiter=LIMIT_HR)
+    tot1 = torch.zeros(5, dtype=torch.float64, device=device)
+    solver.conservation(Ud, tot1)
+    t0, t1, sc = host(tot0), host(tot1), host(scale)
+    V0 = U0.reshape(-1, 5, 9)
+    assert not np.array_equal(host(Ud), U0)
+    print("totals before", t0, "after", t1, "scale", sc)
+    for f in range(4):  # rho, rho u, rho v, E (alpha = 1/(gamma-1) is advected, not conserved)
+        assert abs(t1[f] - t0[f]) <= 1e-13 * nsteps * sc[f], (f, t0[f], t1[f], sc[f])
+    assert abs(t0[0] - 2.0 * float(np.mean(V0[:, 0]))) < 1e-2 * t0[0]  # the mass is about its mean times the area
+
+
+def test_sod_full_run_cfl_mode(device):
+    """1D Sod, 1000 elements, p=2, Rusanov, HR limiter, CFL 0.4 to t=0.4
+    (the two-fluid form with gamma 1.4 on both sides, sodmono in the
+    reference's deck): every Dt, the step count and the final time as the
+    oracle's, 1e-11 per field at the end."""
+    mesh = problems.sod_mesh(nx=1000, order=2, physics=MULTIFLUID)
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, RUSANOV)
+    U0 = problems.sod_state(mesh, MULTIFLUID)
+    cfl = 0.4 * mesh.dx / 5
+    U_ref, n_ref, T_ref, dts = prob.rk4(U0, 1, cfl, tf=0.4, limiter=LIMIT_HR)
+    assert n_ref > 1000
+    solver = Solver(Config(1, 2, MULTIFLUID, RUSANOV), mesh)
+    Ud = dev(U0, device)
+    steps, T = solver.rk4_integrate(Ud, 1, cfl, tf=0.4, limiter=LIMIT_HR)
+    errs = per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s)
+    print(f"Sod: {steps} steps (oracle {n_ref}), T {T!r} (oracle {T_ref!r}), per-field {errs}")
+    assert steps == n_ref and T == T_ref
+    for l2, li in errs:
+        assert l2 < 1e-11 and li < 1e-11, errs
+
+
+def test_output_times_match_oracle(device):
+    """dgx_rk4_integrate_out: Dt clamped to each output time and to the final
+    time (rk.cc:149-150), the output call after each clamped step, and the
+    same steps and state as or_rk4_out (1D Sod, 200 elements, HR)."""
+    mesh = problems.sod_mesh(nx=200, order=2, physics=MULTIFLUID)
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, RUSANOV)
+    U0 = problems.sod_state(mesh, MULTIFLUID)
+    cfl = 0.4 * mesh.dx / 5
+    outs = [0.01, 0.025, 0.04, 0.05]
+    U_ref, n_ref, T_ref, _, o_ref = prob.rk4_out(U0, cfl, outs, limiter=LIMIT_HR)
+    solver = Solver(Config(1, 2, MULTIFLUID, RUSANOV), mesh)
+    Ud = dev(U0, device)
+    seen = []
+
+    def on_output(count, step, time):
+        seen.append((count, step, time))
+        return 0
+
+    steps, T = solver.rk4_integrate_out(Ud, cfl, outs, limiter=LIMIT_HR, on_output=on_output)
+    assert steps == n_ref and T == T_ref == outs[-1]
+    assert [s[0] for s in seen] == [1, 2, 3, 4]
+    assert [s[1] for s in seen] == list(o_ref)
+    for (c, s, t), want in zip(seen, outs):
+        assert abs(t - want) <= 1e-15
+    for l2, li in per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s):
+        assert l2 < 1e-11 and li < 1e-11
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_dt_from_cfl_nan_speeds_like_idamax(device, dim):
+    """Negative pressure in one element (NaN |u|+a): Dt comes from the other
+    nodes, as the reference's idamax passes over a NaN after node 0; a NaN at
+    node 0 gives Dt = NaN (the reference's stop, rk.cc:148)."""
+    from tests.helpers import small_mesh
+
+    mesh = small_mesh(dim, 2, MULTIFLUID, n=(64, 32))
+    prob = oracle.Problem(mesh.tables(), MULTIFLUID, ROE)
+    c = prob.cfg
+    V = problems.riemann_state(mesh, MULTIFLUID, 120).reshape(c.N_E, prob.nf, c.N_s).copy()
+    V[c.N_E // 2, 1 + dim] = -1.0
+    solver = Solver(Config(dim, 2, MULTIFLUID, ROE), mesh)
+    dt = torch.zeros(1, dtype=torch.float64, device=device)
+    solver.dt_from_cfl(dev(V.ravel(), device), 0.7, dt)
+    want = prob.dt_from_cfl(V.ravel(), 0.7)
+    assert math.isfinite(want) and host(dt)[0] == want
+    V[0, 1 + dim] = -1.0
+    solver.dt_from_cfl(dev(V.ravel(), device), 0.7, dt)
+    assert math.isnan(host(dt)[0]) and math.isnan(prob.dt_from_cfl(V.ravel(), 0.7))

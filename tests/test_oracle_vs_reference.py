@@ -17,6 +17,7 @@ import pytest
 import oracle
 from dg_amd import FARFIELD, MULTIFLUID, PERIODIC, REFLECTIVE, STIFFENED, problems
 from tests.helpers import CONFIGS, CONFIGS_PS, cfg_id, small_mesh
+from tests.test_oracle_compositions import ref_hr, ref_hri_sweeps
 
 pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="reference sources not present: _ref not built")
 
@@ -134,19 +135,11 @@ def test_sensors_and_hri_bitwise(c, order):
                         ctypes.c_double(tv[1]), ctypes.c_bool(on[2]), ctypes.c_double(tv[2]), P(prob._keep["neighbors"]),
                         P(uavg), P(sens))
         assert np.array_equal(sens, sens_or)
-        # hri1D on the flagged elements, then copy_detected
-        Unew_ref = np.zeros_like(U)
-        L2M = prob._keep["Lag2Mono" if d == 1 else "Lag2MonoX"]
-        M2L = prob._keep["Mono2Lag" if d == 1 else "MonoX2Lag"]
-        ns1 = order + 1
-        R.Lhri1D(prob.cfg.N_s, prob.cfg.N_E, prob.cfg.N_N, P(prob._keep["neighbors"]), ns1 if d == 2 else prob.cfg.N_s,
-                 ns1 if d == 2 else 1, 0, P(L2M), P(M2L), P(sens), P(U), P(Unew_ref))
-        U_ref = U.copy()
-        R.Lcopy_detected(prob.cfg.N_s, prob.cfg.N_E, P(sens), P(Unew_ref), P(U_ref))
-        if d == 1:
-            U_or, s2 = prob.limit_hri(U, th)
-            assert np.array_equal(s2, sens)
-            assert np.array_equal(U_or, U_ref)
+        # HRIlimiting (limiting.cc:596-634): hri1D on the flagged elements, then copy_detected, per direction
+        U_ref = ref_hri_sweeps(R, prob, U, sens)
+        U_or, s2 = prob.limit_hri(U, th)
+        assert np.array_equal(s2, sens)
+        assert np.array_equal(U_or, U_ref)
 
 
 @pytest.mark.parametrize("c", CONFIGS + [x for x in CONFIGS_PS if x[1] == STIFFENED], ids=cfg_id)
@@ -245,9 +238,10 @@ def test_m2_and_pr_limiting_bitwise(dim, order):
         U_ref = _ref_m2(R, prob, t, U)
         assert np.array_equal(prob.limit_m2(U), U_ref)
         assert not np.array_equal(U_ref, U)
+        # PRlimiting (limiting.cc:102-121): Cons2Prim, HRlimiting composed from the reference launchers, Prim2Cons
         V = U.copy()
         R.LCons2Prim(prob.cfg.N_s, prob.cfg.N_E, P(V))
-        V = prob.limit_hr(V)
+        V = ref_hr(R, prob, V)
         R.LPrim2Cons(prob.cfg.N_s, prob.cfg.N_E, P(V))
         assert np.array_equal(prob.limit_pr(U), V)
         # PRIlimiting (limiting.cc:636-662): sensing (pinned above), Cons2Prim, hri1D + copy_detected, Prim2Cons
