@@ -16,6 +16,22 @@ hipError_t compat_sensors(Solver& tmp, const int* nb, const double* Uavg, int* s
 hipError_t compat_hri(int N_s, int N_E, int NF, int N_N, const int* nb, int N_s1D, int slicenum, int offxy, int D,
                       const double* L2M, const double* M2L, const int* sensors, const double* U, double* Unew);
 hipError_t compat_copy_detected(int N_s, int N_E, int NF, const int* sensors, const double* Uold, double* U);
+hipError_t compat_pressure(const dgx_config& c, int N_s, int N_E, const double* U, double* p);
+hipError_t compat_kinetic(int D, int N_s, int N_E, const double* rho, const double* rhou, const double* rhov, double* K);
+hipError_t compat_internal_energy(int N_s, int N_E, int slicenum, const double* p, const double* g, const double* b,
+                                  double* rhoe);
+hipError_t compat_reconstruct_energy(int N_s, int N_E, int slicenum, const double* rhoeLim, const double* KLim,
+                                     const double* EMono, double* ELim);
+hipError_t compat_average_p0(int N_s, int N_E, int NF, double* DU);
+hipError_t compat_change_basis(int size1, int size2, int N_E, int NF, const double* T, const double* U, double* Unew);
+hipError_t compat_cons_prim(const dgx_config& c, int N_s, int N_E, double* U, int to_prim);
+hipError_t compat_stridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA, int offsetB,
+                              const double* A, double* B);
+hipError_t compat_format_sensor(int N_s, int N_E, const int* sensor, double* out);
+hipError_t compat_p0i(int N_s, int N_E, int NF, int N_G, double refArea, const int* sensors, const double* phi_w,
+                      const double* U, double* Unew);
+hipError_t compat_m2i(int D, int N_s1D, int N_E, int N_N, const int* nb, int offxy, const double* L2M,
+                      const double* M2L, const int* sensors, const double* U, double* Unew);
 
 static dgx_config g_cfg = {1, 2, DGX_MULTIFLUID, DGX_RUSANOV};
 const dgx_config& current_config() { return g_cfg; }
@@ -154,7 +170,9 @@ __global__ void kc_rflctive(int M_s, int M_B, int NF, const int* boundaryMap, co
   }
 }
 
+// solve (rk_kernels.cu:22-56): DU = Dt * (Minv_e f), the sum in node order, no contraction
 __global__ void kc_solve(int N_s, int N_E, int NF, double Dt, const double* Minv, const double* f, double* DU) {
+#pragma clang fp contract(off)
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)N_E * NF * N_s) return;
   int i = (int)(k % N_s);
@@ -176,35 +194,49 @@ __global__ void kc_find_upa(int N_s, int N_E, const double* U, double* UPA) {
   UPA[k] = upa<D, PH>(u);
 }
 
-// BLAS layer: column-major, one thread per C entry, sequential inner sum
+// BLAS layer: column-major, one thread per C entry, each entry in the
+// reference BLAS's order and rounding (no contraction): NN C(i,j) starts from
+// 0, C or beta*C and adds (alpha*B(l,j))*A(i,l) for l = 1..K in turn; TN forms
+// the dot product first, then alpha*temp (+ beta*C).
 __global__ void kc_dgemm(int tA, int M, int N, int K, double alpha, const double* A, int lda, const double* B, int ldb,
                          double beta, double* C, int ldc) {
+#pragma clang fp contract(off)
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (long)M * N) return;
   int i = (int)(k % M), j = (int)(k / M);
-  double s = 0.0;
-  if (!tA)
-    for (int l = 0; l < K; ++l) s += (alpha * B[l + (long)j * ldb]) * A[i + (long)l * lda];
-  else {
-    for (int l = 0; l < K; ++l) s += A[l + (long)i * lda] * B[l + (long)j * ldb];
-    s = alpha * s;
+  double* c = C + i + (long)j * ldc;
+  if (!tA) {
+    double s = beta == 0.0 ? 0.0 : (beta == 1.0 ? *c : beta * *c);
+    for (int l = 0; l < K; ++l) s = s + (alpha * B[l + (long)j * ldb]) * A[i + (long)l * lda];
+    *c = s;
+  } else {
+    double s = 0.0;
+    for (int l = 0; l < K; ++l) s = s + A[l + (long)i * lda] * B[l + (long)j * ldb];
+    *c = (beta == 0.0) ? alpha * s : alpha * s + beta * *c;
   }
-  C[i + (long)j * ldc] = (beta == 0.0) ? s : s + beta * C[i + (long)j * ldc];
 }
 
+// y = y + a*x (reference DAXPY; a zero a leaves y as it is)
 __global__ void kc_daxpy(int n, double a, const double* x, int incx, double* y, int incy) {
+#pragma clang fp contract(off)
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) y[k * incy] = y[k * incy] + a * x[k * incx];
+  if (k < n && a != 0.0) y[k * incy] = y[k * incy] + a * x[k * incx];
 }
 __global__ void kc_dcopy(int n, const double* x, int incx, double* y, int incy) {
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) y[k * incy] = x[k * incx];
 }
-// first index of max |x|: pack (|x| bits, ~index) so that ties pick the lowest index
+// BLAS idamax: the first index of the largest |x|, scanning from x[0] and
+// moving on only to a strictly larger value, so a NaN after x[0] is passed
+// over and a NaN x[0] is the answer.  The non-negative doubles order like
+// their bit patterns, so the maximum over non-NaN |x| is an integer atomicMax;
+// the second kernel takes the lowest index holding it.  A NaN x[0] is caught
+// on the host side of the launcher.
 __global__ void kc_idamax(int n, const double* x, int incx, unsigned long long* best_val, int* best_idx) {
   long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   double v = fabs(x[k * incx]);
+  if (v != v) return;
   unsigned long long bits = (unsigned long long)__double_as_longlong(v);
   unsigned long long old = atomicMax(best_val, bits);
   (void)old;
@@ -362,6 +394,93 @@ void Lcopy_detected(int N_s, int N_E, int* sensors, double* Uold, double* U) {
   finish("Lcopy_detected");
 }
 
+// ----------------------------------------------- helper launchers (m2, pr, p0, output)
+static void run(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    hip_ok(e, what);
+    return;
+  }
+  finish(what);
+}
+
+void Lstridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA, int offsetB, double* A,
+                  double* B) {
+  run(compat_stridedcopy(numblocks, blocklen, strideA, strideB, offsetA, offsetB, A, B), "Lstridedcopy");
+}
+
+void Lreconstruct_energy(int N_s, int N_E, int slicenum, double* rhoeLim, double* KLim, double* EMono, double* ELim) {
+  run(compat_reconstruct_energy(N_s, N_E, slicenum, rhoeLim, KLim, EMono, ELim), "Lreconstruct_energy");
+}
+
+void Linternal_energy_multifluid(int N_s, int N_E, int slicenum, double* p, double* g, double* rhoe) {
+  run(compat_internal_energy(N_s, N_E, slicenum, p, g, nullptr, rhoe), "Linternal_energy_multifluid");
+}
+
+void Linternal_energy_stiffened(int N_s, int N_E, int slicenum, double* p, double* g, double* b, double* rhoe) {
+  run(compat_internal_energy(N_s, N_E, slicenum, p, g, b, rhoe), "Linternal_energy_stiffened");
+}
+
+void Lm2i1D(int N_s, int N_E, int N_N, int* neighbors, int N_s1D, int slicenum, int offxy, double* Lag2Mono,
+            double* Mono2Lag, int* sensors, double* U, double* Unew) {
+  (void)N_s; (void)slicenum;
+  if (g_cfg.physics != DGX_MULTIFLUID || N_s1D < 2 || N_s1D > 4) {
+    set_error("Lm2i1D: two-fluid states of order 1..3 only");
+    return;
+  }
+  run(compat_m2i(g_cfg.dim, N_s1D, N_E, N_N, neighbors, offxy, Lag2Mono, Mono2Lag, sensors, U, Unew), "Lm2i1D");
+}
+
+void Lp0i(int N_s, int N_E, int N_G, double refArea, int* sensors, double* phi_w, double* U, double* Unew) {
+  run(compat_p0i(N_s, N_E, dgx_nfields(&g_cfg), N_G, refArea, sensors, phi_w, U, Unew), "Lp0i");
+}
+
+void LChangeBasis(int size1, int size2, int N_E, double* Transform, double* U, double* Unew) {
+  run(compat_change_basis(size1, size2, N_E, dgx_nfields(&g_cfg), Transform, U, Unew), "LChangeBasis");
+}
+
+void LPrim2Cons(int N_s, int N_E, double* U) {
+  if (g_cfg.physics == DGX_SINGLEFLUID) {
+    set_error("LPrim2Cons: single-fluid states have no alpha field");
+    return;
+  }
+  run(compat_cons_prim(g_cfg, N_s, N_E, U, 0), "LPrim2Cons");
+}
+
+void LCons2Prim(int N_s, int N_E, double* U) {
+  if (g_cfg.physics == DGX_SINGLEFLUID) {
+    set_error("LCons2Prim: single-fluid states have no alpha field");
+    return;
+  }
+  run(compat_cons_prim(g_cfg, N_s, N_E, U, 1), "LCons2Prim");
+}
+
+void Lpressure(int N_s, int N_E, double* U, double* p) { run(compat_pressure(g_cfg, N_s, N_E, U, p), "Lpressure"); }
+
+void Lkinetic_energy1D(int N_s, int N_E, double* rho, double* rhou, double* K) {
+  run(compat_kinetic(1, N_s, N_E, rho, rhou, nullptr, K), "Lkinetic_energy1D");
+}
+
+void Lkinetic_energy2D(int N_s, int N_E, double* rho, double* rhou, double* rhov, double* K) {
+  run(compat_kinetic(2, N_s, N_E, rho, rhou, rhov, K), "Lkinetic_energy2D");
+}
+
+void Laverage_cell_p0(const int N_s, const int N_E, double* DU) {
+  run(compat_average_p0(N_s, N_E, dgx_nfields(&g_cfg), DU), "Laverage_cell_p0");
+}
+
+void Lformater(int N_s, int N_E, double* U, double* output, dgx_bool inverse) {
+  Solver tmp;
+  tmp.cfg = g_cfg;
+  tmp.D = g_cfg.dim;
+  tmp.N_s = N_s;
+  tmp.N_E = N_E;
+  run(launch_format(tmp, U, output, inverse ? 1 : 0, 0), "Lformater");
+}
+
+void Lformat_sensor(int N_s, int N_E, int* sensor, double* output) {
+  run(compat_format_sensor(N_s, N_E, sensor, output), "Lformat_sensor");
+}
+
 void dgx_blasDgemm(char transa, char transb, int M, int N, int K, double alpha, const double* A, int lda,
                    const double* B, int ldb, double beta, double* C, int ldc) {
   (void)transb;
@@ -397,10 +516,13 @@ int dgx_blasIdamax(int n, const double* x, int incx) {
   hipLaunchKernelGGL(kc_idamax, dim3(nb256(n)), dim3(256), 0, 0, n, x, incx, bv, bi);
   hipLaunchKernelGGL(kc_idamax_index, dim3(nb256(n)), dim3(256), 0, 0, n, x, incx, bv, bi);
   int idx = 0;
+  double x0 = 0;
   hipMemcpy(&idx, bi, sizeof(int), hipMemcpyDeviceToHost);
+  hipMemcpy(&x0, x, sizeof(double), hipMemcpyDeviceToHost);
   finish("dgx_blasIdamax");
   hipFree(bv);
   hipFree(bi);
+  if (x0 != x0 || idx >= n) return 1;  // a NaN first entry, or every entry NaN after it
   return idx + 1;
 }
 

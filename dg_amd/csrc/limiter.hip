@@ -23,6 +23,7 @@
 // Sensor-gated HR (Limiting::HRIlimiting, limiting.cc:596-634; hri1D,
 //   limiting_kernels.cu:322-427; copy_detected, sensor_kernels.cu:167-200).
 #pragma clang fp contract(off)
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -443,14 +444,16 @@ __global__ __launch_bounds__(256) void k_m2_reconstruct(int Ns, int N_E, int sli
   for (int q = 2; q < Ns; q += 2) E0 -= 1.0 / ((double)ifact(q + 1)) * (ELim[o + q] - EMono[o + q]);
   ELim[o] = E0;
 }
-// Cons2Prim / Prim2Cons (limiting_kernels.cu:957-1068), two-fluid GAMNCON
-template <int D>
+// Cons2Prim / Prim2Cons (limiting_kernels.cu:957-1068), GAMNCON: alpha is
+// field D+2 (two-fluid 1/(gamma-1); for passive states the reference's
+// expressions take rho*phic there), beta field D+3 for stiffened states, else 0
+template <int D, int PH = 1>
 __global__ __launch_bounds__(256) void k_cons_prim(long n, int Ns, int to_prim, double* __restrict__ U) {
-  constexpr int NF = 2 + D + 1;
+  constexpr int NF = Fields<D, PH>::NF;
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   double* u = U + (k / Ns) * NF * Ns + k % Ns;
-  const double beta = 0;
+  const double beta = PH == 3 ? u[(D + 3) * Ns] : 0.0;
   if (to_prim) {
     const double rho = u[0], rhou = u[Ns], rhov = D == 2 ? u[2 * Ns] : 0.0, E = u[(D + 1) * Ns],
                  alpha = u[(D + 2) * Ns];
@@ -463,6 +466,95 @@ __global__ __launch_bounds__(256) void k_cons_prim(long n, int Ns, int to_prim, 
     if (D == 2) u[2 * Ns] = rho * v;
     u[(D + 1) * Ns] = p * alpha + beta + 0.5 * rho * (uu * uu + v * v);
   }
+}
+
+// ------------------------------------- reference-named helper launchers
+// The kernels behind Lpressure, Lkinetic_energy1D/2D, Linternal_energy_*,
+// Lreconstruct_energy, Laverage_cell_p0, LChangeBasis, Lstridedcopy,
+// Lformat_sensor, Lp0i (compat.hip), in the reference's operation order and
+// without contraction, so their outputs are the reference CPU loops' bits.
+
+// pressure (physics.cu:891-960) for every physics: gamma 1.4 (single fluid,
+// passive), 1 + 1/alpha (two-fluid, stiffened), E - beta (stiffened)
+template <int D, int PH>
+__global__ __launch_bounds__(256) void k_pressure(long n, int Ns, const double* __restrict__ U, double* __restrict__ p) {
+  constexpr int NF = Fields<D, PH>::NF;
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double* u = U + (k / Ns) * NF * Ns + k % Ns;
+  const double rho = u[0], rhou = u[Ns];
+  double E = u[(D + 1) * Ns];
+  double gamma = kGammaSF;
+  if (PH == 1 || PH == 3) gamma = 1.0 + 1.0 / u[(D + 2) * Ns];
+  if (PH == 3) E = E - u[(D + 3) * Ns];
+  if (D == 1) {
+    p[k] = (gamma - 1) * (E - 0.5 * rhou * rhou / rho);
+  } else {
+    const double rhov = u[2 * Ns];
+    p[k] = (gamma - 1) * (E - 0.5 * (rhou * rhou + rhov * rhov) / rho);
+  }
+}
+// internal_energy_stiffened (limiting_kernels.cu:203-243): the two-fluid
+// product plus the limited gamma*pinf/(gamma-1)
+__global__ __launch_bounds__(256) void k_internal_stiffened(int Ns, int N_E, int slicenum, const double* __restrict__ p,
+                                                            const double* __restrict__ g, const double* __restrict__ b,
+                                                            double* __restrict__ rhoe) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)N_E * slicenum) return;
+  const long o = k * Ns;
+  for (int i = 0; i < Ns; i++) {
+    double I = 0;
+    for (int q = 0; q < i + 1; q++) I += (double)binomial_d(i, q) * p[o + i - q] * g[o + q];
+    rhoe[o + i] = I + b[o + i];
+  }
+}
+// average_cell_p0 (rk_kernels.cu:61-92): one thread per (element, field)
+__global__ __launch_bounds__(256) void k_average_p0(long n, int Ns, double* __restrict__ DU) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double* d = DU + k * Ns;
+  double average = 0.0;
+  for (int i = 0; i < Ns; i++) average += d[i];
+  average = average / Ns;
+  for (int i = 0; i < Ns; i++) d[i] = average;
+}
+// ChangeBasis (limiting_kernels.cu:918-955): Unew[(e,fc),i] = sum_ii T[e][i][ii] U[(e,fc),ii]
+__global__ __launch_bounds__(256) void k_change_basis(int size1, int size2, int N_E, int NF, const double* __restrict__ T,
+                                                      const double* __restrict__ U, double* __restrict__ Unew) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)N_E * NF * size1) return;
+  const int i = (int)(k % size1);
+  const long efc = k / size1, e = efc / NF;
+  double sol = 0.0;
+  for (int ii = 0; ii < size2; ii++) sol += T[(e * size1 + i) * size2 + ii] * U[efc * size2 + ii];
+  Unew[k] = sol;
+}
+// stridedcopy (limiting_kernels.cu:51-108): block i of blocklen values
+__global__ __launch_bounds__(256) void k_stridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA,
+                                                     int offsetB, const double* __restrict__ A, double* __restrict__ B) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)numblocks * blocklen) return;
+  const long i = k / blocklen, j = k % blocklen;
+  B[offsetB + i * strideB + j] = A[offsetA + i * strideA + j];
+}
+// format_sensor (printer_sensor.cu:19-43)
+__global__ __launch_bounds__(256) void k_format_sensor(int N_s, int N_E, const int* __restrict__ sensor,
+                                                       double* __restrict__ out) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < (long)N_E * N_s) out[k] = (double)sensor[k / N_s];
+}
+// p0i (limiting_kernels.cu:723-768) into Unew for the flagged elements
+__global__ __launch_bounds__(256) void k_p0i(int N_E, int NF, int Ns, int NG, double refArea,
+                                             const int* __restrict__ sensors, const double* __restrict__ phi_w,
+                                             const double* __restrict__ U, double* __restrict__ Unew) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long)N_E * NF || sensors[k / NF] == 0) return;
+  const double* u = U + k * Ns;
+  double avg = 0;
+  for (int g = 0; g < NG; ++g)
+    for (int i = 0; i < Ns; ++i) avg += phi_w[i * NG + g] * u[i];
+  avg = avg / refArea;
+  for (int i = 0; i < Ns; ++i) Unew[k * Ns + i] = avg;
 }
 
 // ---------------------------------------------------------------- sensors
@@ -746,18 +838,21 @@ __global__ __launch_bounds__(128) void k_hri_list(int NF, int N_N, const int* __
 // columns: kinetic_energy (:1899-1922), internal_energy (:1924-1953),
 // reconstruct_total_energy (:1955-2001).  A block of 64 lanes takes eight
 // elements per pass; the loop bound is uniform over the block.
+// With list == nullptr the kernel walks elements 0 .. N_E-1 and limits those
+// with sensors[e] != 0 (Lm2i1D's own loop).
 template <int D, int P>
 __global__ __launch_bounds__(64) void k_m2i_list(int N_N, const int* __restrict__ nb, int offxy,
                                                  const double* __restrict__ L2M, const double* __restrict__ M2L,
                                                  const int* __restrict__ list, const int* __restrict__ count,
-                                                 const double* __restrict__ U, double* __restrict__ Unew) {
+                                                 const double* __restrict__ U, double* __restrict__ Unew,
+                                                 int N_E = 0, const int* __restrict__ sensors = nullptr) {
   constexpr int Ns = Shape<D, P>::Ns, N = P, N1 = P + 1, NF = D + 3;
   constexpr int SL = (D == 1) ? 1 : N1;
   constexpr int fE = D + 1, fG = D + 2;
   __shared__ double sNod[8][D + 1][Ns];  // limited rho, rho u[, rho v]
   __shared__ double sG[8][Ns], sP[8][Ns];  // limited monomials of 1/(gamma-1) and p
   const int g = threadIdx.x >> 3, c = threadIdx.x & 7;
-  const long n = count[0];
+  const long n = list ? (long)count[0] : (long)N_E;
   auto pressure = [&](long el, int i) -> double {
     const double* u = U + (long)el * NF * Ns;
     const double rho = u[i], ru = u[Ns + i], E = u[fE * Ns + i], ga = u[fG * Ns + i];
@@ -766,8 +861,8 @@ __global__ __launch_bounds__(64) void k_m2i_list(int N_N, const int* __restrict_
     return (E - 0.5 * (ru * ru + rv * rv) / rho) / ga;
   };
   for (long base = (long)blockIdx.x * 8; base < n; base += (long)gridDim.x * 8) {
-    const bool live = base + g < n;
-    const int e = live ? list[base + g] : 0;
+    const bool live = base + g < n && (list || sensors[base + g] != 0);
+    const int e = live ? (list ? list[base + g] : (int)(base + g)) : 0;
     int physical = 0;
     if (live) {
       const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
@@ -1111,11 +1206,11 @@ hipError_t launch_limit_pri(Solver& s, double* U, hipStream_t st) {
   if (e != hipSuccess) return e;
   const long n = (long)s.N_E * s.N_s;
   const dim3 g(nblk(n, 256)), bl(256);
-  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 1, U);
-  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 1, U);
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1, 1>), g, bl, 0, st, n, s.N_s, 1, U);
+  else hipLaunchKernelGGL((k_cons_prim<2, 1>), g, bl, 0, st, n, s.N_s, 1, U);
   if ((e = launch_limit_flagged(s, U, st)) != hipSuccess) return e;
-  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 0, U);
-  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 0, U);
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1, 1>), g, bl, 0, st, n, s.N_s, 0, U);
+  else hipLaunchKernelGGL((k_cons_prim<2, 1>), g, bl, 0, st, n, s.N_s, 0, U);
   return hipGetLastError();
 }
 
@@ -1124,12 +1219,12 @@ hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st) {
   if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;
   const long n = (long)s.N_E * s.N_s;
   const dim3 g(nblk(n, 256)), bl(256);
-  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 1, U);
-  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 1, U);
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1, 1>), g, bl, 0, st, n, s.N_s, 1, U);
+  else hipLaunchKernelGGL((k_cons_prim<2, 1>), g, bl, 0, st, n, s.N_s, 1, U);
   hipError_t e = launch_limit_hr(s, U, st);
   if (e != hipSuccess) return e;
-  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1>), g, bl, 0, st, n, s.N_s, 0, U);
-  else hipLaunchKernelGGL((k_cons_prim<2>), g, bl, 0, st, n, s.N_s, 0, U);
+  if (s.D == 1) hipLaunchKernelGGL((k_cons_prim<1, 1>), g, bl, 0, st, n, s.N_s, 0, U);
+  else hipLaunchKernelGGL((k_cons_prim<2, 1>), g, bl, 0, st, n, s.N_s, 0, U);
   return hipGetLastError();
 }
 
@@ -1332,6 +1427,84 @@ hipError_t compat_copy_detected(int N_s, int N_E, int NF, const int* sensors, co
   int per = NF * N_s;
   hipLaunchKernelGGL(k_copy_detected, dim3(nblk(N_E, 64)), dim3(256), 0, 0, N_E, per, sensors, Uold, U);
   return hipGetLastError();
+}
+
+// ---------------------------------------- reference-named helper launchers
+// Host halves of the launchers of compat.hip whose arithmetic must round like
+// the reference's CPU loops (this file is compiled without contraction).
+hipError_t compat_pressure(const dgx_config& c, int N_s, int N_E, const double* U, double* p) {
+  const long n = (long)N_E * N_s;
+  return by_dim_phys(c.dim, c.physics, [&](auto d, auto ph) -> hipError_t {
+    hipLaunchKernelGGL((k_pressure<decltype(d)::value, decltype(ph)::value>), dim3(nblk(n, 256)), dim3(256), 0, 0, n,
+                       N_s, U, p);
+    return hipGetLastError();
+  });
+}
+hipError_t compat_kinetic(int D, int N_s, int N_E, const double* rho, const double* rhou, const double* rhov, double* K) {
+  const long n = (long)N_E * N_s;
+  if (D == 1) hipLaunchKernelGGL((k_m2_kinetic<1>), dim3(nblk(n, 256)), dim3(256), 0, 0, n, rho, rhou, rhov, K);
+  else hipLaunchKernelGGL((k_m2_kinetic<2>), dim3(nblk(n, 256)), dim3(256), 0, 0, n, rho, rhou, rhov, K);
+  return hipGetLastError();
+}
+hipError_t compat_internal_energy(int N_s, int N_E, int slicenum, const double* p, const double* g, const double* b,
+                                  double* rhoe) {
+  const unsigned nb = nblk((long)N_E * slicenum, 256);
+  if (b) hipLaunchKernelGGL(k_internal_stiffened, dim3(nb), dim3(256), 0, 0, N_s, N_E, slicenum, p, g, b, rhoe);
+  else hipLaunchKernelGGL(k_m2_internal, dim3(nb), dim3(256), 0, 0, N_s, N_E, slicenum, p, g, rhoe);
+  return hipGetLastError();
+}
+hipError_t compat_reconstruct_energy(int N_s, int N_E, int slicenum, const double* rhoeLim, const double* KLim,
+                                     const double* EMono, double* ELim) {
+  hipLaunchKernelGGL(k_m2_reconstruct, dim3(nblk((long)N_E * slicenum, 256)), dim3(256), 0, 0, N_s, N_E, slicenum,
+                     rhoeLim, KLim, EMono, ELim);
+  return hipGetLastError();
+}
+hipError_t compat_average_p0(int N_s, int N_E, int NF, double* DU) {
+  const long n = (long)N_E * NF;
+  hipLaunchKernelGGL(k_average_p0, dim3(nblk(n, 256)), dim3(256), 0, 0, n, N_s, DU);
+  return hipGetLastError();
+}
+hipError_t compat_change_basis(int size1, int size2, int N_E, int NF, const double* T, const double* U, double* Unew) {
+  hipLaunchKernelGGL(k_change_basis, dim3(nblk((long)N_E * NF * size1, 256)), dim3(256), 0, 0, size1, size2, N_E, NF, T,
+                     U, Unew);
+  return hipGetLastError();
+}
+hipError_t compat_cons_prim(const dgx_config& c, int N_s, int N_E, double* U, int to_prim) {
+  if (c.physics == DGX_SINGLEFLUID) return hipErrorInvalidValue;
+  const long n = (long)N_E * N_s;
+  return by_dim_phys(c.dim, c.physics, [&](auto d, auto ph) -> hipError_t {
+    hipLaunchKernelGGL((k_cons_prim<decltype(d)::value, decltype(ph)::value>), dim3(nblk(n, 256)), dim3(256), 0, 0, n,
+                       N_s, to_prim, U);
+    return hipGetLastError();
+  });
+}
+hipError_t compat_stridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA, int offsetB,
+                              const double* A, double* B) {
+  const long n = (long)numblocks * blocklen;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stridedcopy, dim3(nblk(n, 256)), dim3(256), 0, 0, numblocks, blocklen, strideA, strideB,
+                     offsetA, offsetB, A, B);
+  return hipGetLastError();
+}
+hipError_t compat_format_sensor(int N_s, int N_E, const int* sensor, double* out) {
+  hipLaunchKernelGGL(k_format_sensor, dim3(nblk((long)N_E * N_s, 256)), dim3(256), 0, 0, N_s, N_E, sensor, out);
+  return hipGetLastError();
+}
+hipError_t compat_p0i(int N_s, int N_E, int NF, int N_G, double refArea, const int* sensors, const double* phi_w,
+                      const double* U, double* Unew) {
+  hipLaunchKernelGGL(k_p0i, dim3(nblk((long)N_E * NF, 256)), dim3(256), 0, 0, N_E, NF, N_s, N_G, refArea, sensors,
+                     phi_w, U, Unew);
+  return hipGetLastError();
+}
+// m2i1D (two-fluid): the flagged elements of U limited into Unew
+hipError_t compat_m2i(int D, int N_s1D, int N_E, int N_N, const int* nb, int offxy, const double* L2M,
+                      const double* M2L, const int* sensors, const double* U, double* Unew) {
+  return by_order_ns(D, N_s1D - 1, [&](auto d, auto p) -> hipError_t {
+    const unsigned grid = (unsigned)std::min<long>(1024, nblk(N_E, 8));
+    hipLaunchKernelGGL((k_m2i_list<decltype(d)::value, decltype(p)::value>), dim3(grid), dim3(64), 0, 0, N_N, nb,
+                       offxy, L2M, M2L, nullptr, nullptr, U, Unew, N_E, sensors);
+    return hipGetLastError();
+  });
 }
 
 }  // namespace dgx

@@ -108,6 +108,29 @@ void LfindUPA(const int N_s, const int N_E, double* U, double* UPA);
 void Lhrl1D(int N_s, int N_E, int Nfields, int N_N, int slicenum, int* neighbors, int offxy, double* A, double* Alim);
 void Lhri1D(int N_s, int N_E, int N_N, int* neighbors, int N_s1D, int slicenum, int offxy, double* Lag2Mono,
             double* Mono2Lag, int* sensors, double* U, double* Unew);
+/* limiting/limiting_kernels.h:21-39: the helpers of the m2, primitive-variable and p0 limiters
+ * (Limiting::M2limiting, PRlimiting, M2Ilimiting, P0Ilimiting, limiting.cc:102-744).  Lm2i1D takes two-fluid
+ * states (DGX_MULTIFLUID); LCons2Prim / LPrim2Cons take every physics with an alpha field (not single fluid). */
+void Lstridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA, int offsetB, double* A,
+                  double* B);
+void Lreconstruct_energy(int N_s, int N_E, int slicenum, double* rhoeLim, double* KLim, double* EMono, double* ELim);
+void Linternal_energy_multifluid(int N_s, int N_E, int slicenum, double* p, double* g, double* rhoe);
+void Linternal_energy_stiffened(int N_s, int N_E, int slicenum, double* p, double* g, double* b, double* rhoe);
+void Lm2i1D(int N_s, int N_E, int N_N, int* neighbors, int N_s1D, int slicenum, int offxy, double* Lag2Mono,
+            double* Mono2Lag, int* sensors, double* U, double* Unew);
+void Lp0i(int N_s, int N_E, int N_G, double refArea, int* sensors, double* phi_w, double* U, double* Unew);
+void LChangeBasis(int size1, int size2, int N_E, double* Transform, double* U, double* Unew);
+void LPrim2Cons(int N_s, int N_E, double* U);
+void LCons2Prim(int N_s, int N_E, double* U);
+/* physics/physics.h:18-20 */
+void Lpressure(int N_s, int N_E, double* U, double* p);
+void Lkinetic_energy1D(int N_s, int N_E, double* rho, double* rhou, double* K);
+void Lkinetic_energy2D(int N_s, int N_E, double* rho, double* rhou, double* rhov, double* K);
+/* time_integration/rk_kernels.h:15 */
+void Laverage_cell_p0(const int N_s, const int N_E, double* DU);
+/* printer/printer_kernels.h:14-15 (the reference's inverse defaults to false in C++) */
+void Lformater(int N_s, int N_E, double* U, double* output, dgx_bool inverse);
+void Lformat_sensor(int N_s, int N_E, int* sensor, double* output);
 /* sensors/sensor_kernels.h:17-18 (flags follow the sequential CPU order) */
 void Lcalc_sensors(int N_E, int N_N, dgx_bool sensor1, double thresh1, dgx_bool sensor2, double thresh2, dgx_bool sensor3,
                    double thresh3, int* neighbors, double* Uavg, int* sensors);

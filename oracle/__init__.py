@@ -92,6 +92,19 @@ def lib() -> ctypes.CDLL:
         L.or_limit_p0i.restype = None
         L.or_p0i.argtypes = [cp, vp, ctypes.c_double, vp, vp, vp]
         L.or_p0i.restype = None
+        i = ctypes.c_int
+        L.or_stridedcopy.argtypes = [i, i, i, i, i, i, vp, vp]
+        L.or_reconstruct_energy.argtypes = [i, i, i, vp, vp, vp, vp]
+        L.or_internal_energy.argtypes = [i, i, i, vp, vp, vp, vp]
+        L.or_pressure.argtypes = [cp, vp, vp]
+        L.or_kinetic_energy.argtypes = [i, i, vp, vp, vp, vp]
+        L.or_average_cell_p0.argtypes = [cp, vp]
+        L.or_change_basis.argtypes = [cp, i, i, vp, vp, vp]
+        L.or_cons_prim.argtypes = [cp, vp, i]
+        L.or_format_sensor.argtypes = [i, i, vp, vp]
+        for n in ("or_stridedcopy", "or_reconstruct_energy", "or_internal_energy", "or_pressure", "or_kinetic_energy",
+                  "or_average_cell_p0", "or_change_basis", "or_cons_prim", "or_format_sensor"):
+            getattr(L, n).restype = None
         L.or_conservation.argtypes = [cp, vp, vp, vp, vp, vp]
         L.or_conservation.restype = None
         L.or_ref_area.argtypes = [cp]

@@ -1307,6 +1307,118 @@ int or_rk4(const or_cfg* c, const or_tables* t, double* U, int mode, double dt_o
   return or_rk4_out(c, t, U, mode, dt_or_cfl, nsteps, &tf, 1, limiter, on, thresh, t_out, dt_hist, max_hist, nullptr);
 }
 
+// ---- launcher-level restatements of the reference's helper kernels (CPU loop order)
+// stridedcopy (limiting_kernels.cu:51-108)
+void or_stridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA, int offsetB, const double* A,
+                    double* B) {
+  long indA = offsetA, indB = offsetB;
+  for (int i = 0; i < numblocks; i++) {
+    for (int j = 0; j < blocklen; j++) B[indB + j] = A[indA + j];
+    indA += strideA;
+    indB += strideB;
+  }
+}
+// reconstruct_energy (limiting_kernels.cu:111-158)
+void or_reconstruct_energy(int N_s, int N_E, int slicenum, const double* rhoeLim, const double* KLim,
+                           const double* EMono, double* ELim) {
+  for (int e = 0; e < N_E; e++)
+    for (int sl = 0; sl < slicenum; sl++) {
+      const size_t o = ((size_t)e * slicenum + sl) * N_s;
+      for (int i = 1; i < N_s; i++) ELim[o + i] = rhoeLim[o + i] + KLim[o + i];
+      double E0 = EMono[o];
+      for (int k = 2; k < N_s; k += 2) E0 -= 1.0 / ((double)lim_factorial(k + 1)) * (ELim[o + k] - EMono[o + k]);
+      ELim[o] = E0;
+    }
+}
+static int binom(int n, int k) {
+  int r = 1;
+  for (int j = 1; j <= k; j++) r = r * (n - k + j) / j;
+  return r;
+}
+// internal_energy_multifluid / internal_energy_stiffened (limiting_kernels.cu:160-243); b may be NULL
+void or_internal_energy(int N_s, int N_E, int slicenum, const double* p, const double* g, const double* b,
+                        double* rhoe) {
+  for (int e = 0; e < N_E; e++)
+    for (int sl = 0; sl < slicenum; sl++)
+      for (int i = 0; i < N_s; i++) {
+        const size_t o = ((size_t)e * slicenum + sl) * N_s;
+        double I = 0;
+        for (int k = 0; k < i + 1; k++) I += (double)binom(i, k) * p[o + i - k] * g[o + k];
+        rhoe[o + i] = b ? I + b[o + i] : I;
+      }
+}
+// pressure (physics.cu:891-960), every physics (GAMNCON; stiffened E - beta)
+void or_pressure(const or_cfg* c, const double* U, double* p) {
+  const int N_s = c->N_s, N_F = c->N_F, D = c->D, ph = c->physics;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < N_s; i++) {
+      auto u = [&](int f) { return U[((size_t)e * N_F + f) * N_s + i]; };
+      const double rho = u(0), rhou = u(1), rhov = D == 2 ? u(2) : 0.0;
+      double E = u(D + 1);
+      double gamma = GAMMA_SF;
+      if (ph == 1 || ph == 3) gamma = 1.0 + 1.0 / u(D + 2);
+      if (ph == 3) E = E - u(D + 3);
+      p[(size_t)e * N_s + i] = D == 1 ? (gamma - 1) * (E - 0.5 * rhou * rhou / rho)
+                                      : (gamma - 1) * (E - 0.5 * (rhou * rhou + rhov * rhov) / rho);
+    }
+}
+// kinetic_energy1D / kinetic_energy2D (physics.cu:814-890); rhov NULL in 1D
+void or_kinetic_energy(int N_s, int N_E, const double* rho, const double* rhou, const double* rhov, double* K) {
+  for (size_t k = 0; k < (size_t)N_s * N_E; k++)
+    K[k] = rhov ? 0.5 * (rhou[k] * rhou[k] + rhov[k] * rhov[k]) / rho[k] : 0.5 * rhou[k] * rhou[k] / rho[k];
+}
+// average_cell_p0 (rk_kernels.cu:61-92)
+void or_average_cell_p0(const or_cfg* c, double* DU) {
+  const int N_s = c->N_s, N_F = c->N_F;
+  for (int e = 0; e < c->N_E; e++)
+    for (int fc = 0; fc < N_F; fc++) {
+      double average = 0.0;
+      for (int i = 0; i < N_s; i++) average += DU[((size_t)e * N_F + fc) * N_s + i];
+      average = average / N_s;
+      for (int i = 0; i < N_s; i++) DU[((size_t)e * N_F + fc) * N_s + i] = average;
+    }
+}
+// ChangeBasis (limiting_kernels.cu:918-955): per-element transform
+void or_change_basis(const or_cfg* c, int size1, int size2, const double* T, const double* U, double* Unew) {
+  const int N_F = c->N_F;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < size1; i++)
+      for (int fc = 0; fc < N_F; fc++) {
+        double sol = 0.0;
+        for (int ii = 0; ii < size2; ii++)
+          sol += T[((size_t)e * size1 + i) * size2 + ii] * U[((size_t)e * N_F + fc) * size2 + ii];
+        Unew[((size_t)e * N_F + fc) * size1 + i] = sol;
+      }
+}
+// Cons2Prim / Prim2Cons (limiting_kernels.cu:957-1068): alpha is field D+2 (two-fluid 1/(gamma-1), passive rho
+// phic), beta field D+3 for stiffened states, 0 otherwise; GAMNCON
+void or_cons_prim(const or_cfg* c, double* U, int to_prim) {
+  const int N_s = c->N_s, N_F = c->N_F, D = c->D, ph = c->physics;
+  for (int e = 0; e < c->N_E; e++)
+    for (int i = 0; i < N_s; i++) {
+      auto u = [&](int f) -> double& { return U[((size_t)e * N_F + f) * N_s + i]; };
+      const double beta = ph == 3 ? u(D + 3) : 0.0;
+      if (to_prim) {
+        const double rho = u(0), rhou = u(1), rhov = D == 2 ? u(2) : 0.0, E = u(D + 1), alpha = u(D + 2);
+        u(0) = rho;
+        u(1) = rhou / rho;
+        if (D == 2) u(2) = rhov / rho;
+        u(D + 1) = (E - beta - 0.5 * (rhou * rhou + rhov * rhov) / rho) / alpha;
+      } else {
+        const double rho = u(0), uu = u(1), v = D == 2 ? u(2) : 0.0, p = u(D + 1), alpha = u(D + 2);
+        u(0) = rho;
+        u(1) = rho * uu;
+        if (D == 2) u(2) = rho * v;
+        u(D + 1) = p * alpha + beta + 0.5 * rho * (uu * uu + v * v);
+      }
+    }
+}
+// format_sensor (printer_sensor.cu:19-43)
+void or_format_sensor(int N_s, int N_E, const int* sensor, double* output) {
+  for (int e = 0; e < N_E; e++)
+    for (int i = 0; i < N_s; i++) output[(size_t)e * N_s + i] = sensor[e];
+}
+
 // DG_SOLVER::conservation (dg_solver.h:367-389): UgC = phi*U (netlib gemm), then
 // I[fc] += UgC*J_e*w_g over elements and points, one running sum per field.
 void or_conservation(const or_cfg* c, const double* phi, const double* J, const double* weight, const double* U,

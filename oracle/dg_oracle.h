@@ -98,6 +98,23 @@ double or_ref_area(const or_cfg* c); /* 4 for quadrilaterals (main.cc:286), 2 fo
 void or_limit_p0i(const or_cfg* c, const or_tables* t, const int* on, const double* thresh, double* U,
                   int* sensors);
 
+/* Launcher-level restatements of the reference's helper kernels (their CPU loops):
+ * stridedcopy, reconstruct_energy, internal_energy_multifluid (b NULL) / _stiffened, pressure (every physics),
+ * kinetic_energy1D (rhov NULL) / 2D, average_cell_p0, ChangeBasis, Cons2Prim / Prim2Cons (to_prim 1 / 0),
+ * format_sensor. */
+void or_stridedcopy(int numblocks, int blocklen, int strideA, int strideB, int offsetA, int offsetB, const double* A,
+                    double* B);
+void or_reconstruct_energy(int N_s, int N_E, int slicenum, const double* rhoeLim, const double* KLim,
+                           const double* EMono, double* ELim);
+void or_internal_energy(int N_s, int N_E, int slicenum, const double* p, const double* g, const double* b,
+                        double* rhoe);
+void or_pressure(const or_cfg* c, const double* U, double* p);
+void or_kinetic_energy(int N_s, int N_E, const double* rho, const double* rhou, const double* rhov, double* K);
+void or_average_cell_p0(const or_cfg* c, double* DU);
+void or_change_basis(const or_cfg* c, int size1, int size2, const double* T, const double* U, double* Unew);
+void or_cons_prim(const or_cfg* c, double* U, int to_prim);
+void or_format_sensor(int N_s, int N_E, const int* sensor, double* output);
+
 /* DG_SOLVER::conservation (dg_solver.h:367-389): I[fc] = sum_e sum_g (phi*U)[(e,fc),g] * J[e] * weight[g],
  * accumulated one term after another in that order. */
 void or_conservation(const or_cfg* c, const double* phi, const double* J, const double* weight, const double* U,
