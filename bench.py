@@ -1,22 +1,31 @@
 #!/usr/bin/env python
-"""DOF-updates/s of the FP64 DG RK4 step on one MI355X.
+"""DOF-updates/s of the DG RK4 step on one MI355X.
 
-Default workload (vortex): 2D single-fluid isentropic vortex on a periodic
-[0,10]^2 mesh of 512x512 quads, tensor Lagrange p=2 (9 nodes x 4 fields per
-element), Roe flux, no limiting, classic RK4 with a fixed Dt from CFL 0.3
-(deck CFL x Dx/(2p+1) / max(|u|+a)).  Synthetic input (the exact vortex,
-interpolated at the nodes).  One DOF-update = one conserved coefficient
-advanced one full RK4 step (four right-hand sides).
+Default workload (bubble): the 2D two-fluid shock-bubble (Mach 1.22 shock in
+gamma=1.4 gas hitting a circular bubble of radius 0.25, rho=0.138,
+gamma=1.667) on [0,2]x[0,1] with 1024x512 quads, tensor Lagrange p=2, 5
+fields, Roe flux, RK4 with a fixed Dt from CFL 0.4 at t=0.  The default run
+times the reduced-precision form (FP32 map/integrate contractions on the
+matrix cores, FP64 fluxes, RK and limiter; HR applied to the cells the
+learned indicator flags, the indicator trained on the GPU before the timed
+region) and, in the same run and on the same initial state, the plain FP64
+form (FP64 contractions, global HR limiter): `value` is the reduced form's
+throughput, `plain_fp64_form` the plain one's and their ratio.  One
+DOF-update = one conserved coefficient advanced one full RK4 step (four
+right-hand sides).  Synthetic input (the initial condition, interpolated at
+the nodes).
 
-Other workloads (--config): `sod`, the 1D single-fluid Sod tube on 1000
-elements, p=2, Rusanov, HR limiter, CFL 0.4 steps (Dt from the state every
-step); `bubble`, the 2D two-fluid shock-bubble on 1024x512 quads, p=2, 5
-fields, Roe, HR limiter, fixed Dt from CFL 0.4 at t=0; `ensemble`, 256
-independent 1D two-fluid Riemann problems (element counts uniform in
-[200, 4000], orders uniform in {1, 2, 3}, seeded states), Roe, HR limiter,
-each with its own CFL 0.4 step, advanced by one graph launch of K steps.
+Other workloads (--config): `vortex`, the 2D single-fluid isentropic vortex on
+a periodic [0,10]^2 mesh of 512x512 quads, p=2, 4 fields, Roe, no limiter,
+fixed Dt from CFL 0.3; `sod`, the 1D single-fluid Sod tube on 1000 elements,
+p=2, Rusanov, HR limiter, CFL 0.4 steps (Dt from the state every step);
+`ensemble`, 256 independent 1D two-fluid Riemann problems (element counts
+uniform in [200, 4000], orders uniform in {1, 2, 3}, seeded states), Roe, HR
+limiter, each with its own CFL 0.4 step, advanced by one graph launch of K
+steps.  `--precision` and `--limiter` select another form (on `bubble`:
+`--precision fp64 --limiter hr` is the plain form alone).
 
-    python bench.py [--steps K] [--warmup W] [--config vortex|sod|bubble|ensemble] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--config bubble|vortex|sod|ensemble] [--no-cpu-baseline]
 
 Prints one JSON line.  `value` is the timed region's throughput with the
 state already resident in HBM; `roofline` is the dominant kernel's achieved
@@ -41,21 +50,29 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=["vortex", "sod", "bubble", "ensemble"], default="vortex")
+    ap.add_argument("--config", choices=["bubble", "vortex", "sod", "ensemble"], default="bubble")
     ap.add_argument("--n", type=int, default=0, help="vortex: elements per direction (default 512)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=["fp64", "fp32mfma", "splitbf16"], default="fp64",
+    ap.add_argument("--precision", choices=["auto", "fp64", "fp32mfma", "splitbf16"], default="auto",
                     help="tile-kernel volume contractions: FP64, FP32 on MFMA, or split-BF16 products with FP32 "
-                         "accumulation (2D p=2; the rest stays FP64)")
+                         "accumulation (2D p=2; the rest stays FP64); auto: FP32 on the bubble, FP64 elsewhere")
     ap.add_argument("--path", choices=["auto", "general", "tiled"], default="auto",
                     help="kernel path: fused tile kernel (auto on this mesh) or face + element kernels")
     ap.add_argument("--limiter", choices=["auto", "none", "hr", "hri", "learned"], default="auto",
-                    help="limiter between stages: the workload's own (auto), none, global HR, sensor-gated HR, or HR "
-                         "gated by the learned indicator (trained on the GPU before the timed run; 2D two-fluid p=2)")
+                    help="limiter between stages: auto (learned on the bubble, else the workload's own), none, "
+                         "global HR, sensor-gated HR, or HR gated by the learned indicator (trained on the GPU before "
+                         "the timed run; 2D two-fluid p=2)")
+    ap.add_argument("--training", choices=["riemann", "full"], default="riemann",
+                    help="indicator training data: seeded random Riemann mosaics and interfaces (riemann), or those "
+                         "plus random shocked bubbles and smooth flows (full)")
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="CPU sample steps (0: workload default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB buffer 5 times (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.precision == "auto":
+        args.precision = "fp32mfma" if args.config == "bubble" else "fp64"
+    if args.limiter == "auto" and args.config == "bubble":
+        args.limiter = "learned"
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs (--gpus N > 1) are not part of this build "
@@ -113,11 +130,12 @@ def main() -> int:
         solver.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
     if lim == 8:  # train the indicator on the GPU (own forward/backward kernels), outside the timed region
         t_tr = time.perf_counter()
-        data = indicator.training_set()
+        data = indicator.training_set(args.training)
         net, losses, thr = indicator.train(data, epochs=40)
         solver.set_indicator(net, thr)
         extra["indicator"] = {"training_s": round(time.perf_counter() - t_tr, 2), "samples": int(data.y.size),
-                              "final_loss": float(losses[-1]), "threshold": thr}
+                              "final_loss": float(losses[-1]), "threshold": thr,
+                              "training_data": indicator.TRAINING_SETS[args.training]}
 
     def make_advance(sv, Ux, lim_x):
         def adv(k):
@@ -164,8 +182,15 @@ def main() -> int:
         adv_p(args.steps)
         torch.cuda.synchronize()
         ms_plain = 1e3 * (time.perf_counter() - tp) / args.steps
-        extra["plain_fp64_form"] = {"limiter": {0: "none", 1: "hr"}.get(w["limiter"], w["limiter"]),
-                                    "ms_per_step": ms_plain, "speedup_of_this_form": ms_plain / ms_per_step}
+        torch.cuda.synchronize()
+        ap_ = Up.cpu().numpy().reshape(N_E, NF, N_s)
+        af_ = U.cpu().numpy().reshape(N_E, NF, N_s)
+        extra["plain_fp64_form"] = {
+            "limiter": {0: "none", 1: "global HR"}.get(w["limiter"], w["limiter"]), "contractions": "FP64",
+            "ms_per_step": ms_plain, "value": n_dof * args.steps / (ms_plain * 1e-3 * args.steps),
+            "speedup_of_this_form": ms_plain / ms_per_step, "target_speedup": 2.0,
+            # the two forms' states after the same warmup + K steps (limiter and precision differ)
+            "rel_l2_this_form_vs_plain": float(np.linalg.norm(af_ - ap_) / np.linalg.norm(ap_))}
     if args.precision != "fp64":
         # precision error: the same limiter in FP64 from the same state, warmup + K steps
         sf = Solver(cfg, mesh)
@@ -328,7 +353,8 @@ def bench_ensemble(args, device):
             t_cpu += time.perf_counter() - t1
             dof_steps += states[k].size * max(steps, 0)
         out["cpu_baseline"] = {"value": dof_steps / t_cpu, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
-                               "sample": f"16 of the problems, 20 RK4 steps each, FP64 oracle, {t_cpu:.1f} s"}
+                               "sample": f"16 of the problems, 20 RK4 steps each, FP64 oracle, {t_cpu:.1f} s",
+                               **host_cpu()}
     print(json.dumps(out))
     return 0
 
@@ -426,8 +452,28 @@ def cpu_baseline(args, device, w):
         "value": U0.size * steps / el, "unit": "DOF-updates/s", "cores": 1, "kind": "port",
         "sample": f"{steps} RK4 step(s) of the same workload ({U0.size} DOFs), FP64 oracle "
                   f"(oracle/dg_oracle.cc, g++ -O2), {el:.1f} s",
+        **host_cpu(),
     }
     return cpu, err
+
+
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: logical CPUs visible to this process
+    and the CPU model string."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        visible = os.cpu_count()
+    return {"host_nproc": visible, "host_cpu_model": model}
 
 
 if __name__ == "__main__":

@@ -103,6 +103,7 @@ def lib() -> ctypes.CDLL:
         L.dgx_mlp_train.argtypes = [vp, vp, vp, ctypes.c_long, i, i, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_ulonglong, vp, vp]
         L.dgx_solver_set_indicator.argtypes = [vp, vp, ctypes.c_float]
+        L.dgx_solver_set_fused_means.argtypes = [vp, i]
         L.dgx_mlp_features.argtypes = [vp, vp, vp, vp]
         L.dgx_indicator_flags.argtypes = [vp, vp, vp, vp, vp]
         L.dgx_solver_set_path.argtypes = [vp, i]
@@ -355,6 +356,11 @@ class Solver:
         _check(lib().dgx_solver_set_indicator(self._h, mlp._h if mlp else None, threshold),
                "dgx_solver_set_indicator")
 
+    def set_fused_means(self, on: bool) -> None:
+        """With limiter 8 on the tile kernel, let the RK stages write the cell
+        means the indicator reads (default) or leave them to its own pass."""
+        _check(lib().dgx_solver_set_fused_means(self._h, 1 if on else 0), "dgx_solver_set_fused_means")
+
     def features(self, U, X, stream=None) -> None:
         """Indicator features of U into X (float32 device array N_E x 64)."""
         _check(lib().dgx_mlp_features(self._h, _ptr(U), _ptr(X), _stream(stream)), "dgx_mlp_features")
@@ -413,11 +419,14 @@ class Batch:
 
     @property
     def block_sizes(self):
+        """[workgroup size, workgroups in use, requested workgroups (0: all the device holds)]."""
         z = (ctypes.c_int * 3)()
         _check(lib().dgx_batch_block_sizes(self._h, z), "dgx_batch_block_sizes")
         return list(z)
 
     def set_block_sizes(self, sizes) -> None:
+        """sizes = [workgroup size, workgroups (0: all the device holds)]."""
+        sizes = list(sizes) + [0] * (3 - len(sizes))
         z = (ctypes.c_int * 3)(*sizes)
         _check(lib().dgx_batch_set_block_sizes(self._h, z), "dgx_batch_set_block_sizes")
 

@@ -364,7 +364,8 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
          upload(s, &s.t.Lag2MonoY, t.Lag2MonoY.data(), t.Lag2MonoY.size()) &&
          upload(s, &s.t.MonoX2Lag, t.MonoX2Lag.data(), t.MonoX2Lag.size());
   s.hr_const = D == 2 && s.P <= 2 && hr_tables_match(s.P, t.Lag2MonoX.data(), t.MonoX2MonoY.data(), t.MonoY2Lag.data());
-  if (s.cart.on && s.hr_const && s.NF <= 5) {
+  // k_hr2d's tiles are 8 x 8 or 8 x 4 elements whatever the stage tile
+  if (s.cart.on && s.hr_const && s.NF <= 5 && s.cart.nx % 8 == 0 && s.cart.ny % 4 == 0) {
     // the one-launch HR limiter assumes e -/+ 1 and e -/+ nx inside a tile,
     // takes the rows below and above it and the ends of every row from the
     // table, and takes the x neighbours of a halo-row element to be the halo
@@ -561,17 +562,29 @@ static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, 
   // buffers (U -> Ustar -> Ustar2 -> Ustar -> U); the face + element path
   // consumes all traces in the face kernel and updates Ustar in place.
   const bool tiled = s.tiled();
+  // with the learned indicator the tile kernel also writes its output's cell
+  // means, and the indicator that follows skips its own pass over the state
+  double* means = (limiter == 8 && s.mlp && s.fuse_means) ? s.mlp_avg : nullptr;
+  bool means_ready = false;
+  auto limit = [&](double* V) {
+    s.means_of = means_ready ? V : nullptr;
+    const dgx_status q = timed_limit(s, limiter, V, st);
+    s.means_of = nullptr;
+    return q;
+  };
   double* cur = U;
   for (int k = 0; k < 4; ++k) {
     const double* Uin = cur;
-    if (k > 0 && (r = timed_limit(s, limiter, cur, st)) != DGX_OK) return r;
+    if (k > 0 && (r = limit(cur)) != DGX_OK) return r;
     double* Uout = (k == 3) ? U : ((tiled && k == 1) ? s.Ustar2 : s.Ustar);
     cur = Uout;
     if (tiled) {
       KernelTimer kt(s, 4, st);
-      if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "tile kernel")) return DGX_EDEVICE;
+      if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st, means, &means_ready), "tile kernel"))
+        return DGX_EDEVICE;
       continue;
     }
+    means_ready = false;
     {
       KernelTimer kt(s, 0, st);
       if (!hip_ok(launch_face(s, Uin, st), "face kernel")) return DGX_EDEVICE;
@@ -581,7 +594,7 @@ static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, 
       if (!hip_ok(launch_elem(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st), "element kernel")) return DGX_EDEVICE;
     }
   }
-  return timed_limit(s, limiter, U, st);
+  return limit(U);
 }
 
 dgx_status dgx_rk4_step(dgx_solver* h, double* U, const double* dt_dev, int limiter, void* stream) {
@@ -748,6 +761,12 @@ dgx_status dgx_solver_set_indicator(dgx_solver* h, dgx_mlp* mlp, float threshold
   }
   s.mlp = mlp;
   s.mlp_threshold = threshold;
+  return DGX_OK;
+}
+
+dgx_status dgx_solver_set_fused_means(dgx_solver* h, int on) {
+  if (!h) return fail(DGX_EINVAL, "dgx_solver_set_fused_means: null solver");
+  h->s.fuse_means = on != 0;
   return DGX_OK;
 }
 

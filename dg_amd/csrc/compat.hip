@@ -371,20 +371,20 @@ void Lcalc_sensors(int N_E, int N_N, dgx_bool sensor1, double thresh1, dgx_bool 
   tmp.D = g_cfg.dim;
   tmp.N_E = N_E;
   tmp.N_N = N_N;
-  int* scratch = nullptr;
+  int* buf = nullptr;
   int* flag = nullptr;
-  if (!hip_ok(hipMalloc(&scratch, sizeof(int) * 5 * (size_t)N_E), "Lcalc_sensors") ||
+  if (!hip_ok(hipMalloc(&buf, sizeof(int) * 5 * (size_t)N_E), "Lcalc_sensors") ||
       !hip_ok(hipMalloc(&flag, sizeof(int)), "Lcalc_sensors")) {
-    hipFree(scratch);
+    hipFree(buf);
     return;
   }
   tmp.flag = flag;
   int on[3] = {sensor1, sensor2, sensor3};
   double th[3] = {thresh1, thresh2, thresh3};
-  hipError_t e = compat_sensors(tmp, neighbors, Uavg, sensors, on, th, scratch);
+  hipError_t e = compat_sensors(tmp, neighbors, Uavg, sensors, on, th, buf);
   if (e != hipSuccess) hip_ok(e, "Lcalc_sensors");
   finish("Lcalc_sensors");
-  hipFree(scratch);
+  hipFree(buf);
   hipFree(flag);
 }
 

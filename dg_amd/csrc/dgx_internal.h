@@ -101,6 +101,10 @@ struct Solver {
   dgx_mlp* mlp = nullptr;
   float mlp_threshold = 0.0f;
   double* mlp_avg = nullptr;  // [e*3 + {rho, p, alpha}] cell means
+  // the state whose cell means mlp_avg holds, set by the RK driver between a
+  // stage that emitted them and the indicator that reads them (else null)
+  const double* means_of = nullptr;
+  bool fuse_means = true;      // RK stages emit the indicator's cell means (dgx_solver_set_fused_means)
   float* mlp_jumps = nullptr;  // [e*4 + n] neighbour jumps L, R, D, U (-1: boundary)
   float* mlp_x = nullptr;     // [e*64 + k] features
   float* mlp_z = nullptr;     // [e] logits
@@ -118,8 +122,11 @@ hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, 
                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st);
 // fused tile kernel (Cartesian meshes): same modes as launch_elem
 bool stage_tiles(int D, int nx, int ny, int* tx, int* ty);
+// means: also write the output tile's cell means of rho, p and alpha there
+// (2D p=2 two-fluid RK stages); *means_done tells whether the kernel did
 hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
-                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st);
+                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st, double* means = nullptr,
+                        bool* means_done = nullptr);
 hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* dt_dev, hipStream_t st);
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 // Out = HR(In), both 2D sweeps in one launch; In != Out, needs s.hr2d

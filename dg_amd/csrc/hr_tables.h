@@ -8,11 +8,15 @@
 //
 // Each line below is one column l.  Most entries are exact zeros and the
 // rest are small dyadic numbers, so a kernel that knows them at compile time
-// skips the zero terms and the multiplications by +-1.  That gives the same
-// bits as the dense product y[r] = sum_l x[l]*M[r + l*Ns] accumulated from
-// +0.0 in order: a +-0 product added to a non-zero sum leaves it unchanged,
-// and in round-to-nearest such a sum is never -0, to which +0 + (-0) = +0
-// would apply.  The host compares the built tables with these entry by entry
+// skips the zero terms and the multiplications by +-1.  For finite inputs
+// that gives the same bits as the dense product y[r] = sum_l x[l]*M[r + l*Ns]
+// accumulated from +0.0 in order: a +-0 product added to a non-zero sum
+// leaves it unchanged, and in round-to-nearest such a sum is never -0, to
+// which +0 + (-0) = +0 would apply.  An Inf or NaN input differs: the dense
+// product spreads 0*Inf = NaN to every output of its column, while the
+// skipped zero terms leave the outputs that do not depend on that input
+// finite (a blown-up state then comes out partly finite, where the
+// reference's gemm and the two-sweep path give NaN).  The host compares the built tables with these entry by entry
 // (hr_tables_match) before it selects that kernel.
 #pragma once
 

@@ -209,7 +209,8 @@ __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, cons
 // k_hr_sweep in 2D for p <= 2 with the transforms of hr_tables.h as
 // compile-time constants (the host has found the solver's tables equal to
 // them, Solver::hr_const): YS 0 is the x sweep (Lag2MonoX, then MonoX2MonoY),
-// YS 1 the y sweep (MonoY2Lag).  The same bits as k_hr_sweep.
+// YS 1 the y sweep (MonoY2Lag).  The same bits as k_hr_sweep for finite
+// states (the skipped zero terms do not spread an Inf or NaN through a column).
 template <int P, int YS>
 __global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, const int* __restrict__ nb,
                                                     const double* __restrict__ In, double* __restrict__ Out) {
@@ -256,8 +257,9 @@ __global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, co
 //      halo rows, from LDS into LDS;
 //   4. the y sweep (hrl along y, MonoY2Lag) of the tile rows, staged in LDS
 //      and stored row by row (a tile row is TXE*NF*Ns contiguous values).
-// Every column gets the bits of the two k_hr_sweep launches (the products
-// with zero table entries that it leaves out change no sum), and the
+// Every column of a finite state gets the bits of the two k_hr_sweep
+// launches (the products with zero table entries that it leaves out change
+// no finite sum; an Inf or NaN is not spread through its column), and the
 // intermediate state never reaches HBM.  In and Out must differ:
 // neighbouring tiles read In's rows.
 template <int Ns, int N, int SL, int NF, int TXE, int TYE, int BS, bool OUT_VEC2>
@@ -1098,6 +1100,8 @@ hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStre
   // per output column) and ran 167 against 177 us on the 1024 x 512 p = 2
   // two-fluid state
   constexpr int TXE = 8;
+  // every element must lie in a tile (the eligibility check, Solver::hr2d, asks the same)
+  if (s.cart.nx % TXE != 0 || s.cart.ny % 4 != 0) return hipErrorInvalidValue;
   const int tye = s.cart.ny % 8 == 0 ? 8 : 4;
   const int tiles = (s.cart.nx / TXE) * (s.cart.ny / tye);
   return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {

@@ -65,8 +65,10 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 }
 
 // ------------------------------------------------------------- features
-// cell means (tensor-Legendre mode 0) of rho, p and alpha = 1/(gamma-1); a
-// block stages 64 elements with coalesced loads, one lane per element reduces
+// cell means of rho, p and alpha = 1/(gamma-1) (cell_means_rpa); a block
+// stages 64 elements with coalesced loads, one lane per element reduces.  The
+// stage kernel emits the same values from its output tile when the RK driver
+// limits with the indicator (Solver::means_of), and this pass is skipped then
 __global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U, const double* __restrict__ w0, int N_E,
                                                    double* __restrict__ avg) {
   constexpr int NF = 5, Ns = 9, NU = NF * Ns;
@@ -87,20 +89,11 @@ __global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U,
   __syncthreads();
   if ((int)threadIdx.x >= ne) return;
   const int e = e0 + threadIdx.x;
-  const double* u = sU + threadIdx.x * NU;
-  double m[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < Ns; ++i) s += w0[i] * u[f * Ns + i];
-    m[f] = s;
-  }
-  const double gam = 1.0 + 1.0 / m[4];
-  const double p = (gam - 1.0) * (m[3] - 0.5 * (m[1] * m[1] + m[2] * m[2]) / m[0]);
-  avg[e * 3 + 0] = m[0];
-  avg[e * 3 + 1] = p;
-  avg[e * 3 + 2] = m[4];
+  double a3[3];
+  cell_means_rpa(sU + threadIdx.x * NU, w0, a3);
+  avg[e * 3 + 0] = a3[0];
+  avg[e * 3 + 1] = a3[1];
+  avg[e * 3 + 2] = a3[2];
 }
 
 __device__ inline double rel_jump(double a, double b) {
@@ -710,6 +703,8 @@ hipError_t forward(dgx_mlp* m, const float* X, const int* idx, long n, float* z,
 // mean gradient of the weighted BCE over samples idx[0..n) (or rows 0..n)
 hipError_t gradient(dgx_mlp* m, const float* X, const int* idx, const float* y, long n, float pos_weight,
                     hipStream_t st) {
+  // the stored activations and the per-block partials are sized for m->cap samples
+  if (n <= 0 || n > m->cap || !m->z || !m->h1 || !m->h2 || !m->partial) return hipErrorInvalidValue;
   hipError_t e = forward(m, X, idx, n, m->z, m->h1, m->h2, st);
   if (e != hipSuccess) return e;
   const unsigned nb = nblocks_for(n, kSB);
@@ -737,7 +732,8 @@ hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool 
   dgx_mlp* m = s.mlp;
   const int N = s.N_E;
   int* count = with_list ? s.flag + 2 : nullptr;
-  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  if (s.means_of != U)  // else the stage kernel that wrote U left its cell means in s.mlp_avg
+    hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
                      reinterpret_cast<float4*>(s.mlp_jumps), count);
   const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
@@ -865,6 +861,8 @@ dgx_status dgx_mlp_train(dgx_mlp* m, const float* X, const float* y, long n, int
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st, m->loss, 1, 0.0f);
     for (long b0 = 0; b0 < n; b0 += batch) {
       const long nb = std::min<long>(batch, n - b0);
+      if (nb > m->cap || b0 + nb > m->idx_cap)  // the minibatch's rows of idx and its activations exist
+        return mfail(DGX_EINVAL, "dgx_mlp_train: minibatch beyond the training scratch");
       if (!hip_ok(gradient(m, X, m->idx + b0, y, nb, pos_weight, st), "mlp gradient")) return DGX_EDEVICE;
       m->step += 1;
       a.c1 = 1.0f - std::pow(a.b1, (float)m->step);

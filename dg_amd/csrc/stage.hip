@@ -45,6 +45,10 @@ struct StageArgs {
   double* Uout;
   const double* __restrict__ dt;
   double* fout;
+  // 2D p=2 two-fluid RK stages: cell means of rho, p and alpha of the output
+  // tile for the learned indicator ([e*3 + k], cell_means_rpa with w0), or null
+  double* means;
+  const double* __restrict__ w0;
 };
 
 template <int D, int TX, int TY>
@@ -499,7 +503,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       if constexpr (MODE == 0) {
         a.fout[gi] = sU[el * SU + rr];
       } else if (MFW && shared_minv) {
-        rk_combine(a.stage, sU[el * SU + rr], pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+        const double o = rk_combine(a.stage, sU[el * SU + rr], pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+        if constexpr (PH == 1) {
+          if (a.means) sU[el * SU + rr] = o;  // this thread's own entry: kept for the cell means below
+        }
       } else {
         // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
         const double dte = sDte[el];
@@ -511,6 +518,19 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 #pragma unroll
         for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
         rk_combine(a.stage, dte * sol, pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+      }
+    }
+    if constexpr (MODE == 1 && MFW && PH == 1) {
+      if (a.means && shared_minv) {
+        __syncthreads();  // the output tile is in sU
+        if (tid < NT) {
+          double a3[3];
+          cell_means_rpa(sU + tid * SU, a.w0, a3);
+          double* m = a.means + ((size_t)(ey0 + tid / TX) * a.nx + ex0 + tid % TX) * 3;
+          m[0] = a3[0];
+          m[1] = a3[1];
+          m[2] = a3[2];
+        }
       }
     }
     __syncthreads();  // LDS is refilled for the next tile
@@ -566,7 +586,9 @@ static hipError_t dispatch_stage(const dgx_config& c, Fn&& fn) {
 }
 
 hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
-                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st) {
+                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st, double* means,
+                        bool* means_done) {
+  if (means_done) *means_done = false;
   const Cart& c = s.cart;
   StageArgs a{};
   a.phi = s.t.phi; a.dphi = s.t.dphi; a.phi_w = s.t.phi_w; a.dphi_w = s.t.dphi_w;
@@ -579,6 +601,7 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
   a.tiles_y = c.ny / c.ty;
   const int ntiles = a.tiles_x * a.tiles_y;
   a.stage = stage; a.Uin = Uin; a.Un = Un; a.Uacc = Uacc; a.Uout = Uout; a.dt = dt_dev; a.fout = f_out;
+  a.w0 = s.t.mlp_vinv;  // first row: the mode-0 weights
   return dispatch_stage(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
@@ -599,6 +622,15 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
         nb = std::min(ntiles, per_cu * cus);
     }
     a.xcd_bands = (ntiles % 8 == 0 && nb % 8 == 0) ? 1 : 0;
+    // the epilogue's cell means: RK stages of 2D p=2 two-fluid tiles on the
+    // matrix-core element phases with the shared Minv (k_stage's MFW path)
+    constexpr bool mfw = D == 2 && P == 2 && PH == 1 && (TX * TY) % 16 == 0;
+    a.means = nullptr;
+    if (mfw && mode == 1 && means && s.t.mlp_vinv && s.t.minv_stride == 0 &&
+        (s.precision >= 1 || DGX_STAGE_MFMA64)) {
+      a.means = means;
+      if (means_done) *means_done = true;
+    }
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, a);
     return hipGetLastError();
   });

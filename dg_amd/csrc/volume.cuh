@@ -198,17 +198,20 @@ __device__ inline void vol_phase2(const VolTables<D, P, PH>& t, int i, const dou
 
 // RK4 stage combination for one state entry (rk.cc:164-198 with DU fused):
 // un = U^n, acc = accumulator (read for stages 1-3), du = Dt*Minv*f.
-__device__ inline void rk_combine(int st, double du, double un, double acc, double* acc_out, double* out) {
+__device__ inline double rk_combine(int st, double du, double un, double acc, double* acc_out, double* out) {
   const double gk = kRkGamma[st];
+  double o;
   if (st == 0) {
     *acc_out = un + gk * du;
-    *out = un + kRkBeta[1] * du;
+    o = un + kRkBeta[1] * du;
   } else if (st < 3) {
     *acc_out = acc + gk * du;
-    *out = un + kRkBeta[st + 1] * du;
+    o = un + kRkBeta[st + 1] * du;
   } else {
-    *out = acc + gk * du;
+    o = acc + gk * du;
   }
+  *out = o;
+  return o;  // the stage output written
 }
 
 // Phase 3 for one state entry r = f*Ns + i of an element whose right-hand

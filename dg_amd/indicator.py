@@ -262,11 +262,25 @@ def smooth_data(n_problems: int = 16, nx: int = 64, ny: int = 32, snapshots=(0, 
     return Dataset(X=Xa, y=(sa != 0).astype(np.float32), sensor=sa)
 
 
-def training_set(device="cuda") -> Dataset:
-    """The default training data: Riemann mosaics, random interfaces, random
-    shocked bubbles and smooth flows, all labelled by the sensor."""
-    return merge(generate(device=device), interface_data(device=device), bubble_data(device=device),
-                 smooth_data(device=device))
+TRAINING_SETS = {
+    "riemann": "4096 seeded random Riemann problems (64x64 mosaic of 8x8-cell blocks, rho and p log-uniform in "
+               "[0.1, 10], u in [-2, 2], gamma in [1.1, 1.7]) and 32 random multimode interfaces (half shocked), "
+               "labelled by the sensor",
+    "full": "the riemann set plus 24 random shocked bubbles and 16 smooth random flows, labelled by the sensor",
+}
+
+
+def training_set(kind: str = "riemann", device="cuda") -> Dataset:
+    """Training data labelled by the sensor.  `riemann`: the seeded random
+    Riemann mosaics and random multimode interfaces only (no shock-bubble
+    geometry, so the shock-bubble is an out-of-sample evaluation); `full`:
+    those plus random shocked bubbles and smooth flows."""
+    parts = [generate(device=device), interface_data(device=device)]
+    if kind == "full":
+        parts += [bubble_data(device=device), smooth_data(device=device)]
+    elif kind != "riemann":
+        raise ValueError(f"training_set: unknown kind {kind!r}")
+    return merge(*parts)
 
 
 def merge(*parts: Dataset) -> Dataset:

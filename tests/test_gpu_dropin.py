@@ -49,8 +49,15 @@ def D(x, device):
     return torch.from_numpy(np.ascontiguousarray(x)).to(device)
 
 
+class _DevPtr(vp):
+    """A device pointer that keeps its tensor alive for the call it is passed
+    to (a temporary's block could otherwise be handed to the next argument)."""
+
+
 def P(t):
-    return vp(t.data_ptr())
+    p = _DevPtr(t.data_ptr())
+    p.keep = t
+    return p
 
 
 def H(t):
@@ -247,8 +254,10 @@ def test_energy_and_copy_launchers_bitwise(device):
     lib, O = L(), oracle.lib()
     _cfg(lib, (2, STIFFENED, ROE))
     rng = np.random.default_rng(6)
-    A = rng.standard_normal(500)
+    A = rng.standard_normal(600)
     for args in ((2, 3, 9, 3, 0, 0), (7, 4, 20, 5, 3, 1), (12, 9, 45, 9, 18, 2)):
+        nb, bl, sa, sb, oa, ob = args
+        assert oa + (nb - 1) * sa + bl <= A.size and ob + (nb - 1) * sb + bl <= 300  # every copied block in bounds
         B_ref = np.full(300, 7.0)
         O.or_stridedcopy(*args, _vp(A), _vp(B_ref))
         Bd = D(np.full(300, 7.0), device)

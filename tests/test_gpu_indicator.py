@@ -138,3 +138,43 @@ def test_learned_limiter_is_exact_on_its_flags(device):
     wall = (t["neighbors"].reshape(-1, 4) < 0).any(axis=1)  # flagged by rule, as by the sensor
     assert np.array_equal(f, ((zz > thr) | wall).astype(np.int32))
     assert np.array_equal(Ud.cpu().numpy(), prob.limit_flagged(U0, f))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_stage_emitted_means_give_the_same_run(device, prec):
+    """The tile kernel's epilogue writes the cell means the indicator reads
+    (dgx_solver_set_fused_means, on by default): 6 RK4 steps limited by the
+    learned indicator on the shock-bubble at 128x64 give the same state bit
+    for bit with and without it, in FP64 and with the FP32 contractions.  An
+    untrained seeded network with the threshold at the median logit of the
+    initial state flags a large share of the cells."""
+    from dg_amd import PATH_TILED
+
+    mesh = problems.shock_bubble_mesh(128, 64)
+    U0 = problems.shock_bubble_state(mesh, xs=0.24)
+    net = Indicator(seed=3)
+    out = []
+    for fused in (True, False):
+        solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+        assert solver.path == PATH_TILED
+        solver.set_precision(prec)
+        solver.set_indicator(net, 0.0)
+        Ud = torch.from_numpy(U0).to(device)
+        z = torch.zeros(mesh.sizes["N_E"], dtype=torch.float32, device=device)
+        flags = torch.zeros(mesh.sizes["N_E"], dtype=torch.int32, device=device)
+        solver.indicator_flags(Ud, flags, z)
+        torch.cuda.synchronize()
+        thr = float(np.median(z.cpu().numpy()))
+        solver.set_indicator(net, thr)
+        solver.set_fused_means(fused)
+        dt_dev = torch.zeros(1, dtype=torch.float64, device=device)
+        solver.dt_from_cfl(Ud, 0.4 * mesh.dx / 5, dt_dev)
+        dt = float(dt_dev.item())
+        solver.rk4_integrate(Ud, 0, dt, nsteps=6, limiter=LIMIT_LEARNED)
+        solver.indicator_flags(Ud, flags)
+        torch.cuda.synchronize()
+        out.append((Ud.cpu().numpy(), flags.cpu().numpy()))
+    assert 0 < out[0][1].sum() < out[0][1].size
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+    assert not np.array_equal(out[0][0], U0)
