@@ -62,9 +62,9 @@ def main() -> int:
                     help="limiter between stages: auto (learned on the bubble, else the workload's own), none, "
                          "global HR, sensor-gated HR, or HR gated by the learned indicator (trained on the GPU before "
                          "the timed run; 2D two-fluid p=2)")
-    ap.add_argument("--training", choices=["riemann", "full"], default="riemann",
-                    help="indicator training data: seeded random Riemann mosaics and interfaces (riemann), or those "
-                         "plus random shocked bubbles and smooth flows (full)")
+    ap.add_argument("--training", choices=["riemann", "interfaces", "full"], default="riemann",
+                    help="indicator training data: seeded random Riemann mosaics and smooth flows (riemann), plus "
+                         "random multimode interfaces (interfaces), plus random shocked bubbles (full)")
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="CPU sample steps (0: workload default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB buffer 5 times (HBM counter calibration for rocprofv3 --pmc)")

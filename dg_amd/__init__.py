@@ -104,6 +104,7 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_ulonglong, vp, vp]
         L.dgx_solver_set_indicator.argtypes = [vp, vp, ctypes.c_float]
         L.dgx_solver_set_fused_means.argtypes = [vp, i]
+        L.dgx_solver_set_shock_guard.argtypes = [vp, i]
         L.dgx_mlp_features.argtypes = [vp, vp, vp, vp]
         L.dgx_indicator_flags.argtypes = [vp, vp, vp, vp, vp]
         L.dgx_solver_set_path.argtypes = [vp, i]
@@ -361,6 +362,12 @@ class Solver:
         means the indicator reads (default) or leave them to its own pass."""
         _check(lib().dgx_solver_set_fused_means(self._h, 1 if on else 0), "dgx_solver_set_fused_means")
 
+    def set_shock_guard(self, on: bool) -> None:
+        """With the sensor thresholds set, let the indicator also flag every
+        pair of cells the sensor's shock test marks (default), or leave the
+        flags to the network alone."""
+        _check(lib().dgx_solver_set_shock_guard(self._h, 1 if on else 0), "dgx_solver_set_shock_guard")
+
     def features(self, U, X, stream=None) -> None:
         """Indicator features of U into X (float32 device array N_E x 64)."""
         _check(lib().dgx_mlp_features(self._h, _ptr(U), _ptr(X), _stream(stream)), "dgx_mlp_features")
@@ -419,13 +426,13 @@ class Batch:
 
     @property
     def block_sizes(self):
-        """[workgroup size, workgroups in use, requested workgroups (0: all the device holds)]."""
+        """[workgroup size, workgroups (one per problem), 0]."""
         z = (ctypes.c_int * 3)()
         _check(lib().dgx_batch_block_sizes(self._h, z), "dgx_batch_block_sizes")
         return list(z)
 
     def set_block_sizes(self, sizes) -> None:
-        """sizes = [workgroup size, workgroups (0: all the device holds)]."""
+        """sizes = [workgroup size (a multiple of 64 in [64, 1024])]."""
         sizes = list(sizes) + [0] * (3 - len(sizes))
         z = (ctypes.c_int * 3)(*sizes)
         _check(lib().dgx_batch_set_block_sizes(self._h, z), "dgx_batch_set_block_sizes")

@@ -3,23 +3,20 @@
 // its own CFL time step (rk.cc:250-272), the HR limiter (limiting.cc:11-27)
 // and the DG right-hand side of DG_SOLVER::dg_solver (dg_solver.h:292-364).
 //
-// One persistent kernel for every order.  The host splits each problem into
-// contiguous element chunks of about equal work (DOFs) and gives each chunk
-// a workgroup, so that a 4000-element p = 3 problem runs on several CUs while
-// a 200-element p = 1 problem takes one workgroup; the grid never exceeds
-// what the device holds at once.  The workgroups of one problem meet at a
-// barrier of their own (problem_sync: an arrival counter per problem and
-// agent-scope fences) between the phases that read a neighbouring chunk's
-// data: the Dt reduction, the face pass, and the two halves of the limiter.
-// Per step: Dt from the problem's maximum of |u|+a; per stage: a face pass
-// (one lane per interface, Riemann flux integrated into element-major edge
-// slots), an element pass (one lane per element, volume terms in registers
-// with the element-local helpers of volume.cuh, Dt*Minv and the RK4
-// combination), and the HR limiter (Lagrange->Taylor, per-column limiting
-// with the neighbours' Taylor coefficients, Taylor->Lagrange, without FMA
-// contraction as in the CPU path).  The launch is captured once in a hipGraph
-// (after a fill node that zeroes the barrier counters) and replayed; the
-// workgroup size and the number of chunks come from a timing autotuner.
+// One persistent kernel for every order: a workgroup owns one problem and
+// runs all its steps, stages and limiter sweeps, separated by workgroup
+// barriers only (the problems never exchange data).  Problems are dispatched
+// largest first.  Per step: Dt from the workgroup maximum of |u|+a; per
+// stage: a face pass (one lane per interface, Riemann flux integrated into
+// element-major edge slots), an element pass (one lane per element, volume
+// terms accumulated point by point with the element-local helpers of
+// volume.cuh, Dt*Minv and the RK4 combination), and the HR limiter
+// (Lagrange->Taylor, per-column limiting with the neighbours' Taylor
+// coefficients, Taylor->Lagrange, without FMA contraction as in the CPU
+// path).  The launch is captured once in a hipGraph and replayed; the
+// workgroup size (up to 1024 threads, so that the few largest problems, which
+// set the time, keep four waves on every SIMD of their CU) comes from a
+// timing autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,12 +47,7 @@ struct BatchTables {
 
 struct BatchArgs {
   BatchTables tab[3];       // orders 1..3
-  // work split: workgroup w advances elements [wg_e0[w], wg_e1[w]) of problem wg_prob[w]
-  const int* wg_prob;
-  const int* wg_e0;
-  const int* wg_e1;
-  const int* prob_wg0;      // [problem] its first workgroup
-  const int* prob_nwg;      // [problem] its number of workgroups
+  const int* wg_prob;       // [workgroup] its problem (largest first)
   const int* order;         // [problem] 1..3
   const int* n_elem;        // [problem]
   const long* state_off;    // [problem] doubles into the packed state
@@ -73,10 +65,7 @@ struct BatchArgs {
   double* A;    // limiter: Taylor coefficients
   double* Qe;   // [element][edge][field]
   double* T;    // [problem] accumulated time
-  int* status;  // [problem] 0 ok, 1 Dt too small or NaN, 2 a barrier wait timed out
-  unsigned* bar;             // [problem] barrier arrivals (zero at launch)
-  double* dtp;               // [workgroup] partial maxima of |u|+a
-  long long sync_ticks;      // wall-clock ticks after which a barrier wait gives up
+  int* status;  // [problem] 0 ok, 1 Dt too small or NaN
   int nsteps, limiter, limit_initial;
 };
 
@@ -90,44 +79,7 @@ struct BatchShape {
   static constexpr int NF = Fields<1, PH>::NF, Ns = P + 1, NG = P + 1, NU = NF * Ns;
 };
 
-// Barrier of the nwg workgroups that advance one problem.  Arrivals are
-// counted in *ctr, which starts at zero and is never reset within a launch:
-// the g-th barrier completes when the count reaches g*nwg.  Every wave
-// fences at agent scope before the arrival (its global writes reach the
-// other workgroups, whatever XCD they run on) and after the wait (stale
-// cached lines are dropped).  A wait longer than a.sync_ticks gives up and
-// returns false; the grid is sized so that every workgroup is resident, so
-// that only happens when something else holds the device.
-__device__ bool problem_sync(unsigned* ctr, int nwg, unsigned& gen, long long ticks, int* s_ok) {
-  if (nwg == 1) {
-    __syncthreads();
-    return true;
-  }
-  __threadfence();
-  __syncthreads();
-  gen += 1u;
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    const unsigned target = gen * (unsigned)nwg;
-    const unsigned prev = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev + 1u < target) {
-      const long long t0 = wall_clock64();
-      while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        __builtin_amdgcn_s_sleep(2);
-        if (wall_clock64() - t0 > ticks) {
-          ok = 0;
-          break;
-        }
-      }
-    }
-    *s_ok = ok;
-  }
-  __syncthreads();
-  __threadfence();
-  return *s_ok != 0;
-}
-
-// Taylor coefficients of the chunk's columns (the first half of the HR
+// Taylor coefficients of the problem's columns (the first half of the HR
 // limiter, limiting.cc:11-27 in 1D): A = L2M U per (element, field) column.
 template <int P, int PH>
 __device__ void batch_limit_l2m(const double* U, double* A, int e0, int e1, const double* sL2M) {
@@ -149,8 +101,8 @@ __device__ void batch_limit_l2m(const double* U, double* A, int e0, int e1, cons
   }
 }
 
-// The second half: per column of the chunk, hrl1D with the neighbours'
-// Taylor coefficients (another chunk's at the chunk ends), then U = M2L Alim.
+// The second half: per column, hrl1D with the neighbours' Taylor
+// coefficients, then U = M2L Alim.
 template <int P, int PH>
 __device__ void batch_limit_apply(const BatchArgs& a, double* U, const double* A, int NE, int e0, int e1,
                                   const double* sM2L) {
@@ -193,26 +145,20 @@ __device__ void batch_limit_apply(const BatchArgs& a, double* U, const double* A
   }
 }
 
-// One workgroup's share of its problem (order P): every step, stage and
-// limiter application over elements [e0, e1), meeting the problem's other
-// workgroups at problem_sync.
+// One workgroup's problem (order P): every step, stage and limiter
+// application, separated by workgroup barriers.
 template <int P, int PH, int FL>
-__device__ void batch_chunk(const BatchArgs& a, int w) {
+__device__ void batch_problem(const BatchArgs& a, int pid) {
   using B = BatchShape<P, PH>;
   using V = Vol<1, P, PH>;
   constexpr int NF = B::NF, Ns = B::Ns, NG = B::NG, NU = B::NU, NQ = V::NQ;
   __shared__ double sPhi[V::NTAB_PHI], sDphi[PH ? V::NTAB_D : 1], sDw[V::NTAB_D], sPw[PH ? V::NTAB_PHI : 1];
   __shared__ double sMinv[Ns * Ns], sL2M[Ns * Ns], sM2L[Ns * Ns];
-  __shared__ double sRed[8];
-  __shared__ int sOk;
+  __shared__ double sRed[16];
   const VolTables<1, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
   const BatchTables& bt = a.tab[P - 1];
 
-  const int pid = a.wg_prob[w];
   const int NE = a.n_elem[pid];
-  const int e0 = a.wg_e0[w], e1 = a.wg_e1[w];
-  const int w0 = a.prob_wg0[pid], nwg = a.prob_nwg[pid];
-  const bool lead = w == w0;
   const long so = a.state_off[pid], eo = a.elem_off[pid];
   double* U = a.U + so;
   double* Uacc = a.Uacc + so;
@@ -222,14 +168,8 @@ __device__ void batch_chunk(const BatchArgs& a, int w) {
   const double* J = a.J + eo;
   const double* ij = a.ij + eo;
   const double* nrm = a.normals + eo + pid;
-  unsigned* bar = a.bar + pid;
   const double cfl = a.cfl[pid], J0 = a.J0[pid], psiw = bt.psi_w[0];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  unsigned gen = 0;
-  auto sync = [&]() { return problem_sync(bar, nwg, gen, a.sync_ticks, &sOk); };
-  auto give_up = [&]() {
-    if (tid == 0) a.status[pid] = 2;
-  };
 
   load_vol_tables(tb, bt.phi, bt.dphi, bt.phi_w, bt.dphi_w, nullptr, false, tid, nthr);
   for (int k = tid; k < Ns * Ns; k += nthr) {
@@ -238,20 +178,19 @@ __device__ void batch_chunk(const BatchArgs& a, int w) {
     sM2L[k] = bt.M2L[k];
   }
   __syncthreads();
-  if (a.status[pid] != 0) return;  // every workgroup of the problem reads the same value
+  if (a.status[pid] != 0) return;
   auto limit = [&](double* V) {
-    batch_limit_l2m<P, PH>(V, A, e0, e1, sL2M);
-    if (!sync()) return false;  // the neighbouring chunks' Taylor coefficients
-    batch_limit_apply<P, PH>(a, V, A, NE, e0, e1, sM2L);
-    __syncthreads();  // the chunk's limited columns, before this workgroup reads them by node
-    return true;
+    batch_limit_l2m<P, PH>(V, A, 0, NE, sL2M);
+    __syncthreads();  // the neighbours' Taylor coefficients
+    batch_limit_apply<P, PH>(a, V, A, NE, 0, NE, sM2L);
+    __syncthreads();
   };
-  if (a.limiter && a.limit_initial && !limit(U)) return give_up();
+  if (a.limiter && a.limit_initial) limit(U);
 
   for (int step = 0; step < a.nsteps; ++step) {
-    // ---- Dt = cfl / max over the problem's nodes of |u|+a (rk.cc:250-272)
+    // ---- Dt = cfl / max over nodes of |u|+a (rk.cc:250-272)
     double m = 0.0;
-    for (int n = e0 * Ns + tid; n < e1 * Ns; n += nthr) {
+    for (int n = tid; n < NE * Ns; n += nthr) {
       const int e = n / Ns, i = n % Ns;
       double u[NF];
 #pragma unroll
@@ -265,36 +204,23 @@ __device__ void batch_chunk(const BatchArgs& a, int w) {
     if (tid == 0) {
       double mx = sRed[0];
       for (int k = 1; k < (nthr + 63) / 64; ++k) mx = nanmax(mx, sRed[k]);
-      a.dtp[w] = mx;
-    }
-    // the chunks' maxima: every workgroup reduces them in the same order
-    if (!sync()) return give_up();
-    if (tid == 0) {
-      double mx = a.dtp[w0];
-      for (int k = 1; k < nwg; ++k) mx = nanmax(mx, a.dtp[w0 + k]);
       sRed[0] = cfl / mx;
     }
     __syncthreads();
     const double dt = sRed[0];
     __syncthreads();  // sRed is reused by the next step
     if (!(dt >= 1e-14)) {
-      if (lead && tid == 0) a.status[pid] = 1;
+      if (tid == 0) a.status[pid] = 1;
       return;
     }
-    if (lead && tid == 0) a.T[pid] = a.T[pid] + dt;
+    if (tid == 0) a.T[pid] = a.T[pid] + dt;
 
     double* cur = U;
     for (int k = 0; k < 4; ++k) {
-      // the previous stage's output (and, at k = 0, the previous step's
-      // limited state) of the neighbouring chunks is complete here
-      if (k > 0 && a.limiter && !limit(cur)) return give_up();
-      if (k > 0 && !sync()) return give_up();
+      if (k > 0 && a.limiter) limit(cur);
       // ---- faces: interface j between elements j-1 and j; boundary faces
-      // contribute to the boundary element only (dg_functions.cc:225-233).
-      // The chunk owns interfaces e0 .. e1-1, the last chunk also NE.
-      const int nif = e1 - e0 + (e1 == NE ? 1 : 0);
-      for (int jj = tid; jj < nif; jj += nthr) {
-        const int j = e0 + jj;
+      // contribute to the boundary element only (dg_functions.cc:225-233)
+      for (int j = tid; j <= NE; j += nthr) {
         const bool lb = j == 0, rb = j == NE;
         const int ea = lb ? 0 : j - 1, na = lb ? 0 : 1;  // side 0: element and node
         const int eb = (lb || rb) ? ea : j, nb = (lb || rb) ? na : 0;
@@ -313,11 +239,10 @@ __device__ void batch_chunk(const BatchArgs& a, int w) {
           if (!lb && !rb) Qe[(eb * 2 + nb) * NF + f] = psiw * (F[f] + nc[f]);
         }
       }
-      // the left neighbour chunk wrote edge 1 of this chunk's first element
-      if (!sync()) return give_up();
+      __syncthreads();
       // ---- elements: volume terms, face terms, Dt*Minv and the RK4 update
       double* out = (k == 3) ? U : S;
-      for (int e = e0 + tid; e < e1; e += nthr) {
+      for (int e = tid; e < NE; e += nthr) {
         // vol_phase1 at each Gauss point, accumulated at once into every
         // node's sum in vol_phase2's order (point by point), so that the
         // per-point values need no registers of their own
@@ -339,7 +264,7 @@ __device__ void batch_chunk(const BatchArgs& a, int w) {
 #pragma unroll
             for (int f = 0; f < NFX; ++f) R[f * Ns + i] += wd * feg[f];
 #pragma unroll
-            for (int m = 0; m < NNC; ++m) R[(NFX + m) * Ns + i] += tb.pw[g * Ns + i] * feg[NFX + m];
+            for (int mm = 0; mm < NNC; ++mm) R[(NFX + mm) * Ns + i] += tb.pw[g * Ns + i] * feg[NFX + mm];
           }
         }
         // face terms: edge 0 lands on node 0, edge 1 on node 1 (1D
@@ -364,22 +289,21 @@ __device__ void batch_chunk(const BatchArgs& a, int w) {
           rk_combine(k, dte * sol, un, acc, Uacc + idx, out + idx);
         }
       }
-      __syncthreads();  // the chunk's stage output, before this workgroup reads it by column or node
+      __syncthreads();
       cur = out;
     }
-    if (a.limiter && !limit(U)) return give_up();
-    // the next step's Dt reads only this chunk's nodes; its barrier orders
-    // the limited state before the next face pass reads a neighbour's
+    if (a.limiter) limit(U);
   }
 }
 
-template <int PH, int FL>
-__global__ __launch_bounds__(256) void k_batch(BatchArgs a) {
-  const int w = blockIdx.x;
-  switch (a.order[a.wg_prob[w]]) {
-    case 1: batch_chunk<1, PH, FL>(a, w); break;
-    case 2: batch_chunk<2, PH, FL>(a, w); break;
-    default: batch_chunk<3, PH, FL>(a, w); break;
+// LB: the launch bound (512, or 1024 with four waves on every SIMD of the CU)
+template <int PH, int FL, int LB>
+__global__ __launch_bounds__(LB) void k_batch(BatchArgs a) {
+  const int pid = a.wg_prob[blockIdx.x];
+  switch (a.order[pid]) {
+    case 1: batch_problem<1, PH, FL>(a, pid); break;
+    case 2: batch_problem<2, PH, FL>(a, pid); break;
+    default: batch_problem<3, PH, FL>(a, pid); break;
   }
 }
 
@@ -391,13 +315,6 @@ __global__ void k_fill_zero_batch(double* T, int* status, int n) {
   }
 }
 
-// the barrier counters start every launch at zero (a kernel node, not a
-// memset node, in the captured graph)
-__global__ void k_zero_u32(unsigned* p, int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0u;
-}
-
 // ------------------------------------------------------------------- host
 template <typename Fn>
 static hipError_t dispatch_batch(const dgx_config& c, Fn&& fn) {
@@ -405,51 +322,6 @@ static hipError_t dispatch_batch(const dgx_config& c, Fn&& fn) {
   using I1 = std::integral_constant<int, 1>;
   auto by_flux = [&](auto ph) -> hipError_t { return c.flux == 0 ? fn(ph, I0{}) : fn(ph, I1{}); };
   return c.physics == 0 ? by_flux(I0{}) : by_flux(I1{});
-}
-
-// Workgroups of one problem: its element chunks.
-struct BatchPlan {
-  int block = 0, nwg = 0;
-  std::vector<int> wg_prob, wg_e0, wg_e1, prob_wg0, prob_nwg;
-};
-
-// Splits the problems over at most `target` workgroups (at least one each):
-// repeatedly the problem with the most DOFs per workgroup gets one more, as
-// long as its chunks keep >= kMinChunk elements; each problem's elements are
-// then cut into equal chunks.  target <= the problems' count: one each.
-constexpr int kMinChunk = 32;
-BatchPlan make_plan(const std::vector<int>& ne, const std::vector<int>& order, int nf, int block, int target) {
-  const int n = (int)ne.size();
-  BatchPlan pl;
-  pl.block = block;
-  std::vector<int> k(n, 1);
-  std::vector<double> work(n);
-  for (int p = 0; p < n; ++p) work[p] = (double)ne[p] * nf * (order[p] + 1);
-  int total = n;
-  while (total < target) {
-    int best = -1;
-    double bw = 0.0;
-    for (int p = 0; p < n; ++p)
-      if (ne[p] / (k[p] + 1) >= kMinChunk && work[p] / k[p] > bw) {
-        bw = work[p] / k[p];
-        best = p;
-      }
-    if (best < 0) break;
-    ++k[best];
-    ++total;
-  }
-  pl.prob_wg0.resize(n);
-  pl.prob_nwg = k;
-  for (int p = 0; p < n; ++p) {
-    pl.prob_wg0[p] = (int)pl.wg_prob.size();
-    for (int c = 0; c < k[p]; ++c) {
-      pl.wg_prob.push_back(p);
-      pl.wg_e0.push_back((int)((long)ne[p] * c / k[p]));
-      pl.wg_e1.push_back((int)((long)ne[p] * (c + 1) / k[p]));
-    }
-  }
-  pl.nwg = (int)pl.wg_prob.size();
-  return pl;
 }
 
 }  // namespace dgx
@@ -463,20 +335,15 @@ struct dgx_batch {
   std::vector<int> ne, order;
   std::vector<long> state_off, elem_off;
   BatchTables tab[3] = {};
-  int block = 256;   // workgroup size
-  int target = 0;    // workgroups to split the problems over (0: as many as the device holds at once)
-  BatchPlan plan;    // the split in use (uploaded)
-  int wg_cap = 0;    // capacity of the per-workgroup device arrays
-  int *d_ne = nullptr, *d_order = nullptr;
-  int *d_wg_prob = nullptr, *d_wg_e0 = nullptr, *d_wg_e1 = nullptr, *d_prob_wg0 = nullptr, *d_prob_nwg = nullptr;
+  int block = 512;   // workgroup size
+  std::vector<int> order_by_work;  // problems, largest first (workgroup w advances order_by_work[w])
+  int *d_ne = nullptr, *d_order = nullptr, *d_wg_prob = nullptr;
   long *d_state_off = nullptr, *d_elem_off = nullptr;
   double *d_J = nullptr, *d_ij = nullptr, *d_normals = nullptr, *d_minv = nullptr, *d_J0 = nullptr,
          *d_dx = nullptr, *d_cfl = nullptr;
   double *Uacc = nullptr, *S = nullptr, *A = nullptr, *Qe = nullptr;
-  double *tuneU = nullptr, *tuneT = nullptr, *d_dtp = nullptr;
-  unsigned* d_bar = nullptr;
+  double *tuneU = nullptr, *tuneT = nullptr;
   int* tune_status = nullptr;
-  long long sync_ticks = 0;
   std::vector<void*> allocs;
   // graph cache
   hipGraphExec_t exec = nullptr;
@@ -523,69 +390,24 @@ dgx_status bfail(int code, const std::string& msg) {
   return code;
 }
 
-// Workgroups of `block` threads the device holds at once (the kernel's
-// occupancy times the CU count): the most a split may use, since the
-// workgroups of a split problem wait for each other.
-int resident_workgroups(const dgx_batch* b, int block) {
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-    return 0;
-  hipError_t e = dispatch_batch(b->cfg, [&](auto ph, auto fl) -> hipError_t {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch<decltype(ph)::value, decltype(fl)::value>,
-                                                        block, 0);
-  });
-  return e == hipSuccess ? per_cu * cus : 0;
-}
-
-// Makes and uploads the split for (block, target); target 0: the resident limit.
-bool set_plan(dgx_batch* b, int block, int target) {
-  const int cap = resident_workgroups(b, block);
-  if (cap <= 0) return hip_ok(hipErrorInvalidValue, "dgx_batch: occupancy query");
-  const int t = std::min(target > 0 ? target : cap, cap);
-  const int nf = 2 + 1 + b->cfg.physics;
-  BatchPlan pl = make_plan(b->ne, b->order, nf, block, t);
-  if (pl.nwg > b->wg_cap) {
-    const int c = std::max(pl.nwg, 2 * b->wg_cap);
-    if (!bupload<int>(b, &b->d_wg_prob, nullptr, c) || !bupload<int>(b, &b->d_wg_e0, nullptr, c) ||
-        !bupload<int>(b, &b->d_wg_e1, nullptr, c) || !bupload<double>(b, &b->d_dtp, nullptr, c))
-      return false;
-    b->wg_cap = c;
-  }
-  const size_t nw = (size_t)pl.nwg, np = (size_t)b->n;
-  if (!hip_ok(hipMemcpy(b->d_wg_prob, pl.wg_prob.data(), nw * 4, hipMemcpyHostToDevice), "plan upload") ||
-      !hip_ok(hipMemcpy(b->d_wg_e0, pl.wg_e0.data(), nw * 4, hipMemcpyHostToDevice), "plan upload") ||
-      !hip_ok(hipMemcpy(b->d_wg_e1, pl.wg_e1.data(), nw * 4, hipMemcpyHostToDevice), "plan upload") ||
-      !hip_ok(hipMemcpy(b->d_prob_wg0, pl.prob_wg0.data(), np * 4, hipMemcpyHostToDevice), "plan upload") ||
-      !hip_ok(hipMemcpy(b->d_prob_nwg, pl.prob_nwg.data(), np * 4, hipMemcpyHostToDevice), "plan upload"))
-    return false;
-  b->plan = std::move(pl);
-  b->block = block;
-  b->target = target;
-  drop_graph(b);
-  return true;
-}
-
 BatchArgs make_args(dgx_batch* b, double* U, int nsteps, int limiter, int limit_initial, double* T, int* status) {
   BatchArgs a{};
   for (int q = 0; q < 3; ++q) a.tab[q] = b->tab[q];
-  a.wg_prob = b->d_wg_prob; a.wg_e0 = b->d_wg_e0; a.wg_e1 = b->d_wg_e1;
-  a.prob_wg0 = b->d_prob_wg0; a.prob_nwg = b->d_prob_nwg; a.order = b->d_order;
+  a.wg_prob = b->d_wg_prob; a.order = b->d_order;
   a.n_elem = b->d_ne; a.state_off = b->d_state_off; a.elem_off = b->d_elem_off;
   a.J = b->d_J; a.ij = b->d_ij; a.normals = b->d_normals; a.minv = b->d_minv; a.J0 = b->d_J0; a.cfl = b->d_cfl;
   a.bc[0] = b->bc[0]; a.bc[1] = b->bc[1];
   a.U = U; a.Uacc = b->Uacc; a.S = b->S; a.A = b->A; a.Qe = b->Qe; a.T = T; a.status = status;
-  a.bar = b->d_bar; a.dtp = b->d_dtp; a.sync_ticks = b->sync_ticks;
   a.nsteps = nsteps; a.limiter = limiter; a.limit_initial = limit_initial;
   return a;
 }
 
-// the counter fill and the ensemble kernel
-hipError_t launch_batch(dgx_batch* b, const BatchArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_u32, dim3((b->n + 255) / 256), dim3(256), 0, st, b->d_bar, b->n);
+// one workgroup per problem; the 1024-thread instance for the largest workgroups
+hipError_t launch_batch(dgx_batch* b, const BatchArgs& a, int block, hipStream_t st) {
   return dispatch_batch(b->cfg, [&](auto ph, auto fl) -> hipError_t {
-    hipLaunchKernelGGL((k_batch<decltype(ph)::value, decltype(fl)::value>), dim3(b->plan.nwg), dim3(b->plan.block), 0,
-                       st, a);
+    constexpr int PH = decltype(ph)::value, FL = decltype(fl)::value;
+    if (block <= 512) hipLaunchKernelGGL((k_batch<PH, FL, 512>), dim3(b->n), dim3(block), 0, st, a);
+    else hipLaunchKernelGGL((k_batch<PH, FL, 1024>), dim3(b->n), dim3(block), 0, st, a);
     return hipGetLastError();
   });
 }
@@ -698,17 +520,16 @@ dgx_status dgx_batch_create(const dgx_config* cfg, int n, const int* n_elem, con
             bupload<double>(b, &b->Uacc, nullptr, so) && bupload<double>(b, &b->S, nullptr, so) &&
             bupload<double>(b, &b->A, nullptr, so) && bupload<double>(b, &b->Qe, nullptr, (size_t)eo * 2 * NF) &&
             bupload<double>(b, &b->tuneU, nullptr, so) && bupload<double>(b, &b->tuneT, nullptr, n) &&
-            bupload<int>(b, &b->tune_status, nullptr, n) && bupload<unsigned>(b, &b->d_bar, nullptr, n) &&
-            bupload<int>(b, &b->d_prob_wg0, nullptr, n) && bupload<int>(b, &b->d_prob_nwg, nullptr, n);
+            bupload<int>(b, &b->tune_status, nullptr, n);
   if (ok) ok = bupload(b, &b->d_minv, minv.data(), minv.size());
-  // a barrier wait gives up after 2 s of the device's constant-rate clock
-  int dev = 0, khz = 0;
-  if (ok)
-    ok = hip_ok(hipGetDevice(&dev), "hipGetDevice") &&
-         hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
-  b->sync_ticks = 2000LL * (khz > 0 ? khz : 100000);
+  // problems by descending DOFs: the largest, which set the time, start first
+  b->order_by_work.resize(n);
+  std::iota(b->order_by_work.begin(), b->order_by_work.end(), 0);
+  std::stable_sort(b->order_by_work.begin(), b->order_by_work.end(), [&](int x, int y) {
+    return (long)b->ne[x] * (b->order[x] + 1) > (long)b->ne[y] * (b->order[y] + 1);
+  });
+  if (ok) ok = bupload(b, &b->d_wg_prob, b->order_by_work.data(), n);
   if (ok) ok = hip_ok(hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking), "hipStreamCreate");
-  if (ok) ok = set_plan(b, 256, 0);
   if (!ok) return bail(DGX_EDEVICE, "");
   *out = b;
   return DGX_OK;
@@ -727,17 +548,19 @@ long dgx_batch_state_offset(const dgx_batch* b, int k) {
 
 dgx_status dgx_batch_block_sizes(const dgx_batch* b, int* sizes) {
   if (!b || !sizes) return bfail(DGX_EINVAL, "dgx_batch_block_sizes: null argument");
-  sizes[0] = b->plan.block;
-  sizes[1] = b->plan.nwg;
-  sizes[2] = b->target;
+  sizes[0] = b->block;
+  sizes[1] = b->n;
+  sizes[2] = 0;
   return DGX_OK;
 }
 
 dgx_status dgx_batch_set_block_sizes(dgx_batch* b, const int* sizes) {
   if (!b || !sizes) return bfail(DGX_EINVAL, "dgx_batch_set_block_sizes: null argument");
-  if (sizes[0] < 64 || sizes[0] > 256 || sizes[0] % 64 || sizes[1] < 0)
-    return bfail(DGX_EINVAL, "dgx_batch_set_block_sizes: workgroup size 64, 128, 192 or 256; workgroups >= 0");
-  return set_plan(b, sizes[0], sizes[1]) ? DGX_OK : DGX_EDEVICE;
+  if (sizes[0] < 64 || sizes[0] > 1024 || sizes[0] % 64)
+    return bfail(DGX_EINVAL, "dgx_batch_set_block_sizes: workgroup size a multiple of 64 in [64, 1024]");
+  b->block = sizes[0];
+  drop_graph(b);
+  return DGX_OK;
 }
 
 dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int limiter, void* stream) {
@@ -747,46 +570,37 @@ dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int lim
   hipEvent_t t0, t1;
   if (!hip_ok(hipEventCreate(&t0), "hipEventCreate") || !hip_ok(hipEventCreate(&t1), "hipEventCreate"))
     return DGX_EDEVICE;
-  // workgroup sizes x splits: one workgroup per problem, or a half or all of
-  // the workgroups the device holds at once
-  const int sizes[3] = {64, 128, 256};
+  const int sizes[3] = {256, 512, 1024};
   float best = 1e30f;
-  int best_b = b->block, best_t = b->target;
+  int best_b = b->block;
   dgx_status r = DGX_OK;
   for (int bs : sizes) {
-    const int cap = resident_workgroups(b, bs);
-    const int targets[3] = {b->n, cap / 2, 0};
-    for (int tg : targets) {
-      if (tg > 0 && tg < b->n) continue;
-      if (!set_plan(b, bs, tg)) { r = DGX_EDEVICE; break; }
-      float tot = 0.0f;
-      for (int rep = 0; rep < 2 && r == DGX_OK; ++rep) {
-        // scratch copies: the caller's state and times are left untouched
-        if (!hip_ok(hipMemcpyAsync(b->tuneU, U, sizeof(double) * b->state_off[b->n], hipMemcpyDeviceToDevice, st),
-                    "autotune copy")) { r = DGX_EDEVICE; break; }
-        hipLaunchKernelGGL(k_fill_zero_batch, dim3((b->n + 255) / 256), dim3(256), 0, st, b->tuneT, b->tune_status,
-                           b->n);
-        BatchArgs a = make_args(b, b->tuneU, 1, limiter, 0, b->tuneT, b->tune_status);
-        (void)hipEventRecord(t0, st);
-        if (!hip_ok(launch_batch(b, a, st), "autotune launch")) { r = DGX_EDEVICE; break; }
-        (void)hipEventRecord(t1, st);
-        if (!hip_ok(hipEventSynchronize(t1), "autotune sync")) { r = DGX_EDEVICE; break; }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, t0, t1);
-        if (rep == 1) tot += ms;  // first run warms the code object and caches
-      }
-      if (r != DGX_OK) break;
-      if (tot < best) {
-        best = tot;
-        best_b = bs;
-        best_t = tg;
-      }
+    float tot = 0.0f;
+    for (int rep = 0; rep < 2 && r == DGX_OK; ++rep) {
+      // scratch copies: the caller's state and times are left untouched
+      if (!hip_ok(hipMemcpyAsync(b->tuneU, U, sizeof(double) * b->state_off[b->n], hipMemcpyDeviceToDevice, st),
+                  "autotune copy")) { r = DGX_EDEVICE; break; }
+      hipLaunchKernelGGL(k_fill_zero_batch, dim3((b->n + 255) / 256), dim3(256), 0, st, b->tuneT, b->tune_status,
+                         b->n);
+      BatchArgs a = make_args(b, b->tuneU, 1, limiter, 0, b->tuneT, b->tune_status);
+      (void)hipEventRecord(t0, st);
+      if (!hip_ok(launch_batch(b, a, bs, st), "autotune launch")) { r = DGX_EDEVICE; break; }
+      (void)hipEventRecord(t1, st);
+      if (!hip_ok(hipEventSynchronize(t1), "autotune sync")) { r = DGX_EDEVICE; break; }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, t0, t1);
+      if (rep == 1) tot += ms;  // first run warms the code object and caches
     }
     if (r != DGX_OK) break;
+    if (tot < best) {
+      best = tot;
+      best_b = bs;
+    }
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
-  if (r == DGX_OK && !set_plan(b, best_b, best_t)) r = DGX_EDEVICE;
+  b->block = best_b;
+  drop_graph(b);
   return r;
 }
 
@@ -800,11 +614,11 @@ dgx_status dgx_batched_rk4(dgx_batch* b, double* U, double cfl, int nsteps, int 
   if (!same) {
     drop_graph(b);
     if (!upload_cfl(b, cfl)) return DGX_EDEVICE;
-    // the counter fill and the kernel, captured on the internal stream
+    // the kernel, captured on the internal stream
     if (!hip_ok(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
       return DGX_EDEVICE;
     BatchArgs a = make_args(b, U, nsteps, limiter, limit_initial, T, status);
-    const bool ok = hip_ok(launch_batch(b, a, b->cs), "batched launch");
+    const bool ok = hip_ok(launch_batch(b, a, b->block, b->cs), "batched launch");
     hipGraph_t graph = nullptr;
     const bool ended = hip_ok(hipStreamEndCapture(b->cs, &graph), "hipStreamEndCapture");
     if (!ok || !ended) {

@@ -31,6 +31,7 @@
 #include "hr.cuh"
 #include "hr_tables.h"
 #include "physics.cuh"
+#include "sensor.cuh"
 
 namespace dgx {
 
@@ -565,11 +566,7 @@ template <int Ns>
 __global__ __launch_bounds__(256) void k_uavg(const double* __restrict__ U, long ncol, double* __restrict__ Uavg) {
   const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
-  const double w = 1.0 / (double)Ns;
-  double c = 0.0;
-#pragma unroll
-  for (int l = 0; l < Ns; ++l) c = c + (w * U[col * Ns + l]) * 1.0;
-  Uavg[col] = c;
+  Uavg[col] = sensor_nodal_mean<Ns>(U + col * Ns);
 }
 
 struct SensorTh {
@@ -583,70 +580,18 @@ template <int D, int PH>
 __global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int* __restrict__ nb,
                                                     const double* __restrict__ Uavg, SensorTh s,
                                                     int* __restrict__ tgt, int* __restrict__ val) {
-  constexpr int NF = Fields<D, PH>::NF;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N_E) return;
-  double vyL = 0, vyR = 0, betaL = 0, betaR = 0;
-  int fc = 0;
-  double rhoL = Uavg[e * NF + fc]; fc++;
-  double vxL = Uavg[e * NF + fc] / rhoL; fc++;
-  if (D == 2) { vyL = Uavg[e * NF + fc] / rhoL; fc++; }
-  double EtL = Uavg[e * NF + fc]; fc++;
-  // passive states use the single-fluid gamma; stiffened states read beta
-  // after alpha (sensor_kernels.cu:57-64)
-  double alphaL = (PH == 0 || PH == 2) ? 1 / (kGammaSF - 1) : Uavg[e * NF + fc];
-  if (PH == 3) betaL = Uavg[e * NF + fc + 1];
-  double gammaL = 1.0 + 1.0 / alphaL;
-  double pinfL = (1 - 1.0 / gammaL) * betaL;
-  double pL = (gammaL - 1) * (EtL - betaL - 0.5 * rhoL * (vxL * vxL + vyL * vyL));
-  double iL = pL * alphaL;
-  double aL = sqrt((gammaL * (pL + pinfL)) / rhoL);
-  double HL = (EtL + pL) / rhoL;
+  const SensorCell L = sensor_cell<D, PH>(Uavg, e);
   int t_out = -2, v_out = 0;
   for (int nn = 0; nn < N_N; nn++) {
     int right = nb[e * N_N + nn];
     if (right < 0) { t_out = -1; v_out = 1; break; }
-    fc = 0;
-    double rhoR = Uavg[right * NF + fc]; fc++;
-    double vxR = Uavg[right * NF + fc] / rhoR; fc++;
-    if (D == 2) { vyR = Uavg[right * NF + fc] / rhoR; fc++; }
-    double EtR = Uavg[right * NF + fc]; fc++;
-    double alphaR = (PH == 0 || PH == 2) ? 1 / (kGammaSF - 1) : Uavg[right * NF + fc];
-    if (PH == 3) betaR = Uavg[right * NF + fc + 1];
-    double gammaR = 1.0 + 1.0 / alphaR;
-    double pinfR = (1 - 1.0 / gammaR) * betaR;
-    double pR = (gammaR - 1) * (EtR - betaR - 0.5 * rhoR * (vxR * vxR + vyR * vyR));
-    double iR = pR * alphaR;
-    double aR = sqrt((gammaR * (pR + pinfR)) / rhoR);
-    double HR = (EtR + pR) / rhoR;
-    double RT = sqrt(rhoR / rhoL);
-    double vx = (vxL + RT * vxR) / (1 + RT);
-    double vy = (vyL + RT * vyR) / (1 + RT);
-    double alpha = (alphaL + RT * alphaR) / (1 + RT);
-    double gamma = 1 + 1.0 / alpha;
-    double H = (HL + RT * HR) / (1 + RT);
-    double a = sqrt((gamma - 1) * (H - 0.5 * (vx * vx + vy * vy)));
-    double i = (iL + RT * iR) / (1 + RT);
-    if (s.on[0]) {
-      double drho = rhoR - rhoL;
-      double dp = (gamma - 1) * (gamma - 1) * (alpha * (iR - iL) - i * (alphaR - alphaL));
-      double dV2 = drho - dp / (a * a);
-      double G = fabs(dV2) / (rhoL + rhoR);
-      double PHI = 2 * G / ((1 + G) * (1 + G));
-      if (PHI > s.th[0]) { t_out = right; v_out = 1; break; }
-    }
-    if (s.on[1]) {
-      if (((vxL - aL > vx - a) && (vx - a > vxR - aR)) || ((vyL - aL > vy - a) && (vy - a > vyR - aR))) {
-        double PSI = fabs(pR - pL) / (pL + pR);
-        double W = 2 * PSI / ((1 + PSI) * (1 + PSI));
-        if (W > s.th[1]) { t_out = right; v_out = 2; break; }
-      }
-    }
-    if (s.on[2]) {
-      double G = fabs(gammaR - gammaL) / (gammaL + gammaR);
-      double PHI = 2 * G / ((1 + G) * (1 + G));
-      if (PHI > s.th[2]) { t_out = right; v_out = 3; break; }
-    }
+    const SensorCell R = sensor_cell<D, PH>(Uavg, right);
+    const SensorRoe m = sensor_roe(L, R);
+    if (s.on[0] && sensor_contact(L, R, m, s.th[0])) { t_out = right; v_out = 1; break; }
+    if (s.on[1] && sensor_shock(L, R, m, s.th[1])) { t_out = right; v_out = 2; break; }
+    if (s.on[2] && sensor_gamma(L, R, s.th[2])) { t_out = right; v_out = 3; break; }
   }
   tgt[e] = t_out;
   val[e] = v_out;
@@ -1253,6 +1198,15 @@ static hipError_t sensor_resolve(Solver& s, const int* nb, const int* init, cons
   hipLaunchKernelGGL(k_sensor_final, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt, val, act0,
                      sensors);
   return hipGetLastError();
+}
+
+hipError_t launch_uavg(const Solver& s, const double* U, double* Uavg, hipStream_t st) {
+  const long ncol = (long)s.N_E * s.NF;
+  return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    constexpr int Ns = Shape<decltype(d)::value, decltype(p)::value>::Ns;
+    hipLaunchKernelGGL((k_uavg<Ns>), dim3(nblk(ncol, 256)), dim3(256), 0, st, U, ncol, Uavg);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st) {

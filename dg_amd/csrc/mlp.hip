@@ -30,6 +30,7 @@
 
 #include "dgx_internal.h"
 #include "physics.cuh"
+#include "sensor.cuh"
 
 namespace dgx {
 
@@ -157,8 +158,15 @@ __device__ inline void wave_modes(double* u, const double* Vinv, double* inv, in
 // relative jump of the cell means of rho, p and alpha (avg, k_mlp_means),
 // as the float the network input takes; -1 marks a domain boundary (the
 // feature is 0 there and the cell is flagged by rule).
+// With uavg (the sensor's cell averages, k_uavg's [e*5 + f]) the same pass
+// writes the shock guard: guard[e] = 1 when the sensor's shock test
+// (sensor_shock, the same arithmetic as calc_sensors) fires for a pair of e
+// and a neighbour in either order.  Every cell the sensor marks as a shock is
+// in such a pair, so the guard holds all of them, whatever the network says.
 __global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ avg, const int* __restrict__ nb, int N,
-                                                   float4* __restrict__ jumps, int* __restrict__ count_reset) {
+                                                   float4* __restrict__ jumps, int* __restrict__ count_reset,
+                                                   const double* __restrict__ uavg, double th_shock,
+                                                   int* __restrict__ guard) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   // the flagged-element count that k_mlp_indicator appends to starts at zero
   if (count_reset && e == 0) *count_reset = 0;
@@ -166,6 +174,16 @@ __global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ av
   int o[4];
 #pragma unroll
   for (int n = 0; n < 4; ++n) o[n] = nb[e * 4 + n];
+  if (uavg) {
+    const SensorCell C = sensor_cell<2, 1>(uavg, e);
+    int g = 0;
+    for (int n = 0; n < 4; ++n) {
+      if (o[n] < 0) continue;
+      const SensorCell R = sensor_cell<2, 1>(uavg, o[n]);
+      if (sensor_shock(C, R, sensor_roe(C, R), th_shock) || sensor_shock(R, C, sensor_roe(R, C), th_shock)) g = 1;
+    }
+    guard[e] = g;
+  }
   const double r0 = avg[e * 3 + 0], p0 = avg[e * 3 + 1], a0 = avg[e * 3 + 2];
   double an[4][3];
 #pragma unroll
@@ -387,9 +405,9 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
                                                        const uint16_t* __restrict__ w1b,
                                                        const uint16_t* __restrict__ w2b,
                                                        const uint16_t* __restrict__ w3b, const float* __restrict__ P,
-                                                       float threshold, float* __restrict__ z,
-                                                       int* __restrict__ flags, int* __restrict__ list,
-                                                       int* __restrict__ count) {
+                                                       float threshold, const int* __restrict__ guard,
+                                                       float* __restrict__ z, int* __restrict__ flags,
+                                                       int* __restrict__ list, int* __restrict__ count) {
   __shared__ double sU[4][16 * 45];
   __shared__ double sV[81];
   __shared__ double sInv[4][16 * 5];
@@ -487,7 +505,7 @@ __global__ __launch_bounds__(256) void k_mlp_indicator(const double* __restrict_
       if (z) z[e] = zz;
       // cells next to a domain boundary are flagged by rule, as by the
       // sensor (a boundary neighbour flags the cell, sensor_kernels.cu:19-163)
-      fv = (at_wall(jmp) || zz > threshold) ? 1 : 0;
+      fv = (at_wall(jmp) || zz > threshold || (guard && guard[e])) ? 1 : 0;
       flags[e] = fv;
     }
     if (list) {
@@ -732,14 +750,21 @@ hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool 
   dgx_mlp* m = s.mlp;
   const int N = s.N_E;
   int* count = with_list ? s.flag + 2 : nullptr;
-  if (s.means_of != U)  // else the stage kernel that wrote U left its cell means in s.mlp_avg
+  const bool guard = s.shock_guard_on();
+  if (s.means_of != U) {  // else the stage kernel that wrote U left its cell means in s.mlp_avg and s.Uavg
     hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+    if (guard) {
+      hipError_t e = launch_uavg(s, U, s.Uavg, st);
+      if (e != hipSuccess) return e;
+    }
+  }
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
-                     reinterpret_cast<float4*>(s.mlp_jumps), count);
+                     reinterpret_cast<float4*>(s.mlp_jumps), count, guard ? s.Uavg : nullptr, s.thresh[1],
+                     s.mlp_guard);
   const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
   hipLaunchKernelGGL(k_mlp_indicator, dim3(nb), dim3(256), 0, st, U, s.t.mlp_vinv,
                      reinterpret_cast<const float4*>(s.mlp_jumps), N, m->w1b, m->w2b, m->w3b, m->P, s.mlp_threshold,
-                     s.mlp_z, s.sensors, with_list ? s.flag_list : nullptr, count);
+                     guard ? s.mlp_guard : nullptr, s.mlp_z, s.sensors, with_list ? s.flag_list : nullptr, count);
   return hipGetLastError();
 }
 
@@ -747,7 +772,7 @@ hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st) {
   const int N = s.N_E;
   hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
-                     reinterpret_cast<float4*>(s.mlp_jumps), nullptr);
+                     reinterpret_cast<float4*>(s.mlp_jumps), nullptr, nullptr, 0.0, nullptr);
   const unsigned nbf = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
   hipLaunchKernelGGL(k_mlp_features, dim3(nbf), dim3(256), 0, st, U, s.t.mlp_vinv,
                      reinterpret_cast<const float4*>(s.mlp_jumps), N, X);

@@ -620,17 +620,16 @@ __host__ __device__ constexpr int reversed_pos(int j) {
 // Cell means of rho, p and alpha = 1/(gamma-1) of a 2D p=2 two-fluid element
 // (nodal values u[f*9 + i]): the tensor-Legendre mode 0 of rho, the momenta,
 // E and alpha (w0 = the first row of the Lagrange-to-Legendre transform),
-// then p = (gamma-1)(E - (m_x^2 + m_y^2)/(2 rho)).  One operation order and
-// no contraction, so the learned indicator's own pass (k_mlp_means) and the
-// stage kernel's epilogue give the same bits.
+// then p = (gamma-1)(E - (m_x^2 + m_y^2)/(2 rho)).  One function for the
+// learned indicator's own pass (k_mlp_means) and the stage kernel's
+// epilogue, so that both give the same bits (checked by the tests).
 __device__ inline void cell_means_rpa(const double* u, const double* w0, double* avg3) {
-#pragma clang fp contract(off)
   double m[5];
 #pragma unroll
   for (int f = 0; f < 5; ++f) {
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) s = s + w0[i] * u[f * 9 + i];
+    for (int i = 0; i < 9; ++i) s += w0[i] * u[f * 9 + i];
     m[f] = s;
   }
   const double gam = 1.0 + 1.0 / m[4];

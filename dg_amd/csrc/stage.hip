@@ -17,6 +17,7 @@
 
 #include "dgx_internal.h"
 #include "physics.cuh"
+#include "sensor.cuh"
 #include "volume.cuh"
 
 namespace dgx {
@@ -49,6 +50,7 @@ struct StageArgs {
   // tile for the learned indicator ([e*3 + k], cell_means_rpa with w0), or null
   double* means;
   const double* __restrict__ w0;
+  double* uavg;  // with means: also the sensor's cell averages [e*5 + f] (sensor_nodal_mean), or null
 };
 
 template <int D, int TX, int TY>
@@ -526,10 +528,15 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
         if (tid < NT) {
           double a3[3];
           cell_means_rpa(sU + tid * SU, a.w0, a3);
-          double* m = a.means + ((size_t)(ey0 + tid / TX) * a.nx + ex0 + tid % TX) * 3;
+          const size_t eg = (size_t)(ey0 + tid / TX) * a.nx + ex0 + tid % TX;
+          double* m = a.means + eg * 3;
           m[0] = a3[0];
           m[1] = a3[1];
           m[2] = a3[2];
+          if (a.uavg) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) a.uavg[eg * NF + f] = sensor_nodal_mean<Ns>(sU + tid * SU + f * Ns);
+          }
         }
       }
     }
@@ -587,7 +594,7 @@ static hipError_t dispatch_stage(const dgx_config& c, Fn&& fn) {
 
 hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
                         double* Uout, const double* dt_dev, double* f_out, hipStream_t st, double* means,
-                        bool* means_done) {
+                        double* uavg, bool* means_done) {
   if (means_done) *means_done = false;
   const Cart& c = s.cart;
   StageArgs a{};
@@ -626,9 +633,11 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
     // matrix-core element phases with the shared Minv (k_stage's MFW path)
     constexpr bool mfw = D == 2 && P == 2 && PH == 1 && (TX * TY) % 16 == 0;
     a.means = nullptr;
+    a.uavg = nullptr;
     if (mfw && mode == 1 && means && s.t.mlp_vinv && s.t.minv_stride == 0 &&
         (s.precision >= 1 || DGX_STAGE_MFMA64)) {
       a.means = means;
+      a.uavg = uavg;
       if (means_done) *means_done = true;
     }
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, a);

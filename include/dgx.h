@@ -263,17 +263,12 @@ void dgx_batch_free(dgx_batch* b);
 /* Offset (doubles) of problem k in the packed state, k = 0..n (n: total size).
  * Each problem's slice uses the usual layout U[(e*N_F+f)*N_s+i]. */
 long dgx_batch_state_offset(const dgx_batch* b, int k);
-/* The split in use: sizes[0] = workgroup size, sizes[1] = workgroups (each
- * advances a contiguous chunk of one problem's elements; a problem with
- * several chunks syncs them at a barrier of its own), sizes[2] = the
- * requested count (0: as many as the device holds at once). */
+/* sizes[0] = workgroup size (one workgroup per problem), sizes[1] = workgroups
+ * (the number of problems), sizes[2] = 0. */
 dgx_status dgx_batch_block_sizes(const dgx_batch* b, int* sizes);
-/* sizes[0] = workgroup size (64, 128, 192 or 256), sizes[1] = workgroups to
- * split the problems over (0: as many as the device holds at once; never more;
- * at least one per problem). */
+/* sizes[0] = workgroup size, a multiple of 64 in [64, 1024]; the rest is ignored. */
 dgx_status dgx_batch_set_block_sizes(dgx_batch* b, const int* sizes);
-/* Times one step for each candidate workgroup size (64, 128, 256) and split
- * (one workgroup per problem, half or all of the resident workgroups) on
+/* Times one step for each candidate workgroup size (256, 512, 1024) on
  * scratch copies of U (U is not modified) and keeps the fastest. */
 dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int limiter, void* stream);
 /* nsteps RK4 steps of every problem in one graph launch.  Problem k uses
@@ -281,9 +276,7 @@ dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int lim
  * limiter 0 none or 1 hrl; limit_initial limits the initial state first
  * (rk.cc:100-114).  T (double[n], device) accumulates each problem's time;
  * status (int[n], device, zero on entry) becomes 1 for a problem whose Dt
- * fell below 1e-14 or was NaN (rk.cc:147-148); that problem stops.  Status 2:
- * the workgroups of a split problem did not all run at once (another kernel
- * held the device) and gave up after 2 s; that problem's state is invalid. */
+ * fell below 1e-14 or was NaN (rk.cc:147-148); that problem stops. */
 dgx_status dgx_batched_rk4(dgx_batch* b, double* U, double cfl, int nsteps, int limiter, int limit_initial,
                            double* T, int* status, void* stream);
 
@@ -319,6 +312,13 @@ dgx_status dgx_solver_set_indicator(dgx_solver* s, dgx_mlp* m, float threshold);
  * the indicator skips its own pass over the state (on by default; 0 turns
  * it off, the same flags and bits either way). */
 dgx_status dgx_solver_set_fused_means(dgx_solver* s, int on);
+/* Limiter 8 and dgx_indicator_flags also flag every cell of a neighbour pair
+ * for which the sensor's shock test (calc_sensors test 2, the same arithmetic
+ * and cell averages, sensor_kernels.cu:125-136) fires, so that no cell the
+ * sensor marks as a shock goes unflagged, whatever the network says.  Needs
+ * the second sensor threshold (dgx_set_sensor_thresholds with n >= 2).  On by
+ * default; 0 leaves the flags to the network alone. */
+dgx_status dgx_solver_set_shock_guard(dgx_solver* s, int on);
 dgx_status dgx_mlp_features(dgx_solver* s, const double* U, float* X, void* stream);
 /* flags (int[N_E], device) and optionally logits z (float[N_E], device) */
 dgx_status dgx_indicator_flags(dgx_solver* s, const double* U, int* flags, float* z, void* stream);

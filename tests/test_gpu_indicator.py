@@ -143,9 +143,10 @@ def test_learned_limiter_is_exact_on_its_flags(device):
 @pytest.mark.parametrize("prec", [0, 1])
 def test_stage_emitted_means_give_the_same_run(device, prec):
     """The tile kernel's epilogue writes the cell means the indicator reads
-    (dgx_solver_set_fused_means, on by default): 6 RK4 steps limited by the
-    learned indicator on the shock-bubble at 128x64 give the same state bit
-    for bit with and without it, in FP64 and with the FP32 contractions.  An
+    (dgx_solver_set_fused_means, on by default) and the sensor's cell
+    averages the shock guard reads: 6 RK4 steps limited by the learned
+    indicator on the shock-bubble at 128x64 give the same state bit for bit
+    with and without it, in FP64 and with the FP32 contractions.  An
     untrained seeded network with the threshold at the median logit of the
     initial state flags a large share of the cells."""
     from dg_amd import PATH_TILED
@@ -154,10 +155,15 @@ def test_stage_emitted_means_give_the_same_run(device, prec):
     U0 = problems.shock_bubble_state(mesh, xs=0.24)
     net = Indicator(seed=3)
     out = []
+    from dg_amd import indicator
+
     for fused in (True, False):
         solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
         assert solver.path == PATH_TILED
         solver.set_precision(prec)
+        # the sensor thresholds switch the shock guard on: the stages also
+        # write the sensor's cell averages it reads
+        solver.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
         solver.set_indicator(net, 0.0)
         Ud = torch.from_numpy(U0).to(device)
         z = torch.zeros(mesh.sizes["N_E"], dtype=torch.float32, device=device)
@@ -178,3 +184,35 @@ def test_stage_emitted_means_give_the_same_run(device, prec):
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][0], out[1][0])
     assert not np.array_equal(out[0][0], U0)
+
+
+def test_shock_guard_holds_every_sensor_shock_cell(device):
+    """With a threshold no logit reaches, the indicator's flags are the
+    boundary cells and the shock guard alone: on the shock-bubble at 256x128
+    (shock, reflected and transmitted shocks after 40 HR-limited steps) they
+    contain every cell the sensor marks as a shock, bit-identical decisions
+    from the same cell averages; without the guard the untrained network
+    flags almost none of them."""
+    from dg_amd import LIMIT_HR, indicator
+
+    mesh = problems.shock_bubble_mesh(256, 128)
+    solver = Solver(Config(2, 2, MULTIFLUID, ROE), mesh)
+    solver.set_sensor_thresholds(indicator.SENSOR_THRESHOLDS)
+    solver.set_indicator(Indicator(seed=5), 1e30)
+    U = torch.from_numpy(problems.shock_bubble_state(mesh, xs=0.24)).to(device)
+    solver.rk4_integrate(U, 1, 0.4 * mesh.dx / 5, nsteps=40, tf=1e30, limiter=LIMIT_HR)
+    NE = mesh.sizes["N_E"]
+    sens = torch.zeros(NE, dtype=torch.int32, device=device)
+    flags = torch.zeros_like(sens)
+    solver.sensing(U, sens)
+    solver.indicator_flags(U, flags)
+    torch.cuda.synchronize()
+    c = indicator.confusion(flags.cpu().numpy(), sens.cpu().numpy())
+    print("guard alone:", c)
+    assert c["shock_cells"] > 100 and c["missed_shock"] == 0, c
+    solver.set_shock_guard(False)
+    solver.indicator_flags(U, flags)
+    torch.cuda.synchronize()
+    c0 = indicator.confusion(flags.cpu().numpy(), sens.cpu().numpy())
+    print("without the guard:", c0)
+    assert c0["missed_shock"] > 0.9 * c0["shock_cells"], c0
