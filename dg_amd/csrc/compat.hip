@@ -2,6 +2,8 @@
 // functions and its BLAS macro layer).  Device pointers, reference layouts,
 // null stream, synchronous return.  Physics configuration comes from
 // dgx_set_config() because the reference fixes it at compile time.
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 #include "dgx_internal.h"
@@ -34,6 +36,36 @@ hipError_t compat_m2i(int D, int N_s1D, int N_E, int N_N, const int* nb, int off
                       const double* M2L, const int* sensors, const double* U, double* Unew);
 
 static dgx_config g_cfg = {1, 2, DGX_MULTIFLUID, DGX_RUSANOV};
+
+// Fixed device scratch of dgx_calc_sensors_noalloc: five ints per element
+// (hit targets, hit values, two activity vectors, initial flags) for up to
+// kSensorsNoallocMaxElements elements, then the sweep flag.  Each GPU the
+// library runs on holds its own copy (40 MiB) until the process ends.  Host
+// code never uses the array's name as an address: noalloc_scratch() looks up
+// each device's copy once, with that device current.
+constexpr long kSensorsNoallocMaxElements = 1L << 21;
+static __device__ __align__(256) int scratch[5 * kSensorsNoallocMaxElements + 64];
+
+static std::mutex g_scratch_mu;
+static int* g_scratch_dev[64];  // the array's address on each device, once looked up
+
+static bool noalloc_scratch(int** out) {
+  int dev = -1;
+  if (!hip_ok(hipGetDevice(&dev), "dgx_calc_sensors_noalloc: hipGetDevice")) return false;
+  if (dev < 0 || dev >= 64) {
+    set_error("dgx_calc_sensors_noalloc: device ordinal out of range");
+    return false;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "dgx_calc_sensors_noalloc: hipGetSymbolAddress"))
+      return false;
+    g_scratch_dev[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return true;
+}
 const dgx_config& current_config() { return g_cfg; }
 
 static inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
@@ -386,6 +418,41 @@ void Lcalc_sensors(int N_E, int N_N, dgx_bool sensor1, double thresh1, dgx_bool 
   finish("Lcalc_sensors");
   hipFree(buf);
   hipFree(flag);
+}
+
+dgx_status dgx_calc_sensors_noalloc(int N_E, int N_N, dgx_bool sensor1, double thresh1, dgx_bool sensor2,
+                                    double thresh2, dgx_bool sensor3, double thresh3, int* neighbors, double* Uavg,
+                                    int* sensors) {
+  if (N_E < 0 || N_N < 0 || (N_E > 0 && (!neighbors || !Uavg || !sensors))) {
+    set_error("dgx_calc_sensors_noalloc: bad arguments");
+    return DGX_EINVAL;
+  }
+  if (N_E > kSensorsNoallocMaxElements) {
+    set_error("dgx_calc_sensors_noalloc: " + std::to_string(N_E) + " elements exceed the fixed scratch of " +
+              std::to_string(kSensorsNoallocMaxElements) + " elements (use Lcalc_sensors)");
+    return DGX_EINVAL;
+  }
+  if (N_E == 0) return DGX_OK;
+  int* base = nullptr;
+  if (!noalloc_scratch(&base)) return DGX_EDEVICE;
+  Solver tmp;
+  tmp.cfg = g_cfg;
+  tmp.D = g_cfg.dim;
+  tmp.N_E = N_E;
+  tmp.N_N = N_N;
+  // [0, 5*N_E): hit targets and values, two activity vectors, initial flags;
+  // the sweep flag after the largest such span, so its offset never moves
+  tmp.flag = base + 5 * (size_t)kSensorsNoallocMaxElements;
+  int on[3] = {sensor1, sensor2, sensor3};
+  double th[3] = {thresh1, thresh2, thresh3};
+  hipError_t e = compat_sensors(tmp, neighbors, Uavg, sensors, on, th, base);
+  if (e != hipSuccess) {
+    hip_ok(e, "dgx_calc_sensors_noalloc");
+    return DGX_EDEVICE;
+  }
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return hip_ok(e, "dgx_calc_sensors_noalloc") ? DGX_OK : DGX_EDEVICE;
 }
 
 void Lcopy_detected(int N_s, int N_E, int* sensors, double* Uold, double* U) {

@@ -135,6 +135,16 @@ void Lformat_sensor(int N_s, int N_E, int* sensor, double* output);
 void Lcalc_sensors(int N_E, int N_N, dgx_bool sensor1, double thresh1, dgx_bool sensor2, double thresh2, dgx_bool sensor3,
                    double thresh3, int* neighbors, double* Uavg, int* sensors);
 void Lcopy_detected(int N_s, int N_E, int* sensors, double* Uold, double* U);
+/* No counterpart in the reference: Lcalc_sensors for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and flags as
+ * Lcalc_sensors; its scratch is a fixed per-device array (40 MiB) instead of
+ * hipMalloc, so N_E is limited to 2^21 elements (DGX_EINVAL beyond, with
+ * dgx_last_error()).  Calls that overlap on one GPU share that array and race.
+ * Synchronous, null stream, like the reference launchers. */
+dgx_status dgx_calc_sensors_noalloc(int N_E, int N_N, dgx_bool sensor1, double thresh1, dgx_bool sensor2,
+                                    double thresh2, dgx_bool sensor3, double thresh3, int* neighbors, double* Uavg,
+                                    int* sensors);
 /* BLAS macro layer (utils/scalar_def.h:81-87): column-major, device pointers */
 void dgx_blasDgemm(char transa, char transb, int M, int N, int K, double alpha, const double* A, int lda,
                    const double* B, int ldb, double beta, double* C, int ldc);
