@@ -581,7 +581,7 @@ static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, 
     if (tiled) {
       KernelTimer kt(s, 4, st);
       if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, Uout, dt, nullptr, st, means,
-                               means && s.shock_guard_on() ? s.Uavg : nullptr, &means_ready),
+                               means && s.shock_guard_on() ? s.Uavg : nullptr, s.mlp_psens, &means_ready),
                   "tile kernel"))
         return DGX_EDEVICE;
       continue;
@@ -758,7 +758,8 @@ dgx_status dgx_solver_set_indicator(dgx_solver* h, dgx_mlp* mlp, float threshold
     if (!legendre_vinv(h->ref_nodes, s.N_s, vinv)) return fail(DGX_EINVAL, "dgx_solver_set_indicator: singular transform");
     if (!upload(s, &s.t.mlp_vinv, vinv.data(), vinv.size()) || !alloc(s, &s.mlp_avg, (size_t)s.N_E * 3) ||
         !alloc(s, &s.mlp_x, (size_t)s.N_E * 64) || !alloc(s, &s.mlp_z, (size_t)s.N_E) ||
-        !alloc(s, &s.mlp_jumps, (size_t)s.N_E * 4) || !alloc(s, &s.mlp_guard, (size_t)s.N_E))
+        !alloc(s, &s.mlp_jumps, (size_t)s.N_E * 4) || !alloc(s, &s.mlp_psens, (size_t)s.N_E) ||
+        !alloc(s, &s.mlp_guard, (size_t)s.N_E))
       return DGX_EDEVICE;
   }
   s.mlp = mlp;

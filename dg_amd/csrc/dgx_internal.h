@@ -106,6 +106,7 @@ struct Solver {
   const double* means_of = nullptr;
   bool fuse_means = true;      // RK stages emit the indicator's cell means (dgx_solver_set_fused_means)
   bool shock_guard = true;     // the indicator also flags the sensor's shock pairs (dgx_solver_set_shock_guard)
+  double* mlp_psens = nullptr; // [e] the sensor's pressure from the cell averages (shock guard)
   int* mlp_guard = nullptr;    // [e] 1 where the sensor's shock test fires for a pair of e and a neighbour
   // the guard needs the sensor's shock test switched on (a second threshold)
   bool shock_guard_on() const { return shock_guard && n_thresh >= 2; }
@@ -127,11 +128,12 @@ hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, 
 // fused tile kernel (Cartesian meshes): same modes as launch_elem
 bool stage_tiles(int D, int nx, int ny, int* tx, int* ty);
 // means: also write the output tile's cell means of rho, p and alpha there
-// (2D p=2 two-fluid RK stages), and with uavg the sensor's cell averages;
+// (2D p=2 two-fluid RK stages), and with uavg and psens the sensor's cell
+// averages and pressures;
 // *means_done tells whether the kernel did
 hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
                         double* Uout, const double* dt_dev, double* f_out, hipStream_t st, double* means = nullptr,
-                        double* uavg = nullptr, bool* means_done = nullptr);
+                        double* uavg = nullptr, double* psens = nullptr, bool* means_done = nullptr);
 hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* dt_dev, hipStream_t st);
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 // Out = HR(In), both 2D sweeps in one launch; In != Out, needs s.hr2d

@@ -32,6 +32,10 @@
 #include "physics.cuh"
 #include "sensor.cuh"
 
+#ifndef DGX_IND_BLOCKS
+#define DGX_IND_BLOCKS 4096  // most blocks of the flagging pass (each wave loops over groups of 16 elements)
+#endif
+
 namespace dgx {
 
 constexpr int kIn = 49, kInP = 64, kH = 64;
@@ -66,12 +70,14 @@ __host__ __device__ inline float bf2f(uint16_t b) {
 }
 
 // ------------------------------------------------------------- features
-// cell means of rho, p and alpha = 1/(gamma-1) (cell_means_rpa); a block
+// cell means of rho, p and alpha = 1/(gamma-1) (cell_means_rpa), and with
+// uavg the sensor's cell averages and pressure (sensor_averages_2d); a block
 // stages 64 elements with coalesced loads, one lane per element reduces.  The
 // stage kernel emits the same values from its output tile when the RK driver
-// limits with the indicator (Solver::means_of), and this pass is skipped then
+// limits with the indicator (Solver::means_of), and this pass is skipped then.
 __global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U, const double* __restrict__ w0, int N_E,
-                                                   double* __restrict__ avg) {
+                                                   double* __restrict__ avg, double* __restrict__ uavg,
+                                                   double* __restrict__ psens) {
   constexpr int NF = 5, Ns = 9, NU = NF * Ns;
   __shared__ double sU[64 * NU];
   const int e0 = blockIdx.x * 64, ne = min(64, N_E - e0);
@@ -95,6 +101,7 @@ __global__ __launch_bounds__(256) void k_mlp_means(const double* __restrict__ U,
   avg[e * 3 + 0] = a3[0];
   avg[e * 3 + 1] = a3[1];
   avg[e * 3 + 2] = a3[2];
+  if (uavg) sensor_averages_2d(sU + threadIdx.x * NU, uavg + (size_t)e * NF, psens + e);
 }
 
 __device__ inline double rel_jump(double a, double b) {
@@ -158,15 +165,17 @@ __device__ inline void wave_modes(double* u, const double* Vinv, double* inv, in
 // relative jump of the cell means of rho, p and alpha (avg, k_mlp_means),
 // as the float the network input takes; -1 marks a domain boundary (the
 // feature is 0 there and the cell is flagged by rule).
-// With uavg (the sensor's cell averages, k_uavg's [e*5 + f]) the same pass
-// writes the shock guard: guard[e] = 1 when the sensor's shock test
-// (sensor_shock, the same arithmetic as calc_sensors) fires for a pair of e
-// and a neighbour in either order.  Every cell the sensor marks as a shock is
-// in such a pair, so the guard holds all of them, whatever the network says.
+// With uavg (the sensor's cell averages [e*5 + f]) and psens (their sensor
+// pressures) the same pass writes the shock guard: guard[e] = 1 when the
+// sensor's shock test (sensor_shock, the same arithmetic as calc_sensors)
+// fires for a pair of e and a neighbour in either order.  Every cell the
+// sensor marks as a shock is in such a pair, so the guard holds all of them,
+// whatever the network says.  The pressure jump W > th, necessary for the
+// test, is checked first from psens.
 __global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ avg, const int* __restrict__ nb, int N,
                                                    float4* __restrict__ jumps, int* __restrict__ count_reset,
-                                                   const double* __restrict__ uavg, double th_shock,
-                                                   int* __restrict__ guard) {
+                                                   const double* __restrict__ uavg, const double* __restrict__ psens,
+                                                   double th_shock, int* __restrict__ guard) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   // the flagged-element count that k_mlp_indicator appends to starts at zero
   if (count_reset && e == 0) *count_reset = 0;
@@ -175,13 +184,10 @@ __global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ av
 #pragma unroll
   for (int n = 0; n < 4; ++n) o[n] = nb[e * 4 + n];
   if (uavg) {
-    const SensorCell C = sensor_cell<2, 1>(uavg, e);
+    const double pe = psens[e];
     int g = 0;
-    for (int n = 0; n < 4; ++n) {
-      if (o[n] < 0) continue;
-      const SensorCell R = sensor_cell<2, 1>(uavg, o[n]);
-      if (sensor_shock(C, R, sensor_roe(C, R), th_shock) || sensor_shock(R, C, sensor_roe(R, C), th_shock)) g = 1;
-    }
+    for (int n = 0; n < 4; ++n)
+      if (o[n] >= 0 && sensor_shock_pair<2, 1>(uavg, e, o[n], pe, psens[o[n]], th_shock)) g = 1;
     guard[e] = g;
   }
   const double r0 = avg[e * 3 + 0], p0 = avg[e * 3 + 1], a0 = avg[e * 3 + 2];
@@ -751,17 +757,15 @@ hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool 
   const int N = s.N_E;
   int* count = with_list ? s.flag + 2 : nullptr;
   const bool guard = s.shock_guard_on();
-  if (s.means_of != U) {  // else the stage kernel that wrote U left its cell means in s.mlp_avg and s.Uavg
-    hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
-    if (guard) {
-      hipError_t e = launch_uavg(s, U, s.Uavg, st);
-      if (e != hipSuccess) return e;
-    }
-  }
+  // else the stage kernel that wrote U left the cell means (and the sensor's
+  // averages and pressures) in s.mlp_avg, s.Uavg and s.mlp_psens
+  if (s.means_of != U)
+    hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg,
+                       guard ? s.Uavg : nullptr, s.mlp_psens);
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
-                     reinterpret_cast<float4*>(s.mlp_jumps), count, guard ? s.Uavg : nullptr, s.thresh[1],
-                     s.mlp_guard);
-  const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
+                     reinterpret_cast<float4*>(s.mlp_jumps), count, guard ? s.Uavg : nullptr, s.mlp_psens,
+                     s.thresh[1], s.mlp_guard);
+  const unsigned nb = std::max(1u, std::min(nblocks_for(N, 64), (unsigned)DGX_IND_BLOCKS));
   hipLaunchKernelGGL(k_mlp_indicator, dim3(nb), dim3(256), 0, st, U, s.t.mlp_vinv,
                      reinterpret_cast<const float4*>(s.mlp_jumps), N, m->w1b, m->w2b, m->w3b, m->P, s.mlp_threshold,
                      guard ? s.mlp_guard : nullptr, s.mlp_z, s.sensors, with_list ? s.flag_list : nullptr, count);
@@ -770,9 +774,10 @@ hipError_t mlp_indicator_flags(Solver& s, const double* U, hipStream_t st, bool 
 
 hipError_t mlp_features(Solver& s, const double* U, float* X, hipStream_t st) {
   const int N = s.N_E;
-  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg);
+  hipLaunchKernelGGL(k_mlp_means, dim3(nblocks_for(N, 64)), dim3(256), 0, st, U, s.t.mlp_vinv, N, s.mlp_avg, nullptr,
+                     nullptr);
   hipLaunchKernelGGL(k_mlp_jumps, dim3(nblocks_for(N, 256)), dim3(256), 0, st, s.mlp_avg, s.t.neighbors, N,
-                     reinterpret_cast<float4*>(s.mlp_jumps), nullptr, nullptr, 0.0, nullptr);
+                     reinterpret_cast<float4*>(s.mlp_jumps), nullptr, nullptr, nullptr, 0.0, nullptr);
   const unsigned nbf = std::max(1u, std::min(nblocks_for(N, 64), 4096u));
   hipLaunchKernelGGL(k_mlp_features, dim3(nbf), dim3(256), 0, st, U, s.t.mlp_vinv,
                      reinterpret_cast<const float4*>(s.mlp_jumps), N, X);

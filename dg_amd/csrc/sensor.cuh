@@ -81,17 +81,46 @@ __device__ inline bool sensor_contact(const SensorCell& L, const SensorCell& R, 
   return PHI > th;
 }
 
+// The pressure-jump measure of test 2, W = 2 psi / (1 + psi)^2 with
+// psi = |pR - pL| / (pL + pR) (sensor_kernels.cu:131-132).  It does not depend
+// on which cell is tested, so W > th is a necessary condition for the shock
+// test in either order.
+__device__ inline double sensor_shock_w(double pL, double pR) {
+#pragma clang fp contract(off)
+  const double PSI = fabs(pR - pL) / (pL + pR);
+  return 2 * PSI / ((1 + PSI) * (1 + PSI));
+}
+
 // Test 2, shock: converging characteristics and a pressure jump
 // (sensor_kernels.cu:125-136).
 __device__ inline bool sensor_shock(const SensorCell& L, const SensorCell& R, const SensorRoe& m, double th) {
 #pragma clang fp contract(off)
   if (((L.vx - L.a > m.vx - m.a) && (m.vx - m.a > R.vx - R.a)) ||
-      ((L.vy - L.a > m.vy - m.a) && (m.vy - m.a > R.vy - R.a))) {
-    const double PSI = fabs(R.p - L.p) / (L.p + R.p);
-    const double W = 2 * PSI / ((1 + PSI) * (1 + PSI));
-    return W > th;
-  }
+      ((L.vy - L.a > m.vy - m.a) && (m.vy - m.a > R.vy - R.a)))
+    return sensor_shock_w(L.p, R.p) > th;
   return false;
+}
+
+// The shock guard of a pair (e, n): the shock test fires with e as the tested
+// cell or with n as the tested cell.  pe, pn: the cells' sensor pressures
+// (SensorCell::p), which decide the necessary W > th first.
+template <int D, int PH>
+__device__ inline bool sensor_shock_pair(const double* __restrict__ Uavg, int e, int n, double pe, double pn,
+                                         double th) {
+  if (!(sensor_shock_w(pe, pn) > th)) return false;
+  const SensorCell C = sensor_cell<D, PH>(Uavg, e), R = sensor_cell<D, PH>(Uavg, n);
+  return sensor_shock(C, R, sensor_roe(C, R), th) || sensor_shock(R, C, sensor_roe(R, C), th);
+}
+
+// The sensor's averages of a 2D p=2 two-fluid element (nodal values
+// u[f*9 + i]) into ua[0..4], and its sensor pressure (SensorCell::p) into *p:
+// what the shock guard reads, written by the producers of the indicator's
+// cell means.
+__device__ inline void sensor_averages_2d(const double* u, double* ua, double* p) {
+  double v[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) v[f] = ua[f] = sensor_nodal_mean<9>(u + f * 9);
+  *p = sensor_cell<2, 1>(v, 0).p;
 }
 
 // Test 3, material (gamma) interface (sensor_kernels.cu:137-145).

@@ -50,7 +50,8 @@ struct StageArgs {
   // tile for the learned indicator ([e*3 + k], cell_means_rpa with w0), or null
   double* means;
   const double* __restrict__ w0;
-  double* uavg;  // with means: also the sensor's cell averages [e*5 + f] (sensor_nodal_mean), or null
+  double* uavg;   // with means: also the sensor's cell averages [e*5 + f] (sensor_nodal_mean), or null
+  double* psens;  // with uavg: their sensor pressures [e]
 };
 
 template <int D, int TX, int TY>
@@ -533,10 +534,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
           m[0] = a3[0];
           m[1] = a3[1];
           m[2] = a3[2];
-          if (a.uavg) {
-#pragma unroll
-            for (int f = 0; f < NF; ++f) a.uavg[eg * NF + f] = sensor_nodal_mean<Ns>(sU + tid * SU + f * Ns);
-          }
+          if (a.uavg) sensor_averages_2d(sU + tid * SU, a.uavg + eg * NF, a.psens + eg);
         }
       }
     }
@@ -594,7 +592,7 @@ static hipError_t dispatch_stage(const dgx_config& c, Fn&& fn) {
 
 hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
                         double* Uout, const double* dt_dev, double* f_out, hipStream_t st, double* means,
-                        double* uavg, bool* means_done) {
+                        double* uavg, double* psens, bool* means_done) {
   if (means_done) *means_done = false;
   const Cart& c = s.cart;
   StageArgs a{};
@@ -638,6 +636,7 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
         (s.precision >= 1 || DGX_STAGE_MFMA64)) {
       a.means = means;
       a.uavg = uavg;
+      a.psens = psens;
       if (means_done) *means_done = true;
     }
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, a);
