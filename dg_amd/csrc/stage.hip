@@ -13,6 +13,7 @@
 // face/element kernels.  Tiles are mapped to XCDs in contiguous bands so
 // that halo rows are shared through each XCD's L2.
 #include <algorithm>
+#include <cstdint>
 #include <type_traits>
 
 #include "dgx_internal.h"
@@ -222,7 +223,10 @@ __device__ inline int tile_of(const StageArgs& a, int i) {
   return t < ntiles ? t : -1;
 }
 
-template <int D, int P, int PH, int FL, int MODE, int TX, int TY, int PREC>
+// VEC: the tile state, the RK operands and the stage output move as 16-byte
+// pairs (tile rows are contiguous in HBM and in LDS; every state pointer is
+// 16-byte aligned, which the host checks before it picks this instance).
+template <int D, int P, int PH, int FL, int MODE, int TX, int TY, int PREC, bool VEC = false>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   using V = Vol<D, P, PH>;
   using T = Tile<D, TX, TY>;
@@ -243,11 +247,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   constexpr int NGEO = 1 + D * D;
   static_assert(PREC == 0 || MFW, "mixed precision: 2D p=2 tiles of a multiple of 16 elements");
   __shared__ int sNodeQ[18];
-  // per-thread counts of the loaded quantities
-  constexpr int R_U = (NT * NU + 255) / 256, R_H = (NH * HT + 255) / 256;
+  // per-thread counts of the loaded quantities (state entries in groups of VW)
+  static_assert(!VEC || (D == 2 && SU == NU && (TX * NU) % 2 == 0), "paired state rows");
+  constexpr int VW = VEC ? 2 : 1;
+  constexpr int R_UV = (NT * NU / VW + 255) / 256, R_U = R_UV * VW;
+  constexpr int R_H = (NH * HT + 255) / 256;
   constexpr int R_G = (NT * NGEO + 255) / 256, R_F = (NFACE * 4 + 255) / 256;
 
-  __shared__ double sU[NT * SU];
+  __shared__ __align__(16) double sU[NT * SU];
   __shared__ double sH[NH * HT];
   __shared__ double sQ[NT * SQ];
   __shared__ double sW[NW];  // face-point fluxes, then per-point volume values
@@ -279,11 +286,18 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   auto issue = [&](int tile) {
     const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
 #pragma unroll
-    for (int r = 0; r < R_U; ++r) {
-      const int k = tid + r * 256;
+    for (int r = 0; r < R_UV; ++r) {
+      const int k = VW * (tid + r * 256);
       if (k < NT * NU) {
         const int ty = k / (TX * NU);
-        pu[r] = a.Uin[((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU)];
+        const double* src = a.Uin + ((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU);
+        if constexpr (VEC) {
+          const double2 v = *reinterpret_cast<const double2*>(src);
+          pu[2 * r] = v.x;
+          pu[2 * r + 1] = v.y;
+        } else {
+          pu[r] = *src;
+        }
       }
     }
 #pragma unroll
@@ -324,9 +338,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   };
   auto commit = [&]() {
 #pragma unroll
-    for (int r = 0; r < R_U; ++r) {
-      const int k = tid + r * 256;
-      if (k < NT * NU) sU[(k / NU) * SU + k % NU] = pu[r];
+    for (int r = 0; r < R_UV; ++r) {
+      const int k = VW * (tid + r * 256);
+      if (k < NT * NU) {
+        if constexpr (VEC)
+          *reinterpret_cast<double2*>(sU + k) = make_double2(pu[2 * r], pu[2 * r + 1]);  // SU == NU
+        else
+          sU[(k / NU) * SU + k % NU] = pu[r];
+      }
     }
 #pragma unroll
     for (int r = 0; r < R_H; ++r) {
@@ -363,13 +382,27 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     // inputs: all in flight during the compute below
     double pre_un[R_U], pre_acc[R_U];
 #pragma unroll
-    for (int r = 0; r < R_U; ++r) {
-      pre_un[r] = pre_acc[r] = 0.0;
-      const int k = tid + r * 256;
+    for (int r = 0; r < R_UV; ++r) {
+#pragma unroll
+      for (int v = 0; v < VW; ++v) pre_un[r * VW + v] = pre_acc[r * VW + v] = 0.0;
+      const int k = VW * (tid + r * 256);
       if (MODE == 1 && k < NT * NU) {
         const size_t i = gidx(k);
-        if (a.stage < 3) pre_un[r] = a.Un[i];
-        if (a.stage > 0) pre_acc[r] = a.Uacc[i];
+        if constexpr (VEC) {
+          if (a.stage < 3) {
+            const double2 v = *reinterpret_cast<const double2*>(a.Un + i);
+            pre_un[2 * r] = v.x;
+            pre_un[2 * r + 1] = v.y;
+          }
+          if (a.stage > 0) {
+            const double2 v = *reinterpret_cast<const double2*>(a.Uacc + i);
+            pre_acc[2 * r] = v.x;
+            pre_acc[2 * r + 1] = v.y;
+          }
+        } else {
+          if (a.stage < 3) pre_un[r] = a.Un[i];
+          if (a.stage > 0) pre_acc[r] = a.Uacc[i];
+        }
       }
     }
     const int next = DGX_STAGE_PERSIST ? tile_of(a, it + 1) : -1;
@@ -496,31 +529,43 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     }
     }
 
-    // ---- stage output (coalesced along the tile rows)
+    // ---- stage output (coalesced along the tile rows, VW entries per step)
 #pragma unroll
-    for (int r = 0; r < R_U; ++r) {
-      const int k = tid + r * 256;
-      if (k >= NT * NU) break;
-      const int el = k / NU, rr = k % NU;
-      const size_t gi = gidx(k);
-      if constexpr (MODE == 0) {
-        a.fout[gi] = sU[el * SU + rr];
-      } else if (MFW && shared_minv) {
-        const double o = rk_combine(a.stage, sU[el * SU + rr], pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
-        if constexpr (PH == 1) {
-          if (a.means) sU[el * SU + rr] = o;  // this thread's own entry: kept for the cell means below
+    for (int r = 0; r < R_UV; ++r) {
+      const int k0 = VW * (tid + r * 256);
+      if (k0 >= NT * NU) break;
+      const size_t gi = gidx(k0);
+      double ov[VW], av[VW];
+#pragma unroll
+      for (int v = 0; v < VW; ++v) {
+        const int k = k0 + v, el = k / NU, rr = k % NU;
+        if constexpr (MODE == 0) {
+          ov[v] = sU[el * SU + rr];
+        } else if (MFW && shared_minv) {
+          rk_values(a.stage, sU[el * SU + rr], pre_un[r * VW + v], pre_acc[r * VW + v], av[v], ov[v]);
+          if constexpr (PH == 1) {
+            if (a.means) sU[el * SU + rr] = ov[v];  // this thread's own entry: kept for the cell means below
+          }
+        } else {
+          // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
+          const double dte = sDte[el];
+          const double* M = shared_minv ? tb.minv
+                                        : a.Minv + ((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * Ns * Ns;
+          const int f = rr / Ns, i = rr % Ns;
+          const double* R = sU + el * SU + f * Ns;
+          double sol = 0.0;
+#pragma unroll
+          for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
+          rk_values(a.stage, dte * sol, pre_un[r * VW + v], pre_acc[r * VW + v], av[v], ov[v]);
         }
+      }
+      double* out = MODE == 0 ? a.fout + gi : a.Uout + gi;
+      if constexpr (VEC) {
+        *reinterpret_cast<double2*>(out) = make_double2(ov[0], ov[1]);
+        if (MODE == 1 && a.stage < 3) *reinterpret_cast<double2*>(a.Uacc + gi) = make_double2(av[0], av[1]);
       } else {
-        // shared Minv: Minv_e = Minv_0 * J_0 / J_e (affine elements); else per element
-        const double dte = sDte[el];
-        const double* M = shared_minv ? tb.minv
-                                      : a.Minv + ((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * Ns * Ns;
-        const int f = rr / Ns, i = rr % Ns;
-        const double* R = sU + el * SU + f * Ns;
-        double sol = 0.0;
-#pragma unroll
-        for (int ii = 0; ii < Ns; ++ii) sol += M[ii * Ns + i] * R[ii];
-        rk_combine(a.stage, dte * sol, pre_un[r], pre_acc[r], a.Uacc + gi, a.Uout + gi);
+        out[0] = ov[0];
+        if (MODE == 1 && a.stage < 3) a.Uacc[gi] = av[0];
       }
     }
     if constexpr (MODE == 1 && MFW && PH == 1) {
@@ -612,10 +657,21 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
     constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
     auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0> : k_stage<D, P, PH, FL, 1, TX, TY, 0>;
     if constexpr (D == 2 && P == 2 && TX * TY == 32 && PH <= 1) {
-      if (s.precision == 1)
-        kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
-      if (s.precision == 2)
-        kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2> : k_stage<D, P, PH, FL, 1, TX, TY, 2>;
+      // 16-byte pairs when every state pointer allows them
+      auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+      const bool vec = al(Uin) && (mode == 0 ? al(f_out) : al(Un) && al(Uacc) && al(Uout));
+      if (vec) {
+        kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0, true> : k_stage<D, P, PH, FL, 1, TX, TY, 0, true>;
+        if (s.precision == 1)
+          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1, true> : k_stage<D, P, PH, FL, 1, TX, TY, 1, true>;
+        if (s.precision == 2)
+          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2, true> : k_stage<D, P, PH, FL, 1, TX, TY, 2, true>;
+      } else {
+        if (s.precision == 1)
+          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
+        if (s.precision == 2)
+          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2> : k_stage<D, P, PH, FL, 1, TX, TY, 2>;
+      }
     }
     // persistent grid: as many blocks as fit on the device at once
     int nb = ntiles;

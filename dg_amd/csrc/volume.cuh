@@ -198,18 +198,26 @@ __device__ inline void vol_phase2(const VolTables<D, P, PH>& t, int i, const dou
 
 // RK4 stage combination for one state entry (rk.cc:164-198 with DU fused):
 // un = U^n, acc = accumulator (read for stages 1-3), du = Dt*Minv*f.
-__device__ inline double rk_combine(int st, double du, double un, double acc, double* acc_out, double* out) {
+// The values rk_combine stores: the new accumulator (stages 0-2) and the
+// stage output.
+__device__ inline void rk_values(int st, double du, double un, double acc, double& acc_new, double& out) {
   const double gk = kRkGamma[st];
-  double o;
   if (st == 0) {
-    *acc_out = un + gk * du;
-    o = un + kRkBeta[1] * du;
+    acc_new = un + gk * du;
+    out = un + kRkBeta[1] * du;
   } else if (st < 3) {
-    *acc_out = acc + gk * du;
-    o = un + kRkBeta[st + 1] * du;
+    acc_new = acc + gk * du;
+    out = un + kRkBeta[st + 1] * du;
   } else {
-    o = acc + gk * du;
+    acc_new = acc;
+    out = acc + gk * du;
   }
+}
+
+__device__ inline double rk_combine(int st, double du, double un, double acc, double* acc_out, double* out) {
+  double an, o;
+  rk_values(st, du, un, acc, an, o);
+  if (st < 3) *acc_out = an;
   *out = o;
   return o;  // the stage output written
 }
