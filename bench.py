@@ -75,8 +75,8 @@ def main() -> int:
         args.limiter = "learned"
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs (--gpus N > 1) are not part of this build "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; multi-GPU runs (--gpus N > 1) are not supported",
+              file=sys.stderr)
         return 2
 
     import numpy as np
@@ -281,8 +281,9 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": (measured_traffic(args.config, dominant)
-                        if args.precision == "fp64" and args.path != "general" and args.n in (0, 512) else None),
+            "traffic": (measured_traffic(args.config if args.precision == "fp64" else f"{args.config}-{args.precision}",
+                                         dominant)
+                        if args.path != "general" and args.n in (0, 512) else None),
             "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg,
             "kernels_ms": avg,
         },

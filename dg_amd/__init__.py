@@ -104,6 +104,7 @@ def lib() -> ctypes.CDLL:
                                     ctypes.c_ulonglong, vp, vp]
         L.dgx_solver_set_indicator.argtypes = [vp, vp, ctypes.c_float]
         L.dgx_solver_set_fused_means.argtypes = [vp, i]
+        L.dgx_solver_set_device_time.argtypes = [vp, i]
         L.dgx_solver_set_shock_guard.argtypes = [vp, i]
         L.dgx_mlp_features.argtypes = [vp, vp, vp, vp]
         L.dgx_indicator_flags.argtypes = [vp, vp, vp, vp, vp]
@@ -361,6 +362,12 @@ class Solver:
         """With limiter 8 on the tile kernel, let the RK stages write the cell
         means the indicator reads (default) or leave them to its own pass."""
         _check(lib().dgx_solver_set_fused_means(self._h, 1 if on else 0), "dgx_solver_set_fused_means")
+
+    def set_device_time(self, on: bool) -> None:
+        """CFL runs of rk4_integrate with limiter 0 or 1: time control on the
+        device, one read-back per chunk of steps (default), or the per-step
+        host loop."""
+        _check(lib().dgx_solver_set_device_time(self._h, 1 if on else 0), "dgx_solver_set_device_time")
 
     def set_shock_guard(self, on: bool) -> None:
         """With the sensor thresholds set, let the indicator also flag every

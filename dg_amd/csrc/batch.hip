@@ -81,73 +81,106 @@ struct BatchShape {
 
 // Taylor coefficients of the problem's columns (the first half of the HR
 // limiter, limiting.cc:11-27 in 1D): A = L2M U per (element, field) column.
+// KL columns per thread at a time, their loads issued together (a loop that
+// stores one column before it loads the next waits a memory round trip per
+// column).
 template <int P, int PH>
 __device__ void batch_limit_l2m(const double* U, double* A, int e0, int e1, const double* sL2M) {
 #pragma clang fp contract(off)
   using B = BatchShape<P, PH>;
-  constexpr int NF = B::NF, Ns = B::Ns;
-  for (int col = e0 * NF + threadIdx.x; col < e1 * NF; col += blockDim.x) {
-    double x[Ns], y[Ns];
+  constexpr int NF = B::NF, Ns = B::Ns, KL = 4;
+  const int end = e1 * NF, stride = blockDim.x;
+  for (int c0 = e0 * NF + threadIdx.x; c0 < end; c0 += KL * stride) {
+    double x[KL][Ns];
 #pragma unroll
-    for (int l = 0; l < Ns; ++l) x[l] = U[col * Ns + l];
+    for (int k = 0; k < KL; ++k) {
+      const int col = c0 + k * stride;
 #pragma unroll
-    for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+      for (int l = 0; l < Ns; ++l) x[k][l] = col < end ? U[col * Ns + l] : 0.0;
+    }
 #pragma unroll
-    for (int l = 0; l < Ns; ++l)
+    for (int k = 0; k < KL; ++k) {
+      const int col = c0 + k * stride;
+      if (col >= end) break;
+      double y[Ns];
 #pragma unroll
-      for (int r = 0; r < Ns; ++r) y[r] = y[r] + x[l] * sL2M[r + l * Ns];
+      for (int r = 0; r < Ns; ++r) y[r] = 0.0;
 #pragma unroll
-    for (int r = 0; r < Ns; ++r) A[col * Ns + r] = y[r];
+      for (int l = 0; l < Ns; ++l)
+#pragma unroll
+        for (int r = 0; r < Ns; ++r) y[r] = y[r] + x[k][l] * sL2M[r + l * Ns];
+#pragma unroll
+      for (int r = 0; r < Ns; ++r) A[col * Ns + r] = y[r];
+    }
   }
 }
 
 // The second half: per column, hrl1D with the neighbours' Taylor
-// coefficients, then U = M2L Alim.
+// coefficients, then U = M2L Alim; KA columns per thread at a time, each with
+// its neighbours' coefficients loaded together (KA = 2 spilled registers in the
+// order-3 body).
 template <int P, int PH>
 __device__ void batch_limit_apply(const BatchArgs& a, double* U, const double* A, int NE, int e0, int e1,
                                   const double* sM2L) {
 #pragma clang fp contract(off)
   using B = BatchShape<P, PH>;
-  constexpr int NF = B::NF, Ns = B::Ns;
-  for (int col = e0 * NF + threadIdx.x; col < e1 * NF; col += blockDim.x) {
-    const int e = col / NF, fc = col % NF;
-    const int left = e > 0 ? e - 1 : -a.bc[0];
-    const int right = e < NE - 1 ? e + 1 : -a.bc[1];
-    int physical = 0;
-    if (left < 0) physical = -left;
-    if (right < 0) physical = -right;
-    double lim[Ns], ac[Ns];
+  constexpr int NF = B::NF, Ns = B::Ns, KA = 1;
+  const int end = e1 * NF, stride = blockDim.x;
+  for (int c0 = e0 * NF + threadIdx.x; c0 < end; c0 += KA * stride) {
+    double al[KA][Ns], ac[KA][Ns], ar[KA][Ns];
+    int physical[KA];
 #pragma unroll
-    for (int n = 0; n < Ns; ++n) { ac[n] = A[col * Ns + n]; lim[n] = 0.0; }
-    if (physical == 2 || (physical == 3 && fc != 1)) {
-      average_monomial_d<P>(ac, lim);
-    } else if (physical == 3) {
-#pragma unroll
-      for (int n = 0; n < Ns; ++n) lim[n] = ac[n];
-    } else {
-      double al[Ns], ar[Ns];
+    for (int k = 0; k < KA; ++k) {
+      const int col = c0 + k * stride;
+      const int e = col / NF, fc = col % NF;
+      const int left = e > 0 ? e - 1 : -a.bc[0];
+      const int right = e < NE - 1 ? e + 1 : -a.bc[1];
+      int ph = 0;
+      if (left < 0) ph = -left;
+      if (right < 0) ph = -right;
+      physical[k] = ph;
+      const bool in = col < end, inner = in && ph == 0;
 #pragma unroll
       for (int n = 0; n < Ns; ++n) {
-        al[n] = A[(left * NF + fc) * Ns + n];
-        ar[n] = A[(right * NF + fc) * Ns + n];
+        ac[k][n] = in ? A[col * Ns + n] : 0.0;
+        al[k][n] = inner ? A[(left * NF + fc) * Ns + n] : 0.0;
+        ar[k][n] = inner ? A[(right * NF + fc) * Ns + n] : 0.0;
       }
-      limit_monomial_d<P>(al, ac, ar, lim);
     }
-    double y[Ns];
 #pragma unroll
-    for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+    for (int k = 0; k < KA; ++k) {
+      const int col = c0 + k * stride;
+      if (col >= end) break;
+      const int fc = col % NF;
+      double lim[Ns];
 #pragma unroll
-    for (int l = 0; l < Ns; ++l)
+      for (int n = 0; n < Ns; ++n) lim[n] = 0.0;
+      if (physical[k] == 2 || (physical[k] == 3 && fc != 1)) {
+        average_monomial_d<P>(ac[k], lim);
+      } else if (physical[k] == 3) {
 #pragma unroll
-      for (int r = 0; r < Ns; ++r) y[r] = y[r] + lim[l] * sM2L[r + l * Ns];
+        for (int n = 0; n < Ns; ++n) lim[n] = ac[k][n];
+      } else {
+        limit_monomial_d<P>(al[k], ac[k], ar[k], lim);
+      }
+      double y[Ns];
 #pragma unroll
-    for (int r = 0; r < Ns; ++r) U[col * Ns + r] = y[r];
+      for (int r = 0; r < Ns; ++r) y[r] = 0.0;
+#pragma unroll
+      for (int l = 0; l < Ns; ++l)
+#pragma unroll
+        for (int r = 0; r < Ns; ++r) y[r] = y[r] + lim[l] * sM2L[r + l * Ns];
+#pragma unroll
+      for (int r = 0; r < Ns; ++r) U[col * Ns + r] = y[r];
+    }
   }
 }
 
 // One workgroup's problem (order P): every step, stage and limiter
-// application, separated by workgroup barriers.
-template <int P, int PH, int FL>
+// application, separated by workgroup barriers.  PF: the element pass loads
+// the next element's state ahead (the 512-thread instance, whose register
+// budget holds it).
+template <int P, int PH, int FL, bool PF>
 __device__ void batch_problem(const BatchArgs& a, int pid) {
   using B = BatchShape<P, PH>;
   using V = Vol<1, P, PH>;
@@ -189,13 +222,22 @@ __device__ void batch_problem(const BatchArgs& a, int pid) {
 
   for (int step = 0; step < a.nsteps; ++step) {
     // ---- Dt = cfl / max over nodes of |u|+a (rk.cc:250-272)
+    // (KD nodes per thread at a time, loads issued together; max is exact in
+    // any order)
     double m = 0.0;
-    for (int n = tid; n < NE * Ns; n += nthr) {
-      const int e = n / Ns, i = n % Ns;
-      double u[NF];
+    constexpr int KD = 4;
+    for (int n0 = tid; n0 < NE * Ns; n0 += KD * nthr) {
+      double u[KD][NF];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) u[f] = U[(e * NF + f) * Ns + i];
-      m = nanmax(m, upa<1, PH>(u));
+      for (int k = 0; k < KD; ++k) {
+        const int n = n0 + k * nthr;
+        const int e = n / Ns, i = n % Ns;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) u[k][f] = n < NE * Ns ? U[(e * NF + f) * Ns + i] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < KD; ++k)
+        if (n0 + k * nthr < NE * Ns) m = nanmax(m, upa<1, PH>(u[k]));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = nanmax(m, __shfl_xor(m, off));
@@ -219,41 +261,85 @@ __device__ void batch_problem(const BatchArgs& a, int pid) {
     for (int k = 0; k < 4; ++k) {
       if (k > 0 && a.limiter) limit(cur);
       // ---- faces: interface j between elements j-1 and j; boundary faces
-      // contribute to the boundary element only (dg_functions.cc:225-233)
-      for (int j = tid; j <= NE; j += nthr) {
-        const bool lb = j == 0, rb = j == NE;
-        const int ea = lb ? 0 : j - 1, na = lb ? 0 : 1;  // side 0: element and node
-        const int eb = (lb || rb) ? ea : j, nb = (lb || rb) ? na : 0;
-        const int tag = lb ? a.bc[0] : (rb ? a.bc[1] : 0);
-        double ul[NF], ur[NF], F[NF], nc[NF];
+      // contribute to the boundary element only (dg_functions.cc:225-233).
+      // KF faces per thread at a time, their loads issued together.
+      constexpr int KF = 2;
+      for (int j0 = tid; j0 <= NE; j0 += KF * nthr) {
+        double ul[KF][NF], ur[KF][NF];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          ul[f] = cur[(ea * NF + f) * Ns + na];
-          ur[f] = cur[(eb * NF + f) * Ns + nb];
+        for (int q = 0; q < KF; ++q) {
+          const int j = j0 + q * nthr;
+          const bool in = j <= NE, lb = j == 0, rb = j == NE;
+          const int ea = lb ? 0 : j - 1, na = lb ? 0 : 1;  // side 0: element and node
+          const int eb = (lb || rb) ? ea : j, nb = (lb || rb) ? na : 0;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            ul[q][f] = in ? cur[(ea * NF + f) * Ns + na] : 0.0;
+            ur[q][f] = in ? cur[(eb * NF + f) * Ns + nb] : 0.0;
+          }
         }
-        if (tag == BC_REFLECTIVE) ur[1] = -ur[1];  // boundaries.cu:20-35
-        riemann<1, PH, FL>(ul, ur, nrm[j], 0.0, F, nc);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) {
-          Qe[(ea * 2 + na) * NF + f] = psiw * (-F[f] + nc[f]);
-          if (!lb && !rb) Qe[(eb * 2 + nb) * NF + f] = psiw * (F[f] + nc[f]);
+        for (int q = 0; q < KF; ++q) {
+          const int j = j0 + q * nthr;
+          if (j > NE) break;
+          const bool lb = j == 0, rb = j == NE;
+          const int ea = lb ? 0 : j - 1, na = lb ? 0 : 1;
+          const int eb = (lb || rb) ? ea : j, nb = (lb || rb) ? na : 0;
+          const int tag = lb ? a.bc[0] : (rb ? a.bc[1] : 0);
+          double F[NF], nc[NF];
+          if (tag == BC_REFLECTIVE) ur[q][1] = -ur[q][1];  // boundaries.cu:20-35
+          riemann<1, PH, FL>(ul[q], ur[q], nrm[j], 0.0, F, nc);
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            Qe[(ea * 2 + na) * NF + f] = psiw * (-F[f] + nc[f]);
+            if (!lb && !rb) Qe[(eb * 2 + nb) * NF + f] = psiw * (F[f] + nc[f]);
+          }
         }
       }
       __syncthreads();
-      // ---- elements: volume terms, face terms, Dt*Minv and the RK4 update
+      // ---- elements: volume terms, face terms, Dt*Minv and the RK4 update.
+      // The element's RK operands are loaded as batches issued together (a
+      // store to `out`, which may be U, would otherwise order every later
+      // load after it): with PF all of them before the volume terms, and the
+      // next element's state too when it fits the registers (PFU); else half
+      // the fields at a time after the volume terms.
+      constexpr bool PFU = PF && NU <= 12;
+      constexpr int FB = PF ? NF : (NF + 1) / 2;  // fields per operand batch
       double* out = (k == 3) ? U : S;
+      double u[NU];
+      if (PFU && tid < NE) {
+#pragma unroll
+        for (int r = 0; r < NU; ++r) u[r] = cur[tid * NU + r];
+      }
       for (int e = tid; e < NE; e += nthr) {
         // vol_phase1 at each Gauss point, accumulated at once into every
         // node's sum in vol_phase2's order (point by point), so that the
         // per-point values need no registers of their own
         constexpr int NFX = V::NFX, NNC = V::NNC;
-        double u[NU], R[NU];
+        double R[NU], pun[FB * Ns], pacc[FB * Ns];
+        auto operands = [&](int f0) {
 #pragma unroll
-        for (int r = 0; r < NU; ++r) {
-          u[r] = cur[e * NU + r];
-          R[r] = 0.0;
+          for (int r = 0; r < FB * Ns; ++r) {
+            const long idx = (long)e * NU + f0 * Ns + r;
+            const bool in = f0 * Ns + r < NU;
+            pun[r] = (in && k < 3) ? U[idx] : 0.0;
+            pacc[r] = (in && k > 0) ? Uacc[idx] : 0.0;
+          }
+        };
+        if (!PFU) {
+#pragma unroll
+          for (int r = 0; r < NU; ++r) u[r] = cur[e * NU + r];
         }
         const double Je = J[e], ije = ij[e];
+        if (PF) operands(0);
+        double u_next[PFU ? NU : 1];
+        if (PFU) {
+          const int en = e + nthr;
+#pragma unroll
+          for (int r = 0; r < NU; ++r) u_next[r] = en < NE ? cur[en * NU + r] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < NU; ++r) R[r] = 0.0;
 #pragma unroll 1
         for (int g = 0; g < NG; ++g) {
           double feg[NQ];
@@ -278,15 +364,21 @@ __device__ void batch_problem(const BatchArgs& a, int pid) {
         }
         const double dte = dt * (J0 / Je);
 #pragma unroll
-        for (int r = 0; r < NU; ++r) {
-          const int f = r / Ns, i = r % Ns;
-          double sol = 0.0;
+        for (int f0 = 0; f0 < NF; f0 += FB) {
+          if (!PF) operands(f0);
 #pragma unroll
-          for (int ii = 0; ii < Ns; ++ii) sol += sMinv[ii * Ns + i] * R[f * Ns + ii];
-          const long idx = (long)e * NU + r;
-          const double un = k < 3 ? U[idx] : 0.0;
-          const double acc = k > 0 ? Uacc[idx] : 0.0;
-          rk_combine(k, dte * sol, un, acc, Uacc + idx, out + idx);
+          for (int r = f0 * Ns; r < (f0 + FB) * Ns && r < NU; ++r) {
+            const int f = r / Ns, i = r % Ns;
+            double sol = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < Ns; ++ii) sol += sMinv[ii * Ns + i] * R[f * Ns + ii];
+            const long idx = (long)e * NU + r;
+            rk_combine(k, dte * sol, pun[r - f0 * Ns], pacc[r - f0 * Ns], Uacc + idx, out + idx);
+          }
+        }
+        if (PFU) {
+#pragma unroll
+          for (int r = 0; r < NU; ++r) u[r] = u_next[r];
         }
       }
       __syncthreads();
@@ -301,9 +393,9 @@ template <int PH, int FL, int LB>
 __global__ __launch_bounds__(LB) void k_batch(BatchArgs a) {
   const int pid = a.wg_prob[blockIdx.x];
   switch (a.order[pid]) {
-    case 1: batch_problem<1, PH, FL>(a, pid); break;
-    case 2: batch_problem<2, PH, FL>(a, pid); break;
-    default: batch_problem<3, PH, FL>(a, pid); break;
+    case 1: batch_problem<1, PH, FL, LB <= 512>(a, pid); break;
+    case 2: batch_problem<2, PH, FL, LB <= 512>(a, pid); break;
+    default: batch_problem<3, PH, FL, LB <= 512>(a, pid); break;
   }
 }
 

@@ -394,7 +394,8 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
        alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
        alloc(s, &s.sensors, (size_t)NE) && alloc(s, &s.sens_tgt, (size_t)2 * NE) &&
        alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.flag_list, (size_t)NE) &&
-       alloc(s, &s.dscratch, 8) && alloc(s, &s.cons_partial, conservation_scratch_doubles()) &&
+       alloc(s, &s.dscratch, 8) && alloc(s, &s.time_state, 4) &&
+       alloc(s, &s.cons_partial, conservation_scratch_doubles()) &&
        alloc(s, &s.dt_partial, ((size_t)NE * Ns + 255) / 256 + 1) &&
        (!s.cart.on || alloc(s, &s.Ustar2, S));
   if (ok) ok = hip_ok(hipMemset(s.sensors, 0, sizeof(int) * NE), "hipMemset");
@@ -604,6 +605,57 @@ dgx_status dgx_rk4_step(dgx_solver* h, double* U, const double* dt_dev, int limi
   return rk4_step(h->s, U, dt_dev, limiter, as_stream(stream));
 }
 
+// The CFL loop of a run without outputs, its time control on the device:
+// steps are enqueued in chunks without waiting.  k_dt_step computes each
+// step's Dt, stops the run on a tiny or NaN Dt and clamps the last step to the
+// final time; once the run has stopped, every kernel of the steps still
+// queued returns at once (Solver::gate).  The host reads the state back once
+// per chunk; the next chunk covers the time left at the last Dt.
+static dgx_status integrate_cfl_device(Solver& s, double* U, double cfl, int nsteps, double tf, int limiter,
+                                       int* n_out_steps, double* T_out, hipStream_t st) {
+  double* dt_dev = s.dscratch + 1;
+  double* T_dev = s.dscratch + 2;
+  struct Ungate {
+    Solver& s;
+    ~Ungate() { s.gate = nullptr; }
+  } ungate{s};
+  int hs[4] = {0, 0, 0, 0};
+  double hd[2] = {0, 0};  // Dt, T
+  auto finish = [&]() {
+    *n_out_steps = hs[3];
+    *T_out = hd[1];
+  };
+  if (!hip_ok(hipMemsetAsync(s.time_state, 0, sizeof(int) * 4, st), "time state") ||
+      !hip_ok(hipMemsetAsync(T_dev, 0, sizeof(double), st), "time"))
+    return DGX_EDEVICE;
+  dgx_status r;
+  int queued = 0, chunk = 4;
+  for (;;) {
+    const int k = nsteps > 0 ? std::min(chunk, nsteps - queued) : chunk;
+    if (k <= 0) break;
+    s.gate = s.time_state;
+    for (int i = 0; i < k; ++i) {
+      if (!hip_ok(launch_dt_step(s, U, cfl, tf, dt_dev, T_dev, s.time_state, st), "dt step")) return DGX_EDEVICE;
+      if ((r = rk4_step(s, U, dt_dev, limiter, st)) != DGX_OK) return r;
+    }
+    s.gate = nullptr;
+    queued += k;
+    if (!hip_ok(hipMemcpyAsync(hs, s.time_state, sizeof(hs), hipMemcpyDeviceToHost, st), "time state readback") ||
+        !hip_ok(hipMemcpyAsync(hd, dt_dev, sizeof(hd), hipMemcpyDeviceToHost, st), "time readback") ||
+        !hip_ok(hipStreamSynchronize(st), "time readback"))
+      return DGX_EDEVICE;
+    if (hs[2]) {
+      finish();
+      return fail(DGX_EDT, "time step too small or NaN (rk.cc:147-148)");
+    }
+    if (hs[0] || hs[1]) break;
+    const double left = (tf - hd[1]) / hd[0];
+    chunk = left < 1 ? 1 : (int)std::min(256.0, std::ceil(left) + 1);
+  }
+  finish();
+  return DGX_OK;
+}
+
 // RK::RK_integration's CFL branch (rk.cc:100-233): the initial limit, then
 // per step Dt from CFL (one 8-byte read-back, as DtFromCFL's, rk.cc:265-268),
 // the stop on a tiny or NaN Dt (rk.cc:147-148), the clamps to the final time
@@ -623,6 +675,8 @@ static dgx_status integrate_cfl(Solver& s, double* U, double cfl, int nsteps, co
     *T_out = T;
   };
   if ((r = do_limit(s, limiter, U, st)) != DGX_OK) return r;
+  if (s.device_time && !on_output && n_out == 1 && (limiter == 0 || limiter == 1))
+    return integrate_cfl_device(s, U, cfl, nsteps, tf, limiter, n_out_steps, T_out, st);
   bool done = false;
   while (!done && (nsteps <= 0 || n < nsteps)) {
     {
@@ -770,6 +824,12 @@ dgx_status dgx_solver_set_indicator(dgx_solver* h, dgx_mlp* mlp, float threshold
 dgx_status dgx_solver_set_shock_guard(dgx_solver* h, int on) {
   if (!h) return fail(DGX_EINVAL, "dgx_solver_set_shock_guard: null solver");
   h->s.shock_guard = on != 0;
+  return DGX_OK;
+}
+
+dgx_status dgx_solver_set_device_time(dgx_solver* h, int on) {
+  if (!h) return fail(DGX_EINVAL, "dgx_solver_set_device_time: null solver");
+  h->s.device_time = on != 0;
   return DGX_OK;
 }
 

@@ -91,7 +91,10 @@ struct Solver {
   int* flag = nullptr;      // device flag / counters: [0] sensor relaxation, [2] flagged-element count
   int* flag_list = nullptr; // [N_E] flagged elements (launch_limit_flagged)
   double* m2buf = nullptr;  // m2 limiting: 15 per-field arrays of N_E*N_s (allocated on first use)
-  double* dscratch = nullptr;  // device scalars: [1] Dt
+  double* dscratch = nullptr;  // device scalars: [1] Dt, [2] the time of a device-timed CFL run
+  int* time_state = nullptr;   // device-timed CFL run: stop, last step taken, Dt error, steps (k_dt_step)
+  const int* gate = nullptr;   // while steps of such a run are enqueued: time_state, read by their kernels
+  bool device_time = true;     // CFL runs without outputs keep their time control on the device
   double* cons_partial = nullptr;  // conservation totals: per-block double-double partial sums
   double* dt_partial = nullptr;  // per-block maxima of |u|+a (ceil(N_E*N_s/256)), then node 0's |u|+a
   int n_thresh = 0;
@@ -135,6 +138,10 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
                         double* Uout, const double* dt_dev, double* f_out, hipStream_t st, double* means = nullptr,
                         double* uavg = nullptr, double* psens = nullptr, bool* means_done = nullptr);
 hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* dt_dev, hipStream_t st);
+// one step's Dt on the device with the stop, final-time clamp and step count
+// of a CFL run (k_dt_step): state = Solver::time_state, T_dev its time
+hipError_t launch_dt_step(const Solver& s, const double* U, double cfl, double tf, double* dt_dev, double* T_dev,
+                          int* state, hipStream_t st);
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 // Out = HR(In), both 2D sweeps in one launch; In != Out, needs s.hr2d
 hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st);

@@ -36,9 +36,12 @@
 namespace dgx {
 
 // Out[:,col] = M * In[:,col] for every column (e,f), netlib dgemm order.
+// gate: when set and nonzero, the launch does nothing (a step skipped by the
+// device-side time control).
 template <int Ns>
 __global__ __launch_bounds__(256) void k_apply(const double* __restrict__ M, const double* __restrict__ In,
-                                               double* __restrict__ Out, long ncol) {
+                                               double* __restrict__ Out, long ncol, const int* gate) {
+  if (gate && *gate) return;
   const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
   double x[Ns], y[Ns];
@@ -181,7 +184,9 @@ __device__ inline int hr_physical(int left, int right) {
 template <int Ns, int N, int SL>
 __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, const int* __restrict__ nb, int offxy,
                                                   const double* __restrict__ Mfirst, const double* __restrict__ Mnext,
-                                                  const double* __restrict__ In, double* __restrict__ Out) {
+                                                  const double* __restrict__ In, double* __restrict__ Out,
+                                                  const int* gate) {
+  if (gate && *gate) return;
   const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (long)N_E * NF) return;
   const int e = (int)(col / NF), fc = (int)(col % NF);
@@ -214,7 +219,9 @@ __global__ __launch_bounds__(256) void k_hr_sweep(int N_E, int NF, int N_N, cons
 // states (the skipped zero terms do not spread an Inf or NaN through a column).
 template <int P, int YS>
 __global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, const int* __restrict__ nb,
-                                                    const double* __restrict__ In, double* __restrict__ Out) {
+                                                    const double* __restrict__ In, double* __restrict__ Out,
+                                                    const int* gate) {
+  if (gate && *gate) return;
   constexpr int Ns = HrTables2D<P>::Ns, SL = P + 1, offxy = YS ? 2 : 0;
   const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (long)N_E * NF) return;
@@ -262,10 +269,13 @@ __global__ __launch_bounds__(256) void k_hr_sweep_c(int N_E, int NF, int N_N, co
 // launches (the products with zero table entries that it leaves out change
 // no finite sum; an Inf or NaN is not spread through its column), and the
 // intermediate state never reaches HBM.  In and Out must differ:
-// neighbouring tiles read In's rows.
+// neighbouring tiles read In's rows.  At least six waves per SIMD (three
+// 512-thread blocks of the 2D p=2 five-field limiter per CU).
 template <int Ns, int N, int SL, int NF, int TXE, int TYE, int BS, bool OUT_VEC2>
-__global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restrict__ nb,
-                                             const double* __restrict__ In, double* __restrict__ Out) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void k_hr2d(
+    int nx, int N_N, const int* __restrict__ nb, const double* __restrict__ In, double* __restrict__ Out,
+    const int* gate) {
+  if (gate && *gate) return;
   constexpr int ROWS = TYE + 2, W = TXE + 2, EL = NF * Ns, ROW_OUT = TXE * EL;
   // one x-sweep column per thread, so that its result can wait in registers
   // while the Taylor-x coefficients are still read, then take their place
@@ -276,23 +286,59 @@ __global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restr
   __shared__ int sE[ROWS * W];  // element of each slot, -1 if absent
   const int tiles_x = nx / TXE;
   const int ex0 = (blockIdx.x % tiles_x) * TXE, ey0 = (blockIdx.x / tiles_x) * TYE;
-  // slot elements: row 0 below the tile, rows 1..TYE the tile, row TYE+1 above
-  for (int t = threadIdx.x; t < ROWS * TXE; t += BS) {
-    const int i = t % TXE, r = t / TXE;
+  // slot elements: row 0 below the tile, rows 1..TYE the tile, row TYE+1
+  // above; columns 0 and W-1 the left and right neighbours of the row's end
+  // elements.  One thread per slot, so that no thread waits for more than the
+  // two dependent table reads of a halo row's end.
+  static_assert(ROWS * W <= BS, "one thread per slot");
+  if (threadIdx.x < ROWS * W) {
+    const int c = threadIdx.x % W, r = threadIdx.x / W;
+    const int i = c == 0 ? 0 : (c == W - 1 ? TXE - 1 : c - 1);  // the row element it starts from
     int e;
     if (r == 0) e = nb[(ey0 * nx + ex0 + i) * N_N + 2];
     else if (r == ROWS - 1) e = nb[((ey0 + TYE - 1) * nx + ex0 + i) * N_N + 3];
     else e = (ey0 + r - 1) * nx + ex0 + i;
-    sE[r * W + i + 1] = e;
-    if (i == 0) sE[r * W] = e >= 0 ? nb[e * N_N + 0] : -1;
-    if (i == TXE - 1) sE[r * W + W - 1] = e >= 0 ? nb[e * N_N + 1] : -1;
+    if (c == 0 || c == W - 1) e = e >= 0 ? nb[e * N_N + (c == 0 ? 0 : 1)] : -1;
+    sE[threadIdx.x] = e;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < ROWS * W * EL; q += BS) {
-    const int slot = q / EL;
-    const int e = sE[slot];
-    if (e >= 0) sA[q] = In[(long)e * EL + (q - slot * EL)];
+  // every load of the slots in flight at once (a loop that stores each value
+  // before it loads the next waits one memory round trip per value), and the
+  // neighbour entries the two sweeps test with them
+  constexpr int TOT = ROWS * W * EL, PER = (TOT + BS - 1) / BS;
+  const int t2 = threadIdx.x;
+  const int f2 = t2 % NF, i2 = (t2 / NF) % TXE, r2 = t2 / (NF * TXE);
+  const bool on2 = t2 < ROWS * TXE * NF && sE[r2 * W + i2 + 1] >= 0;
+  const bool on3 = t2 < TYE * TXE * NF;
+  double v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int q = t2 + k * BS;
+    v[k] = 0.0;
+    if (q < TOT) {
+      const int slot = q / EL;
+      const int e = sE[slot];
+      if (e >= 0) v[k] = In[(long)e * EL + (q - slot * EL)];
+    }
   }
+  int nbx0 = 0, nbx1 = 0, nby0 = 0, nby1 = 0;
+  if (on2) {
+    const int e = sE[r2 * W + i2 + 1];
+    nbx0 = nb[e * N_N + 0];
+    nbx1 = nb[e * N_N + 1];
+  }
+  if (on3) {
+    const int e = (ey0 + r2) * nx + ex0 + i2;
+    nby0 = nb[e * N_N + 2];
+    nby1 = nb[e * N_N + 3];
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int q = t2 + k * BS;
+    if (q < TOT && sE[q / EL] >= 0) sA[q] = v[k];
+  }
+  // both sweeps' boundary kinds in one register
+  const int phys = hr_physical(nbx0, nbx1) | (hr_physical(nby0, nby1) << 16);
   __syncthreads();
   for (int c = threadIdx.x; c < ROWS * W * NF; c += BS) {
     if (sE[c / NF] < 0) continue;
@@ -307,14 +353,10 @@ __global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restr
   __syncthreads();
   // x sweep of every row present (a wall below or above leaves its halo row out:
   // the y sweep does not read it)
-  const int t2 = threadIdx.x;
-  const int f2 = t2 % NF, i2 = (t2 / NF) % TXE, r2 = t2 / (NF * TXE);
-  const bool on2 = t2 < ROWS * TXE * NF && sE[r2 * W + i2 + 1] >= 0;
   double xs[Ns];
   if (on2) {
     const int slot = r2 * W + i2 + 1;
-    const int e = sE[slot];
-    const int physical = hr_physical(nb[e * N_N + 0], nb[e * N_N + 1]);
+    const int physical = phys & 0xffff;
     double AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
     const double* ac = sA + (slot * NF + f2) * Ns;
 #pragma unroll
@@ -336,11 +378,9 @@ __global__ __launch_bounds__(BS) void k_hr2d(int nx, int N_N, const int* __restr
   }
   __syncthreads();
   // y sweep of the tile rows (TYE*TXE*NF <= BS columns: one pass)
-  const bool on3 = t2 < TYE * TXE * NF;
   if (on3) {
     const int f = f2, i = i2, j = r2;
-    const int e = (ey0 + j) * nx + ex0 + i;
-    const int physical = hr_physical(nb[e * N_N + 2], nb[e * N_N + 3]);
+    const int physical = phys >> 16;
     double AC[Ns], AL[Ns], AR[Ns], Alim[Ns];
     const double* bc = sA + (((j + 1) * TXE + i) * NF + f) * Ns;
 #pragma unroll
@@ -1018,19 +1058,19 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
     constexpr int Ns = Shape<D, P>::Ns, SL = (D == 1) ? 1 : P + 1;
     const dim3 g(nblk(ncol, 256)), b(256);
     if (D == 1) {
-      hipLaunchKernelGGL((k_apply<Ns>), g, b, 0, st, s.t.Lag2Mono, U, s.work, ncol);
+      hipLaunchKernelGGL((k_apply<Ns>), g, b, 0, st, s.t.Lag2Mono, U, s.work, ncol, s.gate);
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, nullptr,
-                         s.t.Mono2Lag, s.work, U);
+                         s.t.Mono2Lag, s.work, U, s.gate);
     } else if (s.hr_const) {
       if constexpr (D == 2 && P <= 2) {
-        hipLaunchKernelGGL((k_hr_sweep_c<P, 0>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, U, s.work);
-        hipLaunchKernelGGL((k_hr_sweep_c<P, 1>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, s.work, U);
+        hipLaunchKernelGGL((k_hr_sweep_c<P, 0>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, U, s.work, s.gate);
+        hipLaunchKernelGGL((k_hr_sweep_c<P, 1>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, s.work, U, s.gate);
       }
     } else {
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 0, s.t.Lag2MonoX,
-                         s.t.MonoX2MonoY, U, s.work);
+                         s.t.MonoX2MonoY, U, s.work, s.gate);
       hipLaunchKernelGGL((k_hr_sweep<Ns, P, SL>), g, b, 0, st, s.N_E, s.NF, s.N_N, s.t.neighbors, 2, nullptr,
-                         s.t.MonoY2Lag, s.work, U);
+                         s.t.MonoY2Lag, s.work, U, s.gate);
     }
     return hipGetLastError();
   });
@@ -1057,10 +1097,10 @@ hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStre
         constexpr int NF = decltype(nf)::value, TYE = decltype(ty)::value, BS = TYE == 4 ? 256 : 512;
         if ((reinterpret_cast<uintptr_t>(Out) & 15) == 0)
           hipLaunchKernelGGL((k_hr2d<Ns, P, SL, NF, TXE, TYE, BS, true>), dim3(tiles), dim3(BS), 0, st, s.cart.nx,
-                             s.N_N, s.t.neighbors, In, Out);
+                             s.N_N, s.t.neighbors, In, Out, s.gate);
         else
           hipLaunchKernelGGL((k_hr2d<Ns, P, SL, NF, TXE, TYE, BS, false>), dim3(tiles), dim3(BS), 0, st, s.cart.nx,
-                             s.N_N, s.t.neighbors, In, Out);
+                             s.N_N, s.t.neighbors, In, Out, s.gate);
         return hipGetLastError();
       };
       using I4 = std::integral_constant<int, 4>;
@@ -1095,7 +1135,7 @@ hipError_t launch_limit_m2(Solver& s, double* U, hipStream_t st) {
     auto get = [&](int f, double* x) { hipLaunchKernelGGL(k_field_get, g, bl, 0, st, n, Ns, NF, f, U, x); };
     auto put = [&](int f, const double* x) { hipLaunchKernelGGL(k_field_put, g, bl, 0, st, n, Ns, NF, f, x, U); };
     auto gemm = [&](const double* M, const double* X, double* Y) {
-      hipLaunchKernelGGL((k_apply<Nsc>), dim3(nblk(NE, 256)), bl, 0, st, M, X, Y, (long)NE);
+      hipLaunchKernelGGL((k_apply<Nsc>), dim3(nblk(NE, 256)), bl, 0, st, M, X, Y, (long)NE, nullptr);
     };
     const int sl = D == 1 ? 1 : n1, hs = D == 1 ? Ns : n1;  // slices and slice length
     auto hrl = [&](int offxy, const double* A, double* Alim) {

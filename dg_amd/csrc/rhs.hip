@@ -38,12 +38,14 @@ struct FaceArgs {
   const double* __restrict__ psi_w;
   double* __restrict__ Qe;
   int M_T;
+  const int* gate;  // set and nonzero: the launch does nothing (a step skipped by the device-side time control)
 };
 
 template <int D, int P, int PH, int FL>
 __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
   using S = Shape<D, P>;
   constexpr int NF = Fields<D, PH>::NF, Ms = S::Ms, MG = S::MG, Ns = S::Ns;
+  if (a.gate && *a.gate) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.M_T) return;
   const int e0 = a.face_elem[2 * t], e1 = a.face_elem[2 * t + 1];
@@ -144,6 +146,7 @@ struct ElemArgs {
   double* Uout;
   const double* __restrict__ dt;
   double* fout;
+  const int* gate;  // as FaceArgs::gate
 };
 
 // One block of 256 threads handles EPB elements.  Phase 1: lane = (element,
@@ -152,6 +155,7 @@ struct ElemArgs {
 // per-point fluxes and the reference tables are staged in LDS.
 template <int D, int P, int PH, int FL, int MODE>
 __global__ __launch_bounds__(256) void k_elem(ElemArgs a) {
+  if (a.gate && *a.gate) return;
   using V = Vol<D, P, PH>;
   constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, NU = V::NU, NQE = V::NQE, NQ = V::NQ;
   constexpr int TPE = V::TPE, EPB = V::EPB;
@@ -238,6 +242,7 @@ constexpr int elem_stride(int n) { return elem_bank_ok(n) ? n : elem_stride(n + 
 
 template <int PH, int FL, int MODE>
 __global__ __launch_bounds__(256) void k_elem_mfma(ElemArgs a) {
+  if (a.gate && *a.gate) return;
   constexpr int D = 2, P = 2, NT = 32, NGEO = 1 + D * D;
   using V = Vol<D, P, PH>;
   constexpr int NF = V::NF, Ns = V::Ns, NU = V::NU, NQE = V::NQE;
@@ -392,6 +397,43 @@ __global__ __launch_bounds__(256) void k_dt_finalize(const double* __restrict__ 
   if (threadIdx.x == 0) dt[0] = cfl / (first != first ? first : m);
 }
 
+// One step's time control on the device, for CFL-stepped runs without
+// outputs (rk.cc:140-150): Dt from the partial maxima exactly as
+// k_dt_finalize, then the stop on a tiny or NaN Dt, the clamp to the final
+// time and the step count.  state[0]: stop (every gated kernel of later
+// steps returns at once), [1]: the clamped last step was taken, [2]: Dt too
+// small or NaN, [3]: steps taken; T[0]: the time reached.
+__global__ __launch_bounds__(256) void k_dt_step(const double* __restrict__ partial, int n, double cfl, double tf,
+                                                 double* dt, double* T, int* state) {
+  if (state[0]) return;
+  __syncthreads();  // every thread has read state[0] before thread 0 may set it
+  if (state[1]) {
+    if (threadIdx.x == 0) state[0] = 1;
+    return;
+  }
+  __shared__ double red[4];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v = skip_nan_max(v, partial[k]);
+  const double m = block_max(v, red);
+  const double first = partial[n];
+  if (threadIdx.x == 0) {
+    double d = cfl / (first != first ? first : m);
+    if (d < 1e-14 || d != d) {
+      state[2] = 1;
+      state[0] = 1;
+      return;
+    }
+    const double t = T[0];
+    if (d > (tf - t)) {
+      d = tf - t;
+      state[1] = 1;
+    }
+    dt[0] = d;
+    T[0] = t + d;
+    state[3] = state[3] + 1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host-side dispatch
 // ---------------------------------------------------------------------------
@@ -425,7 +467,8 @@ static hipError_t dispatch(const dgx_config& c, Fn&& fn) {
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 hipError_t launch_face(const Solver& s, const double* U, hipStream_t st) {
-  FaceArgs a{U, s.t.face_elem, s.t.face_closure, s.t.face_tag, s.t.normals, s.t.JF, s.t.psi, s.t.psi_w, s.Qe, s.M_T};
+  FaceArgs a{U,        s.t.face_elem, s.t.face_closure, s.t.face_tag, s.t.normals, s.t.JF, s.t.psi, s.t.psi_w,
+             s.Qe,     s.M_T,         s.gate};
   return dispatch(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     hipLaunchKernelGGL((k_face<D, P, PH, FL>), dim3(blocks_for(s.M_T, 256)), dim3(256), 0, st, a);
@@ -436,7 +479,7 @@ hipError_t launch_face(const Solver& s, const double* U, hipStream_t st) {
 hipError_t launch_elem(const Solver& s, int mode, int stage, const double* Uin, const double* Un, double* Uacc,
                        double* Uout, const double* dt_dev, double* f_out, hipStream_t st) {
   ElemArgs a{s.t.phi, s.t.dphi, s.t.phi_w, s.t.dphi_w, s.t.J, s.t.invJac, s.t.Minv, s.t.geo_stride,
-             s.t.minv_stride, s.t.J0, s.Qe, s.N_E, stage, Uin, Un, Uacc, Uout, dt_dev, f_out};
+             s.t.minv_stride, s.t.J0, s.Qe, s.N_E, stage, Uin, Un, Uacc, Uout, dt_dev, f_out, s.gate};
   return dispatch(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
     if constexpr (D == 2 && P == 2 && PH <= 1) {
@@ -468,6 +511,21 @@ hipError_t launch_upa_max(const Solver& s, const double* U, double cfl, double* 
   });
   if (err != hipSuccess) return err;
   hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(256), 0, st, s.dt_partial, (int)nb, cfl, dt_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_step(const Solver& s, const double* U, double cfl, double tf, double* dt_dev, double* T_dev,
+                          int* state, hipStream_t st) {
+  const long n = (long)s.N_E * s.N_s;
+  const unsigned nb = blocks_for(n, 256);
+  hipError_t err = dispatch(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
+    constexpr int D = decltype(d)::value, PH = decltype(ph)::value;
+    (void)p; (void)fl;
+    hipLaunchKernelGGL((k_upa_max<D, PH>), dim3(nb), dim3(256), 0, st, U, s.N_E, s.N_s, s.dt_partial);
+    return hipGetLastError();
+  });
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(k_dt_step, dim3(1), dim3(256), 0, st, s.dt_partial, (int)nb, cfl, tf, dt_dev, T_dev, state);
   return hipGetLastError();
 }
 

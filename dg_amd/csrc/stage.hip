@@ -53,6 +53,7 @@ struct StageArgs {
   const double* __restrict__ w0;
   double* uavg;   // with means: also the sensor's cell averages [e*5 + f] (sensor_nodal_mean), or null
   double* psens;  // with uavg: their sensor pressures [e]
+  const int* gate;  // set and nonzero: the launch does nothing (a step skipped by the device-side time control)
 };
 
 template <int D, int TX, int TY>
@@ -228,6 +229,7 @@ __device__ inline int tile_of(const StageArgs& a, int i) {
 // 16-byte aligned, which the host checks before it picks this instance).
 template <int D, int P, int PH, int FL, int MODE, int TX, int TY, int PREC, bool VEC = false>
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
+  if (a.gate && *a.gate) return;
   using V = Vol<D, P, PH>;
   using T = Tile<D, TX, TY>;
   constexpr int NF = V::NF, Ns = V::Ns, NG = V::NG, Ms = V::Ms, MG = Shape<D, P>::MG;
@@ -651,6 +653,7 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
   a.tiles_y = c.ny / c.ty;
   const int ntiles = a.tiles_x * a.tiles_y;
   a.stage = stage; a.Uin = Uin; a.Un = Un; a.Uacc = Uacc; a.Uout = Uout; a.dt = dt_dev; a.fout = f_out;
+  a.gate = s.gate;
   a.w0 = s.t.mlp_vinv;  // first row: the mode-0 weights
   return dispatch_stage(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
     constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;

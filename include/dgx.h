@@ -215,7 +215,11 @@ dgx_status dgx_rk4_step(dgx_solver* s, double* U, const double* dt_dev, int limi
  * mode 1: Dt from CFL = dt_or_cfl (already scaled by Dx/(2p+1)) until tf,
  *         or for at most nsteps steps when nsteps > 0; the initial condition
  *         is limited first (rk.cc:100-114).
- * Returns the step count in *steps and the final time in *t_final. */
+ * Returns the step count in *steps and the final time in *t_final.
+ * In mode 1 with limiter 0 or 1 the time control (Dt, its stop and the clamp
+ * to tf) runs on the device and the steps are enqueued in chunks, with one
+ * read-back per chunk instead of one per step (dgx_solver_set_device_time);
+ * the same steps, times and bits as the per-step loop. */
 dgx_status dgx_rk4_integrate(dgx_solver* s, double* U, int mode, double dt_or_cfl, int nsteps, double tf,
                              int limiter, int* steps, double* t_final, void* stream);
 /* RK::RK_integration's CFL branch with its output times (rk.cc:100-233, rk.h:58-83): out_times[0..n_out-1]
@@ -322,6 +326,11 @@ dgx_status dgx_solver_set_indicator(dgx_solver* s, dgx_mlp* m, float threshold);
  * the indicator skips its own pass over the state (on by default; 0 turns
  * it off, the same flags and bits either way). */
 dgx_status dgx_solver_set_fused_means(dgx_solver* s, int on);
+/* 1 (default): dgx_rk4_integrate's CFL runs with limiter 0 or 1 keep their
+ * time control on the device; 0: the host loop reads every step's Dt back
+ * (as DtFromCFL, rk.cc:250-272).  No counterpart in the reference: a switch
+ * between two schedules of the same computation. */
+dgx_status dgx_solver_set_device_time(dgx_solver* s, int on);
 /* Limiter 8 and dgx_indicator_flags also flag every cell of a neighbour pair
  * for which the sensor's shock test (calc_sensors test 2, the same arithmetic
  * and cell averages, sensor_kernels.cu:125-136) fires, so that no cell the
