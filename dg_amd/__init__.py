@@ -236,6 +236,19 @@ def _ptr(x) -> int:
     return x.data_ptr()
 
 
+def _state_ptr(x, n: int, what: str) -> int:
+    """Device pointer of a state array of n float64 values; a tensor of
+    another size, dtype or layout is refused before anything is launched."""
+    if x is None or isinstance(x, int):
+        return _ptr(x)
+    import torch
+
+    if x.dtype != torch.float64 or x.numel() != n or not x.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous float64 state of {n} values, got {x.dtype} "
+                         f"{tuple(x.shape)}")
+    return x.data_ptr()
+
+
 def _stream(stream):
     if stream is None:
         return None
@@ -253,23 +266,26 @@ class Solver:
         self._h = h
         self.state_size = lib().dgx_solver_state_size(h)
 
+    def _u(self, U) -> int:
+        return _state_ptr(U, self.state_size, "Solver")
+
     def set_sensor_thresholds(self, thresholds) -> None:
         th = (ctypes.c_double * 3)(*(list(thresholds) + [0.0] * (3 - len(thresholds))))
         _check(lib().dgx_set_sensor_thresholds(self._h, len(thresholds), th), "dgx_set_sensor_thresholds")
 
     def rhs(self, U, f_out, stream=None) -> None:
-        _check(lib().dgx_rhs(self._h, _ptr(U), _ptr(f_out), _stream(stream)), "dgx_rhs")
+        _check(lib().dgx_rhs(self._h, self._u(U), self._u(f_out), _stream(stream)), "dgx_rhs")
 
     def limit(self, limiter: int, U, stream=None) -> None:
-        _check(lib().dgx_limit(self._h, limiter, _ptr(U), _stream(stream)), "dgx_limit")
+        _check(lib().dgx_limit(self._h, limiter, self._u(U), _stream(stream)), "dgx_limit")
 
     def limit_into(self, limiter: int, In, Out, stream=None) -> None:
         """Out = limited In (In unchanged); 2D hrl on tile meshes runs both sweeps in one launch."""
-        _check(lib().dgx_limit_into(self._h, limiter, _ptr(In), _ptr(Out), _stream(stream)), "dgx_limit_into")
+        _check(lib().dgx_limit_into(self._h, limiter, self._u(In), self._u(Out), _stream(stream)), "dgx_limit_into")
 
     def format(self, U, out, inverse: bool = False, stream=None) -> None:
         """PRINTER formatter on the device: out = printed fields of U, or U from out when inverse."""
-        _check(lib().dgx_format(self._h, _ptr(U), _ptr(out), 1 if inverse else 0, _stream(stream)), "dgx_format")
+        _check(lib().dgx_format(self._h, self._u(U), _ptr(out), 1 if inverse else 0, _stream(stream)), "dgx_format")
 
     def checkpoint(self, U, prefix: str, step: int, time: float) -> None:
         """Write U as the reference's .pos files (printed fields, one file per field)."""
@@ -291,19 +307,19 @@ class Solver:
         return time
 
     def sensing(self, U, sensors, Uavg=None, stream=None) -> None:
-        _check(lib().dgx_sensing(self._h, _ptr(U), _ptr(sensors), _ptr(Uavg), _stream(stream)), "dgx_sensing")
+        _check(lib().dgx_sensing(self._h, self._u(U), _ptr(sensors), _ptr(Uavg), _stream(stream)), "dgx_sensing")
 
     def dt_from_cfl(self, U, cfl: float, dt_dev, stream=None) -> None:
-        _check(lib().dgx_dt_from_cfl(self._h, _ptr(U), cfl, _ptr(dt_dev), _stream(stream)), "dgx_dt_from_cfl")
+        _check(lib().dgx_dt_from_cfl(self._h, self._u(U), cfl, _ptr(dt_dev), _stream(stream)), "dgx_dt_from_cfl")
 
     def rk4_step(self, U, dt_dev, limiter: int = 0, stream=None) -> None:
-        _check(lib().dgx_rk4_step(self._h, _ptr(U), _ptr(dt_dev), limiter, _stream(stream)), "dgx_rk4_step")
+        _check(lib().dgx_rk4_step(self._h, self._u(U), _ptr(dt_dev), limiter, _stream(stream)), "dgx_rk4_step")
 
     def rk4_integrate(self, U, mode: int, dt_or_cfl: float, nsteps: int = 0, tf: float = 0.0, limiter: int = 0,
                       stream=None):
         steps = ctypes.c_int()
         tfin = ctypes.c_double()
-        _check(lib().dgx_rk4_integrate(self._h, _ptr(U), mode, dt_or_cfl, nsteps, tf, limiter, ctypes.byref(steps),
+        _check(lib().dgx_rk4_integrate(self._h, self._u(U), mode, dt_or_cfl, nsteps, tf, limiter, ctypes.byref(steps),
                                        ctypes.byref(tfin), _stream(stream)), "dgx_rk4_integrate")
         return steps.value, tfin.value
 
@@ -324,14 +340,14 @@ class Solver:
         fn = OUTPUT_FN(cb)
         steps = ctypes.c_int()
         tfin = ctypes.c_double()
-        _check(lib().dgx_rk4_integrate_out(self._h, _ptr(U), cfl, outs, len(out_times), limiter, fn, None,
+        _check(lib().dgx_rk4_integrate_out(self._h, self._u(U), cfl, outs, len(out_times), limiter, fn, None,
                                            ctypes.byref(steps), ctypes.byref(tfin), _stream(stream)),
                "dgx_rk4_integrate_out")
         return steps.value, tfin.value
 
     def conservation(self, U, totals, stream=None) -> None:
         """DG_SOLVER::conservation: the integral of each field, into totals (device float64[N_F])."""
-        _check(lib().dgx_conservation(self._h, _ptr(U), _ptr(totals), _stream(stream)), "dgx_conservation")
+        _check(lib().dgx_conservation(self._h, self._u(U), _ptr(totals), _stream(stream)), "dgx_conservation")
 
     def set_path(self, path: int) -> None:
         """PATH_AUTO, PATH_GENERAL (face + element kernels) or PATH_TILED (one fused kernel per stage)."""
@@ -377,10 +393,10 @@ class Solver:
 
     def features(self, U, X, stream=None) -> None:
         """Indicator features of U into X (float32 device array N_E x 64)."""
-        _check(lib().dgx_mlp_features(self._h, _ptr(U), _ptr(X), _stream(stream)), "dgx_mlp_features")
+        _check(lib().dgx_mlp_features(self._h, self._u(U), _ptr(X), _stream(stream)), "dgx_mlp_features")
 
     def indicator_flags(self, U, flags, z=None, stream=None) -> None:
-        _check(lib().dgx_indicator_flags(self._h, _ptr(U), _ptr(flags), _ptr(z), _stream(stream)),
+        _check(lib().dgx_indicator_flags(self._h, self._u(U), _ptr(flags), _ptr(z), _stream(stream)),
                "dgx_indicator_flags")
 
     def profile(self, enable: bool = True) -> None:

@@ -397,7 +397,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
        alloc(s, &s.dscratch, 8) && alloc(s, &s.time_state, 4) &&
        alloc(s, &s.cons_partial, conservation_scratch_doubles()) &&
        alloc(s, &s.dt_partial, ((size_t)NE * Ns + 255) / 256 + 1) &&
-       (!s.cart.on || alloc(s, &s.Ustar2, S));
+       ((!s.cart.on && D != 1) || alloc(s, &s.Ustar2, S));
   if (ok) ok = hip_ok(hipMemset(s.sensors, 0, sizeof(int) * NE), "hipMemset");
   if (!ok) {
     free_solver(s);
@@ -477,7 +477,8 @@ dgx_status dgx_limit_into(dgx_solver* h, int limiter, const double* In, double* 
   if (In == Out) return fail(DGX_EINVAL, "dgx_limit_into: In and Out must differ");
   Solver& s = h->s;
   hipStream_t st = as_stream(stream);
-  if (limiter == 1 && s.hr2d) return hip_ok(launch_limit_hr_oop(s, In, Out, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
+  if (limiter == 1 && (s.hr2d || s.D == 1))
+    return hip_ok(launch_limit_hr_oop(s, In, Out, st), "HR limiter") ? DGX_OK : DGX_EDEVICE;
   if (!hip_ok(hipMemcpyAsync(Out, In, sizeof(double) * s.state_size(), hipMemcpyDeviceToDevice, st), "copy"))
     return DGX_EDEVICE;
   return do_limit(s, limiter, Out, st);
@@ -543,18 +544,30 @@ static dgx_status timed_hr_oop(Solver& s, const double* In, double* Out, hipStre
 
 static dgx_status rk4_step(Solver& s, double* U, const double* dt, int limiter, hipStream_t st) {
   dgx_status r;
-  if (DGX_RK_HR2D && s.tiled() && s.hr2d && limiter == 1) {
-    // One-launch HR limiter (out of place): every stage writes Ustar, the
-    // limiter takes Ustar to Ustar2 (the next stage's input) and, after the
-    // last stage, to U.  U keeps the step's initial state until then.
+  if (DGX_RK_HR2D && limiter == 1 && ((s.tiled() && s.hr2d) || s.D == 1)) {
+    // One-launch HR limiter (out of place; 2D tiles, or 1D): every stage
+    // writes Ustar, the limiter takes Ustar to Ustar2 (the next stage's input)
+    // and, after the last stage, to U.  U keeps the step's initial state
+    // until then.
     const double* Uin = U;
     for (int k = 0; k < 4; ++k) {
       if (k > 0) {
         if ((r = timed_hr_oop(s, s.Ustar, s.Ustar2, st)) != DGX_OK) return r;
         Uin = s.Ustar2;
       }
-      KernelTimer kt(s, 4, st);
-      if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, s.Ustar, dt, nullptr, st), "tile kernel")) return DGX_EDEVICE;
+      if (s.tiled()) {
+        KernelTimer kt(s, 4, st);
+        if (!hip_ok(launch_stage(s, 1, k, Uin, U, s.Uacc, s.Ustar, dt, nullptr, st), "tile kernel"))
+          return DGX_EDEVICE;
+        continue;
+      }
+      {
+        KernelTimer kt(s, 0, st);
+        if (!hip_ok(launch_face(s, Uin, st), "face kernel")) return DGX_EDEVICE;
+      }
+      KernelTimer kt(s, 1, st);
+      if (!hip_ok(launch_elem(s, 1, k, Uin, U, s.Uacc, s.Ustar, dt, nullptr, st), "element kernel"))
+        return DGX_EDEVICE;
     }
     return timed_hr_oop(s, s.Ustar, U, st);
   }

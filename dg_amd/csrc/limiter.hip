@@ -1079,7 +1079,23 @@ hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st) {
 // Out = HR(In) with both 2D sweeps in one launch (k_hr2d), p <= 2; needs the
 // tile layout and a neighbour table that follows it (Solver::hr2d).
 hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st) {
-  if (s.D != 2 || !s.hr2d || In == Out || (s.NF != 4 && s.NF != 5)) return hipErrorInvalidValue;
+  if (In == Out) return hipErrorInvalidValue;
+  if (s.D == 1) {
+    // 1D: one sweep that forms its own and its neighbours' Taylor
+    // coefficients (col_apply, the order of k_apply: the same bits as the
+    // two launches of launch_limit_hr)
+    const long ncol = (long)s.N_E * s.NF;
+    return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+      constexpr int D = decltype(d)::value, P = decltype(p)::value;
+      if constexpr (D == 1) {
+        constexpr int Ns = Shape<D, P>::Ns;
+        hipLaunchKernelGGL((k_hr_sweep<Ns, P, 1>), dim3(nblk(ncol, 256)), dim3(256), 0, st, s.N_E, s.NF, s.N_N,
+                           s.t.neighbors, 0, s.t.Lag2Mono, s.t.Mono2Lag, In, Out, s.gate);
+      }
+      return hipGetLastError();
+    });
+  }
+  if (s.D != 2 || !s.hr2d || (s.NF != 4 && s.NF != 5)) return hipErrorInvalidValue;
   // 8 x 8 tiles (512 threads) where the rows allow, else 8 x 4: the taller
   // tile recomputes fewer halo columns (1.56 against 1.88 Taylor-x columns
   // per output column) and ran 167 against 177 us on the 1024 x 512 p = 2

@@ -593,8 +593,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
 // ------------------------------------------------------------------- host
 template <int D>
 struct TileShape;
+// 1D: 128 elements per tile, or 8 where 128 does not divide the mesh
 template <>
-struct TileShape<1> { static constexpr int TX = 128, TY = 1; };
+struct TileShape<1> { static constexpr int TX = 128, TX_SMALL = 8, TY = 1; };
 #ifndef DGX_TILE2_TX
 #define DGX_TILE2_TX 8
 #endif
@@ -605,7 +606,7 @@ template <>
 struct TileShape<2> { static constexpr int TX = DGX_TILE2_TX, TY = DGX_TILE2_TY; };
 
 bool stage_tiles(int D, int nx, int ny, int* tx, int* ty) {
-  *tx = D == 1 ? TileShape<1>::TX : TileShape<2>::TX;
+  *tx = D == 1 ? (nx % TileShape<1>::TX == 0 ? TileShape<1>::TX : TileShape<1>::TX_SMALL) : TileShape<2>::TX;
   *ty = D == 1 ? TileShape<1>::TY : TileShape<2>::TY;
   return nx % *tx == 0 && (D == 1 || ny % *ty == 0);
 }
@@ -656,50 +657,57 @@ hipError_t launch_stage(const Solver& s, int mode, int stage, const double* Uin,
   a.gate = s.gate;
   a.w0 = s.t.mlp_vinv;  // first row: the mode-0 weights
   return dispatch_stage(s.cfg, [&](auto d, auto p, auto ph, auto fl) -> hipError_t {
-    constexpr int D = decltype(d)::value, P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
-    constexpr int TX = TileShape<D>::TX, TY = TileShape<D>::TY;
-    auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0> : k_stage<D, P, PH, FL, 1, TX, TY, 0>;
-    if constexpr (D == 2 && P == 2 && TX * TY == 32 && PH <= 1) {
-      // 16-byte pairs when every state pointer allows them
-      auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-      const bool vec = al(Uin) && (mode == 0 ? al(f_out) : al(Un) && al(Uacc) && al(Uout));
-      if (vec) {
-        kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0, true> : k_stage<D, P, PH, FL, 1, TX, TY, 0, true>;
-        if (s.precision == 1)
-          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1, true> : k_stage<D, P, PH, FL, 1, TX, TY, 1, true>;
-        if (s.precision == 2)
-          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2, true> : k_stage<D, P, PH, FL, 1, TX, TY, 2, true>;
-      } else {
-        if (s.precision == 1)
-          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
-        if (s.precision == 2)
-          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2> : k_stage<D, P, PH, FL, 1, TX, TY, 2>;
+    constexpr int D = decltype(d)::value;
+    auto go = [&](auto txc) -> hipError_t {
+      constexpr int P = decltype(p)::value, PH = decltype(ph)::value, FL = decltype(fl)::value;
+      constexpr int TX = decltype(txc)::value, TY = TileShape<D>::TY;
+      auto kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0> : k_stage<D, P, PH, FL, 1, TX, TY, 0>;
+      if constexpr (D == 2 && P == 2 && TX * TY == 32 && PH <= 1) {
+        // 16-byte pairs when every state pointer allows them
+        auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+        const bool vec = al(Uin) && (mode == 0 ? al(f_out) : al(Un) && al(Uacc) && al(Uout));
+        if (vec) {
+          kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 0, true> : k_stage<D, P, PH, FL, 1, TX, TY, 0, true>;
+          if (s.precision == 1)
+            kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1, true> : k_stage<D, P, PH, FL, 1, TX, TY, 1, true>;
+          if (s.precision == 2)
+            kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2, true> : k_stage<D, P, PH, FL, 1, TX, TY, 2, true>;
+        } else {
+          if (s.precision == 1)
+            kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 1> : k_stage<D, P, PH, FL, 1, TX, TY, 1>;
+          if (s.precision == 2)
+            kern = mode == 0 ? k_stage<D, P, PH, FL, 0, TX, TY, 2> : k_stage<D, P, PH, FL, 1, TX, TY, 2>;
+        }
       }
+      // persistent grid: as many blocks as fit on the device at once
+      int nb = ntiles;
+      if (DGX_STAGE_PERSIST) {
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) == hipSuccess && per_cu > 0)
+          nb = std::min(ntiles, per_cu * cus);
+      }
+      a.xcd_bands = (ntiles % 8 == 0 && nb % 8 == 0) ? 1 : 0;
+      // the epilogue's cell means: RK stages of 2D p=2 two-fluid tiles on the
+      // matrix-core element phases with the shared Minv (k_stage's MFW path)
+      constexpr bool mfw = D == 2 && P == 2 && PH == 1 && (TX * TY) % 16 == 0;
+      a.means = nullptr;
+      a.uavg = nullptr;
+      if (mfw && mode == 1 && means && s.t.mlp_vinv && s.t.minv_stride == 0 &&
+          (s.precision >= 1 || DGX_STAGE_MFMA64)) {
+        a.means = means;
+        a.uavg = uavg;
+        a.psens = psens;
+        if (means_done) *means_done = true;
+      }
+      hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, a);
+      return hipGetLastError();
+    };
+    if constexpr (D == 1) {
+      if (c.tx == TileShape<1>::TX_SMALL) return go(std::integral_constant<int, TileShape<1>::TX_SMALL>{});
     }
-    // persistent grid: as many blocks as fit on the device at once
-    int nb = ntiles;
-    if (DGX_STAGE_PERSIST) {
-      int per_cu = 0, dev = 0, cus = 0;
-      if (hipGetDevice(&dev) == hipSuccess &&
-          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-          hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) == hipSuccess && per_cu > 0)
-        nb = std::min(ntiles, per_cu * cus);
-    }
-    a.xcd_bands = (ntiles % 8 == 0 && nb % 8 == 0) ? 1 : 0;
-    // the epilogue's cell means: RK stages of 2D p=2 two-fluid tiles on the
-    // matrix-core element phases with the shared Minv (k_stage's MFW path)
-    constexpr bool mfw = D == 2 && P == 2 && PH == 1 && (TX * TY) % 16 == 0;
-    a.means = nullptr;
-    a.uavg = nullptr;
-    if (mfw && mode == 1 && means && s.t.mlp_vinv && s.t.minv_stride == 0 &&
-        (s.precision >= 1 || DGX_STAGE_MFMA64)) {
-      a.means = means;
-      a.uavg = uavg;
-      a.psens = psens;
-      if (means_done) *means_done = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, a);
-    return hipGetLastError();
+    return go(std::integral_constant<int, TileShape<D>::TX>{});
   });
 }
 

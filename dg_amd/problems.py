@@ -17,7 +17,10 @@ GAMMA = 1.4
 
 
 def _pack(mesh: Mesh, fields: list[np.ndarray]) -> np.ndarray:
-    """fields[f] has shape (N_E, N_s) -> U[(e*NF+f)*N_s+i]."""
+    """fields[f] has shape (N_E, N_s) -> U[(e*NF+f)*N_s+i]; one field per
+    field of the mesh."""
+    if len(fields) != mesh.sizes["N_F"]:
+        raise ValueError(f"{len(fields)} fields for a mesh of {mesh.sizes['N_F']}")
     return np.ascontiguousarray(np.stack(fields, axis=1).reshape(-1))
 
 
@@ -67,13 +70,18 @@ def sod_mesh(nx: int = 1000, order: int = 2, physics: int = MULTIFLUID) -> Mesh:
 
 def sod_state(mesh: Mesh, physics: int = MULTIFLUID) -> np.ndarray:
     """Sod tube, gamma=1.4 both sides, split on each element's first node
-    (buildLRstates_multifluid, init_cond.cc:351-397)."""
+    (buildLRstates_multifluid, init_cond.cc:351-397).  Passive scalars: 1 on
+    the left, 0 on the right; stiffened gas: pinf = 0."""
     x, _ = node_xy(mesh)
     left = (x[:, :1] < 1e-8) & np.ones_like(x, dtype=bool)
     rho = np.where(left, 1.0, 0.125)
     p = np.where(left, 1.0, 0.1)
     u = np.zeros_like(x)
-    return _pack(mesh, conserved_1d(rho, u, p, GAMMA, physics))
+    if physics in (SINGLEFLUID, MULTIFLUID):
+        return _pack(mesh, conserved_1d(rho, u, p, GAMMA, physics))
+    phi = np.where(left, 1.0, 0.0)
+    extra = dict(phic=phi, phinc=phi) if physics == PASSIVE else dict(pinf=np.zeros_like(x))
+    return _pack(mesh, [rho, rho * u] + _closure(physics, rho, p, np.full_like(x, GAMMA), 0.5 * rho * u * u, **extra))
 
 
 # -------------------------------------------------------- 2D Shu vortex

@@ -31,7 +31,7 @@ BCS_1D = [(FARFIELD, REFLECTIVE), (PERIODIC, PERIODIC), (REFLECTIVE, FARFIELD)]
 BCS_2D = [(PERIODIC, PERIODIC, REFLECTIVE, FARFIELD), (FARFIELD, REFLECTIVE, PERIODIC, PERIODIC),
           (REFLECTIVE, REFLECTIVE, REFLECTIVE, REFLECTIVE), (PERIODIC, PERIODIC, PERIODIC, PERIODIC)]
 # element counts: tile count not a multiple of 8, then a multiple of 8 (XCD bands)
-SIZES_1D = [256, 1024]
+SIZES_1D = [256, 1024, 200]  # 200: tiles of 8 elements (128 does not divide it)
 SIZES_2D = [(16, 8), (32, 16)]
 
 
@@ -47,7 +47,7 @@ def tiled_mesh(d, order, ph, bc, n):
 def test_tiled_rhs_matches_oracle_and_general_path(device, c, order):
     d, ph, fl = c
     for k, bc in enumerate(BCS_1D if d == 1 else BCS_2D):
-        n = (SIZES_1D if d == 1 else SIZES_2D)[k % 2]
+        n = SIZES_1D[k % 3] if d == 1 else SIZES_2D[k % 2]
         mesh = tiled_mesh(d, order, ph, bc, n)
         prob = oracle.Problem(mesh.tables(), ph, fl)
         solver = Solver(Config(d, order, ph, fl), mesh)
@@ -125,6 +125,65 @@ def test_one_launch_hr_limiter_bitwise(device, ph, order):
         solver.limit_into(LIMIT_HR, Ud, Ud)
 
 
+@pytest.mark.parametrize("c", [x for x in CONFIGS + CONFIGS_PS if x[0] == 1], ids=cfg_id)
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_small_1d_tiles_cfl_run_with_hr(device, c, order):
+    """1D meshes that 128 does not divide take tiles of 8 elements; with the
+    HR limiter the RK driver limits out of place in one launch per stage.
+    A CFL Sod run to a final time (76-182 steps) has the oracle's step count
+    and time, and every field within 1e-11 of the oracle's state or, where the
+    limited shock amplifies rounding beyond that, within twice the largest
+    distance between the oracle's run and its own runs from the initial state
+    moved by one ulp (all up, all down, and two seeded mixes of both; with
+    p = 3 and Rusanov these spread over 2e-11..1.3e-10 in the density's L2)."""
+    d, ph, fl = c
+    mesh = tiled_mesh(1, order, ph, BCS_1D[0], 200)
+    prob = oracle.Problem(mesh.tables(), ph, fl)
+    solver = Solver(Config(1, order, ph, fl), mesh)
+    assert solver.path == PATH_TILED
+    U0 = problems.sod_state(mesh, ph)
+    cfl = 0.4 * mesh.dx / (2 * order + 1)
+    U_ref, n_ref, T_ref, _ = prob.rk4(U0, 1, cfl, tf=0.05, limiter=LIMIT_HR)
+    assert n_ref > 0
+    up, down = np.nextafter(U0, np.inf), np.nextafter(U0, -np.inf)
+    floor = None
+    for U1 in (up, down) + tuple(np.where(np.random.default_rng(k).random(U0.size) < 0.5, up, down) for k in (0, 1)):
+        U_ulp, n_ulp, _, _ = prob.rk4(U1, 1, cfl, tf=0.05, limiter=LIMIT_HR)
+        assert n_ulp == n_ref
+        d = per_field_rel(U_ulp, U_ref, prob.nf, prob.cfg.N_s)
+        floor = d if floor is None else [(max(a, c), max(b, e)) for (a, b), (c, e) in zip(floor, d)]
+    Ud = dev(U0, device)
+    n, T = solver.rk4_integrate(Ud, 1, cfl, tf=0.05, limiter=LIMIT_HR)
+    assert n == n_ref and abs(T - T_ref) <= 1e-14
+    errs = per_field_rel(host(Ud), U_ref, prob.nf, prob.cfg.N_s)
+    for (l2, li), (f2, fi) in zip(errs, floor):
+        assert l2 <= max(2 * f2, 1e-11) and li <= max(2 * fi, 1e-11), (errs, floor)
+
+
+@pytest.mark.parametrize("ph", [0, MULTIFLUID])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_one_launch_hr_limiter_1d_bitwise(device, ph, order):
+    """1D dgx_limit_into (one sweep forming its own and the neighbours'
+    Taylor coefficients) equals the oracle's HR limiter and the in-place
+    two-launch limiter bit for bit, on every boundary combination."""
+    for k, bc in enumerate(BCS_1D):
+        mesh = tiled_mesh(1, order, ph, bc, 200)
+        prob = oracle.Problem(mesh.tables(), ph, ROE)
+        solver = Solver(Config(1, order, ph, ROE), mesh)
+        U0 = problems.riemann_state(mesh, ph, seed=40 + k)
+        x = np.arange(U0.size, dtype=np.float64)
+        U0 = U0 * (1.0 + 0.01 * np.sin(0.37 * x + k))
+        U_ref = prob.limit_hr(U0)
+        assert not np.array_equal(U_ref, U0)
+        Ud = dev(U0, device)
+        out = torch.full_like(Ud, float("nan"))
+        solver.limit_into(LIMIT_HR, Ud, out)
+        assert np.array_equal(host(out), U_ref), (bc, order)
+        assert np.array_equal(host(Ud), U0)
+        solver.limit(LIMIT_HR, Ud)
+        assert np.array_equal(host(Ud), U_ref)
+
+
 def test_tiled_hri_cfl_run(device):
     """Sensor-gated limiting with CFL time steps through the tile kernel."""
     d, ph, fl, order = 2, MULTIFLUID, ROE, 2
@@ -157,8 +216,10 @@ def test_path_selection(device):
     assert solver.path == PATH_TILED
     solver.set_path(PATH_GENERAL)
     assert solver.path == PATH_GENERAL
-    mesh = tiled_mesh(1, 2, ph, BCS_1D[0], 200)
+    mesh = tiled_mesh(1, 2, ph, BCS_1D[0], 203)
     assert Solver(Config(1, 2, ph, ROE), mesh).path == PATH_GENERAL
+    mesh = tiled_mesh(1, 2, ph, BCS_1D[0], 200)
+    assert Solver(Config(1, 2, ph, ROE), mesh).path == PATH_TILED
 
 
 def test_one_launch_hr_full_size_equals_two_sweeps(device):

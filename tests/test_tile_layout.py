@@ -26,6 +26,7 @@ def test_cartesian_meshes_qualify(order, ph):
 
 
 def test_ragged_meshes_do_not_qualify():
-    assert not Mesh(1, 2, 200, 1, -1.0, 1.0, bc=BCS_1D[0], nfields=4).tileable
+    assert not Mesh(1, 2, 203, 1, -1.0, 1.0, bc=BCS_1D[0], nfields=4).tileable
+    assert Mesh(1, 2, 200, 1, -1.0, 1.0, bc=BCS_1D[0], nfields=4).tileable  # tiles of 8 elements
     assert not Mesh(2, 2, 12, 7, 0.0, 1.0, 0.0, 1.0, bc=BCS_2D[0], nfields=5).tileable
     assert not Mesh(2, 2, 16, 6, 0.0, 1.0, 0.0, 1.0, bc=BCS_2D[0], nfields=5).tileable
