@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
         L.dgx_batch_state_offset.restype = ctypes.c_long
         L.dgx_batch_block_sizes.argtypes = [vp, ip]
         L.dgx_batch_set_block_sizes.argtypes = [vp, ip]
+        L.dgx_batch_set_min_chunk.argtypes = [vp, i]
         L.dgx_batch_autotune.argtypes = [vp, vp, dp, i, vp]
         L.dgx_batched_rk4.argtypes = [vp, vp, dp, i, i, i, vp, vp, vp]
         L.dgx_mlp_param_count.restype = i
@@ -449,7 +450,7 @@ class Batch:
 
     @property
     def block_sizes(self):
-        """[workgroup size, workgroups (one per problem), 0]."""
+        """[workgroup size, workgroups of a launch, problems cut into chunks]."""
         z = (ctypes.c_int * 3)()
         _check(lib().dgx_batch_block_sizes(self._h, z), "dgx_batch_block_sizes")
         return list(z)
@@ -459,6 +460,10 @@ class Batch:
         sizes = list(sizes) + [0] * (3 - len(sizes))
         z = (ctypes.c_int * 3)(*sizes)
         _check(lib().dgx_batch_set_block_sizes(self._h, z), "dgx_batch_set_block_sizes")
+
+    def set_min_chunk(self, n: int) -> None:
+        """Elements per chunk of a cut problem at least (0: no problem is cut)."""
+        _check(lib().dgx_batch_set_min_chunk(self._h, n), "dgx_batch_set_min_chunk")
 
     def autotune(self, U, cfl: float, limiter: int = LIMIT_HR, stream=None) -> None:
         _check(lib().dgx_batch_autotune(self._h, _ptr(U), cfl, limiter, _stream(stream)), "dgx_batch_autotune")

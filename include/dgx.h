@@ -277,9 +277,17 @@ void dgx_batch_free(dgx_batch* b);
 /* Offset (doubles) of problem k in the packed state, k = 0..n (n: total size).
  * Each problem's slice uses the usual layout U[(e*N_F+f)*N_s+i]. */
 long dgx_batch_state_offset(const dgx_batch* b, int k);
-/* sizes[0] = workgroup size (one workgroup per problem), sizes[1] = workgroups
- * (the number of problems), sizes[2] = 0. */
-dgx_status dgx_batch_block_sizes(const dgx_batch* b, int* sizes);
+/* sizes[0] = workgroup size, sizes[1] = workgroups of a launch, sizes[2] =
+ * problems cut into chunks.  Without a cut problem every problem has its own
+ * workgroup, largest first.  A problem whose work (N_E (p+1)^2) is well above
+ * the ensemble's average per resident workgroup, and that has at least two
+ * chunks' elements (dgx_batch_set_min_chunk), is cut into chunks of
+ * consecutive elements on consecutive workgroups; the other problems then
+ * share the remaining resident workgroups, longest first to the least
+ * loaded.  Cut or not, every problem gets the same bits. */
+dgx_status dgx_batch_block_sizes(dgx_batch* b, int* sizes);
+/* Elements per chunk at least (default 64; 0: no problem is cut; else >= 8). */
+dgx_status dgx_batch_set_min_chunk(dgx_batch* b, int min_chunk);
 /* sizes[0] = workgroup size, a multiple of 64 in [64, 1024]; the rest is ignored. */
 dgx_status dgx_batch_set_block_sizes(dgx_batch* b, const int* sizes);
 /* Times one step for each candidate workgroup size (256, 512, 1024) on
@@ -290,7 +298,9 @@ dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int lim
  * limiter 0 none or 1 hrl; limit_initial limits the initial state first
  * (rk.cc:100-114).  T (double[n], device) accumulates each problem's time;
  * status (int[n], device, zero on entry) becomes 1 for a problem whose Dt
- * fell below 1e-14 or was NaN (rk.cc:147-148); that problem stops. */
+ * fell below 1e-14 or was NaN (rk.cc:147-148); that problem stops.  Status 2
+ * marks a cut problem whose chunks stopped waiting for each other (a bounded
+ * wait that timed out; not expected when the device runs nothing else). */
 dgx_status dgx_batched_rk4(dgx_batch* b, double* U, double cfl, int nsteps, int limiter, int limit_initial,
                            double* T, int* status, void* stream);
 

@@ -147,3 +147,45 @@ def test_large_workgroups_match_oracle(device):
         assert abs(out[1][1][k] - T_ref) <= 1e-12 * T_ref
         for l2, li in per_field_rel(Uh[k], U_ref, prob.nf, prob.cfg.N_s):
             assert l2 < 1e-11 and li < 1e-11, (k, p[k], ne[k], l2, li)
+
+
+@pytest.mark.parametrize("limiter,limit_initial", [(LIMIT_HR, True), (LIMIT_HR, False), (0, False)])
+def test_cut_problems_equal_uncut(device, limiter, limit_initial):
+    """Problems cut into chunks on several workgroups (exchanging their end
+    elements every stage and gathering Dt every step) give the bits of the
+    same problems on one workgroup each: state, time and status, over two
+    graph launches; with the limiter, every problem also equals its own
+    oracle run to 1e-11."""
+    n, cfl, nsteps = 6, 0.4, 10
+    ne, p, meshes, states = _ensemble(n, seed=23, ne_range=(120, 400), bc=(REFLECTIVE, FARFIELD))
+    if not limiter:  # smooth states: the Riemann states lose positivity without the limiter
+        states = [problems.smooth_random_state(m, MULTIFLUID, seed=40 + k) for k, m in enumerate(meshes)]
+    batch = Batch(Config(1, 1, MULTIFLUID, ROE), ne, p, bc=(REFLECTIVE, FARFIELD))
+    U0 = torch.from_numpy(batch.pack(states)).to(device)
+    out = {}
+    for min_chunk in (0, 16):
+        batch.set_min_chunk(min_chunk)
+        batch.set_block_sizes([256])
+        sizes = batch.block_sizes
+        if min_chunk == 0:
+            assert sizes[1:] == [n, 0]
+        else:
+            assert sizes[2] > 0 and sizes[1] > n, sizes
+        U = U0.clone()
+        T = torch.zeros(n, dtype=torch.float64, device=device)
+        st = torch.zeros(n, dtype=torch.int32, device=device)
+        for c in range(2):
+            batch.rk4(U, cfl, nsteps // 2, T, st, limiter=limiter, limit_initial=limit_initial and c == 0)
+        torch.cuda.synchronize()
+        out[min_chunk] = (U.cpu().numpy(), T.cpu().numpy(), st.cpu().numpy())
+    (Ua, Ta, sa), (Ub, Tb, sb) = out[0], out[16]
+    assert not sa.any() and not sb.any()
+    assert np.array_equal(Ta, Tb) and np.array_equal(Ua, Ub)
+    if limiter == LIMIT_HR and limit_initial:
+        ref = _oracle_run(meshes, states, p, cfl, nsteps)
+        Uh = batch.unpack(Ub)
+        for k in range(n):
+            U_ref, n_ref, T_ref, prob = ref[k]
+            assert n_ref == nsteps and abs(Tb[k] - T_ref) <= 1e-12 * T_ref
+            for l2, li in per_field_rel(Uh[k], U_ref, prob.nf, prob.cfg.N_s):
+                assert l2 < 1e-11 and li < 1e-11, (k, p[k], ne[k], l2, li)
