@@ -712,14 +712,22 @@ int resident_workgroups(dgx_batch* b, int block) {
   return per_cu * cus;
 }
 
+// Work per element and step by order, relative to order 1 (per-workgroup
+// times of the 256-problem two-fluid ensemble on MI355X: about 1 : 1.75 :
+// 2.7; the flux and limiter work per element does not grow as (p+1)^2 at
+// these sizes, where latency sets the time).
+constexpr double kOrderCost[3] = {1.0, 1.75, 2.7};
+
 // The segments of every workgroup for workgroups of `block` threads.  Work
-// of a problem: N_E (p+1)^2.  With R workgroups resident at once and the
-// average load L (a little above W/R until everything fits), a problem of
-// more than L and at least two chunks' elements is cut into
-// min(ceil(w/L), N_E/min_chunk) chunks on consecutive workgroups; the others
-// go, longest first, to the least loaded of the remaining workgroups.
-// Without a cut problem every problem has its own workgroup, largest first,
-// and the workgroups need not be resident together.
+// of a problem: N_E kOrderCost[p-1].  With R workgroups resident at once
+// and the average load L (a little above W/R until the cut problems leave
+// room), a problem of more than L and at least two chunks' elements is cut
+// into min(ceil(w/L), N_E/min_chunk) chunks on consecutive workgroups, the
+// first ones of the launch, so that all of them are resident together (at
+// most R-1 of them); every other problem has a workgroup of its own after
+// them, largest first, which the device starts as soon as a slot is free.
+// (Counting rounds of one element per thread instead of elements, and
+// packing the uncut problems into static bins, both measured slower.)
 bool make_partition(dgx_batch* b, int block) {
   if (b->part_block == block && b->part_min_chunk == b->min_chunk && b->d_seg) return true;
   drop_graph(b);
@@ -727,64 +735,47 @@ bool make_partition(dgx_batch* b, int block) {
   const int n = b->n;
   std::vector<double> w(n);
   double W = 0;
-  for (int k = 0; k < n; ++k) W += (w[k] = (double)b->ne[k] * (b->order[k] + 1) * (b->order[k] + 1));
+  for (int k = 0; k < n; ++k) W += (w[k] = (double)b->ne[k] * kOrderCost[b->order[k] - 1]);
   std::vector<int> chunks(n, 1);
   const int R = b->min_chunk > 0 ? resident_workgroups(b, block) : 0;
   bool split = false;
   if (R > 1 && b->min_chunk > 0) {
     for (double f = 1.0; f < 4.0 && !split; f *= 1.1) {
       const double L = f * W / R;
-      long S = 0, nsmall = 0;
+      long S = 0;
       double Wsmall = 0;
       for (int k = 0; k < n; ++k) {
         const long c = std::min((long)std::ceil(w[k] / L), (long)(b->ne[k] / b->min_chunk));
         chunks[k] = (w[k] > L && c >= 2) ? (int)c : 1;
         if (chunks[k] > 1) S += chunks[k];
-        else { ++nsmall; Wsmall += w[k]; }
+        else Wsmall += w[k];
       }
-      const long B = R - S;
-      if (S > 0 && B >= 1 && (B >= nsmall || Wsmall <= B * L)) split = true;
+      if (S > 0 && S < R && Wsmall <= (R - S) * L) split = true;
     }
     if (!split) std::fill(chunks.begin(), chunks.end(), 1);
   }
   std::vector<BatchSeg> seg;
   std::vector<int> wg_seg{0};
-  int nsplit = 0;
-  if (!split) {
-    std::vector<int> ord(n);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return w[x] > w[y]; });
-    for (int k : ord) {
-      seg.push_back(BatchSeg{k, 0, b->ne[k], -1, -1, (int)seg.size(), 1});
+  int nsplit = 0, wg = 0;
+  std::vector<int> whole;
+  for (int k = 0; k < n; ++k) {
+    if (chunks[k] == 1) {
+      whole.push_back(k);
+      continue;
+    }
+    const int c = chunks[k], ne = b->ne[k];
+    ++nsplit;
+    for (int i = 0; i < c; ++i) {
+      seg.push_back(BatchSeg{k, (int)((long)i * ne / c), (int)((long)(i + 1) * ne / c), i > 0 ? wg + i - 1 : -1,
+                             i < c - 1 ? wg + i + 1 : -1, wg, c});
       wg_seg.push_back((int)seg.size());
     }
-  } else {
-    int wg = 0;
-    std::vector<int> small;
-    for (int k = 0; k < n; ++k) {
-      if (chunks[k] == 1) { small.push_back(k); continue; }
-      const int c = chunks[k], ne = b->ne[k];
-      ++nsplit;
-      for (int i = 0; i < c; ++i) {
-        seg.push_back(BatchSeg{k, (int)((long)i * ne / c), (int)((long)(i + 1) * ne / c), i > 0 ? wg + i - 1 : -1,
-                               i < c - 1 ? wg + i + 1 : -1, wg, c});
-        wg_seg.push_back((int)seg.size());
-      }
-      wg += c;
-    }
-    std::stable_sort(small.begin(), small.end(), [&](int x, int y) { return w[x] > w[y]; });
-    const int B = std::min<int>((int)small.size(), R - wg);
-    std::vector<std::vector<int>> bins(B);
-    std::vector<double> load(B, 0.0);
-    for (int k : small) {
-      const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      bins[j].push_back(k);
-      load[j] += w[k];
-    }
-    for (int j = 0; j < B; ++j) {
-      for (int k : bins[j]) seg.push_back(BatchSeg{k, 0, b->ne[k], -1, -1, wg + j, 1});
-      wg_seg.push_back((int)seg.size());
-    }
+    wg += c;
+  }
+  std::stable_sort(whole.begin(), whole.end(), [&](int x, int y) { return w[x] > w[y]; });
+  for (int k : whole) {
+    seg.push_back(BatchSeg{k, 0, b->ne[k], -1, -1, wg++, 1});
+    wg_seg.push_back((int)seg.size());
   }
   const int G = (int)wg_seg.size() - 1;
   const bool ok = palloc(b, &b->d_seg, seg.data(), seg.size()) && palloc(b, &b->d_wg_seg, wg_seg.data(), G + 1) &&

@@ -279,7 +279,8 @@ void dgx_batch_free(dgx_batch* b);
 long dgx_batch_state_offset(const dgx_batch* b, int k);
 /* sizes[0] = workgroup size, sizes[1] = workgroups of a launch, sizes[2] =
  * problems cut into chunks.  Without a cut problem every problem has its own
- * workgroup, largest first.  A problem whose work (N_E (p+1)^2) is well above
+ * workgroup, largest first.  A problem whose work (N_E times a measured cost
+ * per element of order p) is well above
  * the ensemble's average per resident workgroup, and that has at least two
  * chunks' elements (dgx_batch_set_min_chunk), is cut into chunks of
  * consecutive elements on consecutive workgroups; the other problems then
