@@ -485,6 +485,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       const int jf = (D == 2 && cl >= 4) ? reversed_pos<P>(j) : j;
       double* q = sQ + dst * SQ + edge * NF * Ms;
       const double jfd = sFG[fi * 4 + 2 + d];
+      // the side's sign: fma(-1, F, nc) is -F + nc rounded once, the same
+      // bits as the sum, without computing both sums and selecting per lane
+      const double sgn = d ? 1.0 : -1.0;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         double s = 0.0;
@@ -493,7 +496,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
           const double* w = sW + (fi * MG + g) * NWP;
           const double ncf = f >= NF - NNC ? w[NF + f - (NF - NNC)] : 0.0;
           // q_L = (-F + nc) JF_0, q_R = (F + nc) JF_1 (physics.cu:800-804, kernels.cu:199-228)
-          const double qv = (d ? (w[f] + ncf) : (-w[f] + ncf)) * jfd;
+          const double qv = __builtin_fma(sgn, w[f], ncf) * jfd;
           s += sPsiw[g + j * MG] * qv;
         }
         q[f * Ms + jf] = s;
