@@ -270,6 +270,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
   __shared__ double sDte[NT];         // per element: Dt (J_0/J_e with the shared Minv)
   __shared__ double sFG[NFACE * 4];   // per tile face: n_x, n_y, JF side 0, JF side 1
   __shared__ int sFD[NFACE];          // per tile face: receiving slot, edge and closure of each side (packed)
+  __shared__ int sFS[NFACE];          // per tile face: source slot of each side (+128, 8 bits each), boundary tag << 16
+  static_assert(NT <= 128 && NH <= 128, "face sources fit 8 bits");
   const VolTables<D, P, PH> tb{sPhi, sDphi, sDw, sPw, sMinv};
 
   const int tid = threadIdx.x;
@@ -366,8 +368,22 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     }
   };
 
+  // the descriptor of every tile face, once per face (one lane each, while
+  // the tile's loads are in flight); the face and integration phases read it
+  auto descs = [&](int tile) {
+    const int ex0 = (tile % a.tiles_x) * TX, ey0 = (tile / a.tiles_x) * TY;
+    for (int fi = tid; fi < NFACE; fi += 256) {
+      const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
+      sFD[fi] = ((fd.dst[0] + 1) | (fd.edge[0] << 8) | (fd.cl[0] << 10)) |
+                (((fd.dst[1] + 1) | (fd.edge[1] << 8) | (fd.cl[1] << 10)) << 16);
+      sFS[fi] = (fd.src[0] + 128) | ((fd.src[1] + 128) << 8) | (fd.tag << 16);
+    }
+  };
   int tile = tile_of(a, 0);
-  if (tile >= 0) issue(tile);
+  if (tile >= 0) {
+    issue(tile);
+    descs(tile);
+  }
   tf.store(tb, MODE == 1 && shared_minv, tid);
   if (tid < MG * Ms) { sPsi[tid] = psi_r; sPsiw[tid] = psiw_r; }
   for (int it = 0; tile >= 0; ++it) {
@@ -377,6 +393,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       return ((size_t)(ey0 + ty) * a.nx + ex0) * NU + (k - ty * TX * NU);
     };
     commit();
+    if (it > 0) descs(tile);  // the previous tile's descriptors were read up to its last barrier
     __syncthreads();
     // the output's Dt factor once per element (read after several barriers)
     if (MODE == 1 && tid < NT) sDte[tid] = shared_minv ? dt * (a.J0 / sGeo[tid * NGEO]) : dt;
@@ -412,21 +429,21 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     // ---- faces: one lane per (face, Gauss point)
     for (int itf = tid; itf < NFACE * MG; itf += 256) {
       const int fi = itf / MG, g = itf % MG;
-      const FaceDesc fd = face_desc<D, TX, TY>(fi, ex0, ey0, a);
-      // what the integration below needs of the descriptor, per side: (dst + 1) | edge << 8 | cl << 10
-      if (g == 0)
-        sFD[fi] = ((fd.dst[0] + 1) | (fd.edge[0] << 8) | (fd.cl[0] << 10)) |
-                  (((fd.dst[1] + 1) | (fd.edge[1] << 8) | (fd.cl[1] << 10)) << 16);
+      // descriptor: per side (dst + 1) | edge << 8 | cl << 10 in sFD, source slot + 128 in sFS
+      const int fdw = sFD[fi], fsw = sFS[fi];
+      const int fsrc[2] = {(fsw & 255) - 128, ((fsw >> 8) & 255) - 128};
+      const int fcl[2] = {(fdw >> 10) & 7, (fdw >> 26) & 7};
+      const int ftag = fsw >> 16;
       const double* fg = sFG + fi * 4;
       const double nxv = fg[0], nyv = (D == 2) ? fg[1] : 0.0, jf0 = fg[2], jf1 = fg[3];
       double tr[2][NF][Ms];
 #pragma unroll
       for (int d = 0; d < 2; ++d) {
-        const int s = fd.src[d];
+        const int s = fsrc[d];
 #pragma unroll
         for (int j = 0; j < Ms; ++j) {
           if (s >= 0) {
-            const int node = closure_node<D, P>(fd.cl[d], j);
+            const int node = closure_node<D, P>(fcl[d], j);
 #pragma unroll
             for (int f = 0; f < NF; ++f) tr[d][f][j] = sU[s * SU + f * Ns + node];
           } else {
@@ -435,7 +452,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
           }
         }
       }
-      if (fd.tag == BC_REFLECTIVE) {  // boundaries.cu:20-67 on the nodal face values
+      if (ftag == BC_REFLECTIVE) {  // boundaries.cu:20-67 on the nodal face values
 #pragma unroll
         for (int j = 0; j < Ms; ++j) {
           if constexpr (D == 1) {

@@ -533,18 +533,27 @@ __device__ void vol_mfma_wave(const VolTables<2, 2, PH>& t, const double* __rest
 
 // Offsets (kk*NF*Ms + jj) of the at most two face slots that land on node i
 // (closure_node(kk, jj) == i, edges in order), -1 when absent; field f adds f*Ms.
+// Built at compile time; each block copies it into LDS.
+template <int NF>
+struct NodeQ {
+  int v[18];
+  constexpr NodeQ() : v{} {
+    for (int i = 0; i < 9; ++i) {
+      int n = 0, o[2] = {-1, -1};
+      for (int kk = 0; kk < 4; ++kk)
+        for (int jj = 0; jj < 3; ++jj)
+          if (closure_node<2, 2>(kk, jj) == i && n < 2) o[n++] = kk * NF * 3 + jj;
+      v[2 * i] = o[0];
+      v[2 * i + 1] = o[1];
+    }
+  }
+};
+template <int NF>
+__device__ constexpr NodeQ<NF> kNodeQ{};
+
 template <int NF>
 __device__ inline void node_q_table(int* t, int tid) {
-  if (tid < 9) {
-    int n = 0, o[2] = {-1, -1};
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-      for (int jj = 0; jj < 3; ++jj)
-        if (closure_node<2, 2>(kk, jj) == tid && n < 2) o[n++] = kk * NF * 3 + jj;
-    t[2 * tid] = o[0];
-    t[2 * tid + 1] = o[1];
-  }
+  if (tid < 18) t[tid] = kNodeQ<NF>.v[tid];
 }
 
 }  // namespace dgx
