@@ -623,19 +623,24 @@ __host__ __device__ constexpr int reversed_pos(int j) {
 // then p = (gamma-1)(E - (m_x^2 + m_y^2)/(2 rho)).  One function for the
 // learned indicator's own pass (k_mlp_means) and the stage kernel's
 // epilogue, so that both give the same bits (checked by the tests).
-__device__ inline void cell_means_rpa(const double* u, const double* w0, double* avg3) {
-  double m[5];
+// Split in two for callers that form the five field means on separate lanes.
+__device__ inline double cell_mode0(const double* uf, const double* w0) {
+  double s = 0.0;
 #pragma unroll
-  for (int f = 0; f < 5; ++f) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) s += w0[i] * u[f * 9 + i];
-    m[f] = s;
-  }
+  for (int i = 0; i < 9; ++i) s += w0[i] * uf[i];
+  return s;
+}
+__device__ inline void rpa_of_means(const double* m, double* avg3) {
   const double gam = 1.0 + 1.0 / m[4];
   avg3[0] = m[0];
   avg3[1] = (gam - 1.0) * (m[3] - 0.5 * (m[1] * m[1] + m[2] * m[2]) / m[0]);
   avg3[2] = m[4];
+}
+__device__ inline void cell_means_rpa(const double* u, const double* w0, double* avg3) {
+  double m[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) m[f] = cell_mode0(u + f * 9, w0);
+  rpa_of_means(m, avg3);
 }
 
 }  // namespace dgx

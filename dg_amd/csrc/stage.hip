@@ -592,20 +592,38 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     }
     if constexpr (MODE == 1 && MFW && PH == 1) {
       if (a.means && shared_minv) {
-        __syncthreads();  // the output tile is in sU
+        // cell means (cell_means_rpa) and sensor averages (sensor_averages_2d)
+        // of the output tile: one lane per (element, field) forms the field's
+        // two means, then one lane per element the derived values (the same
+        // functions and order as the one-lane forms, so the same bits)
+        static_assert(NW >= 2 * NT * NF, "means staging fits the flux buffer");
+        __syncthreads();  // the output tile is in sU, sW is free
+        double* sM = sW;            // [el*NF + f] mode-0 means
+        double* sA = sW + NT * NF;  // [el*NF + f] sensor nodal means
+        for (int k = tid; k < NT * NF; k += 256) {
+          const int el = k / NF, f = k % NF;
+          const double* uf = sU + el * SU + f * Ns;
+          sM[k] = cell_mode0(uf, a.w0);
+          if (a.uavg) {
+            const double v = sensor_nodal_mean<Ns>(uf);
+            sA[k] = v;
+            a.uavg[((size_t)(ey0 + el / TX) * a.nx + ex0 + el % TX) * NF + f] = v;
+          }
+        }
+        __syncthreads();
         if (tid < NT) {
           double a3[3];
-          cell_means_rpa(sU + tid * SU, a.w0, a3);
+          rpa_of_means(sM + tid * NF, a3);
           const size_t eg = (size_t)(ey0 + tid / TX) * a.nx + ex0 + tid % TX;
           double* m = a.means + eg * 3;
           m[0] = a3[0];
           m[1] = a3[1];
           m[2] = a3[2];
-          if (a.uavg) sensor_averages_2d(sU + tid * SU, a.uavg + eg * NF, a.psens + eg);
+          if (a.uavg) a.psens[eg] = sensor_cell<2, 1>(sA + tid * NF, 0).p;
         }
       }
     }
-    __syncthreads();  // LDS is refilled for the next tile
+    if (next >= 0) __syncthreads();  // LDS is refilled for the next tile
     tile = next;
   }
 }
