@@ -778,6 +778,35 @@ __device__ inline void hri_sweep_column(int e, int fc, int NF, int N_N, const in
   hri_limit<D, P>(physical, fc, offxy, L2M, M2L, UL, UC, UR);
 }
 
+// The same column of a second sweep with each element's values taken from W
+// where the element is flagged (the first sweep's result for it) and from U
+// elsewhere: the values U holds after the first sweep's copy-back.
+template <int D, int P>
+__device__ inline void hri_sweep_column_sel(int e, int fc, int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                            const double* __restrict__ L2M, const double* __restrict__ M2L,
+                                            const int* __restrict__ sensors, const double* U,
+                                            const double* __restrict__ W, double* UC) {
+  constexpr int Ns = Shape<D, P>::Ns;
+  const int left = nb[e * N_N + offxy + 0], right = nb[e * N_N + offxy + 1];
+  int physical = 0;
+  if (left < 0) physical = -left;
+  if (right < 0) physical = -right;
+  const double* Se = sensors[e] ? W : U;
+#pragma unroll
+  for (int i = 0; i < Ns; ++i) UC[i] = Se[((long)e * NF + fc) * Ns + i];
+  double UL[Ns], UR[Ns];
+  if (physical == 0) {
+    const double* Sl = sensors[left] ? W : U;
+    const double* Sr = sensors[right] ? W : U;
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) {
+      UL[i] = Sl[((long)left * NF + fc) * Ns + i];
+      UR[i] = Sr[((long)right * NF + fc) * Ns + i];
+    }
+  }
+  hri_limit<D, P>(physical, fc, offxy, L2M, M2L, UL, UC, UR);
+}
+
 // hri1D (limiting_kernels.cu:322-427) for one column (element, field) of a
 // flagged element: HR in Lagrange space through Lag2Mono / Mono2Lag, or the
 // boundary rules; Unew receives the column.
@@ -815,6 +844,28 @@ __global__ __launch_bounds__(128) void k_hri_list(int NF, int N_N, const int* __
   const long n = (long)count[0] * NF;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
     hri_column<D, P>(list[t / NF], (int)(t % NF), NF, N_N, nb, offxy, L2M, M2L, U, Unew);
+}
+
+// The second sweep over the list without either copy-back: the listed
+// (flagged) elements' first-sweep values are in W, so the result goes straight
+// to U[e] -- no thread reads U of a flagged element here.  Same bits as the
+// two sweeps with their copy-backs.
+template <int D, int P>
+__global__ __launch_bounds__(128) void k_hri_list_second(int NF, int N_N, const int* __restrict__ nb, int offxy,
+                                                         const double* __restrict__ L2M,
+                                                         const double* __restrict__ M2L, const int* __restrict__ list,
+                                                         const int* __restrict__ count,
+                                                         const int* __restrict__ sensors, double* U,
+                                                         const double* __restrict__ W) {
+  constexpr int Ns = Shape<D, P>::Ns;
+  const long n = (long)count[0] * NF;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const int e = list[t / NF], fc = (int)(t % NF);
+    double UC[Ns];
+    hri_sweep_column_sel<D, P>(e, fc, NF, N_N, nb, offxy, L2M, M2L, sensors, U, W, UC);
+#pragma unroll
+    for (int i = 0; i < Ns; ++i) U[((long)e * NF + fc) * Ns + i] = UC[i];
+  }
 }
 
 // m2i1D (limiting_kernels.cu:431-722) over the listed elements (two-fluid:
@@ -1317,11 +1368,19 @@ hipError_t launch_limit_flagged(Solver& s, double* U, hipStream_t st) {
   return launch_limit_listed(s, U, st);
 }
 
+// 2D: the x sweep into s.work, then the y sweep from s.work (flagged
+// elements) and U (the rest) straight into U (k_hri_list_second): no copy-back.
 hipError_t launch_limit_listed(Solver& s, double* U, hipStream_t st) {
-  hipError_t e;
   if (s.D == 1) return hri_list(s, 0, s.t.Lag2Mono, s.t.Mono2Lag, U, st);
-  if ((e = hri_list(s, 0, s.t.Lag2MonoX, s.t.MonoX2Lag, U, st)) != hipSuccess) return e;
-  return hri_list(s, 2, s.t.Lag2MonoY, s.t.MonoY2Lag, U, st);
+  const int* count = s.flag + 2;
+  return by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
+    constexpr int D = decltype(d)::value, P = decltype(p)::value;
+    hipLaunchKernelGGL((k_hri_list<D, P>), dim3(2048), dim3(128), 0, st, s.NF, s.N_N, s.t.neighbors, 0,
+                       s.t.Lag2MonoX, s.t.MonoX2Lag, s.flag_list, count, U, s.work);
+    hipLaunchKernelGGL((k_hri_list_second<D, P>), dim3(2048), dim3(128), 0, st, s.NF, s.N_N, s.t.neighbors, 2,
+                       s.t.Lag2MonoY, s.t.MonoY2Lag, s.flag_list, count, s.sensors, U, s.work);
+    return hipGetLastError();
+  });
 }
 
 // Limiting::M2Ilimiting (limiting.cc:682-720), two-fluid: sensing, the
