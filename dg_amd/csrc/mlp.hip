@@ -227,6 +227,13 @@ __device__ inline float feature_of(int k, const double* m, const double* inv, fl
 // and the jumps L, R, D at 45..47 (q = 3, j = 1..3); k = 48 is the jump U
 // (q = 0, j = 0); the rest are zero.
 __device__ inline double clamp_feature(double v) { return v > 8.0 ? 8.0 : (v < -8.0 ? -8.0 : (v == v ? v : 0.0)); }
+// (float)clamp_feature(v), clamped after the conversion: the conversion is
+// monotonic and +-8 are floats, so the same value for every v (infinities to
+// +-8, NaN to +0, -0 kept), in single-precision instructions.
+__device__ inline float clamp_feature_f(double v) {
+  const float x = (float)v;
+  return x == x ? fminf(fmaxf(x, -8.0f), 8.0f) : 0.0f;
+}
 __device__ inline float jump_feature(float j) { return j < 0.0f ? 0.0f : j; }
 __device__ inline void element_features(int q, const double* m, const double* inv, float4 jmp, bf16x4 (&bx)[4]) {
 #pragma unroll
@@ -234,12 +241,12 @@ __device__ inline void element_features(int q, const double* m, const double* in
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = 16 * s + 4 * q + j;
-      bx[s][j] = (short)f2bf((float)clamp_feature(m[k] * inv[k / 9]));
+      bx[s][j] = (short)f2bf(clamp_feature_f(m[k] * inv[k / 9]));
     }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int k = min(32 + 4 * q + j, 44);
-    const float mf = (float)clamp_feature(m[k] * inv[k / 9]);
+    const float mf = clamp_feature_f(m[k] * inv[k / 9]);
     if (j == 0) {
       bx[2][j] = (short)f2bf(mf);
     } else {
