@@ -148,12 +148,18 @@ __device__ inline void wave_modes(double* u, const double* Vinv, double* inv, in
       const int mode = q + 4 * v;
       if (mode < 9) u[c * 45 + f * 9 + mode] = d[f][v];
     }
-  if (q == 0) {
-    const double sc_mom = sqrt(fabs(d[0][0]) * fabs(d[3][0]));
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      const double sc = (f == 1 || f == 2) ? sc_mom : fabs(d[f][0]);
-      inv[c * 5 + f] = sc > 0.0 ? 1.0 / sc : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();  // the modes are in place
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // the scales from the modes 0 in place, lane group q taking field q (and
+  // q = 0 field 4 too): two divisions per lane instead of five on one group
+  {
+    const double* m = u + c * 45;
+    const double sc = (q == 1 || q == 2) ? sqrt(fabs(m[0]) * fabs(m[27])) : fabs(m[q * 9]);
+    inv[c * 5 + q] = sc > 0.0 ? 1.0 / sc : 0.0;
+    if (q == 0) {
+      const double s4 = fabs(m[36]);
+      inv[c * 5 + 4] = s4 > 0.0 ? 1.0 / s4 : 0.0;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
