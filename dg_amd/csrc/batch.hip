@@ -560,11 +560,18 @@ __device__ __forceinline__ bool batch_segment(const BatchArgs& a, const BatchSeg
 }
 
 // LB: the launch bound (512, or 1024 with four waves on every SIMD of the
-// CU).  The arguments live in device memory: a by-value argument block of
+// CU), and W the waves per SIMD the registers must allow: 256-thread
+// workgroups are built for three (168 registers, a few spilled), so that
+// three of them share a CU where the unbounded build's 238 registers allow
+// two.  The arguments live in device memory: a by-value argument block of
 // this size was copied to scratch, and its pointers took the registers of
 // the element pass.
-template <int PH, int FL, int LB>
-__global__ __launch_bounds__(LB) void k_batch(const BatchArgs* __restrict__ pa) {
+#ifndef DGX_BATCH_W256
+#define DGX_BATCH_W256 3
+#endif
+template <int PH, int FL, int LB, int W = 1>
+__global__ __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(W))) void k_batch(
+    const BatchArgs* __restrict__ pa) {
   const BatchArgs& a = *pa;
   const int wg = blockIdx.x;
   for (int k = a.wg_seg[wg]; k < a.wg_seg[wg + 1]; ++k) {
@@ -702,8 +709,9 @@ int resident_workgroups(dgx_batch* b, int block) {
   int per_cu = 0, dev = 0, cus = 0;
   const hipError_t e = dispatch_batch(b->cfg, [&](auto ph, auto fl) -> hipError_t {
     constexpr int PH = decltype(ph)::value, FL = decltype(fl)::value;
-    const void* k = block <= 512 ? reinterpret_cast<const void*>(&k_batch<PH, FL, 512>)
-                                 : reinterpret_cast<const void*>(&k_batch<PH, FL, 1024>);
+    const void* k = block <= 256   ? reinterpret_cast<const void*>(&k_batch<PH, FL, 256, DGX_BATCH_W256>)
+                    : block <= 512 ? reinterpret_cast<const void*>(&k_batch<PH, FL, 512>)
+                                   : reinterpret_cast<const void*>(&k_batch<PH, FL, 1024>);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0);
   });
   if (e != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
@@ -821,7 +829,8 @@ hipError_t launch_batch(dgx_batch* b, int block, hipStream_t st) {
   return dispatch_batch(b->cfg, [&](auto ph, auto fl) -> hipError_t {
     constexpr int PH = decltype(ph)::value, FL = decltype(fl)::value;
     const BatchArgs* pa = b->d_args;
-    if (block <= 512) hipLaunchKernelGGL((k_batch<PH, FL, 512>), dim3(b->G), dim3(block), 0, st, pa);
+    if (block <= 256) hipLaunchKernelGGL((k_batch<PH, FL, 256, DGX_BATCH_W256>), dim3(b->G), dim3(block), 0, st, pa);
+    else if (block <= 512) hipLaunchKernelGGL((k_batch<PH, FL, 512>), dim3(b->G), dim3(block), 0, st, pa);
     else hipLaunchKernelGGL((k_batch<PH, FL, 1024>), dim3(b->G), dim3(block), 0, st, pa);
     return hipGetLastError();
   });
