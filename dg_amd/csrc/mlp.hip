@@ -178,7 +178,8 @@ __device__ inline void wave_modes(double* u, const double* Vinv, double* inv, in
 // sensor marks as a shock is in such a pair, so the guard holds all of them,
 // whatever the network says.  The pressure jump W > th, necessary for the
 // test, is checked first from psens.
-__global__ __launch_bounds__(256) void k_mlp_jumps(const double* __restrict__ avg, const int* __restrict__ nb, int N,
+// eight waves per SIMD (62 registers): the pass waits on dependent gathers
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_mlp_jumps(const double* __restrict__ avg, const int* __restrict__ nb, int N,
                                                    float4* __restrict__ jumps, int* __restrict__ count_reset,
                                                    const double* __restrict__ uavg, const double* __restrict__ psens,
                                                    double th_shock, int* __restrict__ guard) {
