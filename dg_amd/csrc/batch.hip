@@ -996,7 +996,7 @@ dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int lim
   hipEvent_t t0, t1;
   if (!hip_ok(hipEventCreate(&t0), "hipEventCreate") || !hip_ok(hipEventCreate(&t1), "hipEventCreate"))
     return DGX_EDEVICE;
-  const int sizes[3] = {256, 512, 1024};
+  const int sizes[5] = {64, 128, 256, 512, 1024};
   float best = 1e30f;
   int best_b = b->block;
   dgx_status r = DGX_OK;

@@ -291,7 +291,7 @@ dgx_status dgx_batch_block_sizes(dgx_batch* b, int* sizes);
 dgx_status dgx_batch_set_min_chunk(dgx_batch* b, int min_chunk);
 /* sizes[0] = workgroup size, a multiple of 64 in [64, 1024]; the rest is ignored. */
 dgx_status dgx_batch_set_block_sizes(dgx_batch* b, const int* sizes);
-/* Times ten steps for each candidate workgroup size (256, 512, 1024; the
+/* Times ten steps for each candidate workgroup size (64 to 1024; the
  * fastest of two timed runs after a warm-up run) on scratch copies of U (U is
  * not modified) and keeps the fastest size. */
 dgx_status dgx_batch_autotune(dgx_batch* b, const double* U, double cfl, int limiter, void* stream);

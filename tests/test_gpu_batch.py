@@ -85,7 +85,7 @@ def test_batched_graph_replay_and_autotune(device):
     assert np.array_equal(sa, ss) and np.array_equal(Ta, Ts) and np.array_equal(Ua, Us)
     batch.autotune(U0, cfl)
     sizes = batch.block_sizes
-    assert sizes[0] in (256, 512, 1024) and sizes[1] == n
+    assert sizes[0] in (64, 128, 256, 512, 1024) and sizes[1] == n
     Uc, Tc, sc = run(None, 1, 16)
     assert np.array_equal(sa, sc)
     scale = np.abs(Ua).max()
