@@ -724,7 +724,15 @@ int resident_workgroups(dgx_batch* b, int block) {
 // times of the 256-problem two-fluid ensemble on MI355X: about 1 : 1.75 :
 // 2.7; the flux and limiter work per element does not grow as (p+1)^2 at
 // these sizes, where latency sets the time).
-constexpr double kOrderCost[3] = {1.0, 1.75, 2.7};
+// (overridable in tuning builds; at 128-thread workgroups 1.5 : 2.2 measured the
+// same, 2.0 : 3.2 and 1.75 : 3.5 4-5 % slower)
+#ifndef DGX_BATCH_COST2
+#define DGX_BATCH_COST2 1.75
+#endif
+#ifndef DGX_BATCH_COST3
+#define DGX_BATCH_COST3 2.7
+#endif
+constexpr double kOrderCost[3] = {1.0, DGX_BATCH_COST2, DGX_BATCH_COST3};
 
 // The segments of every workgroup for workgroups of `block` threads.  Work
 // of a problem: N_E kOrderCost[p-1].  With R workgroups resident at once
