@@ -73,3 +73,30 @@ def test_features_of_polynomial_and_uniform_states():
     Xc = om.features(Uc.ravel(), t["neighbors"], vinv)
     assert np.allclose(Xc[:, 45:49], 0.0)
     assert np.allclose(Xc[:, [0, 27, 36]], 1.0)
+
+
+def test_clamp_after_conversion_equals_clamp_before():
+    """The indicator clamps a scaled mode to [-8, 8] after its conversion to
+    float (single-precision min/max) where the features of the reference
+    form clamp the double first: the conversion is monotonic and +-8 are
+    floats, so both give the same float for every double, infinities to +-8,
+    NaN to +0 and -0 kept.  Checked over the edges (values that round to +-8,
+    the largest doubles, subnormals, zeros, NaN) and random values."""
+    def before(v):
+        c = 8.0 if v > 8.0 else (-8.0 if v < -8.0 else (v if v == v else 0.0))
+        return np.float32(c)
+
+    def after(v):
+        x = np.float32(v)
+        return np.float32(min(max(x, np.float32(-8.0)), np.float32(8.0))) if x == x else np.float32(0.0)
+
+    eps = np.finfo(np.float64).eps
+    edges = [0.0, -0.0, 8.0, -8.0, 8.0 * (1 + eps), -8.0 * (1 + eps), 8.0 * (1 - eps / 2), np.nextafter(8.0, 9.0),
+             np.nextafter(8.0, 0.0), np.nextafter(-8.0, 0.0), 8.0000001, -8.0000001, 7.9999999, np.inf, -np.inf,
+             np.nan, 5e-324, -5e-324, 1e-310, np.finfo(np.float64).max, -np.finfo(np.float64).max, 3.4e38, 1e39]
+    rng = np.random.default_rng(7)
+    vals = edges + list(rng.standard_normal(20000) * 6.0) + list(rng.standard_normal(2000) * 1e-30)
+    with np.errstate(over="ignore", invalid="ignore"):
+        for v in vals:
+            b, a = before(v), after(v)
+            assert b.tobytes() == a.tobytes(), (v, b, a)
