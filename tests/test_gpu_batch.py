@@ -59,8 +59,8 @@ def test_batched_rk4_matches_oracle(device, bc):
 
 def test_batched_graph_replay_and_autotune(device):
     """Two replays of the captured graph equal one run of twice the steps;
-    workgroups of 64 and 1024 threads per problem give the same bits, and the
-    autotuner's choice agrees."""
+    workgroups of 64, 128, 256, 512 and 1024 threads per problem give the
+    same bits, and the autotuner's choice agrees."""
     n, cfl = 9, 0.4
     ne, p, meshes, states = _ensemble(n, seed=5)
     batch = Batch(Config(1, 1, MULTIFLUID, ROE), ne, p)
@@ -83,6 +83,9 @@ def test_batched_graph_replay_and_autotune(device):
     assert np.array_equal(Ua, Ub) and np.array_equal(Ta, Tb) and np.array_equal(sa, sb)
     Us, Ts, ss = run([1024], 2, 8)  # the same arithmetic per element, whatever the workgroup size
     assert np.array_equal(sa, ss) and np.array_equal(Ta, Ts) and np.array_equal(Ua, Us)
+    for bs in (128, 256, 512):  # the three-wave instance (<= 256 threads) and the two-wave one
+        Uw, Tw, sw = run([bs], 1, 16)
+        assert np.array_equal(sa, sw) and np.array_equal(Ta, Tw) and np.array_equal(Ua, Uw), bs
     batch.autotune(U0, cfl)
     sizes = batch.block_sizes
     assert sizes[0] in (64, 128, 256, 512, 1024) and sizes[1] == n
