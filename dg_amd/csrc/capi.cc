@@ -393,7 +393,7 @@ dgx_status dgx_solver_create(const dgx_config* cfg, const dgx_mesh* mesh, dgx_so
   ok = ok && alloc(s, &s.Qe, (size_t)NE * NN * s.NF * s.M_s) && alloc(s, &s.Uacc, S) && alloc(s, &s.Ustar, S) &&
        alloc(s, &s.work, S) && alloc(s, &s.work2, S) && alloc(s, &s.Uavg, (size_t)NE * s.NF) &&
        alloc(s, &s.sensors, (size_t)NE) && alloc(s, &s.sens_tgt, (size_t)2 * NE) &&
-       alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.flag_list, (size_t)NE) &&
+       alloc(s, &s.sens_act, (size_t)3 * NE) && alloc(s, &s.flag, 4) && alloc(s, &s.sens_changed, kRelaxBatch) && alloc(s, &s.flag_list, (size_t)NE) &&
        alloc(s, &s.dscratch, 8) && alloc(s, &s.time_state, 4) &&
        alloc(s, &s.cons_partial, conservation_scratch_doubles()) &&
        alloc(s, &s.dt_partial, ((size_t)NE * Ns + 255) / 256 + 1) &&

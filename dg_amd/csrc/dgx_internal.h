@@ -66,6 +66,10 @@ struct Cart {
 
 enum KernelPath { PATH_AUTO = 0, PATH_GENERAL = 1, PATH_TILED = 2 };
 
+// Sensor relaxation sweeps queued per host read-back (the 1024x512
+// shock-bubble needs about three).
+constexpr int kRelaxBatch = 4;
+
 struct Solver {
   dgx_config cfg{};
   int D = 1, P = 1, NF = 3, N_s = 0, N_G = 0, M_s = 0, M_G = 0, N_N = 0, N_E = 0, M_T = 0, N1 = 0;
@@ -89,6 +93,7 @@ struct Solver {
   int* sens_tgt = nullptr;  // sensor resolution scratch [e*2]
   int* sens_act = nullptr;  // [e*2] (ping-pong)
   int* flag = nullptr;      // device flag / counters: [0] sensor relaxation, [2] flagged-element count
+  int* sens_changed = nullptr;  // [kRelaxBatch] change flags of a batch of sensor relaxation sweeps
   int* flag_list = nullptr; // [N_E] flagged elements (launch_limit_flagged)
   double* m2buf = nullptr;  // m2 limiting: 15 per-field arrays of N_E*N_s (allocated on first use)
   double* dscratch = nullptr;  // device scalars: [1] Dt, [2] the time of a device-timed CFL run

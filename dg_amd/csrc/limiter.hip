@@ -638,10 +638,13 @@ __global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int*
 }
 
 // act_in/act_out: 1 if the element runs its tests in the sequential order.
+// prev: the previous sweep's change flag when sweeps are queued in batches;
+// zero there means the fixed point is reached and this sweep does nothing.
 __global__ __launch_bounds__(256) void k_sensor_relax(int N_E, int N_N, const int* __restrict__ nb,
                                                       const int* __restrict__ init, const int* __restrict__ tgt,
                                                       const int* __restrict__ act_in, int* __restrict__ act_out,
-                                                      int* __restrict__ changed) {
+                                                      int* __restrict__ changed, const int* prev = nullptr) {
+  if (prev && *prev == 0) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N_E) return;
   int a = init[e] == 0 ? 1 : 0;
@@ -1291,6 +1294,29 @@ static hipError_t sensor_resolve(Solver& s, const int* nb, const int* init, cons
   int* act1 = s.sens_act + s.N_E;
   hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, act0, (long)s.N_E, 1);
   // Jacobi sweeps converge to the sequential fixed point after (longest chain + 1) sweeps.
+  if (s.sens_changed) {
+    // batches of kRelaxBatch sweeps with one read-back each: a sweep after one
+    // that changed nothing returns at once (its input already equals its
+    // output, so either buffer holds the fixed point); the batch length is
+    // even, so every batch starts from act0
+    static_assert(kRelaxBatch % 2 == 0, "batches end on act0");
+    for (int it = 0; it <= s.N_E; it += kRelaxBatch) {
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kRelaxBatch), 0, st, s.sens_changed, (long)kRelaxBatch, 0);
+      for (int i = 0; i < kRelaxBatch; ++i) {
+        hipLaunchKernelGGL(k_sensor_relax, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt,
+                           (i % 2) ? act1 : act0, (i % 2) ? act0 : act1, s.sens_changed + i,
+                           i ? s.sens_changed + i - 1 : nullptr);
+      }
+      int h = 0;
+      hipError_t e = hipMemcpyAsync(&h, s.sens_changed + kRelaxBatch - 1, sizeof(int), hipMemcpyDeviceToHost, st);
+      if (e != hipSuccess) return e;
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      if (!h) break;
+    }
+    hipLaunchKernelGGL(k_sensor_final, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt, val,
+                       act0, sensors);
+    return hipGetLastError();
+  }
   for (int it = 0; it <= s.N_E; ++it) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag, 1L, 0);
     hipLaunchKernelGGL(k_sensor_relax, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt, act0,
