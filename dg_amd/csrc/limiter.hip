@@ -617,11 +617,19 @@ struct SensorTh {
 // First test that fires for element e (as if e is tested): tgt = neighbour
 // index, -1 for a boundary neighbour (flags only e), -2 for none.
 template <int D, int PH>
+// With init and act (the solver's sensing), the pass also sets init[e] = 0
+// (sensors start at zero, sensor.cc:17-20) and act[e] = 1 (every element
+// active before the first relaxation sweep).
 __global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int* __restrict__ nb,
                                                     const double* __restrict__ Uavg, SensorTh s,
-                                                    int* __restrict__ tgt, int* __restrict__ val) {
+                                                    int* __restrict__ tgt, int* __restrict__ val,
+                                                    int* __restrict__ init = nullptr, int* __restrict__ act = nullptr) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N_E) return;
+  if (init) {
+    init[e] = 0;
+    act[e] = 1;
+  }
   const SensorCell L = sensor_cell<D, PH>(Uavg, e);
   int t_out = -2, v_out = 0;
   for (int nn = 0; nn < N_N; nn++) {
@@ -1288,11 +1296,12 @@ hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st) {
 }
 
 // Resolve sequential-order flags from per-element first hits.
+// act_set: the hit pass has already set every element active in s.sens_act.
 static hipError_t sensor_resolve(Solver& s, const int* nb, const int* init, const int* tgt, const int* val,
-                                 int* sensors, hipStream_t st) {
+                                 int* sensors, hipStream_t st, bool act_set = false) {
   int* act0 = s.sens_act;
   int* act1 = s.sens_act + s.N_E;
-  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, act0, (long)s.N_E, 1);
+  if (!act_set) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, act0, (long)s.N_E, 1);
   // Jacobi sweeps converge to the sequential fixed point after (longest chain + 1) sweeps.
   if (s.sens_changed) {
     // batches of kRelaxBatch sweeps with one read-back each: a sweep after one
@@ -1355,17 +1364,16 @@ hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg
   for (int k = 0; k < 3; ++k) { th.on[k] = k < s.n_thresh; th.th[k] = s.thresh[k]; }
   int* tgt = s.sens_tgt;
   int* val = s.sens_tgt + s.N_E;
+  // sensors start at zero (sensor.cc:17-20): init, set by the hit pass with the first active set
+  int* init = s.sens_act + 2 * (long)s.N_E;
   auto hit = [&](auto d, auto ph) -> hipError_t {
     hipLaunchKernelGGL((k_sensor_hit<decltype(d)::value, decltype(ph)::value>), dim3(nblk(s.N_E, 256)), dim3(256), 0,
-                       st, s.N_E, s.N_N, s.t.neighbors, Uavg, th, tgt, val);
+                       st, s.N_E, s.N_N, s.t.neighbors, Uavg, th, tgt, val, init, s.sens_act);
     return hipGetLastError();
   };
   e = by_dim_phys(s.D, s.cfg.physics, hit);
   if (e != hipSuccess) return e;
-  // sensors start at zero (sensor.cc:17-20)
-  int* init = s.sens_act + 2 * (long)s.N_E;
-  hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, init, (long)s.N_E, 0);
-  return sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st);
+  return sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st, true);
 }
 
 // One sweep of the flagged HR over the compacted list (see launch_limit_flagged).
