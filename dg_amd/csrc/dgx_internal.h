@@ -150,7 +150,9 @@ hipError_t launch_dt_step(const Solver& s, const double* U, double cfl, double t
 hipError_t launch_limit_hr(Solver& s, double* U, hipStream_t st);
 // Out = HR(In), both 2D sweeps in one launch; In != Out, needs s.hr2d
 hipError_t launch_limit_hr_oop(Solver& s, const double* In, double* Out, hipStream_t st);
-hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st);
+// with list: the flagged elements also appended to list[0 .. *count)
+hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st, int* list = nullptr,
+                          int* count = nullptr);
 // the sensor's cell averages alone (k_uavg), [e*NF + f]
 hipError_t launch_uavg(const Solver& s, const double* U, double* Uavg, hipStream_t st);
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st);

@@ -619,12 +619,15 @@ struct SensorTh {
 template <int D, int PH>
 // With init and act (the solver's sensing), the pass also sets init[e] = 0
 // (sensors start at zero, sensor.cc:17-20) and act[e] = 1 (every element
-// active before the first relaxation sweep).
+// active before the first relaxation sweep); with count, zeroes the flag
+// list's count that k_sensor_final appends to.
 __global__ __launch_bounds__(256) void k_sensor_hit(int N_E, int N_N, const int* __restrict__ nb,
                                                     const double* __restrict__ Uavg, SensorTh s,
                                                     int* __restrict__ tgt, int* __restrict__ val,
-                                                    int* __restrict__ init = nullptr, int* __restrict__ act = nullptr) {
+                                                    int* __restrict__ init = nullptr, int* __restrict__ act = nullptr,
+                                                    int* __restrict__ count = nullptr) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (count && e == 0) *count = 0;
   if (e >= N_E) return;
   if (init) {
     init[e] = 0;
@@ -671,20 +674,38 @@ __global__ __launch_bounds__(256) void k_fill_i32(int* p, long n, int v) {
   if (i < n) p[i] = v;
 }
 
-// last writer in element order wins; untouched elements keep init.
+// last writer in element order wins; untouched elements keep init.  With
+// list, the flagged elements are also appended to list[0 .. *count) (one
+// atomic per block, as k_flag_list; *count zeroed by the hit pass).
 __global__ __launch_bounds__(256) void k_sensor_final(int N_E, int N_N, const int* __restrict__ nb,
                                                       const int* __restrict__ init, const int* __restrict__ tgt,
                                                       const int* __restrict__ val, const int* __restrict__ act,
-                                                      int* __restrict__ sensors) {
+                                                      int* __restrict__ sensors, int* __restrict__ list = nullptr,
+                                                      int* __restrict__ count = nullptr) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= N_E) return;
-  int best = -1, bval = init[x];
-  if (act[x] && tgt[x] != -2) { best = x; bval = val[x]; }
-  for (int nn = 0; nn < N_N; ++nn) {
-    int o = nb[x * N_N + nn];
-    if (o >= 0 && act[o] && tgt[o] == x && o > best) { best = o; bval = val[o]; }
+  int bval = 0;
+  if (x < N_E) {
+    int best = -1;
+    bval = init[x];
+    if (act[x] && tgt[x] != -2) { best = x; bval = val[x]; }
+    for (int nn = 0; nn < N_N; ++nn) {
+      int o = nb[x * N_N + nn];
+      if (o >= 0 && act[o] && tgt[o] == x && o > best) { best = o; bval = val[o]; }
+    }
+    sensors[x] = bval;
   }
-  sensors[x] = bval;
+  if (list) {  // uniform over the launch
+    __shared__ int sN, sBase;
+    __shared__ int sIdx[256];
+    if (threadIdx.x == 0) sN = 0;
+    __syncthreads();
+    const bool f = x < N_E && bval != 0;
+    if (f) sIdx[atomicAdd(&sN, 1)] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) sBase = sN ? atomicAdd(count, sN) : 0;
+    __syncthreads();
+    if ((int)threadIdx.x < sN) list[sBase + threadIdx.x] = sIdx[threadIdx.x];
+  }
 }
 
 // set2average (limiting_kernels.cu:1792-1828) on one nodal column of Ns
@@ -1298,7 +1319,8 @@ hipError_t launch_limit_pr(Solver& s, double* U, hipStream_t st) {
 // Resolve sequential-order flags from per-element first hits.
 // act_set: the hit pass has already set every element active in s.sens_act.
 static hipError_t sensor_resolve(Solver& s, const int* nb, const int* init, const int* tgt, const int* val,
-                                 int* sensors, hipStream_t st, bool act_set = false) {
+                                 int* sensors, hipStream_t st, bool act_set = false, int* list = nullptr,
+                                 int* count = nullptr) {
   int* act0 = s.sens_act;
   int* act1 = s.sens_act + s.N_E;
   if (!act_set) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, act0, (long)s.N_E, 1);
@@ -1323,7 +1345,7 @@ static hipError_t sensor_resolve(Solver& s, const int* nb, const int* init, cons
       if (!h) break;
     }
     hipLaunchKernelGGL(k_sensor_final, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.N_N, nb, init, tgt, val,
-                       act0, sensors);
+                       act0, sensors, list, count);
     return hipGetLastError();
   }
   for (int it = 0; it <= s.N_E; ++it) {
@@ -1351,7 +1373,8 @@ hipError_t launch_uavg(const Solver& s, const double* U, double* Uavg, hipStream
   });
 }
 
-hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st) {
+hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg, hipStream_t st, int* list,
+                          int* count) {
   if (!Uavg) Uavg = s.Uavg;
   long ncol = (long)s.N_E * s.NF;
   hipError_t e = by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
@@ -1368,12 +1391,20 @@ hipError_t launch_sensing(Solver& s, const double* U, int* sensors, double* Uavg
   int* init = s.sens_act + 2 * (long)s.N_E;
   auto hit = [&](auto d, auto ph) -> hipError_t {
     hipLaunchKernelGGL((k_sensor_hit<decltype(d)::value, decltype(ph)::value>), dim3(nblk(s.N_E, 256)), dim3(256), 0,
-                       st, s.N_E, s.N_N, s.t.neighbors, Uavg, th, tgt, val, init, s.sens_act);
+                       st, s.N_E, s.N_N, s.t.neighbors, Uavg, th, tgt, val, init, s.sens_act,
+                       list && s.sens_changed ? count : nullptr);
     return hipGetLastError();
   };
   e = by_dim_phys(s.D, s.cfg.physics, hit);
   if (e != hipSuccess) return e;
-  return sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st, true);
+  if (list && !s.sens_changed) {  // the per-sweep loop below lists nothing: list afterwards
+    hipError_t r = sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st, true);
+    if (r != hipSuccess) return r;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, count, 1L, 0);
+    hipLaunchKernelGGL(k_flag_list, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, sensors, list, count);
+    return hipGetLastError();
+  }
+  return sensor_resolve(s, s.t.neighbors, init, tgt, val, sensors, st, true, list, count);
 }
 
 // One sweep of the flagged HR over the compacted list (see launch_limit_flagged).
@@ -1458,10 +1489,11 @@ hipError_t launch_limit_p0i(Solver& s, double* U, hipStream_t st) {
   return hipGetLastError();
 }
 
+// the sensing lists the flagged elements itself (k_sensor_final)
 hipError_t launch_limit_hri(Solver& s, double* U, hipStream_t st) {
-  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st, s.flag_list, s.flag + 2);
   if (e != hipSuccess) return e;
-  return launch_limit_flagged(s, U, st);
+  return launch_limit_listed(s, U, st);
 }
 
 // exported for the reference-named launchers (compat.hip)
