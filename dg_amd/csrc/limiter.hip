@@ -1448,18 +1448,14 @@ hipError_t launch_limit_listed(Solver& s, double* U, hipStream_t st) {
   });
 }
 
-// Limiting::M2Ilimiting (limiting.cc:682-720), two-fluid: sensing, the
-// flagged elements listed once, then m2i1D + copy_detected per direction over
-// the list.
+// Limiting::M2Ilimiting (limiting.cc:682-720), two-fluid: sensing, which
+// lists the flagged elements once, then m2i1D + copy_detected per direction
+// over the list.
 hipError_t launch_limit_m2i(Solver& s, double* U, hipStream_t st) {
   if (s.cfg.physics != DGX_MULTIFLUID) return hipErrorInvalidValue;
-  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st, s.flag_list, s.flag + 2);  // also lists the flags
   if (e != hipSuccess) return e;
   const int* count = s.flag + 2;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag + 2, 1L, 0);
-  hipLaunchKernelGGL(k_flag_list, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.sensors, s.flag_list,
-                     s.flag + 2);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
   auto sweep = [&](int offxy, const double* L2M, const double* M2L) -> hipError_t {
     hipError_t r = by_order_ns(s.D, s.P, [&](auto d, auto p) -> hipError_t {
       hipLaunchKernelGGL((k_m2i_list<decltype(d)::value, decltype(p)::value>), dim3(1024), dim3(64), 0, st, s.N_N,
@@ -1479,11 +1475,8 @@ hipError_t launch_limit_m2i(Solver& s, double* U, hipStream_t st) {
 // elements set to their cell averages.  refArea is 4 for the quadrilateral
 // (main.cc:286); the reference leaves it unset for segments, here 2.
 hipError_t launch_limit_p0i(Solver& s, double* U, hipStream_t st) {
-  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st);
+  hipError_t e = launch_sensing(s, U, s.sensors, s.Uavg, st, s.flag_list, s.flag + 2);  // also lists the flags
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, st, s.flag + 2, 1L, 0);
-  hipLaunchKernelGGL(k_flag_list, dim3(nblk(s.N_E, 256)), dim3(256), 0, st, s.N_E, s.sensors, s.flag_list,
-                     s.flag + 2);
   hipLaunchKernelGGL(k_p0i_list, dim3(1024), dim3(256), 0, st, s.NF, s.N_s, s.N_G, s.D == 1 ? 2.0 : 4.0, s.t.phi_w,
                      s.flag_list, s.flag + 2, U);
   return hipGetLastError();
